@@ -1,0 +1,231 @@
+/*
+ * zsba.h — C ABI of the MI355X bundle-adjustment engine (libzsba.so).
+ *
+ * The engine is a drop-in for the Levenberg–Marquardt / Schur bundle-adjustment
+ * optimiser of ZeroSLAM's factor graph:
+ *   factor_graph::factor_graph::{add_vertex, add_edge, remove_vertex, remove_edge,
+ *   solve, get_current_chi}            (reference source/factor_graph.hpp:385-629)
+ * specialised, as the reference's solver itself is, to SE(3) pose vertices
+ * (vertex_pose, factor_graph.hpp:938-961), 3-D point vertices (vertex_point_xyz,
+ * factor_graph.hpp:963-969, always marginalised) and 2-D pinhole reprojection
+ * edges (edge_reprojection<camera::pinhole>, factor_graph.hpp:973-1049).
+ *
+ * Conventions
+ *   - Pose parameters are the reference layout [tx, ty, tz, qx, qy, qz, qw] of a
+ *     world-to-camera transform (factor_graph.hpp:941, 953-959).
+ *   - Ids are insertion indices (0, 1, 2, ...) per kind and are never reused.
+ *     Slot order in the solve is insertion order of live items, as
+ *     factor_graph::set_ordering (factor_graph.hpp:633-645) assigns it.
+ *   - Observation order is insertion order (factor_graph.hpp:444-454, 664).
+ *   - No function throws or aborts.  Functions returning int32_t return a value
+ *     >= 0 on success and a negative ZSBA_E* code on failure; zsba_last_error()
+ *     then holds a message.  Handles are not thread-safe: use one per host thread.
+ *   - All input arrays are host memory and are copied; the caller keeps ownership.
+ */
+#ifndef ZSBA_H
+#define ZSBA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZSBA_ABI_VERSION 1
+
+/* Error codes (negative returns). */
+#define ZSBA_EINVAL   (-1) /* bad argument (null pointer, id out of range, NaN size ...) */
+#define ZSBA_ESTATE   (-2) /* call not valid in the handle's current state            */
+#define ZSBA_EDEVICE  (-3) /* HIP runtime error (no device, launch failure, ...)      */
+#define ZSBA_ENOMEM   (-4) /* device or host allocation failed                         */
+#define ZSBA_EUNSUP   (-5) /* a graph shape the engine does not support                */
+
+/* Robust loss kinds: factor_graph.hpp:851-934.  ZSBA_LOSS_NONE is an edge with no
+ * loss object (loss_function == nullptr, factor_graph.hpp:222-224, 255-258). */
+#define ZSBA_LOSS_NONE    0
+#define ZSBA_LOSS_TRIVIAL 1
+#define ZSBA_LOSS_HUBER   2
+#define ZSBA_LOSS_CAUCHY  3
+#define ZSBA_LOSS_TUKEY   4
+
+/* Arithmetic of a solve.
+ *   FP64:  every kernel in binary64 with the reference's operation order per
+ *          observation and no contraction; the result matches the reference
+ *          optimiser to rounding of the reductions.
+ *   MIXED: residual/Jacobian/Hessian-block/Schur work in binary32 with binary32
+ *          accumulation; the LM cost, its comparison, the reduced-camera Cholesky
+ *          and iterative refinement of the reduced solve stay binary64. */
+#define ZSBA_PRECISION_FP64  0
+#define ZSBA_PRECISION_MIXED 1
+
+typedef struct zsba_options {
+    int32_t device;       /* HIP device ordinal (default 0)                          */
+    int32_t verbose;      /* 1: print the reference's [INIT]/[GOOD]/[ BAD] lines     */
+    int32_t refine_steps; /* MIXED: refinement steps of the reduced solve (default 2)*/
+    int32_t reserved[5];
+} zsba_options;
+
+/* Per-solve statistics (filled by zsba_get_stats after zsba_solve). */
+typedef struct zsba_stats {
+    int32_t iterations;      /* outer LM iterations executed (factor_graph.hpp:538)   */
+    int32_t attempts;        /* linear solves attempted                               */
+    int32_t accepted;        /* accepted steps (the value zsba_solve returns)         */
+    int32_t rejected;        /* steps evaluated and rejected                          */
+    int32_t solve_failures;  /* attempts whose Cholesky decompose/solve failed        */
+    int32_t precision;       /* precision of the last solve                           */
+    int64_t n_obs, n_poses, n_points; /* live sizes of the last solve                 */
+    double initial_chi2;     /* chi2 at the start of the last solve                   */
+    double final_chi2;       /* chi2 after the last accepted step                     */
+    double lambda;           /* damping at exit                                       */
+    double solve_ms;         /* wall time of the numeric solve, device-synchronised   */
+    double setup_ms;         /* upload + structure analysis (0 when cached)           */
+    double reserved[8];
+} zsba_stats;
+
+typedef struct zsba_graph zsba_graph;
+
+/* ---- lifetime ---------------------------------------------------------------- */
+int32_t     zsba_abi_version(void);
+zsba_graph* zsba_create(const zsba_options* options /* NULL = defaults */);
+void        zsba_destroy(zsba_graph* graph);
+const char* zsba_last_error(const zsba_graph* graph);
+
+/* ---- graph construction --------------------------------------------------------
+ * zsba_add_pose / zsba_add_point replace factor_graph::add_vertex
+ * (factor_graph.hpp:385-402) for vertex_pose (not marginalised) and
+ * vertex_point_xyz (marginalised, map.hpp:158-162).  They return the new id. */
+int32_t zsba_add_pose(zsba_graph* graph, const double params[7], int32_t fixed);
+int32_t zsba_add_point(zsba_graph* graph, const double xyz[3], int32_t fixed);
+int32_t zsba_add_poses(zsba_graph* graph, int32_t count, const double* params /* count*7 */,
+                       const uint8_t* fixed /* count, NULL = all free */);
+int32_t zsba_add_points(zsba_graph* graph, int32_t count, const double* xyz /* count*3 */,
+                        const uint8_t* fixed /* count, NULL = all free */);
+
+/* zsba_add_observation replaces constructing edge_reprojection<pinhole>(camera)
+ * with set_observation / set_information / set_loss_function / add_vertex(pose),
+ * add_vertex(point) and factor_graph::add_edge (factor_graph.hpp:198-276, 444-454,
+ * 984-987).  info = row-major 2x2 information (NULL = identity,
+ * factor_graph.hpp:181); K = (fx, fy, cx, cy) (camera.hpp:71-80).  Returns the id. */
+int32_t zsba_add_observation(zsba_graph* graph, int32_t pose_id, int32_t point_id,
+                             const double uv[2], const double info[4] /* NULL = I */,
+                             const double K[4], int32_t loss_kind, double loss_delta);
+/* Bulk form with one camera and one loss for all observations.  Returns first id. */
+int32_t zsba_add_observations(zsba_graph* graph, int32_t count, const int32_t* pose_ids,
+                              const int32_t* point_ids, const double* uv /* count*2 */,
+                              const double* info /* count*4, NULL = identity */,
+                              const double K[4], int32_t loss_kind, double loss_delta);
+
+/* Removal: factor_graph::remove_edge / remove_vertex (factor_graph.hpp:405-470).
+ * Removing a vertex removes its observations.  Return 0, or ZSBA_EINVAL if the id
+ * is unknown or already removed (the reference returns false). */
+int32_t zsba_remove_observation(zsba_graph* graph, int32_t obs_id);
+int32_t zsba_remove_pose(zsba_graph* graph, int32_t pose_id);
+int32_t zsba_remove_point(zsba_graph* graph, int32_t point_id);
+
+/* Parameter access (vertex_base::set_parameters / get_parameters / set_fixed,
+ * factor_graph.hpp:107-129). */
+int32_t zsba_set_pose(zsba_graph* graph, int32_t pose_id, const double params[7]);
+int32_t zsba_set_point(zsba_graph* graph, int32_t point_id, const double xyz[3]);
+int32_t zsba_set_pose_fixed(zsba_graph* graph, int32_t pose_id, int32_t fixed);
+int32_t zsba_set_point_fixed(zsba_graph* graph, int32_t point_id, int32_t fixed);
+/* counts[0..2] = poses, points, observations ever added (array sizes for getters). */
+int32_t zsba_counts(const zsba_graph* graph, int64_t counts[3]);
+/* Copy current parameters, indexed by id (removed ids are left untouched). */
+int32_t zsba_get_poses(const zsba_graph* graph, double* params /* n_added*7 */);
+int32_t zsba_get_points(const zsba_graph* graph, double* xyz /* n_added*3 */);
+
+/* ---- optimisation -------------------------------------------------------------
+ * zsba_solve replaces factor_graph::solve(iterations, use_relative_convergence)
+ * (factor_graph.hpp:489-615) and returns the accepted-step count; 0 for a graph
+ * without observations or vertices (factor_graph.hpp:494-499). */
+int32_t zsba_solve(zsba_graph* graph, int32_t iterations, int32_t use_relative_convergence,
+                   int32_t precision);
+/* factor_graph::get_current_chi(recompute) (factor_graph.hpp:618-629).  With
+ * recompute = 0 it sums the residuals stored by the last evaluation.  Returns NaN
+ * on error. */
+double  zsba_current_chi(zsba_graph* graph, int32_t recompute);
+/* Inlier flag per observation id: 1 when the robust loss is in its quadratic
+ * region (e2 <= delta^2 for Huber, factor_graph.hpp:873) at the current state,
+ * 1 for every observation without a loss.  Removed ids are left untouched. */
+int32_t zsba_get_inlier_flags(zsba_graph* graph, uint8_t* flags /* n_obs_added */);
+/* Per-observation residuals (obs - projection, factor_graph.hpp:990-1009) at the
+ * current state, indexed by id.  */
+int32_t zsba_get_residuals(zsba_graph* graph, double* residuals /* n_obs_added*2 */);
+int32_t zsba_get_stats(const zsba_graph* graph, zsba_stats* stats);
+
+/* ---- batched local BA (no reference counterpart: many independent graphs) -----
+ * Solves every graph with the same iteration count, convergence rule and
+ * precision in one device-side launch sequence.  accepted[i] receives what
+ * zsba_solve would have returned for graphs[i]. */
+int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations,
+                         int32_t use_relative_convergence, int32_t precision,
+                         int32_t* accepted /* count */);
+
+/* ---- implicit differentiation (no reference counterpart) ----------------------
+ * At the converged state of the last FP64 solve, propagate dL/d(points) (and
+ * optionally dL/d(pose local increments)) to the observations and the camera
+ * intrinsics through the optimality condition of the final normal equations.
+ * Requires one camera for all observations.  Outputs are indexed by id. */
+int32_t zsba_adjoint(zsba_graph* graph, const double* dL_dpoints /* n_points_added*3 */,
+                     const double* dL_dposes /* n_poses_added*6 or NULL */,
+                     double* dL_dobs /* n_obs_added*2 */, double dL_dK[4]);
+
+/* ---- diagnostics used by the parity tests -------------------------------------
+ * Per-observation FP64 evaluation on the device at the current state, in
+ * observation-id order, 27 doubles each:
+ *   [0..1] residual, [2..13] pose Jacobian 2x6, [14..19] point Jacobian 2x3,
+ *   [20] rho' (drho), [21..24] robust information 2x2, [25] chi2, [26] robust chi2
+ * (factor_graph.hpp:217-269, 990-1048). */
+#define ZSBA_EDGE_EVAL_STRIDE 27
+int32_t zsba_debug_eval_edges(zsba_graph* graph, double* out /* n_obs_added*27 */);
+/* Builds the damped reduced camera system at the current state with damping
+ * lambda, in the given precision: S (n x n row-major, lower triangle valid),
+ * b_s (n), the pose step (n) and the point step (3 * n_points), where
+ * n = 6 * live poses.  Returns 1 if the Cholesky decompose/solve succeeded, 0 if
+ * it failed (factor_graph.hpp:736-806). */
+int32_t zsba_debug_reduced_system(zsba_graph* graph, double lambda, int32_t precision,
+                                  double* S, double* b_s, double* dx_pose, double* dx_point);
+
+/* ---- synthetic problems (benchmark and test workloads) ------------------------ */
+typedef struct zsba_synth_params {
+    int32_t layout;       /* 0 circle look-at, 1 forward walk, 2 double loop facing out */
+    int32_t n_frames;
+    int32_t n_points;
+    int32_t track_min;    /* track length range of a landmark (frames)             */
+    int32_t track_max;
+    int32_t track_geometric; /* 1: geometric track lengths on [min,max], mean track_mean */
+    double  track_mean;
+    double  revisit_rate; /* layout 2: share of landmarks also seen on the other lap  */
+    double  noise_px;     /* sigma of the pixel noise                                   */
+    double  outlier_rate; /* share of observations replaced by uniform image points    */
+    int32_t loss_kind;
+    double  loss_delta;
+    double  rot_perturb;  /* rad, applied to non-gauge poses                            */
+    double  trans_perturb;/* m                                                          */
+    double  point_perturb;/* sigma (m) per coordinate                                   */
+    int32_t n_fixed;      /* gauge: the first n_fixed poses are fixed                   */
+    double  K[4];
+    int32_t width, height;
+    int32_t reserved[4];
+} zsba_synth_params;
+
+typedef struct zsba_synth zsba_synth;
+/* Defaults for the five benchmark configurations (0..4) of the engine. */
+int32_t     zsba_synth_default_params(int32_t config, zsba_synth_params* params);
+zsba_synth* zsba_synth_create(const zsba_synth_params* params, uint64_t seed);
+void        zsba_synth_destroy(zsba_synth* synth);
+/* sizes[0..2] = poses, points, observations */
+void        zsba_synth_sizes(const zsba_synth* synth, int64_t sizes[3]);
+/* Any output pointer may be NULL.  true_* are the unperturbed values; outlier
+ * flags mark observations replaced by uniform image points. */
+void        zsba_synth_copy(const zsba_synth* synth, double* poses, uint8_t* pose_fixed,
+                            double* points, uint8_t* point_fixed, int32_t* obs_pose,
+                            int32_t* obs_point, double* obs_uv, double* true_poses,
+                            double* true_points, uint8_t* outlier);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZSBA_H */

@@ -1,0 +1,188 @@
+"""TEST INFRASTRUCTURE — Python access to the CPU oracle.
+
+Loads ``oracle/libzsba_oracle.so`` (the binary64 restatement of the reference
+optimiser, ``zsba_oracle.cpp``) and, where it was built, ``oracle/_ref/libzsba_ref.so``
+(the reference's own factor graph compiled from its sources).  Only tests, the
+smoke check and the benchmark's CPU-baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "libzsba_oracle.so"
+REF_SO = HERE / "_ref" / "libzsba_ref.so"
+
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_U8 = C.POINTER(C.c_uint8)
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def _bind(lib, prefix: str) -> None:
+    sig = {
+        "create": (C.c_void_p, [C.c_int32, _D, _U8, C.c_int32, _D, _U8, C.c_int32, _I32, _I32, _D, _D, _D, _I32, _D,
+                                C.c_int32]),
+        "destroy": (None, [C.c_void_p]),
+        "solve": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32]),
+        "current_chi": (C.c_double, [C.c_void_p, C.c_int32]),
+        "get_state": (None, [C.c_void_p, _D, _D]),
+        "set_state": (None, [C.c_void_p, _D, _D]),
+        "edge_eval": (None, [C.c_void_p, C.c_int32, _D]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, prefix + name)
+        fn.restype, fn.argtypes = res, args
+    if prefix == "zo_":
+        extra = {
+            "zo_reduced_system": (C.c_int32, [C.c_void_p, C.c_double, _D, _D, _D, _D]),
+            "zo_hessian": (None, [C.c_void_p, _D, _D, _D]),
+            "zo_stats": (None, [C.c_void_p, _I32, _D, _D]),
+            "zo_trace": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_int64]),
+            "zo_cholesky": (C.c_int32, [_D, C.c_int32, _D]),
+            "zo_cholesky_solve": (C.c_int32, [_D, _D, C.c_int32, _D]),
+            "zo_invert3": (C.c_int32, [_D, _D]),
+            "zo_pose_plus": (None, [_D, _D]),
+            "zo_pose_apply": (None, [_D, _D, _D]),
+            "zo_so3_exp": (None, [_D, _D]),
+            "zo_so3_left_jacobian": (None, [_D, _D]),
+            "zo_loss": (None, [C.c_int32, C.c_double, C.c_double, _D]),
+        }
+        for name, (res, args) in extra.items():
+            fn = getattr(lib, name)
+            fn.restype, fn.argtypes = res, args
+
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def ensure_built() -> None:
+    if not ORACLE_SO.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE), "oracle"], check=True)
+
+
+def oracle_lib() -> C.CDLL:
+    if "zo" not in _libs:
+        ensure_built()
+        lib = C.CDLL(str(ORACLE_SO))
+        _bind(lib, "zo_")
+        _libs["zo"] = lib
+    return _libs["zo"]
+
+
+def have_reference() -> bool:
+    return REF_SO.exists()
+
+
+def reference_lib() -> C.CDLL:
+    if "zr" not in _libs:
+        if not REF_SO.exists():
+            raise RuntimeError("oracle/_ref was not built (needs the reference checkout)")
+        lib = C.CDLL(str(REF_SO))
+        _bind(lib, "zr_")
+        _libs["zr"] = lib
+    return _libs["zr"]
+
+
+class CpuGraph:
+    """One factor graph in the oracle (kind='oracle') or the compiled reference
+    (kind='reference'), built from a Problem in insertion order."""
+
+    def __init__(self, problem, kind: str = "oracle", verbose: int = 0):
+        self.kind = kind
+        self.lib = oracle_lib() if kind == "oracle" else reference_lib()
+        self.pre = "zo_" if kind == "oracle" else "zr_"
+        pr = problem
+        self._keep = [np.ascontiguousarray(a) for a in (
+            pr.poses, pr.pose_fixed, pr.points, pr.point_fixed, pr.obs_pose, pr.obs_point, pr.obs_uv,
+            pr.obs_K, pr.obs_loss, pr.obs_delta)]
+        info = None if pr.obs_info is None else np.ascontiguousarray(pr.obs_info, dtype=np.float64)
+        poses, pfix, points, lfix, opose, opoint, uv, K, loss, delta = self._keep
+        self.n_poses, self.n_points, self.n_obs = len(poses), len(points), len(opose)
+        self.h = getattr(self.lib, self.pre + "create")(
+            self.n_poses, _p(poses), _p(pfix, C.c_uint8), self.n_points, _p(points), _p(lfix, C.c_uint8),
+            self.n_obs, _p(opose, C.c_int32), _p(opoint, C.c_int32), _p(uv), _p(info), _p(K),
+            _p(loss, C.c_int32), _p(delta), int(verbose))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            getattr(self.lib, self.pre + "destroy")(h)
+            self.h = None
+
+    def solve(self, iterations: int, use_relative_convergence: bool = False) -> int:
+        return getattr(self.lib, self.pre + "solve")(self.h, int(iterations), int(use_relative_convergence))
+
+    def solve_capture(self, iterations: int, use_relative_convergence: bool = False) -> tuple[int, str]:
+        """Solve with the process's stderr captured (the reference prints its
+        verbose LM trace there)."""
+        import tempfile
+        with tempfile.TemporaryFile(mode="w+b") as tmp:
+            saved = os.dup(2)
+            os.dup2(tmp.fileno(), 2)
+            try:
+                accepted = self.solve(iterations, use_relative_convergence)
+            finally:
+                C.CDLL(None).fflush(None)
+                os.dup2(saved, 2)
+                os.close(saved)
+            tmp.seek(0)
+            return accepted, tmp.read().decode()
+
+    def current_chi(self, recompute: bool = True) -> float:
+        return getattr(self.lib, self.pre + "current_chi")(self.h, int(recompute))
+
+    def state(self) -> tuple[np.ndarray, np.ndarray]:
+        poses = np.zeros((self.n_poses, 7))
+        points = np.zeros((self.n_points, 3))
+        getattr(self.lib, self.pre + "get_state")(self.h, _p(poses), _p(points))
+        return poses, points
+
+    def set_state(self, poses, points) -> None:
+        poses = np.ascontiguousarray(poses, dtype=np.float64)
+        points = np.ascontiguousarray(points, dtype=np.float64)
+        getattr(self.lib, self.pre + "set_state")(self.h, _p(poses), _p(points))
+
+    def edge_eval(self, edges=None) -> np.ndarray:
+        idx = range(self.n_obs) if edges is None else edges
+        out = np.zeros((len(idx), 27))
+        fn = getattr(self.lib, self.pre + "edge_eval")
+        for k, e in enumerate(idx):
+            fn(self.h, int(e), _p(out[k]))
+        return out
+
+    # oracle-only diagnostics
+    def reduced_system(self, lam: float):
+        P, M = 6 * self.n_poses, 3 * self.n_points
+        S = np.zeros((P, P)); bs = np.zeros(P); xp = np.zeros(P); xl = np.zeros(M)
+        ok = self.lib.zo_reduced_system(self.h, float(lam), _p(S), _p(bs), _p(xp), _p(xl))
+        return bool(ok), S, bs, xp, xl
+
+    def hessian(self):
+        hpp = np.zeros((self.n_poses, 6, 6)); hll = np.zeros((self.n_points, 3, 3))
+        b = np.zeros(6 * self.n_poses + 3 * self.n_points)
+        self.lib.zo_hessian(self.h, _p(hpp), _p(hll), _p(b))
+        return hpp, hll, b
+
+    def stats(self) -> dict:
+        out = np.zeros(5, np.int32); ic = C.c_double(); lam = C.c_double()
+        self.lib.zo_stats(self.h, _p(out, C.c_int32), C.byref(ic), C.byref(lam))
+        keys = ("iterations", "attempts", "accepted", "rejected", "solve_failures")
+        d = {k: int(v) for k, v in zip(keys, out)}
+        d["initial_chi2"] = ic.value
+        d["lambda"] = lam.value
+        return d
+
+    def trace(self) -> str:
+        n = self.lib.zo_trace(self.h, None, 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        self.lib.zo_trace(self.h, buf, n + 1)
+        return buf.value.decode()
