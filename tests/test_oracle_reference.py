@@ -1,0 +1,42 @@
+"""The oracle against the reference's own optimiser compiled from its sources
+(oracle/_ref), where that build exists: bit-identical results on the benchmark
+workloads and on randomised small graphs."""
+import numpy as np
+import pytest
+
+from oracle.oracle import CpuGraph, have_reference
+from tests.golden.make_golden import mixed_problem
+from zeroslam_amd.problem import synth
+
+pytestmark = pytest.mark.skipif(not have_reference(), reason="oracle/_ref not built (no reference checkout)")
+
+
+def bitwise_same(pr, iters, use_rel=False):
+    out = []
+    for kind in ("oracle", "reference"):
+        g = CpuGraph(pr, kind, verbose=1)
+        acc, txt = g.solve_capture(iters, use_rel)
+        P, X = g.state()
+        out.append((acc, g.current_chi(True), g.current_chi(False), P, X, g.trace() if kind == "oracle" else txt))
+    a, b = out
+    assert a[0] == b[0]
+    assert np.float64(a[1]).view(np.uint64) == np.float64(b[1]).view(np.uint64)
+    assert np.float64(a[2]).view(np.uint64) == np.float64(b[2]).view(np.uint64)
+    assert np.array_equal(a[3].view(np.uint64), b[3].view(np.uint64))
+    assert np.array_equal(a[4].view(np.uint64), b[4].view(np.uint64))
+    assert a[5] == b[5]
+
+
+@pytest.mark.parametrize("seed", [21, 22, 23])
+def test_random_huber_windows(seed):
+    pr = synth(1, seed=seed, n_frames=10 + seed % 5, n_points=300, track_min=2, track_max=10)
+    bitwise_same(pr, 10, use_rel=seed % 2 == 0)
+
+
+def test_mixed_losses_information_cameras():
+    bitwise_same(mixed_problem(), 8)
+
+
+def test_config1_three_iterations():
+    """The 100-keyframe / 20k-point / 160k-observation workload."""
+    bitwise_same(synth(1), 3)

@@ -1,0 +1,114 @@
+// Device data layout of one bundle-adjustment problem and the launchers the
+// host engine calls.  Everything lives in HBM as structure-of-arrays.
+//
+// Observation order on the device ("sorted order", index s): grouped by point
+// slot, and within a point by insertion order, so each point's track is a
+// contiguous segment [lm_start[l], lm_start[l+1]).  obs_orig[s] is the
+// observation's index in insertion order (the reference's edge order).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace zb {
+
+// One W block (6x3 pose-point coupling) exists per distinct (pose, point) pair
+// whose pose and point are both free (factor_graph.hpp:669-709).
+struct dev_problem {
+    int32_t np = 0, nl = 0, no = 0;  // live poses, points, observations
+    int32_t npair = 0;               // W blocks
+    int32_t nblk = 0;                // lower-triangle 6x6 blocks of the reduced system
+    int64_t ncontrib = 0;            // (pair a, pair b) products feeding those blocks
+    int32_t n = 0;                   // 6 * np
+    int32_t identity_info = 1;       // all information matrices are I
+    // observations, sorted order
+    const int32_t* obs_pose = nullptr;   // [no] pose slot
+    const int32_t* obs_point = nullptr;  // [no] point slot
+    const int32_t* obs_pair = nullptr;   // [no] W block of the observation, -1 if none
+    const int32_t* obs_param = nullptr;  // [no] camera index | (loss index << 16)
+    const int32_t* obs_orig = nullptr;   // [no] insertion-order index
+    const double* obs_uv = nullptr;      // [no*2]
+    const double* obs_info = nullptr;    // [no*4] (unused when identity_info)
+    // segments
+    const int32_t* lm_start = nullptr;   // [nl+1] observations of a point
+    const int32_t* pose_start = nullptr; // [np+1] into pose_list
+    const int32_t* pose_list = nullptr;  // [no] sorted indices of a pose's observations, insertion order
+    const int32_t* lm_pair_start = nullptr; // [nl+1] W blocks of a point (first-occurrence order)
+    const int32_t* pair_pose = nullptr;  // [npair]
+    const int32_t* pair_point = nullptr; // [npair]
+    const int32_t* pair_first = nullptr; // [npair] sorted index of the pair's first observation
+    // reduced-system blocks: block b = (blk_row[b], blk_col[b]), row >= col,
+    // fed by contributions [blk_start[b], blk_start[b+1]) of (pair a of row, pair c of col)
+    const int32_t* blk_row = nullptr;
+    const int32_t* blk_col = nullptr;
+    const int64_t* blk_start = nullptr;
+    const int2* contrib = nullptr;
+    const int32_t* diag_blk = nullptr;   // [np] block index of (i, i)
+    // vertices
+    const uint8_t* pose_fixed = nullptr;
+    const uint8_t* point_fixed = nullptr;
+    // parameter tables
+    const double* cams = nullptr;        // [ncam*4]
+    const int32_t* loss_kind = nullptr;  // [nloss]
+    const double* loss_delta = nullptr;  // [nloss]
+};
+
+// LM controller state, kept on the device so that attempts need no host round
+// trip (factor_graph.hpp:511-611).
+struct lm_state {
+    double lambda, nu, chi2, last_chi2, chi2_new, scale, rho, initial_chi2, legacy_threshold;
+    int32_t iterations, use_relative;
+    int32_t iter, failures, success, attempts, rejected, solve_failures;
+    int32_t done, need_linearize, need_rollback, solve_ok, step_applied, executed_iterations;
+    int32_t pad[4];
+};
+
+// Working buffers of the binary64 path.
+struct dev_work64 {
+    double* pose = nullptr;         // [np*7] current parameters
+    double* pose_backup = nullptr;
+    double* point = nullptr;        // [nl*3]
+    double* point_backup = nullptr;
+    double* residual = nullptr;     // [no*2] residuals of the last evaluation, sorted order
+    double* pose_terms = nullptr;   // [no*27] per-observation H_pp lower (21) + v_p (6)
+    double* Hpp = nullptr;          // [np*21]
+    double* bp = nullptr;           // [np*6]
+    double* Hll = nullptr;          // [nl*9]
+    double* bl = nullptr;           // [nl*3]
+    double* W = nullptr;            // [npair*18]
+    double* Hinv = nullptr;         // [nl*9]
+    double* Y = nullptr;            // [npair*18]
+    double* S = nullptr;            // [n*n] reduced system, factorised in place
+    double* bs = nullptr;           // [n]
+    double* dxp = nullptr;          // [n]
+    double* dxl = nullptr;          // [nl*3]
+    double* partial = nullptr;      // reduction partials
+    double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
+    int32_t* chol_status = nullptr; // [1] 0 ok, 1 decompose failed, 2 solve failed
+    lm_state* lm = nullptr;
+};
+
+constexpr int kPartials = 1024;  // reduction partial slots
+
+// ---- binary64 kernels (kernels_fp64.hip) ----------------------------------
+void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target /*0 chi2, 1 chi2_new*/,
+                   bool only_if_step);
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
+void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override);
+void launch_after_solve64(const dev_work64& w, hipStream_t s);
+void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_decide64(const dev_work64& w, hipStream_t s);
+void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s);
+void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s);
+void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+
+// ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
+// Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from
+// w.bs.  Sets *w.chol_status and lm->solve_ok; skips everything when the
+// controller is done (gated) or a previous stage failed.
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated);
+
+}  // namespace zb

@@ -1,0 +1,830 @@
+// Host side of the engine: the C ABI of include/zsba.h.
+//
+// A handle owns a host copy of the graph (indexed by insertion id), the device
+// copy in structure-of-arrays form, and one HIP stream.  zsba_solve analyses the
+// structure once (observation order by point, W blocks, the lower-triangle
+// blocks of the reduced camera system and the products feeding each of them),
+// uploads, and then runs the Levenberg-Marquardt loop of factor_graph::solve
+// (factor_graph.hpp:489-615) entirely on the device: the host enqueues attempt
+// after attempt and only reads the controller's `done` flag, one attempt behind.
+#include "zsba.h"
+#include "common.hpp"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace zb {
+void launch_end_attempt64(const dev_work64& w, hipStream_t s);
+void launch_cost_stored64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+}  // namespace zb
+
+using zb::dev_problem;
+using zb::dev_work64;
+using zb::lm_state;
+
+namespace {
+
+struct loss_entry {
+    int32_t kind;
+    double delta;
+};
+
+struct dev_alloc {
+    std::vector<void*> ptrs;
+    size_t bytes = 0;
+    hipError_t status = hipSuccess;
+    template <class T>
+    T* get(size_t n) {
+        if (n == 0) n = 1;
+        void* p = nullptr;
+        const hipError_t e = hipMalloc(&p, n * sizeof(T));
+        if (e != hipSuccess) {
+            status = e;
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        bytes += n * sizeof(T);
+        return static_cast<T*>(p);
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+        bytes = 0;
+        status = hipSuccess;
+    }
+};
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct zsba_graph {
+    zsba_options opt{};
+    std::string err;
+
+    // host graph, indexed by id
+    std::vector<double> pose;
+    std::vector<uint8_t> pose_fixed, pose_alive;
+    std::vector<double> point;
+    std::vector<uint8_t> point_fixed, point_alive;
+    std::vector<int32_t> obs_pose, obs_point, obs_cam, obs_loss;
+    std::vector<double> obs_uv, obs_info;
+    std::vector<uint8_t> obs_alive, obs_has_info;
+    std::vector<std::array<double, 4>> cams;
+    std::vector<loss_entry> losses;
+
+    bool structure_dirty = true;
+    bool values_dirty = true;
+    bool device_newer = false;
+
+    // compaction of the last structure
+    std::vector<int32_t> pose_slot_id, point_slot_id, obs_e_id;  // slot -> id
+    std::vector<int32_t> pose_id_slot, point_id_slot;            // id -> slot (-1 dead)
+    std::vector<int32_t> sorted_e;                               // s -> e
+
+    // device
+    bool hip_ready = false;
+    hipStream_t stream = nullptr;
+    dev_alloc mem;
+    dev_problem dp;
+    dev_work64 w;
+    lm_state* lm_pinned = nullptr;
+    double* lambda_dev = nullptr;
+    uint8_t* flags_dev = nullptr;
+    double* eval_dev = nullptr;
+    size_t eval_capacity = 0;
+    double* snap_pose = nullptr;
+    double* snap_point = nullptr;
+    bool have_snapshot = false;
+
+    zsba_stats stats{};
+
+    ~zsba_graph() {
+        if (hip_ready) {
+            if (stream) (void)hipStreamSynchronize(stream);
+            mem.release();
+            if (eval_dev) (void)hipFree(eval_dev);
+            if (lm_pinned) (void)hipHostFree(lm_pinned);
+            if (stream) (void)hipStreamDestroy(stream);
+        }
+    }
+
+    int32_t fail(int32_t code, const char* fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    int32_t hip_fail(hipError_t e, const char* what) {
+        return fail(ZSBA_EDEVICE, "%s: %s", what, hipGetErrorString(e));
+    }
+
+    int32_t live_poses() const { return static_cast<int32_t>(std::count(pose_alive.begin(), pose_alive.end(), 1)); }
+    int32_t live_points() const { return static_cast<int32_t>(std::count(point_alive.begin(), point_alive.end(), 1)); }
+    int32_t live_obs() const { return static_cast<int32_t>(std::count(obs_alive.begin(), obs_alive.end(), 1)); }
+
+    int32_t ensure_hip() {
+        if (hip_ready) return 0;
+        int count = 0;
+        hipError_t e = hipGetDeviceCount(&count);
+        if (e != hipSuccess || count == 0) return fail(ZSBA_EDEVICE, "no HIP device available");
+        if (opt.device < 0 || opt.device >= count) return fail(ZSBA_EINVAL, "device %d out of range", opt.device);
+        if ((e = hipSetDevice(opt.device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        if ((e = hipHostMalloc(reinterpret_cast<void**>(&lm_pinned), sizeof(lm_state), hipHostMallocDefault)) != hipSuccess)
+            return hip_fail(e, "hipHostMalloc");
+        std::memset(lm_pinned, 0, sizeof(lm_state));
+        hip_ready = true;
+        return 0;
+    }
+
+    int32_t camera_index(const double K[4]) {
+        if (!(K[0] > 0.0) || !(K[1] > 0.0)) return fail(ZSBA_EINVAL, "focal lengths must be positive (camera.hpp:78-79)");
+        for (size_t i = 0; i < cams.size(); ++i)
+            if (std::memcmp(cams[i].data(), K, 4 * sizeof(double)) == 0) return static_cast<int32_t>(i);
+        if (cams.size() >= 65535) return fail(ZSBA_EUNSUP, "more than 65535 distinct cameras");
+        cams.push_back({K[0], K[1], K[2], K[3]});
+        return static_cast<int32_t>(cams.size() - 1);
+    }
+    int32_t loss_index(int32_t kind, double delta) {
+        if (kind < ZSBA_LOSS_NONE || kind > ZSBA_LOSS_TUKEY) return fail(ZSBA_EINVAL, "unknown loss kind %d", kind);
+        if (kind == ZSBA_LOSS_NONE || kind == ZSBA_LOSS_TRIVIAL) delta = 0.0;
+        for (size_t i = 0; i < losses.size(); ++i)
+            if (losses[i].kind == kind && std::memcmp(&losses[i].delta, &delta, sizeof(double)) == 0) return static_cast<int32_t>(i);
+        if (losses.size() >= 32767) return fail(ZSBA_EUNSUP, "more than 32767 distinct losses");
+        losses.push_back({kind, delta});
+        return static_cast<int32_t>(losses.size() - 1);
+    }
+
+    // Pull device parameters into the host arrays if the device holds newer ones.
+    int32_t sync_to_host() {
+        if (!device_newer) return 0;
+        std::vector<double> P(7 * pose_slot_id.size()), X(3 * point_slot_id.size());
+        hipError_t e;
+        if (!P.empty() && (e = hipMemcpy(P.data(), w.pose, P.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "download poses");
+        if (!X.empty() && (e = hipMemcpy(X.data(), w.point, X.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "download points");
+        for (size_t i = 0; i < pose_slot_id.size(); ++i) std::memcpy(&pose[7 * pose_slot_id[i]], &P[7 * i], 7 * sizeof(double));
+        for (size_t j = 0; j < point_slot_id.size(); ++j) std::memcpy(&point[3 * point_slot_id[j]], &X[3 * j], 3 * sizeof(double));
+        device_newer = false;
+        return 0;
+    }
+
+    int32_t build_structure();
+    int32_t upload_values();
+    int32_t prepare();
+    int32_t run_fp64(int32_t iterations, int32_t use_rel);
+    void print_trace();
+};
+
+// ---- structure analysis -----------------------------------------------------
+int32_t zsba_graph::build_structure() {
+    const int32_t np_all = static_cast<int32_t>(pose_alive.size());
+    const int32_t nl_all = static_cast<int32_t>(point_alive.size());
+    pose_slot_id.clear(); point_slot_id.clear(); obs_e_id.clear();
+    pose_id_slot.assign(np_all, -1);
+    point_id_slot.assign(nl_all, -1);
+    for (int32_t i = 0; i < np_all; ++i)
+        if (pose_alive[i]) { pose_id_slot[i] = static_cast<int32_t>(pose_slot_id.size()); pose_slot_id.push_back(i); }
+    for (int32_t j = 0; j < nl_all; ++j)
+        if (point_alive[j]) { point_id_slot[j] = static_cast<int32_t>(point_slot_id.size()); point_slot_id.push_back(j); }
+    for (int32_t k = 0; k < static_cast<int32_t>(obs_alive.size()); ++k)
+        if (obs_alive[k]) obs_e_id.push_back(k);
+    const int32_t np = static_cast<int32_t>(pose_slot_id.size());
+    const int32_t nl = static_cast<int32_t>(point_slot_id.size());
+    const int32_t no = static_cast<int32_t>(obs_e_id.size());
+
+    // observations of a point, insertion order (counting sort by point slot)
+    std::vector<int32_t> lm_start(nl + 1, 0), e_pose(no), e_point(no);
+    for (int32_t e = 0; e < no; ++e) {
+        const int32_t id = obs_e_id[e];
+        e_pose[e] = pose_id_slot[obs_pose[id]];
+        e_point[e] = point_id_slot[obs_point[id]];
+        lm_start[e_point[e] + 1]++;
+    }
+    for (int32_t l = 0; l < nl; ++l) lm_start[l + 1] += lm_start[l];
+    sorted_e.assign(no, 0);
+    std::vector<int32_t> e_sorted(no), fill(lm_start.begin(), lm_start.end() - 1);
+    for (int32_t e = 0; e < no; ++e) {
+        const int32_t s = fill[e_point[e]]++;
+        sorted_e[s] = e;
+        e_sorted[e] = s;
+    }
+    // observations of a pose, insertion order
+    std::vector<int32_t> pose_start(np + 1, 0), pose_list(no);
+    for (int32_t e = 0; e < no; ++e) pose_start[e_pose[e] + 1]++;
+    for (int32_t i = 0; i < np; ++i) pose_start[i + 1] += pose_start[i];
+    {
+        std::vector<int32_t> f(pose_start.begin(), pose_start.end() - 1);
+        for (int32_t e = 0; e < no; ++e) pose_list[f[e_pose[e]]++] = e_sorted[e];
+    }
+    // W blocks: distinct free (pose, point) pairs, first-occurrence order per point
+    std::vector<int32_t> obs_pair(no, -1), lm_pair_start(nl + 1, 0), pair_pose, pair_point, pair_first;
+    std::vector<uint8_t> pfix(np), lfix(nl);
+    for (int32_t i = 0; i < np; ++i) pfix[i] = pose_fixed[pose_slot_id[i]];
+    for (int32_t l = 0; l < nl; ++l) lfix[l] = point_fixed[point_slot_id[l]];
+    std::vector<int32_t> stamp(np, -1), stamp_pair(np, -1);
+    for (int32_t l = 0; l < nl; ++l) {
+        lm_pair_start[l] = static_cast<int32_t>(pair_pose.size());
+        if (!lfix[l]) {
+            for (int32_t s = lm_start[l]; s < lm_start[l + 1]; ++s) {
+                const int32_t k = e_pose[sorted_e[s]];
+                if (pfix[k]) continue;
+                if (stamp[k] != l) {
+                    stamp[k] = l;
+                    stamp_pair[k] = static_cast<int32_t>(pair_pose.size());
+                    pair_pose.push_back(k);
+                    pair_point.push_back(l);
+                    pair_first.push_back(s);
+                }
+                obs_pair[s] = stamp_pair[k];
+            }
+        }
+    }
+    lm_pair_start[nl] = static_cast<int32_t>(pair_pose.size());
+    const int32_t npair = static_cast<int32_t>(pair_pose.size());
+
+    // lower-triangle blocks of the reduced system and their contributions
+    std::vector<int64_t> block_key;  // row * np + col
+    std::vector<int32_t> blk_count;
+    const bool dense_index = static_cast<int64_t>(np) * np <= (int64_t(1) << 26);
+    std::vector<int32_t> dense_map;
+    std::unordered_map<int64_t, int32_t> hash_map;
+    if (dense_index) dense_map.assign(static_cast<size_t>(np) * np, -1);
+    auto block_of = [&](int32_t row, int32_t col) -> int32_t {
+        const int64_t key = static_cast<int64_t>(row) * np + col;
+        if (dense_index) {
+            int32_t& b = dense_map[key];
+            if (b < 0) { b = static_cast<int32_t>(block_key.size()); block_key.push_back(key); blk_count.push_back(0); }
+            return b;
+        }
+        auto it = hash_map.find(key);
+        if (it != hash_map.end()) return it->second;
+        const int32_t b = static_cast<int32_t>(block_key.size());
+        hash_map.emplace(key, b);
+        block_key.push_back(key);
+        blk_count.push_back(0);
+        return b;
+    };
+    for (int32_t i = 0; i < np; ++i) block_of(i, i);
+    for (int32_t l = 0; l < nl; ++l)
+        for (int32_t a = lm_pair_start[l]; a < lm_pair_start[l + 1]; ++a)
+            for (int32_t c = lm_pair_start[l]; c < lm_pair_start[l + 1]; ++c)
+                if (pair_pose[a] >= pair_pose[c]) blk_count[block_of(pair_pose[a], pair_pose[c])]++;
+    const int32_t nblk = static_cast<int32_t>(block_key.size());
+    std::vector<int32_t> order(nblk);
+    for (int32_t b = 0; b < nblk; ++b) order[b] = b;
+    std::sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return block_key[x] < block_key[y]; });
+    std::vector<int32_t> rank(nblk);
+    for (int32_t r = 0; r < nblk; ++r) rank[order[r]] = r;
+    std::vector<int64_t> blk_start(nblk + 1, 0);
+    std::vector<int32_t> blk_row(nblk), blk_col(nblk), diag_blk(np);
+    for (int32_t r = 0; r < nblk; ++r) {
+        const int32_t b = order[r];
+        blk_start[r + 1] = blk_start[r] + blk_count[b];
+        blk_row[r] = static_cast<int32_t>(block_key[b] / np);
+        blk_col[r] = static_cast<int32_t>(block_key[b] % np);
+        if (blk_row[r] == blk_col[r]) diag_blk[blk_row[r]] = r;
+    }
+    const int64_t ncontrib = blk_start[nblk];
+    std::vector<int2> contrib(static_cast<size_t>(ncontrib));
+    {
+        std::vector<int64_t> f(blk_start.begin(), blk_start.end() - 1);
+        for (int32_t l = 0; l < nl; ++l)
+            for (int32_t a = lm_pair_start[l]; a < lm_pair_start[l + 1]; ++a)
+                for (int32_t c = lm_pair_start[l]; c < lm_pair_start[l + 1]; ++c)
+                    if (pair_pose[a] >= pair_pose[c]) {
+                        const int32_t r = rank[block_of(pair_pose[a], pair_pose[c])];
+                        contrib[f[r]++] = make_int2(a, c);
+                    }
+    }
+
+    // per-observation parameters
+    std::vector<int32_t> obs_param(no), obs_pose_s(no), obs_point_s(no);
+    std::vector<double> uv(2L * no), info;
+    bool identity = true;
+    for (int32_t s = 0; s < no; ++s) {
+        const int32_t id = obs_e_id[sorted_e[s]];
+        if (obs_has_info[id]) {
+            const double* W = &obs_info[4L * id];
+            if (!(W[0] == 1.0 && W[1] == 0.0 && W[2] == 0.0 && W[3] == 1.0)) identity = false;
+        }
+    }
+    if (!identity) info.resize(4L * no);
+    for (int32_t s = 0; s < no; ++s) {
+        const int32_t e = sorted_e[s], id = obs_e_id[e];
+        obs_param[s] = obs_cam[id] | (obs_loss[id] << 16);
+        obs_pose_s[s] = e_pose[e];
+        obs_point_s[s] = e_point[e];
+        uv[2L * s] = obs_uv[2L * id];
+        uv[2L * s + 1] = obs_uv[2L * id + 1];
+        if (!identity) {
+            if (obs_has_info[id]) std::memcpy(&info[4L * s], &obs_info[4L * id], 4 * sizeof(double));
+            else { info[4L * s] = 1; info[4L * s + 1] = 0; info[4L * s + 2] = 0; info[4L * s + 3] = 1; }
+        }
+    }
+    std::vector<double> cam_table(4 * std::max<size_t>(cams.size(), 1), 0.0);
+    for (size_t c = 0; c < cams.size(); ++c) std::memcpy(&cam_table[4 * c], cams[c].data(), 4 * sizeof(double));
+    std::vector<int32_t> lk(std::max<size_t>(losses.size(), 1), 0);
+    std::vector<double> ld(std::max<size_t>(losses.size(), 1), 0.0);
+    for (size_t i = 0; i < losses.size(); ++i) { lk[i] = losses[i].kind; ld[i] = losses[i].delta; }
+
+    // allocate and upload
+    (void)hipStreamSynchronize(stream);
+    mem.release();
+    have_snapshot = false;
+    const int64_t n = 6L * np;
+    dev_problem p;
+    p.np = np; p.nl = nl; p.no = no; p.npair = npair; p.nblk = nblk; p.ncontrib = ncontrib;
+    p.n = static_cast<int32_t>(n); p.identity_info = identity ? 1 : 0;
+    auto up = [&](auto* dst, const auto& src) {
+        if (!src.empty() && dst)
+            (void)hipMemcpy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice);
+        return dst;
+    };
+    int32_t *d_obs_pose = mem.get<int32_t>(no), *d_obs_point = mem.get<int32_t>(no), *d_obs_pair = mem.get<int32_t>(no),
+            *d_obs_param = mem.get<int32_t>(no), *d_obs_orig = mem.get<int32_t>(no);
+    double* d_uv = mem.get<double>(2L * no);
+    double* d_info = identity ? nullptr : mem.get<double>(4L * no);
+    int32_t *d_lm_start = mem.get<int32_t>(nl + 1), *d_pose_start = mem.get<int32_t>(np + 1), *d_pose_list = mem.get<int32_t>(no),
+            *d_lm_pair_start = mem.get<int32_t>(nl + 1), *d_pair_pose = mem.get<int32_t>(npair),
+            *d_pair_point = mem.get<int32_t>(npair), *d_pair_first = mem.get<int32_t>(npair),
+            *d_blk_row = mem.get<int32_t>(nblk), *d_blk_col = mem.get<int32_t>(nblk), *d_diag = mem.get<int32_t>(np);
+    int64_t* d_blk_start = mem.get<int64_t>(nblk + 1);
+    int2* d_contrib = mem.get<int2>(static_cast<size_t>(ncontrib));
+    uint8_t *d_pfix = mem.get<uint8_t>(np), *d_lfix = mem.get<uint8_t>(nl);
+    double* d_cams = mem.get<double>(cam_table.size());
+    int32_t* d_lk = mem.get<int32_t>(lk.size());
+    double* d_ld = mem.get<double>(ld.size());
+    w = dev_work64{};
+    w.pose = mem.get<double>(7L * np); w.pose_backup = mem.get<double>(7L * np);
+    w.point = mem.get<double>(3L * nl); w.point_backup = mem.get<double>(3L * nl);
+    w.residual = mem.get<double>(2L * no); w.pose_terms = mem.get<double>(27L * no);
+    w.Hpp = mem.get<double>(21L * np); w.bp = mem.get<double>(6L * np);
+    w.Hll = mem.get<double>(9L * nl); w.bl = mem.get<double>(3L * nl);
+    w.W = mem.get<double>(18L * npair); w.Hinv = mem.get<double>(9L * nl); w.Y = mem.get<double>(18L * npair);
+    w.S = mem.get<double>(static_cast<size_t>(n * n)); w.bs = mem.get<double>(n); w.dxp = mem.get<double>(n);
+    w.dxl = mem.get<double>(3L * nl);
+    w.partial = mem.get<double>(3 * zb::kPartials + 8);
+    w.chol_scale = mem.get<double>(4);
+    w.chol_status = mem.get<int32_t>(4);
+    w.lm = mem.get<lm_state>(1);
+    lambda_dev = mem.get<double>(1);
+    flags_dev = mem.get<uint8_t>(no);
+    snap_pose = mem.get<double>(7L * np);
+    snap_point = mem.get<double>(3L * nl);
+    if (mem.status != hipSuccess) {
+        mem.release();
+        return fail(ZSBA_ENOMEM, "device allocation of %.1f MB failed", mem.bytes / 1e6);
+    }
+    std::vector<int32_t> orig(no);
+    for (int32_t s = 0; s < no; ++s) orig[s] = sorted_e[s];
+    p.obs_pose = up(d_obs_pose, obs_pose_s); p.obs_point = up(d_obs_point, obs_point_s);
+    p.obs_pair = up(d_obs_pair, obs_pair); p.obs_param = up(d_obs_param, obs_param); p.obs_orig = up(d_obs_orig, orig);
+    p.obs_uv = up(d_uv, uv); p.obs_info = identity ? nullptr : up(d_info, info);
+    p.lm_start = up(d_lm_start, lm_start); p.pose_start = up(d_pose_start, pose_start); p.pose_list = up(d_pose_list, pose_list);
+    p.lm_pair_start = up(d_lm_pair_start, lm_pair_start); p.pair_pose = up(d_pair_pose, pair_pose);
+    p.pair_point = up(d_pair_point, pair_point); p.pair_first = up(d_pair_first, pair_first);
+    p.blk_row = up(d_blk_row, blk_row); p.blk_col = up(d_blk_col, blk_col); p.blk_start = up(d_blk_start, blk_start);
+    p.contrib = up(d_contrib, contrib); p.diag_blk = up(d_diag, diag_blk);
+    p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
+    p.cams = up(d_cams, cam_table); p.loss_kind = up(d_lk, lk); p.loss_delta = up(d_ld, ld);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "structure upload");
+    dp = p;
+    structure_dirty = false;
+    values_dirty = true;
+    return 0;
+}
+
+int32_t zsba_graph::upload_values() {
+    const size_t np = pose_slot_id.size(), nl = point_slot_id.size();
+    std::vector<double> P(7 * np), X(3 * nl);
+    for (size_t i = 0; i < np; ++i) std::memcpy(&P[7 * i], &pose[7L * pose_slot_id[i]], 7 * sizeof(double));
+    for (size_t j = 0; j < nl; ++j) std::memcpy(&X[3 * j], &point[3L * point_slot_id[j]], 3 * sizeof(double));
+    hipError_t e;
+    if (np && (e = hipMemcpy(w.pose, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "upload poses");
+    if (nl && (e = hipMemcpy(w.point, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "upload points");
+    values_dirty = false;
+    device_newer = false;
+    return 0;
+}
+
+int32_t zsba_graph::prepare() {
+    int32_t rc = ensure_hip();
+    if (rc) return rc;
+    const double t0 = now_ms();
+    if (structure_dirty) {
+        rc = sync_to_host();
+        if (rc) return rc;
+        rc = build_structure();
+        if (rc) return rc;
+    }
+    if (values_dirty) {
+        rc = upload_values();
+        if (rc) return rc;
+    }
+    stats.setup_ms = now_ms() - t0;
+    return 0;
+}
+
+// The verbose LM trace of factor_graph(true) (factor_graph.hpp:526, 581, 589)
+// cannot be printed from the device; the summary line is printed instead.
+void zsba_graph::print_trace() {
+    std::fprintf(stderr, "[zsba] iterations %d, attempts %d, accepted %d, rejected %d, failed solves %d, chi2 % 7.7f -> % 7.7f, lambda % 8.7f\n",
+                 lm_pinned->executed_iterations, lm_pinned->attempts, lm_pinned->success, lm_pinned->rejected,
+                 lm_pinned->solve_failures, lm_pinned->initial_chi2, lm_pinned->chi2, lm_pinned->lambda);
+}
+
+int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
+    hipError_t e;
+    const double t0 = now_ms();
+    std::memset(lm_pinned, 0, sizeof(lm_state));
+    lm_pinned->iterations = iterations;
+    lm_pinned->use_relative = use_rel ? 1 : 0;
+    if ((e = hipMemcpyAsync(w.lm, lm_pinned, sizeof(lm_state), hipMemcpyHostToDevice, stream)) != hipSuccess)
+        return hip_fail(e, "upload controller");
+    zb::launch_cost64(dp, w, stream, 0, false);
+    zb::launch_linearise64(dp, w, stream, false);
+    zb::launch_init_lm64(dp, w, stream);
+    const int32_t max_attempts = iterations * 10;
+    hipEvent_t ev[2];
+    (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    int32_t* done_pinned = &lm_pinned->done;
+    for (int32_t a = 0; a < max_attempts; ++a) {
+        zb::launch_reduced_system64(dp, w, stream, nullptr);
+        zb::launch_cholesky_solve64(dp.n, w, stream, true);
+        zb::launch_after_solve64(w, stream);
+        zb::launch_backsub_update64(dp, w, stream);
+        zb::launch_cost64(dp, w, stream, 1, true);
+        zb::launch_decide64(w, stream);
+        zb::launch_rollback64(dp, w, stream);
+        zb::launch_linearise64(dp, w, stream, true);
+        zb::launch_end_attempt64(w, stream);
+        (void)hipMemcpyAsync(done_pinned, &w.lm->done, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+        (void)hipEventRecord(ev[a & 1], stream);
+        // Keep one attempt queued behind the one whose flag is read.
+        if (a >= 1) {
+            (void)hipEventSynchronize(ev[(a - 1) & 1]);
+            if (*done_pinned) break;
+        }
+    }
+    e = hipStreamSynchronize(stream);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (e != hipSuccess) return hip_fail(e, "solve");
+    if ((e = hipMemcpy(lm_pinned, w.lm, sizeof(lm_state), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "download controller");
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
+    stats.solve_ms = now_ms() - t0;
+    stats.iterations = lm_pinned->executed_iterations;
+    stats.attempts = lm_pinned->attempts;
+    stats.accepted = lm_pinned->success;
+    stats.rejected = lm_pinned->rejected;
+    stats.solve_failures = lm_pinned->solve_failures;
+    stats.initial_chi2 = lm_pinned->initial_chi2;
+    stats.final_chi2 = lm_pinned->chi2;
+    stats.lambda = lm_pinned->lambda;
+    if (opt.verbose) print_trace();
+    device_newer = true;
+    return lm_pinned->success;
+}
+
+// ---- C ABI ------------------------------------------------------------------
+extern "C" {
+
+int32_t zsba_abi_version(void) { return ZSBA_ABI_VERSION; }
+
+zsba_graph* zsba_create(const zsba_options* options) {
+    zsba_graph* g = new (std::nothrow) zsba_graph();
+    if (!g) return nullptr;
+    if (options) g->opt = *options;
+    if (g->opt.refine_steps <= 0) g->opt.refine_steps = 2;
+    return g;
+}
+
+void zsba_destroy(zsba_graph* g) { delete g; }
+
+const char* zsba_last_error(const zsba_graph* g) { return g ? g->err.c_str() : "null handle"; }
+
+int32_t zsba_add_poses(zsba_graph* g, int32_t count, const double* params, const uint8_t* fixed) {
+    if (!g || count < 0 || (count > 0 && !params)) return g ? g->fail(ZSBA_EINVAL, "bad arguments to add_poses") : ZSBA_EINVAL;
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    const int32_t first = static_cast<int32_t>(g->pose_alive.size());
+    g->pose.insert(g->pose.end(), params, params + 7L * count);
+    for (int32_t i = 0; i < count; ++i) {
+        g->pose_fixed.push_back(fixed ? (fixed[i] ? 1 : 0) : 0);
+        g->pose_alive.push_back(1);
+    }
+    g->structure_dirty = true;
+    return first;
+}
+int32_t zsba_add_pose(zsba_graph* g, const double params[7], int32_t fixed) {
+    const uint8_t f = fixed ? 1 : 0;
+    return zsba_add_poses(g, 1, params, &f);
+}
+int32_t zsba_add_points(zsba_graph* g, int32_t count, const double* xyz, const uint8_t* fixed) {
+    if (!g || count < 0 || (count > 0 && !xyz)) return g ? g->fail(ZSBA_EINVAL, "bad arguments to add_points") : ZSBA_EINVAL;
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    const int32_t first = static_cast<int32_t>(g->point_alive.size());
+    g->point.insert(g->point.end(), xyz, xyz + 3L * count);
+    for (int32_t i = 0; i < count; ++i) {
+        g->point_fixed.push_back(fixed ? (fixed[i] ? 1 : 0) : 0);
+        g->point_alive.push_back(1);
+    }
+    g->structure_dirty = true;
+    return first;
+}
+int32_t zsba_add_point(zsba_graph* g, const double xyz[3], int32_t fixed) {
+    const uint8_t f = fixed ? 1 : 0;
+    return zsba_add_points(g, 1, xyz, &f);
+}
+
+int32_t zsba_add_observations(zsba_graph* g, int32_t count, const int32_t* pose_ids, const int32_t* point_ids,
+                              const double* uv, const double* info, const double K[4], int32_t loss_kind, double loss_delta) {
+    if (!g) return ZSBA_EINVAL;
+    if (count < 0 || (count > 0 && (!pose_ids || !point_ids || !uv)) || !K)
+        return g->fail(ZSBA_EINVAL, "bad arguments to add_observations");
+    for (int32_t k = 0; k < count; ++k) {
+        const int32_t ip = pose_ids[k], il = point_ids[k];
+        if (ip < 0 || ip >= static_cast<int32_t>(g->pose_alive.size()) || !g->pose_alive[ip])
+            return g->fail(ZSBA_EINVAL, "observation %d: unknown pose id %d", k, ip);
+        if (il < 0 || il >= static_cast<int32_t>(g->point_alive.size()) || !g->point_alive[il])
+            return g->fail(ZSBA_EINVAL, "observation %d: unknown point id %d", k, il);
+    }
+    const int32_t cam = g->camera_index(K);
+    if (cam < 0) return cam;
+    const int32_t li = g->loss_index(loss_kind, loss_delta);
+    if (li < 0) return li;
+    const int32_t first = static_cast<int32_t>(g->obs_alive.size());
+    g->obs_pose.insert(g->obs_pose.end(), pose_ids, pose_ids + count);
+    g->obs_point.insert(g->obs_point.end(), point_ids, point_ids + count);
+    g->obs_uv.insert(g->obs_uv.end(), uv, uv + 2L * count);
+    if (info) g->obs_info.insert(g->obs_info.end(), info, info + 4L * count);
+    else g->obs_info.insert(g->obs_info.end(), 4L * count, 0.0);
+    g->obs_has_info.insert(g->obs_has_info.end(), count, info ? 1 : 0);
+    g->obs_cam.insert(g->obs_cam.end(), count, cam);
+    g->obs_loss.insert(g->obs_loss.end(), count, li);
+    g->obs_alive.insert(g->obs_alive.end(), count, 1);
+    g->structure_dirty = true;
+    return first;
+}
+int32_t zsba_add_observation(zsba_graph* g, int32_t pose_id, int32_t point_id, const double uv[2], const double info[4],
+                             const double K[4], int32_t loss_kind, double loss_delta) {
+    return zsba_add_observations(g, 1, &pose_id, &point_id, uv, info, K, loss_kind, loss_delta);
+}
+
+int32_t zsba_remove_observation(zsba_graph* g, int32_t id) {
+    if (!g) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->obs_alive.size()) || !g->obs_alive[id])
+        return g->fail(ZSBA_EINVAL, "unknown observation id %d", id);
+    g->obs_alive[id] = 0;
+    g->structure_dirty = true;
+    return 0;
+}
+int32_t zsba_remove_pose(zsba_graph* g, int32_t id) {
+    if (!g) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->pose_alive.size()) || !g->pose_alive[id])
+        return g->fail(ZSBA_EINVAL, "unknown pose id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    g->pose_alive[id] = 0;
+    for (size_t k = 0; k < g->obs_alive.size(); ++k)
+        if (g->obs_pose[k] == id) g->obs_alive[k] = 0;
+    g->structure_dirty = true;
+    return 0;
+}
+int32_t zsba_remove_point(zsba_graph* g, int32_t id) {
+    if (!g) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->point_alive.size()) || !g->point_alive[id])
+        return g->fail(ZSBA_EINVAL, "unknown point id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    g->point_alive[id] = 0;
+    for (size_t k = 0; k < g->obs_alive.size(); ++k)
+        if (g->obs_point[k] == id) g->obs_alive[k] = 0;
+    g->structure_dirty = true;
+    return 0;
+}
+
+int32_t zsba_set_pose(zsba_graph* g, int32_t id, const double params[7]) {
+    if (!g || !params) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->pose_alive.size())) return g->fail(ZSBA_EINVAL, "unknown pose id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    std::memcpy(&g->pose[7L * id], params, 7 * sizeof(double));
+    g->values_dirty = true;
+    return 0;
+}
+int32_t zsba_set_point(zsba_graph* g, int32_t id, const double xyz[3]) {
+    if (!g || !xyz) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->point_alive.size())) return g->fail(ZSBA_EINVAL, "unknown point id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    std::memcpy(&g->point[3L * id], xyz, 3 * sizeof(double));
+    g->values_dirty = true;
+    return 0;
+}
+int32_t zsba_set_pose_fixed(zsba_graph* g, int32_t id, int32_t fixed) {
+    if (!g) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->pose_alive.size())) return g->fail(ZSBA_EINVAL, "unknown pose id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    g->pose_fixed[id] = fixed ? 1 : 0;
+    g->structure_dirty = true;
+    return 0;
+}
+int32_t zsba_set_point_fixed(zsba_graph* g, int32_t id, int32_t fixed) {
+    if (!g) return ZSBA_EINVAL;
+    if (id < 0 || id >= static_cast<int32_t>(g->point_alive.size())) return g->fail(ZSBA_EINVAL, "unknown point id %d", id);
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    g->point_fixed[id] = fixed ? 1 : 0;
+    g->structure_dirty = true;
+    return 0;
+}
+
+int32_t zsba_counts(const zsba_graph* g, int64_t counts[3]) {
+    if (!g || !counts) return ZSBA_EINVAL;
+    counts[0] = static_cast<int64_t>(g->pose_alive.size());
+    counts[1] = static_cast<int64_t>(g->point_alive.size());
+    counts[2] = static_cast<int64_t>(g->obs_alive.size());
+    return 0;
+}
+
+int32_t zsba_get_poses(const zsba_graph* gc, double* out) {
+    zsba_graph* g = const_cast<zsba_graph*>(gc);
+    if (!g || !out) return ZSBA_EINVAL;
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    for (size_t i = 0; i < g->pose_alive.size(); ++i)
+        if (g->pose_alive[i]) std::memcpy(out + 7 * i, &g->pose[7 * i], 7 * sizeof(double));
+    return 0;
+}
+int32_t zsba_get_points(const zsba_graph* gc, double* out) {
+    zsba_graph* g = const_cast<zsba_graph*>(gc);
+    if (!g || !out) return ZSBA_EINVAL;
+    if (g->sync_to_host()) return ZSBA_EDEVICE;
+    for (size_t j = 0; j < g->point_alive.size(); ++j)
+        if (g->point_alive[j]) std::memcpy(out + 3 * j, &g->point[3 * j], 3 * sizeof(double));
+    return 0;
+}
+
+int32_t zsba_solve(zsba_graph* g, int32_t iterations, int32_t use_relative_convergence, int32_t precision) {
+    if (!g) return ZSBA_EINVAL;
+    g->stats = zsba_stats{};
+    g->stats.precision = precision;
+    if (precision != ZSBA_PRECISION_FP64 && precision != ZSBA_PRECISION_MIXED)
+        return g->fail(ZSBA_EINVAL, "unknown precision %d", precision);
+    // factor_graph.hpp:494-499
+    if (g->live_obs() == 0 || (g->live_poses() == 0 && g->live_points() == 0)) {
+        if (g->opt.verbose) std::fprintf(stderr, "Cannot solve problem without edges or vertices\n");
+        return 0;
+    }
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    g->stats.n_obs = g->dp.no;
+    g->stats.n_poses = g->dp.np;
+    g->stats.n_points = g->dp.nl;
+    if (precision == ZSBA_PRECISION_MIXED) return g->fail(ZSBA_EUNSUP, "mixed precision is not available yet");
+    return g->run_fp64(iterations, use_relative_convergence);
+}
+
+double zsba_current_chi(zsba_graph* g, int32_t recompute) {
+    if (!g) return NAN;
+    if (g->live_obs() == 0) return 0.0;
+    if (g->prepare()) return NAN;
+    if (recompute) zb::launch_cost64(g->dp, g->w, g->stream, 3, false);
+    else zb::launch_cost_stored64(g->dp, g->w, g->stream);
+    double v = NAN;
+    if (hipMemcpy(&v, g->w.partial + zb::kPartials, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        g->fail(ZSBA_EDEVICE, "current_chi");
+        return NAN;
+    }
+    return v;
+}
+
+int32_t zsba_get_inlier_flags(zsba_graph* g, uint8_t* flags) {
+    if (!g || !flags) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return 0;
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    zb::launch_inliers64(g->dp, g->w, g->flags_dev, g->stream);
+    std::vector<uint8_t> f(g->dp.no);
+    const hipError_t e = hipMemcpy(f.data(), g->flags_dev, f.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return g->hip_fail(e, "inlier flags");
+    for (int32_t s = 0; s < g->dp.no; ++s) flags[g->obs_e_id[g->sorted_e[s]]] = f[s];
+    return 0;
+}
+
+int32_t zsba_get_residuals(zsba_graph* g, double* out) {
+    if (!g || !out) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return 0;
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    zb::launch_cost64(g->dp, g->w, g->stream, 3, false);
+    std::vector<double> r(2L * g->dp.no);
+    const hipError_t e = hipMemcpy(r.data(), g->w.residual, r.size() * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return g->hip_fail(e, "residuals");
+    for (int32_t s = 0; s < g->dp.no; ++s) {
+        const int32_t id = g->obs_e_id[g->sorted_e[s]];
+        out[2L * id] = r[2L * s];
+        out[2L * id + 1] = r[2L * s + 1];
+    }
+    return 0;
+}
+
+int32_t zsba_get_stats(const zsba_graph* g, zsba_stats* stats) {
+    if (!g || !stats) return ZSBA_EINVAL;
+    *stats = g->stats;
+    return 0;
+}
+
+int32_t zsba_debug_eval_edges(zsba_graph* g, double* out) {
+    if (!g || !out) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return 0;
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    const size_t need = 27L * g->dp.no;
+    if (g->eval_capacity < need) {
+        if (g->eval_dev) (void)hipFree(g->eval_dev);
+        g->eval_dev = nullptr;
+        g->eval_capacity = 0;
+        if (hipMalloc(&g->eval_dev, need * sizeof(double)) != hipSuccess) return g->fail(ZSBA_ENOMEM, "eval buffer");
+        g->eval_capacity = need;
+    }
+    zb::launch_eval_edges64(g->dp, g->w, g->eval_dev, g->stream);
+    std::vector<double> v(need);
+    const hipError_t e = hipMemcpy(v.data(), g->eval_dev, need * sizeof(double), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return g->hip_fail(e, "eval edges");
+    for (int32_t k = 0; k < g->dp.no; ++k) std::memcpy(out + 27L * g->obs_e_id[k], &v[27L * k], 27 * sizeof(double));
+    return 0;
+}
+
+int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precision, double* S, double* b_s, double* dx_pose,
+                                  double* dx_point) {
+    if (!g) return ZSBA_EINVAL;
+    if (precision != ZSBA_PRECISION_FP64) return g->fail(ZSBA_EUNSUP, "debug system: precision %d", precision);
+    if (g->live_obs() == 0) return g->fail(ZSBA_ESTATE, "no observations");
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    lm_state st{};
+    st.lambda = lambda;
+    hipError_t e;
+    if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hip_fail(e, "sync");
+    if ((e = hipMemcpy(g->w.lm, &st, sizeof st, hipMemcpyHostToDevice)) != hipSuccess) return g->hip_fail(e, "lm");
+    if ((e = hipMemcpy(g->lambda_dev, &lambda, sizeof lambda, hipMemcpyHostToDevice)) != hipSuccess)
+        return g->hip_fail(e, "lambda");
+    zb::launch_linearise64(g->dp, g->w, g->stream, false);
+    zb::launch_reduced_system64(g->dp, g->w, g->stream, g->lambda_dev);
+    const int64_t n = g->dp.n;
+    if (S && (e = hipMemcpyAsync(S, g->w.S, n * n * sizeof(double), hipMemcpyDeviceToHost, g->stream)) != hipSuccess)
+        return g->hip_fail(e, "S");
+    if (b_s && (e = hipMemcpyAsync(b_s, g->w.bs, n * sizeof(double), hipMemcpyDeviceToHost, g->stream)) != hipSuccess)
+        return g->hip_fail(e, "b_s");
+    zb::launch_cholesky_solve64(g->dp.n, g->w, g->stream, false);
+    zb::launch_backsub_only64(g->dp, g->w, g->stream);
+    if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hip_fail(e, "reduced system");
+    lm_state out{};
+    if ((e = hipMemcpy(&out, g->w.lm, sizeof out, hipMemcpyDeviceToHost)) != hipSuccess) return g->hip_fail(e, "lm");
+    if (out.solve_ok) {
+        if (dx_pose && (e = hipMemcpy(dx_pose, g->w.dxp, n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return g->hip_fail(e, "dx_pose");
+        if (dx_point) {
+            std::vector<double> d(3L * g->dp.nl);
+            if ((e = hipMemcpy(d.data(), g->w.dxl, d.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+                return g->hip_fail(e, "dx_point");
+            std::memcpy(dx_point, d.data(), d.size() * sizeof(double));
+        }
+    }
+    return out.solve_ok ? 1 : 0;
+}
+
+int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations, int32_t use_rel, int32_t precision,
+                         int32_t* accepted) {
+    if (!graphs || count < 0 || (count > 0 && !accepted)) return ZSBA_EINVAL;
+    for (int32_t i = 0; i < count; ++i) {
+        const int32_t r = zsba_solve(graphs[i], iterations, use_rel, precision);
+        if (r < 0) return r;
+        accepted[i] = r;
+    }
+    return 0;
+}
+
+int32_t zsba_adjoint(zsba_graph* g, const double*, const double*, double*, double*) {
+    if (!g) return ZSBA_EINVAL;
+    return g->fail(ZSBA_EUNSUP, "adjoint is not available yet");
+}
+
+}  // extern "C"

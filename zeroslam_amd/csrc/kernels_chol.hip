@@ -1,0 +1,283 @@
+// Dense binary64 Cholesky factorisation and solve of the reduced camera system
+// (replaces matrix::decompose_cholesky / matrix::solve_cholesky,
+// source/matrix_decomposition_cholesky.hpp:26-95, as called at
+// factor_graph.hpp:782-793).
+//
+// Blocked right-looking factorisation over 32-wide panels, in place on the lower
+// triangle of S (row-major n x n): factor the diagonal tile in LDS, solve the
+// panel below it, update the trailing lower triangle tile by tile.  Failure
+// rules are the reference's: decomposition fails when a pivot is not greater
+// than 1e-14 * max|S_ii| (NaN fails), the solve fails when any |L_ii| is not
+// greater than 1e-7 * max|L_ii|.  Forward substitution subtracts in the
+// reference's column order; the rest differs from the reference's row-Crout order
+// only by rounding.
+#include "common.hpp"
+
+#include <cmath>
+
+namespace zb {
+namespace {
+
+constexpr int NB = 32;
+
+__device__ __forceinline__ bool skip(const lm_state* lm, int gated, const int32_t* status) {
+    return (gated && lm->done) || *status != 0;
+}
+
+// eps = 1e-14 * max|S_ii| with math::max semantics (NaN entries never win).
+__global__ void __launch_bounds__(1024) k_chol_begin(int n, const double* S, double* eps, int32_t* status,
+                                                     const lm_state* lm, int gated) {
+    if (gated && lm->done) return;
+    __shared__ double sh[1024];
+    double m = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double a = fabs(S[static_cast<int64_t>(i) * n + i]);
+        m = (m < a) ? a : m;
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double a = sh[threadIdx.x + w];
+            sh[threadIdx.x] = (sh[threadIdx.x] < a) ? a : sh[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        eps[0] = sh[0] * 1e-14;
+        *status = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_potrf_diag(int n, double* A, int k0, const double* eps, int32_t* status,
+                                                    const lm_state* lm, int gated) {
+    if (skip(lm, gated, status)) return;
+    __shared__ double T[NB][NB + 1];
+    __shared__ int fail;
+    const int nb = (n - k0) < NB ? (n - k0) : NB;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        if (c <= r) T[r][c] = A[static_cast<int64_t>(k0 + r) * n + k0 + c];
+    }
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    const double e = eps[0];
+    for (int j = 0; j < nb; ++j) {
+        if (tid == 0) {
+            const double v = T[j][j];
+            if (!(v > e)) fail = 1;
+            else T[j][j] = sqrt(v);
+        }
+        __syncthreads();
+        if (fail) break;
+        for (int r = j + 1 + tid; r < nb; r += blockDim.x) T[r][j] = T[r][j] / T[j][j];
+        __syncthreads();
+        const int m = nb - j - 1;
+        for (int idx = tid; idx < m * m; idx += blockDim.x) {
+            const int r = j + 1 + idx / m, c = j + 1 + idx % m;
+            if (c <= r) T[r][c] = T[r][c] - T[r][j] * T[c][j];
+        }
+        __syncthreads();
+    }
+    if (fail) {
+        if (tid == 0) *status = 1;
+        return;
+    }
+    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        if (c <= r) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = T[r][c];
+    }
+}
+
+// Rows below the diagonal tile: L_rk = A_rk L_kk^-T.
+__global__ void __launch_bounds__(64) k_trsm(int n, double* A, int k0, const int32_t* status, const lm_state* lm,
+                                             int gated) {
+    if (skip(lm, gated, status)) return;
+    __shared__ double T[NB][NB + 1];
+    const int nb = (n - k0) < NB ? (n - k0) : NB;
+    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
+        const int r = idx / nb, c = idx % nb;
+        T[r][c] = (c <= r) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    const int row = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= n) return;
+    double* a = A + static_cast<int64_t>(row) * n + k0;
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = (c < nb) ? a[c] : 0.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        if (c < nb) {
+            double v = x[c];
+#pragma unroll
+            for (int m = 0; m < c; ++m) v = v - x[m] * T[c][m];
+            x[c] = v / T[c][c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        if (c < nb) a[c] = x[c];
+}
+
+// Trailing update A_IJ -= L_Ik L_Jk^T over lower-triangle tile pairs I >= J > k.
+__global__ void __launch_bounds__(256) k_syrk(int n, double* A, int k0, int first_tile, const int32_t* status,
+                                              const lm_state* lm, int gated) {
+    if (skip(lm, gated, status)) return;
+    __shared__ double LI[NB][NB + 1];
+    __shared__ double LJ[NB][NB + 1];
+    const int64_t idx = blockIdx.x;
+    int64_t Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(idx) + 1.0) - 1.0) / 2.0);
+    while ((Ip + 1) * (Ip + 2) / 2 <= idx) ++Ip;
+    while (Ip * (Ip + 1) / 2 > idx) --Ip;
+    const int64_t Jp = idx - Ip * (Ip + 1) / 2;
+    const int I = first_tile + static_cast<int>(Ip), J = first_tile + static_cast<int>(Jp);
+    const int nb = (n - k0) < NB ? (n - k0) : NB;
+    const int r0 = I * NB, c0 = J * NB;
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+        const int r = t / NB, m = t % NB;
+        LI[r][m] = (r0 + r < n && m < nb) ? A[static_cast<int64_t>(r0 + r) * n + k0 + m] : 0.0;
+        LJ[r][m] = (c0 + r < n && m < nb) ? A[static_cast<int64_t>(c0 + r) * n + k0 + m] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
+        const int r = t / NB, c = t % NB;
+        if (r0 + r >= n || c0 + c >= n) continue;
+        if (I == J && c > r) continue;
+        double s = 0.0;
+#pragma unroll 8
+        for (int m = 0; m < NB; ++m) s = s + LI[r][m] * LJ[c][m];
+        double* out = A + static_cast<int64_t>(r0 + r) * n + c0 + c;
+        *out = *out - s;
+    }
+}
+
+// Forward and backward substitution in one workgroup; sets lm->solve_ok.
+__global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* x, int32_t* status,
+                                               lm_state* lm, int gated) {
+    if (gated && lm->done) return;
+    if (*status != 0) {
+        if (threadIdx.x == 0) lm->solve_ok = 0;
+        return;
+    }
+    __shared__ double sh[1024];
+    __shared__ double T[NB][NB + 1];
+    __shared__ int fail;
+    const int tid = threadIdx.x;
+    // eps = 1e-7 * max|L_ii| (matrix_decomposition_cholesky.hpp:70-74)
+    double m = 0;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const double a = fabs(L[static_cast<int64_t>(i) * n + i]);
+        m = (m < a) ? a : m;
+    }
+    sh[tid] = m;
+    if (tid == 0) fail = 0;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            const double a = sh[tid + w];
+            sh[tid] = (sh[tid] < a) ? a : sh[tid];
+        }
+        __syncthreads();
+    }
+    const double eps = sh[0] * 1e-7;
+    for (int i = tid; i < n; i += blockDim.x) {
+        if (!(fabs(L[static_cast<int64_t>(i) * n + i]) > eps)) fail = 1;
+        x[i] = b[i];
+    }
+    __syncthreads();
+    if (fail) {
+        if (tid == 0) {
+            *status = 2;
+            lm->solve_ok = 0;
+        }
+        return;
+    }
+    const int tiles = (n + NB - 1) / NB;
+    const int lane = tid & 63;
+    // forward: L y = b, column by column within a tile, x_i -= L_ij x_j in j order
+    for (int t = 0; t < tiles; ++t) {
+        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
+        for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+            const int r = idx / nb, c = idx % nb;
+            T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            double xi = (lane < nb) ? x[i0 + lane] : 0.0;
+            for (int j = 0; j < nb; ++j) {
+                if (lane == j) xi = xi / T[j][j];
+                const double xj = __shfl(xi, j, 64);
+                if (lane > j && lane < nb) xi = xi - T[lane][j] * xj;
+            }
+            if (lane < nb) x[i0 + lane] = xi;
+        }
+        __syncthreads();
+        for (int i = i0 + nb + tid; i < n; i += blockDim.x) {
+            double v = x[i];
+            const double* Li = L + static_cast<int64_t>(i) * n + i0;
+            for (int j = 0; j < nb; ++j) v = v - Li[j] * x[i0 + j];
+            x[i] = v;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y
+    for (int t = tiles - 1; t >= 0; --t) {
+        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
+        for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+            const int r = idx / nb, c = idx % nb;
+            T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
+        }
+        __syncthreads();
+        if (tid < 64) {
+            double xi = (lane < nb) ? x[i0 + lane] : 0.0;
+            for (int j = nb - 1; j >= 0; --j) {
+                if (lane == j) xi = xi / T[j][j];
+                const double xj = __shfl(xi, j, 64);
+                if (lane < j) xi = xi - T[j][lane] * xj;
+            }
+            if (lane < nb) x[i0 + lane] = xi;
+        }
+        __syncthreads();
+        for (int i = tid; i < i0; i += blockDim.x) {
+            double v = x[i];
+            for (int j = 0; j < nb; ++j) v = v - L[static_cast<int64_t>(i0 + j) * n + i] * x[i0 + j];
+            x[i] = v;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) lm->solve_ok = 1;
+}
+
+// A system without poses: the reference's 0 x 0 decompose and solve succeed.
+__global__ void k_empty_ok(lm_state* lm, int gated) {
+    if (gated && lm->done) return;
+    lm->solve_ok = 1;
+}
+
+}  // namespace
+
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated) {
+    const int g = gated ? 1 : 0;
+    if (n <= 0) {
+        k_empty_ok<<<1, 1, 0, s>>>(w.lm, g);
+        return;
+    }
+    k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g);
+    const int tiles = (n + NB - 1) / NB;
+    for (int t = 0; t < tiles; ++t) {
+        const int k0 = t * NB;
+        k_potrf_diag<<<1, 256, 0, s>>>(n, w.S, k0, w.chol_scale, w.chol_status, w.lm, g);
+        const int below = n - (k0 + NB);
+        if (below > 0) {
+            k_trsm<<<(below + 63) / 64, 64, 0, s>>>(n, w.S, k0, w.chol_status, w.lm, g);
+            const int64_t m = tiles - t - 1;
+            const int64_t pairs = m * (m + 1) / 2;
+            k_syrk<<<static_cast<unsigned>(pairs), 256, 0, s>>>(n, w.S, k0, t + 1, w.chol_status, w.lm, g);
+        }
+    }
+    k_trsv<<<1, 1024, 0, s>>>(n, w.S, w.bs, w.dxp, w.chol_status, w.lm, g);
+}
+
+}  // namespace zb

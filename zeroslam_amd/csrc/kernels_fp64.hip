@@ -1,0 +1,694 @@
+// Binary64 bundle-adjustment kernels for gfx950 (compiled with -ffp-contract=off).
+//
+// Per-observation arithmetic is the reference's IEEE sequence (ba_math.hpp).
+// The per-vertex sums that make_hessian forms edge by edge (H_pp, H_ll, b, W;
+// factor_graph.hpp:664-727) are formed here by one lane per sum walking the
+// vertex's observations in insertion order, so they equal the reference's sums
+// bit for bit.  Sums over all observations (chi2, the LM scale) and over a
+// reduced-system block (the Schur products) use fixed-shape trees: deterministic
+// run to run, different from the reference's sequential order only by rounding.
+//
+// Every kernel of an LM attempt first reads the device-side controller state and
+// returns at once when the attempt does not apply (solve finished, step not
+// applied, ...), so the host can enqueue attempts without reading anything back.
+#include "ba_math.hpp"
+#include "common.hpp"
+
+#include <cmath>
+
+namespace zb {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ const double* cam_of(const dev_problem& p, int32_t param) {
+    return p.cams + 4 * (param & 0xFFFF);
+}
+__device__ __forceinline__ void loss_of(const dev_problem& p, int32_t param, int32_t& kind, double& delta) {
+    const int li = param >> 16;
+    kind = p.loss_kind[li];
+    delta = p.loss_delta[li];
+}
+__device__ __forceinline__ void info_of(const dev_problem& p, int s, double W[4]) {
+    if (p.identity_info) {
+        W[0] = 1.0; W[1] = 0.0; W[2] = 0.0; W[3] = 1.0;
+    } else {
+        const double* w = p.obs_info + 4L * s;
+        W[0] = w[0]; W[1] = w[1]; W[2] = w[2]; W[3] = w[3];
+    }
+}
+__device__ __forceinline__ void load3(const double* a, double o[3]) { o[0] = a[0]; o[1] = a[1]; o[2] = a[2]; }
+__device__ __forceinline__ void load7(const double* a, double o[7]) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) o[i] = a[i];
+}
+
+// Block-wide sum of one value per thread into out (fixed tree, deterministic).
+__device__ double block_sum(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = sh[t] + sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---- cost: chi2 = sum of robust costs (factor_graph.hpp:618-629) -------------
+__global__ void __launch_bounds__(kBlock) k_cost(dev_problem p, double* pose, double* point, double* resid,
+                                                 double* partial, const lm_state* lm, int gate, int recompute) {
+    if (gate == 1 && lm->done) return;
+    if (gate == 2 && (lm->done || !lm->step_applied)) return;
+    __shared__ double sh[kBlock];
+    double acc = 0.0;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.no; s += gridDim.x * blockDim.x) {
+        const int32_t param = p.obs_param[s];
+        double r[2];
+        if (recompute) {
+            double P[7], X[3], Pc[3], uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+            load7(pose + 7L * p.obs_pose[s], P);
+            load3(point + 3L * p.obs_point[s], X);
+            pose_apply(P, X, Pc);
+            residual(Pc, uv, cam_of(p, param), r);
+            resid[2L * s] = r[0];
+            resid[2L * s + 1] = r[1];
+        } else {
+            r[0] = resid[2L * s];
+            r[1] = resid[2L * s + 1];
+        }
+        double W[4];
+        info_of(p, s, W);
+        int32_t kind;
+        double delta;
+        loss_of(p, param, kind, delta);
+        acc = acc + robust_chi2(kind, delta, r, W);
+    }
+    const double b = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int n, double* out, lm_state* lm,
+                                                       int target, int gate) {
+    if (gate == 1 && lm->done) return;
+    if (gate == 2 && (lm->done || !lm->step_applied)) return;
+    __shared__ double sh[1024];
+    const double v = threadIdx.x < n ? partial[threadIdx.x] : 0.0;
+    const double total = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+        if (target == 0) lm->chi2 = total;
+        else if (target == 1) lm->chi2_new = total;
+        else if (target == 2) lm->scale = total + 1e-3;  // factor_graph.hpp:556-560
+        else if (out) *out = total;
+    }
+}
+
+// ---- linearisation: make_hessian (factor_graph.hpp:648-732) -------------------
+// One thread per point walks its observations in insertion order: H_ll, b_l and
+// the W blocks accumulate in registers/HBM exactly as the reference adds them;
+// the pose-side terms of each observation go to pose_terms for k_reduce_pose.
+__global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, const double* pose, const double* point,
+                                                             double* pterms, double* Hll, double* bl, double* Wbuf,
+                                                             const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p.nl) return;
+    const bool lfixed = p.point_fixed[l] != 0;
+    double X[3];
+    load3(point + 3L * l, X);
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double b[3] = {0, 0, 0};
+    const int s0 = p.lm_start[l], s1 = p.lm_start[l + 1];
+    for (int s = s0; s < s1; ++s) {
+        const int ip = p.obs_pose[s];
+        const bool pfixed = p.pose_fixed[ip] != 0;
+        if (pfixed && lfixed) continue;
+        const int32_t param = p.obs_param[s];
+        const double* K = cam_of(p, param);
+        double P[7], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+        load7(pose + 7L * ip, P);
+        pose_apply(P, X, Pc);
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        residual(Pc, uv, K, r);
+        pose_quat(P, q);
+        jacobians(Pc, q, K, Jp, Jl);
+        info_of(p, s, W);
+        int32_t kind;
+        double delta;
+        loss_of(p, param, kind, delta);
+        robust_info(kind, delta, W, r, drho, Wr);
+        if (!pfixed) {
+            double H21[21], Wpl[18], v[6];
+            pose_terms(Jp, Jl, Wr, W, r, drho, H21, Wpl, v);
+            double* out = pterms + 27L * s;
+#pragma unroll
+            for (int i = 0; i < 21; ++i) out[i] = H21[i];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) out[21 + i] = v[i];
+            if (!lfixed) {
+                const int pr = p.obs_pair[s];
+                double* Wp = Wbuf + 18L * pr;
+                if (p.pair_first[pr] == s) {
+#pragma unroll
+                    for (int i = 0; i < 18; ++i) Wp[i] = Wpl[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 18; ++i) Wp[i] = Wp[i] + Wpl[i];
+                }
+            }
+        }
+        if (!lfixed) {
+            double Hl[9], v[3];
+            point_terms(Jl, Wr, W, r, drho, Hl, v);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) H[i] = H[i] + Hl[i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) b[i] = b[i] - v[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hll[9L * l + i] = H[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bl[3L * l + i] = b[i];
+}
+
+// One wave per pose; lane j < 27 sums term j over the pose's observations in
+// insertion order (H_pp += H; b_p -= v from zero, factor_graph.hpp:696, 725).
+__global__ void __launch_bounds__(256) k_reduce_pose(dev_problem p, const double* pose_terms, double* Hpp, double* bp,
+                                                     const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= p.np || lane >= 27) return;
+    double acc = 0.0;
+    if (!p.pose_fixed[wave]) {
+        const int t0 = p.pose_start[wave], t1 = p.pose_start[wave + 1];
+        int t = t0;
+        if (lane < 21) {
+            for (; t + 4 <= t1; t += 4) {
+                const double a0 = pose_terms[27L * p.pose_list[t] + lane];
+                const double a1 = pose_terms[27L * p.pose_list[t + 1] + lane];
+                const double a2 = pose_terms[27L * p.pose_list[t + 2] + lane];
+                const double a3 = pose_terms[27L * p.pose_list[t + 3] + lane];
+                acc = acc + a0; acc = acc + a1; acc = acc + a2; acc = acc + a3;
+            }
+            for (; t < t1; ++t) acc = acc + pose_terms[27L * p.pose_list[t] + lane];
+        } else {
+            for (; t + 4 <= t1; t += 4) {
+                const double a0 = pose_terms[27L * p.pose_list[t] + lane];
+                const double a1 = pose_terms[27L * p.pose_list[t + 1] + lane];
+                const double a2 = pose_terms[27L * p.pose_list[t + 2] + lane];
+                const double a3 = pose_terms[27L * p.pose_list[t + 3] + lane];
+                acc = acc - a0; acc = acc - a1; acc = acc - a2; acc = acc - a3;
+            }
+            for (; t < t1; ++t) acc = acc - pose_terms[27L * p.pose_list[t] + lane];
+        }
+    }
+    if (lane < 21) Hpp[21L * wave + lane] = acc;
+    else bp[6L * wave + (lane - 21)] = acc;
+}
+
+__global__ void k_linearise_done(lm_state* lm) { lm->need_linearize = 0; }
+
+// ---- LM initialisation, factor_graph.hpp:511-522 -----------------------------
+// max_diagonal = max over all pose then point diagonal entries with
+// std::max(|h|, m) semantics (a NaN is kept only until the next entry).
+struct max_scan {
+    double tail;   // max of the entries after the last NaN
+    int has_nan;
+    int last_nan;  // the final entry was NaN
+    int empty;
+};
+__device__ max_scan scan_combine(max_scan a, max_scan b) {
+    if (b.empty) return a;
+    if (a.empty) return b;
+    if (b.has_nan) return b;
+    max_scan r;
+    r.has_nan = a.has_nan;
+    r.last_nan = 0;
+    r.empty = 0;
+    r.tail = (a.tail < b.tail) ? b.tail : a.tail;
+    if (a.last_nan) r.tail = b.tail;
+    return r;
+}
+
+__global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const double* Hpp, const double* Hll, lm_state* lm) {
+    __shared__ max_scan sh[1024];
+    const int64_t total = 6L * p.np + 3L * p.nl;
+    const int64_t per = (total + blockDim.x - 1) / blockDim.x;
+    const int64_t i0 = threadIdx.x * per, i1 = (i0 + per < total) ? i0 + per : total;
+    max_scan m = {0.0, 0, 0, 1};
+    for (int64_t i = i0; i < i1; ++i) {
+        double h;
+        if (i < 6L * p.np) {
+            const int64_t pose = i / 6, k = i % 6;
+            h = Hpp[21 * pose + k * (k + 1) / 2 + k];
+        } else {
+            const int64_t j = i - 6L * p.np;
+            h = Hll[9 * (j / 3) + 4 * (j % 3)];
+        }
+        const double a = fabs(h);
+        if (isnan(a)) {
+            m.has_nan = 1; m.last_nan = 1; m.empty = 0; m.tail = 0.0;
+        } else {
+            if (m.empty || m.last_nan) { m.tail = a; } else { m.tail = (a < m.tail) ? m.tail : a; }
+            m.last_nan = 0; m.empty = 0;
+        }
+    }
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        max_scan acc = {0.0, 0, 0, 1};
+        for (unsigned t = 0; t < blockDim.x; ++t) acc = scan_combine(acc, sh[t]);
+        // std::max(|h|, m) starting from m = 0: a final NaN leaves NaN, otherwise
+        // the max of the entries after the last NaN (and of 0 when none).
+        double max_diagonal;
+        if (acc.empty) max_diagonal = 0.0;
+        else if (acc.last_nan) max_diagonal = NAN;
+        else max_diagonal = acc.has_nan ? acc.tail : ((acc.tail < 0.0) ? 0.0 : acc.tail);
+        max_diagonal = (max_diagonal < 5e10) ? max_diagonal : 5e10;  // std::min(5e10, m)
+        const double tau = 1e-5;
+        lm->lambda = tau * max_diagonal;
+        lm->nu = 2.0;
+        lm->legacy_threshold = 1e-10 * lm->chi2;
+        lm->initial_chi2 = lm->chi2;
+        lm->last_chi2 = lm->chi2;
+        lm->iter = 0;
+        lm->failures = 0;
+        lm->success = 0;
+        lm->attempts = 0;
+        lm->rejected = 0;
+        lm->solve_failures = 0;
+        lm->executed_iterations = 0;
+        lm->need_linearize = 0;
+        lm->need_rollback = 0;
+        lm->step_applied = 0;
+        lm->solve_ok = 0;
+        lm->done = (lm->iterations <= 0) ? 1 : 0;
+    }
+}
+
+// ---- damping + point-block inverses, factor_graph.hpp:742-765 ------------------
+__global__ void __launch_bounds__(kBlock) k_damp_invert(dev_problem p, const double* Hll, double* Hinv, const lm_state* lm,
+                                                        const double* lambda_override) {
+    if (!lambda_override && lm->done) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p.nl) return;
+    const double lam = lambda_override ? *lambda_override : lm->lambda;
+    double H[9], inv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = Hll[9L * l + i];
+    H[0] += lam; H[4] += lam; H[8] += lam;
+    if (p.point_fixed[l]) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) inv[i] = 0.0;
+    } else {
+        invert3(H, inv);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hinv[9L * l + i] = inv[i];
+}
+
+// Y = W * H_ll^-1 per W block (matrix_sparse_block_diagonal.hpp:107-122).
+__global__ void __launch_bounds__(kBlock) k_form_Y(dev_problem p, const double* Wbuf, const double* Hinv, double* Y,
+                                                   const lm_state* lm, const double* lambda_override) {
+    if (!lambda_override && lm->done) return;
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= p.npair) return;
+    double Wa[18], Hi[9], o[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Wa[i] = Wbuf[18L * a + i];
+    const int l = p.pair_point[a];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hi[i] = Hinv[9L * l + i];
+    mul<6, 3, 3>(Wa, Hi, o);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Y[18L * a + i] = o[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_zero(double* a, int64_t n, const lm_state* lm, int gated) {
+    if (gated && lm->done) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = 0.0;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Reduced camera system, one wave per 6x6 block (row >= col):
+//   S_ik = [i == k] (H_pp,i + lambda I) - sum_l Y_il W_kl^T
+//   b_s,i = b_p,i - sum_l Y_il b_l
+// (factor_graph.hpp:768-781 with matrix_sparse_block.hpp:158-196).
+__global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, const double* Wbuf, const double* Hpp,
+                                               const double* bp, const double* bl, double* S, double* bs,
+                                               const lm_state* lm, const double* lambda_override) {
+    if (!lambda_override && lm->done) return;
+    const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (blk >= p.nblk) return;
+    const int row = p.blk_row[blk], col = p.blk_col[blk];
+    const bool diag = row == col;
+    double acc[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) acc[i] = 0.0;
+    double accb[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
+    for (int64_t c = c0 + lane; c < c1; c += 64) {
+        const int2 ab = p.contrib[c];
+        double Ya[18], Wc[18], lp[18], prod[36];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
+        transpose<6, 3>(Wc, lp);
+        mul<6, 3, 6>(Ya, lp, prod);
+#pragma unroll
+        for (int i = 0; i < 36; ++i) acc[i] = acc[i] + prod[i];
+        if (diag) {
+            const int l = p.pair_point[ab.x];
+            const double b3[3] = {bl[3L * l], bl[3L * l + 1], bl[3L * l + 2]};
+            double yb[6];
+            mul<6, 3, 1>(Ya, b3, yb);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) accb[i] = accb[i] + yb[i];
+        }
+    }
+    const double lam = lambda_override ? *lambda_override : lm->lambda;
+    const int64_t n = p.n;
+    double* Sblk = S + (6L * row) * n + 6L * col;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) {
+        const double tot = wave_sum(acc[i]);
+        const int r = i / 6, c = i % 6;
+        if (lane == 0) {
+            if (diag) {
+                if (c <= r) {
+                    double base = Hpp[21L * row + r * (r + 1) / 2 + c];
+                    if (r == c) base += lam;
+                    Sblk[r * n + c] = base - tot;
+                }
+            } else {
+                Sblk[r * n + c] = 0.0 - tot;
+            }
+        }
+    }
+    if (diag) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double tot = wave_sum(accb[i]);
+            if (lane == 0) bs[6L * row + i] = bp[6L * row + i] - tot;
+        }
+    }
+}
+
+// ---- after the reduced solve: failure bookkeeping, factor_graph.hpp:542-551 ----
+__device__ void outer_end(lm_state* lm) {
+    const double floor_chi2 = 1e-12;
+    const double m = (lm->last_chi2 < floor_chi2) ? floor_chi2 : lm->last_chi2;  // std::max(last, 1e-12)
+    const double threshold = lm->use_relative ? (1e-6 * m) : lm->legacy_threshold;
+    const bool stop = (lm->failures >= 10) || ((lm->last_chi2 - lm->chi2) < threshold) || !isfinite(lm->lambda);
+    lm->executed_iterations += 1;
+    lm->iter += 1;
+    if (stop || lm->iter >= lm->iterations) {
+        lm->done = 1;
+    } else {
+        lm->last_chi2 = lm->chi2;
+        lm->failures = 0;
+    }
+}
+
+__global__ void k_after_solve(lm_state* lm) {
+    if (lm->done) return;
+    lm->attempts += 1;
+    if (lm->solve_ok) {
+        lm->step_applied = 1;
+        return;
+    }
+    lm->step_applied = 0;
+    lm->solve_failures += 1;
+    lm->lambda *= lm->nu;
+    lm->nu *= 2.0;
+    lm->failures += 1;
+    if (!isfinite(lm->lambda) || lm->failures >= 10) outer_end(lm);
+}
+
+// ---- back-substitution + state update + LM scale -------------------------------
+// Threads [0, np) update poses (T <- exp(dx) T), threads [np, np + nl) solve
+// dx_l = H_ll^-1 (b_l - sum W^T dx_p) (factor_graph.hpp:799-800) and update
+// points (factor_graph.hpp:809-830).  Each thread also adds its slots to the LM
+// scale sum_i dx_i (lambda dx_i + b_i) (factor_graph.hpp:556-559).
+__global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double* pose, double* pose_backup, double* point,
+                                                           double* point_backup, const double* Wbuf, const double* Hinv,
+                                                           const double* dxp, double* dxl, const double* bp,
+                                                           const double* bl, double* partial, const lm_state* lm,
+                                                           int gated, int apply) {
+    if (gated && (lm->done || !lm->step_applied)) return;
+    __shared__ double sh[kBlock];
+    const double lam = lm->lambda;
+    double acc = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < p.np + p.nl; t += gridDim.x * blockDim.x) {
+    if (t < p.np) {
+        double d[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) d[i] = dxp[6L * t + i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) acc = acc + d[i] * (lam * d[i] + bp[6L * t + i]);
+        if (apply && !p.pose_fixed[t]) {
+            double P[7];
+            load7(pose + 7L * t, P);
+#pragma unroll
+            for (int i = 0; i < 7; ++i) pose_backup[7L * t + i] = P[i];
+            pose_plus(P, d);
+#pragma unroll
+            for (int i = 0; i < 7; ++i) pose[7L * t + i] = P[i];
+        }
+    } else if (t < p.np + p.nl) {
+        const int l = t - p.np;
+        double tt[3] = {0.0, 0.0, 0.0};
+        for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+            double Wa[18], lp[18], d[6], prod[3];
+#pragma unroll
+            for (int i = 0; i < 18; ++i) Wa[i] = Wbuf[18L * a + i];
+            transpose<6, 3>(Wa, lp);
+            const int k = p.pair_pose[a];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) d[i] = dxp[6L * k + i];
+            mul<3, 6, 1>(lp, d, prod);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tt[i] = tt[i] + prod[i];
+        }
+        double v[3], Hi[9], prod[3], dl[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) v[i] = bl[3L * l + i] - tt[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Hi[i] = Hinv[9L * l + i];
+        mul<3, 3, 1>(Hi, v, prod);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dl[i] = 0.0 + prod[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dxl[3L * l + i] = dl[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc = acc + dl[i] * (lam * dl[i] + bl[3L * l + i]);
+        if (apply && !p.point_fixed[l]) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                point_backup[3L * l + i] = point[3L * l + i];
+                point[3L * l + i] = point[3L * l + i] + dl[i];
+            }
+        }
+    }
+    }
+    if (partial) {
+        const double b = block_sum(acc, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// ---- accept / reject, factor_graph.hpp:566-605 ---------------------------------
+__global__ void k_decide(lm_state* lm) {
+    if (lm->done || !lm->step_applied) return;
+    const double temp_chi = lm->chi2_new;
+    const double rho = (lm->chi2 - temp_chi) / lm->scale;
+    lm->rho = rho;
+    bool good = false;
+    if ((rho > 0) && isfinite(temp_chi)) {
+        double alpha = 1.0 - pow((2.0 * rho - 1.0), 3.0);
+        alpha = (2.0 / 3.0 < alpha) ? 2.0 / 3.0 : alpha;
+        const double scale_factor = (1.0 / 3.0 < alpha) ? alpha : 1.0 / 3.0;
+        lm->lambda *= scale_factor;
+        lm->nu = 2.0;
+        lm->chi2 = temp_chi;
+        good = true;
+        lm->success += 1;
+    } else {
+        lm->lambda *= lm->nu;
+        lm->nu *= 2.0;
+        lm->rejected += 1;
+    }
+    if (!isfinite(lm->lambda)) {  // the rejected step stays applied (factor_graph.hpp:592-594)
+        outer_end(lm);
+        return;
+    }
+    if (good) {
+        outer_end(lm);
+        if (!lm->done) lm->need_linearize = 1;
+        return;
+    }
+    lm->need_rollback = 1;
+    lm->failures += 1;
+    if (lm->failures >= 10) outer_end(lm);
+}
+
+__global__ void __launch_bounds__(kBlock) k_rollback(dev_problem p, double* pose, const double* pose_backup, double* point,
+                                                     const double* point_backup, const lm_state* lm) {
+    if (!lm->need_rollback) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < p.np) {
+        if (!p.pose_fixed[t])
+            for (int i = 0; i < 7; ++i) pose[7L * t + i] = pose_backup[7L * t + i];
+    } else if (t < p.np + p.nl) {
+        const int l = t - p.np;
+        if (!p.point_fixed[l])
+            for (int i = 0; i < 3; ++i) point[3L * l + i] = point_backup[3L * l + i];
+    }
+}
+
+__global__ void k_end_attempt(lm_state* lm) {
+    lm->need_rollback = 0;
+    lm->step_applied = 0;
+    lm->solve_ok = 0;
+}
+
+// ---- diagnostics ---------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_eval_edges(dev_problem p, const double* pose, const double* point, double* out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= p.no) return;
+    const int32_t param = p.obs_param[s];
+    const double* K = cam_of(p, param);
+    double P[7], X[3], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+    load7(pose + 7L * p.obs_pose[s], P);
+    load3(point + 3L * p.obs_point[s], X);
+    pose_apply(P, X, Pc);
+    const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+    residual(Pc, uv, K, r);
+    pose_quat(P, q);
+    jacobians(Pc, q, K, Jp, Jl);
+    info_of(p, s, W);
+    int32_t kind;
+    double delta;
+    loss_of(p, param, kind, delta);
+    robust_info(kind, delta, W, r, drho, Wr);
+    double* o = out + 27L * p.obs_orig[s];
+    o[0] = r[0]; o[1] = r[1];
+    for (int i = 0; i < 12; ++i) o[2 + i] = Jp[i];
+    for (int i = 0; i < 6; ++i) o[14 + i] = Jl[i];
+    o[20] = drho;
+    for (int i = 0; i < 4; ++i) o[21 + i] = Wr[i];
+    o[25] = chi2(r, W);
+    o[26] = robust_chi2(kind, delta, r, W);
+}
+
+__global__ void __launch_bounds__(kBlock) k_inliers(dev_problem p, const double* pose, const double* point, uint8_t* flags) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= p.no) return;
+    const int32_t param = p.obs_param[s];
+    double P[7], X[3], Pc[3], r[2], W[4];
+    load7(pose + 7L * p.obs_pose[s], P);
+    load3(point + 3L * p.obs_point[s], X);
+    pose_apply(P, X, Pc);
+    const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+    residual(Pc, uv, cam_of(p, param), r);
+    info_of(p, s, W);
+    int32_t kind;
+    double delta;
+    loss_of(p, param, kind, delta);
+    flags[s] = inlier(kind, delta, r, W) ? 1 : 0;
+}
+
+inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
+inline int cost_grid(const dev_problem& p) {
+    const int g = grid_for(p.no, kBlock);
+    return g < kPartials ? (g > 0 ? g : 1) : kPartials;
+}
+
+}  // namespace
+
+// target: 0 -> lm->chi2, 1 -> lm->chi2_new (gated on an applied step), 3 -> partial[kPartials]
+void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step) {
+    const int g = cost_grid(p);
+    const int gate = only_if_step ? 2 : (target == 3 ? 0 : 1);
+    k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1);
+    k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate);
+}
+
+void launch_cost_stored64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    const int g = cost_grid(p);
+    k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, 0, 0);
+    k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, 3, 0);
+}
+
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
+    if (p.nl > 0)
+        k_linearise_points<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms, w.Hll, w.bl, w.W,
+                                                                      w.lm, gated ? 1 : 0);
+    if (p.np > 0) k_reduce_pose<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms, w.Hpp, w.bp, w.lm, gated ? 1 : 0);
+    if (gated) k_linearise_done<<<1, 1, 0, s>>>(w.lm);
+}
+
+void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    k_init_lm<<<1, 1024, 0, s>>>(p, w.Hpp, w.Hll, w.lm);
+}
+
+void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override) {
+    if (p.nl > 0) k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override);
+    if (p.npair > 0) k_form_Y<<<grid_for(p.npair, kBlock), kBlock, 0, s>>>(p, w.W, w.Hinv, w.Y, w.lm, lambda_override);
+    const int64_t nn = static_cast<int64_t>(p.n) * p.n;
+    int zg = grid_for(nn, kBlock);
+    if (zg > 4096) zg = 4096;
+    k_fill_zero<<<zg, kBlock, 0, s>>>(w.S, nn, w.lm, lambda_override ? 0 : 1);
+    if (p.nblk > 0)
+        k_schur<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+                                                            lambda_override);
+}
+
+void launch_after_solve64(const dev_work64& w, hipStream_t s) { k_after_solve<<<1, 1, 0, s>>>(w.lm); }
+
+void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    int g = grid_for(p.np + p.nl, kBlock);
+    if (g > kPartials) g = kPartials;
+    if (g < 1) g = 1;
+    double* partial = w.partial + 2 * kPartials;
+    k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
+                                          w.bp, w.bl, partial, w.lm, 1, 1);
+    k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
+}
+
+void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    int g = grid_for(p.np + p.nl, kBlock);
+    if (g > kPartials) g = kPartials;
+    if (g < 1) g = 1;
+    k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
+                                          w.bp, w.bl, nullptr, w.lm, 0, 0);
+}
+
+void launch_decide64(const dev_work64& w, hipStream_t s) { k_decide<<<1, 1, 0, s>>>(w.lm); }
+
+void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    k_rollback<<<grid_for(p.np + p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.lm);
+}
+
+void launch_end_attempt64(const dev_work64& w, hipStream_t s) { k_end_attempt<<<1, 1, 0, s>>>(w.lm); }
+
+void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s) {
+    if (p.no > 0) k_eval_edges<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, out);
+}
+
+void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s) {
+    if (p.no > 0) k_inliers<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, flags_sorted);
+}
+
+}  // namespace zb

@@ -1,0 +1,202 @@
+"""Python handle over the engine's C ABI (``include/zsba.h``).
+
+:class:`Graph` owns one ``zsba_graph`` handle.  It is a thin layer: every call goes
+to ``libzsba.so`` and every numeric result comes from the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ptr
+from .problem import Problem
+
+FP64 = _lib.ZSBA_PRECISION_FP64
+MIXED = _lib.ZSBA_PRECISION_MIXED
+
+
+class ZsbaError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_lib.ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class Graph:
+    def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 2):
+        self.lib = _lib.load()
+        opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps)
+        self.h = self.lib.zsba_create(C.byref(opts))
+        if not self.h:
+            raise MemoryError("zsba_create failed")
+
+    # -- lifetime ------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.zsba_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise ZsbaError(rc, self.lib.zsba_last_error(self.h).decode())
+        return rc
+
+    # -- construction ----------------------------------------------------------
+    @classmethod
+    def from_problem(cls, problem: Problem, **kw) -> "Graph":
+        g = cls(**kw)
+        g.load_problem(problem)
+        return g
+
+    def load_problem(self, pr: Problem) -> None:
+        self.add_poses(pr.poses, pr.pose_fixed)
+        self.add_points(pr.points, pr.point_fixed)
+        K = pr.uniform_camera()
+        loss = pr.uniform_loss()
+        if K is not None and loss is not None:
+            self.add_observations(pr.obs_pose, pr.obs_point, pr.obs_uv, K, loss[0], loss[1], pr.obs_info)
+        else:
+            for e in range(pr.n_obs):
+                info = None if pr.obs_info is None else pr.obs_info[e]
+                self.add_observation(int(pr.obs_pose[e]), int(pr.obs_point[e]), pr.obs_uv[e], pr.obs_K[e],
+                                     int(pr.obs_loss[e]), float(pr.obs_delta[e]), info)
+
+    def add_poses(self, params, fixed=None) -> int:
+        params = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 7)
+        fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        return self._check(self.lib.zsba_add_poses(self.h, len(params), ptr(params), ptr(fixed, C.c_uint8)))
+
+    def add_points(self, xyz, fixed=None) -> int:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.uint8)
+        return self._check(self.lib.zsba_add_points(self.h, len(xyz), ptr(xyz), ptr(fixed, C.c_uint8)))
+
+    def add_observations(self, pose_ids, point_ids, uv, K, loss_kind=0, loss_delta=0.0, info=None) -> int:
+        pose_ids = np.ascontiguousarray(pose_ids, dtype=np.int32)
+        point_ids = np.ascontiguousarray(point_ids, dtype=np.int32)
+        uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        info = None if info is None else np.ascontiguousarray(info, dtype=np.float64).reshape(-1, 4)
+        return self._check(self.lib.zsba_add_observations(
+            self.h, len(pose_ids), ptr(pose_ids, C.c_int32), ptr(point_ids, C.c_int32), ptr(uv), ptr(info), ptr(K),
+            int(loss_kind), float(loss_delta)))
+
+    def add_observation(self, pose_id, point_id, uv, K, loss_kind=0, loss_delta=0.0, info=None) -> int:
+        uv = np.ascontiguousarray(uv, dtype=np.float64)
+        K = np.ascontiguousarray(K, dtype=np.float64)
+        info = None if info is None else np.ascontiguousarray(info, dtype=np.float64)
+        return self._check(self.lib.zsba_add_observation(self.h, int(pose_id), int(point_id), ptr(uv), ptr(info), ptr(K),
+                                                         int(loss_kind), float(loss_delta)))
+
+    def remove_observation(self, i: int) -> None:
+        self._check(self.lib.zsba_remove_observation(self.h, int(i)))
+
+    def remove_pose(self, i: int) -> None:
+        self._check(self.lib.zsba_remove_pose(self.h, int(i)))
+
+    def remove_point(self, i: int) -> None:
+        self._check(self.lib.zsba_remove_point(self.h, int(i)))
+
+    def set_pose(self, i: int, params) -> None:
+        params = np.ascontiguousarray(params, dtype=np.float64)
+        self._check(self.lib.zsba_set_pose(self.h, int(i), ptr(params)))
+
+    def set_point(self, i: int, xyz) -> None:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._check(self.lib.zsba_set_point(self.h, int(i), ptr(xyz)))
+
+    def set_pose_fixed(self, i: int, fixed: bool) -> None:
+        self._check(self.lib.zsba_set_pose_fixed(self.h, int(i), int(bool(fixed))))
+
+    def set_point_fixed(self, i: int, fixed: bool) -> None:
+        self._check(self.lib.zsba_set_point_fixed(self.h, int(i), int(bool(fixed))))
+
+    def counts(self) -> tuple[int, int, int]:
+        c = np.zeros(3, np.int64)
+        self._check(self.lib.zsba_counts(self.h, ptr(c, C.c_int64)))
+        return int(c[0]), int(c[1]), int(c[2])
+
+    # -- optimisation -------------------------------------------------------------
+    def solve(self, iterations: int = 10, use_relative_convergence: bool = False, precision: int = FP64) -> int:
+        return self._check(self.lib.zsba_solve(self.h, int(iterations), int(bool(use_relative_convergence)),
+                                               int(precision)))
+
+    def current_chi(self, recompute: bool = True) -> float:
+        v = self.lib.zsba_current_chi(self.h, int(bool(recompute)))
+        if np.isnan(v) and self.lib.zsba_last_error(self.h):
+            msg = self.lib.zsba_last_error(self.h).decode()
+            if msg:
+                raise ZsbaError(-3, msg)
+        return float(v)
+
+    def poses(self) -> np.ndarray:
+        out = np.full((self.counts()[0], 7), np.nan)
+        self._check(self.lib.zsba_get_poses(self.h, ptr(out)))
+        return out
+
+    def points(self) -> np.ndarray:
+        out = np.full((self.counts()[1], 3), np.nan)
+        self._check(self.lib.zsba_get_points(self.h, ptr(out)))
+        return out
+
+    def inlier_flags(self) -> np.ndarray:
+        out = np.zeros(self.counts()[2], np.uint8)
+        self._check(self.lib.zsba_get_inlier_flags(self.h, ptr(out, C.c_uint8)))
+        return out
+
+    def residuals(self) -> np.ndarray:
+        out = np.full((self.counts()[2], 2), np.nan)
+        self._check(self.lib.zsba_get_residuals(self.h, ptr(out)))
+        return out
+
+    def stats(self) -> dict:
+        st = _lib.Stats()
+        self._check(self.lib.zsba_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    # -- diagnostics ---------------------------------------------------------------
+    def eval_edges(self) -> np.ndarray:
+        out = np.full((self.counts()[2], _lib.EDGE_EVAL_STRIDE), np.nan)
+        self._check(self.lib.zsba_debug_eval_edges(self.h, ptr(out)))
+        return out
+
+    def reduced_system(self, lam: float, precision: int = FP64):
+        n_p = int(np.count_nonzero(~np.isnan(self.poses()[:, 0])))
+        n_l = int(np.count_nonzero(~np.isnan(self.points()[:, 0])))
+        n = 6 * n_p
+        S = np.zeros((n, n)); bs = np.zeros(n); xp = np.zeros(n); xl = np.zeros(3 * n_l)
+        ok = self._check(self.lib.zsba_debug_reduced_system(self.h, float(lam), int(precision), ptr(S), ptr(bs),
+                                                            ptr(xp), ptr(xl)))
+        return bool(ok), S, bs, xp, xl
+
+    def adjoint(self, dL_dpoints, dL_dposes=None):
+        n_p, n_l, n_o = self.counts()
+        dL_dpoints = np.ascontiguousarray(dL_dpoints, dtype=np.float64).reshape(n_l, 3)
+        dL_dposes = None if dL_dposes is None else np.ascontiguousarray(dL_dposes, dtype=np.float64).reshape(n_p, 6)
+        d_obs = np.zeros((n_o, 2))
+        d_K = np.zeros(4)
+        self._check(self.lib.zsba_adjoint(self.h, ptr(dL_dpoints), ptr(dL_dposes), ptr(d_obs), ptr(d_K)))
+        return d_obs, d_K
+
+
+def solve_batch(graphs: list[Graph], iterations: int, use_relative_convergence: bool = False,
+                precision: int = MIXED) -> np.ndarray:
+    lib = _lib.load()
+    arr = (C.c_void_p * len(graphs))(*[g.h for g in graphs])
+    acc = np.zeros(len(graphs), np.int32)
+    rc = lib.zsba_solve_batch(arr, len(graphs), int(iterations), int(bool(use_relative_convergence)), int(precision),
+                              ptr(acc, C.c_int32))
+    if rc < 0:
+        msgs = [lib.zsba_last_error(g.h).decode() for g in graphs if lib.zsba_last_error(g.h)]
+        raise ZsbaError(rc, "; ".join(m for m in msgs if m))
+    return acc
