@@ -133,6 +133,14 @@ struct zsba_graph {
         return fail(ZSBA_EDEVICE, "%s: %s", what, hipGetErrorString(e));
     }
 
+    // Copies are ordered on the handle's stream and complete before returning.
+    hipError_t copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+        if (bytes == 0) return hipSuccess;
+        const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
+        if (e != hipSuccess) return e;
+        return hipStreamSynchronize(stream);
+    }
+
     int32_t live_poses() const { return static_cast<int32_t>(std::count(pose_alive.begin(), pose_alive.end(), 1)); }
     int32_t live_points() const { return static_cast<int32_t>(std::count(point_alive.begin(), point_alive.end(), 1)); }
     int32_t live_obs() const { return static_cast<int32_t>(std::count(obs_alive.begin(), obs_alive.end(), 1)); }
@@ -175,9 +183,9 @@ struct zsba_graph {
         if (!device_newer) return 0;
         std::vector<double> P(7 * pose_slot_id.size()), X(3 * point_slot_id.size());
         hipError_t e;
-        if (!P.empty() && (e = hipMemcpy(P.data(), w.pose, P.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        if (!P.empty() && (e = copy(P.data(), w.pose, P.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(e, "download poses");
-        if (!X.empty() && (e = hipMemcpy(X.data(), w.point, X.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        if (!X.empty() && (e = copy(X.data(), w.point, X.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
             return hip_fail(e, "download points");
         for (size_t i = 0; i < pose_slot_id.size(); ++i) std::memcpy(&pose[7 * pose_slot_id[i]], &P[7 * i], 7 * sizeof(double));
         for (size_t j = 0; j < point_slot_id.size(); ++j) std::memcpy(&point[3 * point_slot_id[j]], &X[3 * j], 3 * sizeof(double));
@@ -354,7 +362,7 @@ int32_t zsba_graph::build_structure() {
     p.n = static_cast<int32_t>(n); p.identity_info = identity ? 1 : 0;
     auto up = [&](auto* dst, const auto& src) {
         if (!src.empty() && dst)
-            (void)hipMemcpy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice);
+            (void)copy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice);
         return dst;
     };
     int32_t *d_obs_pose = mem.get<int32_t>(no), *d_obs_point = mem.get<int32_t>(no), *d_obs_pair = mem.get<int32_t>(no),
@@ -418,9 +426,9 @@ int32_t zsba_graph::upload_values() {
     for (size_t i = 0; i < np; ++i) std::memcpy(&P[7 * i], &pose[7L * pose_slot_id[i]], 7 * sizeof(double));
     for (size_t j = 0; j < nl; ++j) std::memcpy(&X[3 * j], &point[3L * point_slot_id[j]], 3 * sizeof(double));
     hipError_t e;
-    if (np && (e = hipMemcpy(w.pose, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    if (np && (e = copy(w.pose, P.data(), P.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "upload poses");
-    if (nl && (e = hipMemcpy(w.point, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+    if (nl && (e = copy(w.point, X.data(), X.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "upload points");
     values_dirty = false;
     device_newer = false;
@@ -491,7 +499,7 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     if (e != hipSuccess) return hip_fail(e, "solve");
-    if ((e = hipMemcpy(lm_pinned, w.lm, sizeof(lm_state), hipMemcpyDeviceToHost)) != hipSuccess)
+    if ((e = copy(lm_pinned, w.lm, sizeof(lm_state), hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "download controller");
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
     stats.solve_ms = now_ms() - t0;
@@ -708,7 +716,7 @@ double zsba_current_chi(zsba_graph* g, int32_t recompute) {
     if (recompute) zb::launch_cost64(g->dp, g->w, g->stream, 3, false);
     else zb::launch_cost_stored64(g->dp, g->w, g->stream);
     double v = NAN;
-    if (hipMemcpy(&v, g->w.partial + zb::kPartials, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+    if (g->copy(&v, g->w.partial + zb::kPartials, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
         g->fail(ZSBA_EDEVICE, "current_chi");
         return NAN;
     }
@@ -722,7 +730,7 @@ int32_t zsba_get_inlier_flags(zsba_graph* g, uint8_t* flags) {
     if (rc) return rc;
     zb::launch_inliers64(g->dp, g->w, g->flags_dev, g->stream);
     std::vector<uint8_t> f(g->dp.no);
-    const hipError_t e = hipMemcpy(f.data(), g->flags_dev, f.size(), hipMemcpyDeviceToHost);
+    const hipError_t e = g->copy(f.data(), g->flags_dev, f.size(), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return g->hip_fail(e, "inlier flags");
     for (int32_t s = 0; s < g->dp.no; ++s) flags[g->obs_e_id[g->sorted_e[s]]] = f[s];
     return 0;
@@ -735,7 +743,7 @@ int32_t zsba_get_residuals(zsba_graph* g, double* out) {
     if (rc) return rc;
     zb::launch_cost64(g->dp, g->w, g->stream, 3, false);
     std::vector<double> r(2L * g->dp.no);
-    const hipError_t e = hipMemcpy(r.data(), g->w.residual, r.size() * sizeof(double), hipMemcpyDeviceToHost);
+    const hipError_t e = g->copy(r.data(), g->w.residual, r.size() * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return g->hip_fail(e, "residuals");
     for (int32_t s = 0; s < g->dp.no; ++s) {
         const int32_t id = g->obs_e_id[g->sorted_e[s]];
@@ -766,7 +774,7 @@ int32_t zsba_debug_eval_edges(zsba_graph* g, double* out) {
     }
     zb::launch_eval_edges64(g->dp, g->w, g->eval_dev, g->stream);
     std::vector<double> v(need);
-    const hipError_t e = hipMemcpy(v.data(), g->eval_dev, need * sizeof(double), hipMemcpyDeviceToHost);
+    const hipError_t e = g->copy(v.data(), g->eval_dev, need * sizeof(double), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return g->hip_fail(e, "eval edges");
     for (int32_t k = 0; k < g->dp.no; ++k) std::memcpy(out + 27L * g->obs_e_id[k], &v[27L * k], 27 * sizeof(double));
     return 0;
@@ -783,8 +791,8 @@ int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precisio
     st.lambda = lambda;
     hipError_t e;
     if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hip_fail(e, "sync");
-    if ((e = hipMemcpy(g->w.lm, &st, sizeof st, hipMemcpyHostToDevice)) != hipSuccess) return g->hip_fail(e, "lm");
-    if ((e = hipMemcpy(g->lambda_dev, &lambda, sizeof lambda, hipMemcpyHostToDevice)) != hipSuccess)
+    if ((e = g->copy(g->w.lm, &st, sizeof st, hipMemcpyHostToDevice)) != hipSuccess) return g->hip_fail(e, "lm");
+    if ((e = g->copy(g->lambda_dev, &lambda, sizeof lambda, hipMemcpyHostToDevice)) != hipSuccess)
         return g->hip_fail(e, "lambda");
     zb::launch_linearise64(g->dp, g->w, g->stream, false);
     zb::launch_reduced_system64(g->dp, g->w, g->stream, g->lambda_dev);
@@ -797,13 +805,13 @@ int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precisio
     zb::launch_backsub_only64(g->dp, g->w, g->stream);
     if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hip_fail(e, "reduced system");
     lm_state out{};
-    if ((e = hipMemcpy(&out, g->w.lm, sizeof out, hipMemcpyDeviceToHost)) != hipSuccess) return g->hip_fail(e, "lm");
+    if ((e = g->copy(&out, g->w.lm, sizeof out, hipMemcpyDeviceToHost)) != hipSuccess) return g->hip_fail(e, "lm");
     if (out.solve_ok) {
-        if (dx_pose && (e = hipMemcpy(dx_pose, g->w.dxp, n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        if (dx_pose && (e = g->copy(dx_pose, g->w.dxp, n * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
             return g->hip_fail(e, "dx_pose");
         if (dx_point) {
             std::vector<double> d(3L * g->dp.nl);
-            if ((e = hipMemcpy(d.data(), g->w.dxl, d.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            if ((e = g->copy(d.data(), g->w.dxl, d.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
                 return g->hip_fail(e, "dx_point");
             std::memcpy(dx_point, d.data(), d.size() * sizeof(double));
         }
