@@ -63,8 +63,23 @@ typedef struct zsba_options {
     int32_t device;       /* HIP device ordinal (default 0)                          */
     int32_t verbose;      /* 1: print the reference's [INIT]/[GOOD]/[ BAD] lines     */
     int32_t refine_steps; /* MIXED: refinement steps of the reduced solve (default 2)*/
-    int32_t reserved[5];
+    int32_t timing;       /* 1: time every kernel launch with HIP events (stats)      */
+    int32_t reserved[4];
 } zsba_options;
+
+/* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */
+#define ZSBA_K_LINEARISE   0  /* per-point linearisation: residual, Jacobians, H_ll, b_l, W */
+#define ZSBA_K_POSE_REDUCE 1  /* per-pose H_pp / b_p sums                                   */
+#define ZSBA_K_COST        2  /* residuals + robust chi2 (+ its reduction)                  */
+#define ZSBA_K_INVERT      3  /* damping + 3x3 point-block inverses                         */
+#define ZSBA_K_FORM_Y      4  /* Y = W H_ll^-1                                              */
+#define ZSBA_K_FILL        5  /* clearing the reduced system                                */
+#define ZSBA_K_SCHUR       6  /* reduced camera system blocks and b_s                       */
+#define ZSBA_K_CHOLESKY    7  /* dense factorisation of the reduced system                  */
+#define ZSBA_K_TRSV        8  /* forward / backward substitution                            */
+#define ZSBA_K_BACKSUB     9  /* point back-substitution, state update, LM scale            */
+#define ZSBA_K_CONTROL    10  /* LM controller, rollback and bookkeeping                    */
+#define ZSBA_K_COUNT      16
 
 /* Per-solve statistics (filled by zsba_get_stats after zsba_solve). */
 typedef struct zsba_stats {
@@ -80,6 +95,8 @@ typedef struct zsba_stats {
     double lambda;           /* damping at exit                                       */
     double solve_ms;         /* wall time of the numeric solve, device-synchronised   */
     double setup_ms;         /* upload + structure analysis (0 when cached)           */
+    double kernel_ms[ZSBA_K_COUNT];        /* summed device time per kernel class     */
+    int64_t kernel_launches[ZSBA_K_COUNT]; /* launches per kernel class               */
     double reserved[8];
 } zsba_stats;
 
@@ -153,6 +170,13 @@ int32_t zsba_get_inlier_flags(zsba_graph* graph, uint8_t* flags /* n_obs_added *
  * current state, indexed by id.  */
 int32_t zsba_get_residuals(zsba_graph* graph, double* residuals /* n_obs_added*2 */);
 int32_t zsba_get_stats(const zsba_graph* graph, zsba_stats* stats);
+
+/* ---- workload control (benchmarks) -------------------------------------------
+ * zsba_snapshot copies the current device parameters aside; zsba_restore puts
+ * them back without any host transfer, so repeated solves start from the same
+ * state with their inputs already resident in device memory. */
+int32_t zsba_snapshot(zsba_graph* graph);
+int32_t zsba_restore(zsba_graph* graph);
 
 /* ---- batched local BA (no reference counterpart: many independent graphs) -----
  * Solves every graph with the same iteration count, convergence rule and

@@ -14,12 +14,15 @@ ZSBA_LOSS_NONE, ZSBA_LOSS_TRIVIAL, ZSBA_LOSS_HUBER, ZSBA_LOSS_CAUCHY, ZSBA_LOSS_
 ZSBA_PRECISION_FP64, ZSBA_PRECISION_MIXED = 0, 1
 EDGE_EVAL_STRIDE = 27
 
+KERNEL_NAMES = ("linearise", "pose_reduce", "cost", "invert", "form_Y", "fill", "schur", "cholesky", "trsv",
+                "backsub", "control")
+
 ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EDEVICE", -4: "ENOMEM", -5: "EUNSUP"}
 
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("refine_steps", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("timing", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
@@ -27,10 +30,14 @@ class Stats(C.Structure):
                 ("rejected", C.c_int32), ("solve_failures", C.c_int32), ("precision", C.c_int32),
                 ("n_obs", C.c_int64), ("n_poses", C.c_int64), ("n_points", C.c_int64),
                 ("initial_chi2", C.c_double), ("final_chi2", C.c_double), ("lambda_", C.c_double),
-                ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("reserved", C.c_double * 8)]
+                ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("kernel_ms", C.c_double * 16),
+                ("kernel_launches", C.c_int64 * 16), ("reserved", C.c_double * 8)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "kernel_ms", "kernel_launches")}
+        d["kernel_ms"] = {name: self.kernel_ms[i] for i, name in enumerate(KERNEL_NAMES)}
+        d["kernel_launches"] = {name: int(self.kernel_launches[i]) for i, name in enumerate(KERNEL_NAMES)}
+        return d
 
 
 class SynthParams(C.Structure):
@@ -75,6 +82,8 @@ _SIGNATURES = {
     "zsba_get_inlier_flags": (C.c_int32, [_P, _U8]),
     "zsba_get_residuals": (C.c_int32, [_P, _D]),
     "zsba_get_stats": (C.c_int32, [_P, C.POINTER(Stats)]),
+    "zsba_snapshot": (C.c_int32, [_P]),
+    "zsba_restore": (C.c_int32, [_P]),
     "zsba_solve_batch": (C.c_int32, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32]),
     "zsba_adjoint": (C.c_int32, [_P, _D, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),

@@ -10,8 +10,71 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace zb {
+
+// Per-kernel-class device time from HIP events recorded around launches on the
+// launching stream (enabled by zsba_options.timing).
+struct kernel_timer {
+    static constexpr int kClasses = 16;
+    struct record {
+        hipEvent_t start, stop;
+        int id;
+    };
+    bool on = false;
+    std::vector<record> pending;
+    std::vector<hipEvent_t> pool;
+    double ms[kClasses] = {};
+    int64_t launches[kClasses] = {};
+
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    void begin(int id, hipStream_t s) {
+        if (!on) return;
+        record r{take(), take(), id};
+        (void)hipEventRecord(r.start, s);
+        pending.push_back(r);
+    }
+    void end(hipStream_t s) {
+        if (!on || pending.empty()) return;
+        (void)hipEventRecord(pending.back().stop, s);
+    }
+    // Call after the stream has been synchronised.
+    void collect() {
+        for (const record& r : pending) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, r.start, r.stop) == hipSuccess) {
+                ms[r.id] += t;
+                launches[r.id] += 1;
+            }
+            pool.push_back(r.start);
+            pool.push_back(r.stop);
+        }
+        pending.clear();
+    }
+    void reset() {
+        collect();
+        for (int i = 0; i < kClasses; ++i) { ms[i] = 0.0; launches[i] = 0; }
+    }
+    ~kernel_timer() {
+        for (const record& r : pending) { (void)hipEventDestroy(r.start); (void)hipEventDestroy(r.stop); }
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+enum kernel_class : int {
+    K_LINEARISE = 0, K_POSE_REDUCE = 1, K_COST = 2, K_INVERT = 3, K_FORM_Y = 4, K_FILL = 5, K_SCHUR = 6,
+    K_CHOLESKY = 7, K_TRSV = 8, K_BACKSUB = 9, K_CONTROL = 10
+};
 
 // One W block (6x3 pose-point coupling) exists per distinct (pose, point) pair
 // whose pose and point are both free (factor_graph.hpp:669-709).
@@ -87,7 +150,11 @@ struct dev_work64 {
     double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
     int32_t* chol_status = nullptr; // [1] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
+    kernel_timer* timer = nullptr;  // host object; null = no timing
 };
+
+#define ZB_TIME_BEGIN(w, id, s) do { if ((w).timer) (w).timer->begin((id), (s)); } while (0)
+#define ZB_TIME_END(w, s) do { if ((w).timer) (w).timer->end((s)); } while (0)
 
 constexpr int kPartials = 1024;  // reduction partial slots
 

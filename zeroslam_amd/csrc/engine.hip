@@ -109,6 +109,7 @@ struct zsba_graph {
     bool have_snapshot = false;
 
     zsba_stats stats{};
+    zb::kernel_timer timer;
 
     ~zsba_graph() {
         if (hip_ready) {
@@ -380,6 +381,8 @@ int32_t zsba_graph::build_structure() {
     int32_t* d_lk = mem.get<int32_t>(lk.size());
     double* d_ld = mem.get<double>(ld.size());
     w = dev_work64{};
+    w.timer = opt.timing ? &timer : nullptr;
+    timer.on = opt.timing != 0;
     w.pose = mem.get<double>(7L * np); w.pose_backup = mem.get<double>(7L * np);
     w.point = mem.get<double>(3L * nl); w.point_backup = mem.get<double>(3L * nl);
     w.residual = mem.get<double>(2L * no); w.pose_terms = mem.get<double>(27L * no);
@@ -463,6 +466,7 @@ void zsba_graph::print_trace() {
 
 int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
     hipError_t e;
+    timer.reset();
     const double t0 = now_ms();
     std::memset(lm_pinned, 0, sizeof(lm_state));
     lm_pinned->iterations = iterations;
@@ -503,6 +507,11 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
         return hip_fail(e, "download controller");
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
     stats.solve_ms = now_ms() - t0;
+    timer.collect();
+    for (int k = 0; k < zb::kernel_timer::kClasses && k < ZSBA_K_COUNT; ++k) {
+        stats.kernel_ms[k] = timer.ms[k];
+        stats.kernel_launches[k] = timer.launches[k];
+    }
     stats.iterations = lm_pinned->executed_iterations;
     stats.attempts = lm_pinned->attempts;
     stats.accepted = lm_pinned->success;
@@ -817,6 +826,33 @@ int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precisio
         }
     }
     return out.solve_ok ? 1 : 0;
+}
+
+int32_t zsba_snapshot(zsba_graph* g) {
+    if (!g) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return g->fail(ZSBA_ESTATE, "no observations");
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    hipError_t e;
+    if ((e = g->copy(g->snap_pose, g->w.pose, 7L * g->dp.np * sizeof(double), hipMemcpyDeviceToDevice)) != hipSuccess)
+        return g->hip_fail(e, "snapshot");
+    if ((e = g->copy(g->snap_point, g->w.point, 3L * g->dp.nl * sizeof(double), hipMemcpyDeviceToDevice)) != hipSuccess)
+        return g->hip_fail(e, "snapshot");
+    g->have_snapshot = true;
+    return 0;
+}
+
+int32_t zsba_restore(zsba_graph* g) {
+    if (!g) return ZSBA_EINVAL;
+    if (!g->have_snapshot || g->structure_dirty) return g->fail(ZSBA_ESTATE, "no snapshot of the current structure");
+    if (g->values_dirty) return g->fail(ZSBA_ESTATE, "parameters were set after the snapshot");
+    hipError_t e;
+    if ((e = g->copy(g->w.pose, g->snap_pose, 7L * g->dp.np * sizeof(double), hipMemcpyDeviceToDevice)) != hipSuccess)
+        return g->hip_fail(e, "restore");
+    if ((e = g->copy(g->w.point, g->snap_point, 3L * g->dp.nl * sizeof(double), hipMemcpyDeviceToDevice)) != hipSuccess)
+        return g->hip_fail(e, "restore");
+    g->device_newer = true;
+    return 0;
 }
 
 int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations, int32_t use_rel, int32_t precision,

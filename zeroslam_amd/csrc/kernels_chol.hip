@@ -264,6 +264,7 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
         k_empty_ok<<<1, 1, 0, s>>>(w.lm, g);
         return;
     }
+    ZB_TIME_BEGIN(w, K_CHOLESKY, s);
     k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g);
     const int tiles = (n + NB - 1) / NB;
     for (int t = 0; t < tiles; ++t) {
@@ -277,7 +278,10 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
             k_syrk<<<static_cast<unsigned>(pairs), 256, 0, s>>>(n, w.S, k0, t + 1, w.chol_status, w.lm, g);
         }
     }
+    ZB_TIME_END(w, s);
+    ZB_TIME_BEGIN(w, K_TRSV, s);
     k_trsv<<<1, 1024, 0, s>>>(n, w.S, w.bs, w.dxp, w.chol_status, w.lm, g);
+    ZB_TIME_END(w, s);
 }
 
 }  // namespace zb

@@ -621,8 +621,10 @@ inline int cost_grid(const dev_problem& p) {
 void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step) {
     const int g = cost_grid(p);
     const int gate = only_if_step ? 2 : (target == 3 ? 0 : 1);
+    ZB_TIME_BEGIN(w, K_COST, s);
     k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1);
     k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate);
+    ZB_TIME_END(w, s);
 }
 
 void launch_cost_stored64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
@@ -632,10 +634,17 @@ void launch_cost_stored64(const dev_problem& p, const dev_work64& w, hipStream_t
 }
 
 void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
-    if (p.nl > 0)
+    if (p.nl > 0) {
+        ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_linearise_points<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms, w.Hll, w.bl, w.W,
                                                                       w.lm, gated ? 1 : 0);
-    if (p.np > 0) k_reduce_pose<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms, w.Hpp, w.bp, w.lm, gated ? 1 : 0);
+        ZB_TIME_END(w, s);
+    }
+    if (p.np > 0) {
+        ZB_TIME_BEGIN(w, K_POSE_REDUCE, s);
+        k_reduce_pose<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms, w.Hpp, w.bp, w.lm, gated ? 1 : 0);
+        ZB_TIME_END(w, s);
+    }
     if (gated) k_linearise_done<<<1, 1, 0, s>>>(w.lm);
 }
 
@@ -644,27 +653,46 @@ void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s) 
 }
 
 void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override) {
-    if (p.nl > 0) k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override);
-    if (p.npair > 0) k_form_Y<<<grid_for(p.npair, kBlock), kBlock, 0, s>>>(p, w.W, w.Hinv, w.Y, w.lm, lambda_override);
+    if (p.nl > 0) {
+        ZB_TIME_BEGIN(w, K_INVERT, s);
+        k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override);
+        ZB_TIME_END(w, s);
+    }
+    if (p.npair > 0) {
+        ZB_TIME_BEGIN(w, K_FORM_Y, s);
+        k_form_Y<<<grid_for(p.npair, kBlock), kBlock, 0, s>>>(p, w.W, w.Hinv, w.Y, w.lm, lambda_override);
+        ZB_TIME_END(w, s);
+    }
     const int64_t nn = static_cast<int64_t>(p.n) * p.n;
     int zg = grid_for(nn, kBlock);
     if (zg > 4096) zg = 4096;
+    ZB_TIME_BEGIN(w, K_FILL, s);
     k_fill_zero<<<zg, kBlock, 0, s>>>(w.S, nn, w.lm, lambda_override ? 0 : 1);
-    if (p.nblk > 0)
+    ZB_TIME_END(w, s);
+    if (p.nblk > 0) {
+        ZB_TIME_BEGIN(w, K_SCHUR, s);
         k_schur<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
                                                             lambda_override);
+        ZB_TIME_END(w, s);
+    }
 }
 
-void launch_after_solve64(const dev_work64& w, hipStream_t s) { k_after_solve<<<1, 1, 0, s>>>(w.lm); }
+void launch_after_solve64(const dev_work64& w, hipStream_t s) {
+    ZB_TIME_BEGIN(w, K_CONTROL, s);
+    k_after_solve<<<1, 1, 0, s>>>(w.lm);
+    ZB_TIME_END(w, s);
+}
 
 void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
     int g = grid_for(p.np + p.nl, kBlock);
     if (g > kPartials) g = kPartials;
     if (g < 1) g = 1;
     double* partial = w.partial + 2 * kPartials;
+    ZB_TIME_BEGIN(w, K_BACKSUB, s);
     k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
                                           w.bp, w.bl, partial, w.lm, 1, 1);
     k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
+    ZB_TIME_END(w, s);
 }
 
 void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
@@ -675,13 +703,23 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
                                           w.bp, w.bl, nullptr, w.lm, 0, 0);
 }
 
-void launch_decide64(const dev_work64& w, hipStream_t s) { k_decide<<<1, 1, 0, s>>>(w.lm); }
-
-void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
-    k_rollback<<<grid_for(p.np + p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.lm);
+void launch_decide64(const dev_work64& w, hipStream_t s) {
+    ZB_TIME_BEGIN(w, K_CONTROL, s);
+    k_decide<<<1, 1, 0, s>>>(w.lm);
+    ZB_TIME_END(w, s);
 }
 
-void launch_end_attempt64(const dev_work64& w, hipStream_t s) { k_end_attempt<<<1, 1, 0, s>>>(w.lm); }
+void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    ZB_TIME_BEGIN(w, K_CONTROL, s);
+    k_rollback<<<grid_for(p.np + p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.lm);
+    ZB_TIME_END(w, s);
+}
+
+void launch_end_attempt64(const dev_work64& w, hipStream_t s) {
+    ZB_TIME_BEGIN(w, K_CONTROL, s);
+    k_end_attempt<<<1, 1, 0, s>>>(w.lm);
+    ZB_TIME_END(w, s);
+}
 
 void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s) {
     if (p.no > 0) k_eval_edges<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, out);

@@ -24,9 +24,9 @@ class ZsbaError(RuntimeError):
 
 
 class Graph:
-    def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 2):
+    def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 2, timing: bool = False):
         self.lib = _lib.load()
-        opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps)
+        opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps, timing=int(bool(timing)))
         self.h = self.lib.zsba_create(C.byref(opts))
         if not self.h:
             raise MemoryError("zsba_create failed")
@@ -130,6 +130,14 @@ class Graph:
     def solve(self, iterations: int = 10, use_relative_convergence: bool = False, precision: int = FP64) -> int:
         return self._check(self.lib.zsba_solve(self.h, int(iterations), int(bool(use_relative_convergence)),
                                                int(precision)))
+
+    def snapshot(self) -> None:
+        """Keep the current device parameters aside (no host transfer)."""
+        self._check(self.lib.zsba_snapshot(self.h))
+
+    def restore(self) -> None:
+        """Put the snapshot back on the device (no host transfer)."""
+        self._check(self.lib.zsba_restore(self.h))
 
     def current_chi(self, recompute: bool = True) -> float:
         v = self.lib.zsba_current_chi(self.h, int(bool(recompute)))
