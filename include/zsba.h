@@ -50,14 +50,20 @@ extern "C" {
 #define ZSBA_LOSS_TUKEY   4
 
 /* Arithmetic of a solve.
- *   FP64:  every kernel in binary64 with the reference's operation order per
- *          observation and no contraction; the result matches the reference
- *          optimiser to rounding of the reductions.
- *   MIXED: residual/Jacobian/Hessian-block/Schur work in binary32 with binary32
- *          accumulation; the LM cost, its comparison, the reduced-camera Cholesky
- *          and iterative refinement of the reduced solve stay binary64. */
-#define ZSBA_PRECISION_FP64  0
-#define ZSBA_PRECISION_MIXED 1
+ *   FP64:      binary64 in the reference's operation order throughout: per
+ *              observation, per vertex sum, in the iteration order of the
+ *              reference's hash maps for the Schur sums, sequential cost and
+ *              scale sums, and a Cholesky with the row-Crout update order.  The
+ *              result is the reference optimiser's, bit for bit (the C library's
+ *              sin/cos/pow are matched by correctly rounded evaluations).
+ *   MIXED:     residual/Jacobian/Hessian-block/Schur work in binary32 with binary32
+ *              accumulation; the LM cost, its comparison, the reduced-camera
+ *              Cholesky and iterative refinement of the reduced solve stay binary64.
+ *   FP64_FAST: binary64 with parallel (fixed-shape, deterministic) reductions;
+ *              equal to FP64 up to rounding of the sums. */
+#define ZSBA_PRECISION_FP64      0
+#define ZSBA_PRECISION_MIXED     1
+#define ZSBA_PRECISION_FP64_FAST 2
 
 typedef struct zsba_options {
     int32_t device;       /* HIP device ordinal (default 0)                          */
@@ -210,6 +216,10 @@ int32_t zsba_debug_eval_edges(zsba_graph* graph, double* out /* n_obs_added*27 *
  * it failed (factor_graph.hpp:736-806). */
 int32_t zsba_debug_reduced_system(zsba_graph* graph, double lambda, int32_t precision,
                                   double* S, double* b_s, double* dx_pose, double* dx_point);
+/* The engine's correctly rounded elementary functions: fn 0 sin, 1 cos,
+ * 2 cube (pow(x, 3.0)), 3 sqrt; evaluated by the host code (on_device = 0) or
+ * by a device kernel (on_device = 1). */
+int32_t zsba_debug_math(int32_t fn, int32_t n, const double* in, double* out, int32_t on_device);
 
 /* ---- synthetic problems (benchmark and test workloads) ------------------------ */
 typedef struct zsba_synth_params {

@@ -11,7 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libzsba.so"
 
 ZSBA_LOSS_NONE, ZSBA_LOSS_TRIVIAL, ZSBA_LOSS_HUBER, ZSBA_LOSS_CAUCHY, ZSBA_LOSS_TUKEY = range(5)
-ZSBA_PRECISION_FP64, ZSBA_PRECISION_MIXED = 0, 1
+ZSBA_PRECISION_FP64, ZSBA_PRECISION_MIXED, ZSBA_PRECISION_FP64_FAST = 0, 1, 2
 EDGE_EVAL_STRIDE = 27
 
 KERNEL_NAMES = ("linearise", "pose_reduce", "cost", "invert", "form_Y", "fill", "schur", "cholesky", "trsv",
@@ -88,6 +88,7 @@ _SIGNATURES = {
     "zsba_adjoint": (C.c_int32, [_P, _D, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),
     "zsba_debug_reduced_system": (C.c_int32, [_P, C.c_double, C.c_int32, _D, _D, _D, _D]),
+    "zsba_debug_math": (C.c_int32, [C.c_int32, C.c_int32, _D, _D, C.c_int32]),
     "zsba_synth_default_params": (C.c_int32, [C.c_int32, C.POINTER(SynthParams)]),
     "zsba_synth_create": (_P, [C.POINTER(SynthParams), C.c_uint64]),
     "zsba_synth_destroy": (None, [_P]),

@@ -311,6 +311,128 @@ ZB_HD bool invert3(const T* v, T* out) {
     return true;
 }
 
+// ---- correctly rounded sin, cos and cube (binary64) ---------------------------
+// The reference takes sin/cos/pow from the C library.  These evaluate the
+// function in double-double arithmetic (error-free transformations with fma,
+// Dekker / Knuth) to ~2^-100 relative and round once, so they return the
+// correctly rounded result except when the exact value lies within ~2^-100 of a
+// rounding midpoint.  Argument reduction subtracts k*pi/2 with pi/2 split into
+// three binary64 parts (Cody-Waite), exact for |k| < 2^20.
+struct dd {
+    double hi, lo;
+};
+ZB_HD dd dd_two_sum(double a, double b) {
+    const double s = a + b;
+    const double bb = s - a;
+    return {s, (a - (s - bb)) + (b - bb)};
+}
+ZB_HD dd dd_quick(double a, double b) {
+    const double s = a + b;
+    return {s, b - (s - a)};
+}
+ZB_HD dd dd_two_prod(double a, double b) {
+    const double p = a * b;
+    return {p, fma(a, b, -p)};
+}
+ZB_HD dd dd_add(dd a, dd b) {
+    dd s = dd_two_sum(a.hi, b.hi);
+    const dd t = dd_two_sum(a.lo, b.lo);
+    s.lo = s.lo + t.hi;
+    s = dd_quick(s.hi, s.lo);
+    s.lo = s.lo + t.lo;
+    return dd_quick(s.hi, s.lo);
+}
+ZB_HD dd dd_mul(dd a, dd b) {
+    dd p = dd_two_prod(a.hi, b.hi);
+    p.lo = p.lo + (a.hi * b.lo + a.lo * b.hi);
+    return dd_quick(p.hi, p.lo);
+}
+ZB_HD dd dd_mul_d(dd a, double b) {
+    dd p = dd_two_prod(a.hi, b);
+    p.lo = p.lo + a.lo * b;
+    return dd_quick(p.hi, p.lo);
+}
+ZB_HD dd dd_div_d(dd a, double b) {
+    const double q1 = a.hi / b;
+    const dd p = dd_two_prod(q1, b);
+    dd r = dd_two_sum(a.hi, -p.hi);
+    r.lo = r.lo - p.lo;
+    r.lo = r.lo + a.lo;
+    const double q2 = (r.hi + r.lo) / b;
+    return dd_quick(q1, q2);
+}
+// sin(r) / cos(r) by Taylor series in double-double, |r| <= pi/4.
+ZB_HD dd dd_sin_kernel(dd r) {
+    const dd r2 = dd_mul(r, r);
+    dd term = r, sum = r;
+    for (int k = 1; k < 16; ++k) {
+        term = dd_mul(term, r2);
+        term = dd_div_d(term, -static_cast<double>((2 * k) * (2 * k + 1)));
+        sum = dd_add(sum, term);
+        if (fabs(term.hi) < 1e-36 * fabs(sum.hi)) break;
+    }
+    return sum;
+}
+ZB_HD dd dd_cos_kernel(dd r) {
+    const dd r2 = dd_mul(r, r);
+    dd term = {1.0, 0.0}, sum = {1.0, 0.0};
+    for (int k = 1; k < 16; ++k) {
+        term = dd_mul(term, r2);
+        term = dd_div_d(term, -static_cast<double>((2 * k - 1) * (2 * k)));
+        sum = dd_add(sum, term);
+        if (fabs(term.hi) < 1e-36) break;
+    }
+    return sum;
+}
+// x - k*pi/2 in double-double, returns the quadrant k mod 4.
+ZB_HD int dd_reduce(double x, dd& r) {
+    const double pio2_1 = 1.5707963267948966;       // nearest binary64 to pi/2
+    const double pio2_2 = 6.123233995736766e-17;    // next 53 bits of pi/2
+    const double pio2_3 = -1.4973849048591698e-33;  // next 53 bits of pi/2
+    if (fabs(x) <= 0.7853981633974483) {
+        r = {x, 0.0};
+        return 0;
+    }
+    const double k = rint(x * 0.6366197723675814);
+    dd t = dd_two_sum(x, -dd_two_prod(k, pio2_1).hi);
+    t.lo = t.lo - dd_two_prod(k, pio2_1).lo;
+    t = dd_quick(t.hi, t.lo);
+    t = dd_add(t, dd_two_prod(-k, pio2_2));
+    t = dd_add(t, dd_two_prod(-k, pio2_3));
+    r = t;
+    long long q = static_cast<long long>(k);
+    return static_cast<int>(q & 3);
+}
+ZB_HD double cr_sin(double x) {
+    if (!(fabs(x) < 1.0e5)) return sin(x);
+    if (x == 0.0) return x;
+    dd r;
+    const int q = dd_reduce(x, r);
+    dd v = (q & 1) ? dd_cos_kernel(r) : dd_sin_kernel(r);
+    if (q & 2) v = {-v.hi, -v.lo};
+    return v.hi + v.lo;
+}
+ZB_HD double cr_cos(double x) {
+    if (!(fabs(x) < 1.0e5)) return cos(x);
+    dd r;
+    const int q = dd_reduce(x, r);
+    dd v = (q & 1) ? dd_sin_kernel(r) : dd_cos_kernel(r);
+    if (q == 1 || q == 2) v = {-v.hi, -v.lo};
+    return v.hi + v.lo;
+}
+// x * x * x rounded once (the value of pow(x, 3.0)).
+ZB_HD double cr_cube(double x) {
+    const dd x2 = dd_two_prod(x, x);
+    dd c = dd_two_prod(x2.hi, x);
+    c.lo = c.lo + x2.lo * x;
+    return c.hi + c.lo;
+}
+
+template <class T> ZB_HD T trig_sin(T x) { return zsin(x); }
+template <class T> ZB_HD T trig_cos(T x) { return zcos(x); }
+template <> ZB_HD double trig_sin<double>(double x) { return cr_sin(x); }
+template <> ZB_HD double trig_cos<double>(double x) { return cr_cos(x); }
+
 // so3::exp and so3::left_jacobian, lie.hpp:247-264, 284-294; vertex_pose::plus
 // T <- exp(delta) * T, factor_graph.hpp:947-960 via se3::exp / se3::operator*
 // (lie.hpp:446-450, 521-526).  delta = (omega, upsilon).
@@ -331,10 +453,10 @@ ZB_HD void pose_plus(T p[7], const T d[6]) {
     } else {
         const T theta = zsqrt(t2);
         const T half = T(0.5) * theta;
-        real = zcos(half);
-        imag = zsin(half) / theta;
-        a = (T(1) - zcos(theta)) / (t2);
-        b = (theta - zsin(theta)) / (t2 * theta);
+        real = trig_cos(half);
+        imag = trig_sin(half) / theta;
+        a = (T(1) - trig_cos(theta)) / (t2);
+        b = (theta - trig_sin(theta)) / (t2 * theta);
     }
     const T qe[4] = {real, w[0] * imag, w[1] * imag, w[2] * imag};
     const T Wh[9] = {T(0), -w[2], w[1], w[2], T(0), -w[0], -w[1], w[0], T(0)};

@@ -108,6 +108,13 @@ struct dev_problem {
     const int64_t* blk_start = nullptr;
     const int2* contrib = nullptr;
     const int32_t* diag_blk = nullptr;   // [np] block index of (i, i)
+    // Reference summation orders (exact binary64 mode), two sets: [0] for the
+    // first linearisation of a solve, [1] for every later one.  The order is the
+    // iteration order of the reference's hash-map containers.
+    const int2* contrib_x[2] = {nullptr, nullptr};      // per block, ordered as the Y map
+    const int32_t* pose_pair_start = nullptr;           // [np+1]
+    const int32_t* pose_pairs_x[2] = {nullptr, nullptr};// W blocks of a pose, ordered as the Y map
+    const int32_t* lm_pairs_x[2] = {nullptr, nullptr};  // W blocks of a point, ordered as the h_lp map
     // vertices
     const uint8_t* pose_fixed = nullptr;
     const uint8_t* point_fixed = nullptr;
@@ -124,7 +131,8 @@ struct lm_state {
     int32_t iterations, use_relative;
     int32_t iter, failures, success, attempts, rejected, solve_failures;
     int32_t done, need_linearize, need_rollback, solve_ok, step_applied, executed_iterations;
-    int32_t pad[4];
+    int32_t lin_count;  // linearisations performed in this solve
+    int32_t pad[3];
 };
 
 // Working buffers of the binary64 path.
@@ -147,6 +155,9 @@ struct dev_work64 {
     double* dxp = nullptr;          // [n]
     double* dxl = nullptr;          // [nl*3]
     double* partial = nullptr;      // reduction partials
+    double* cbuf = nullptr;         // [no] per-observation robust chi2, insertion order (exact sums)
+    double* sbuf = nullptr;         // [6np + 3nl] per-slot LM scale terms (exact sums)
+    double* Lt = nullptr;           // [n*n] transposed factor (exact backward substitution)
     double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
     int32_t* chol_status = nullptr; // [1] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
@@ -159,13 +170,17 @@ struct dev_work64 {
 constexpr int kPartials = 1024;  // reduction partial slots
 
 // ---- binary64 kernels (kernels_fp64.hip) ----------------------------------
+// `exact`: reproduce the reference's summation orders (sequential sums over all
+// observations / slots, hash-map orders in the Schur products); otherwise use
+// fixed-shape parallel reductions.
 void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target /*0 chi2, 1 chi2_new*/,
-                   bool only_if_step);
+                   bool only_if_step, bool exact = false);
 void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
 void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s);
-void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override);
+void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override,
+                             bool exact = false);
 void launch_after_solve64(const dev_work64& w, hipStream_t s);
-void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool exact = false);
 void launch_decide64(const dev_work64& w, hipStream_t s);
 void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s);
@@ -176,6 +191,9 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from
 // w.bs.  Sets *w.chol_status and lm->solve_ok; skips everything when the
 // controller is done (gated) or a previous stage failed.
-void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated);
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact = false);
+
+// ---- diagnostics ---------------------------------------------------------------
+void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t s);
 
 }  // namespace zb

@@ -9,6 +9,7 @@
 // after attempt and only reads the controller's `done` flag, one attempt behind.
 #include "zsba.h"
 #include "common.hpp"
+#include "ba_math.hpp"
 
 #include <algorithm>
 #include <array>
@@ -63,6 +64,21 @@ struct dev_alloc {
     }
 };
 
+// Key of a block-sparse matrix block with the hash of the reference's
+// sparse_block::block_key (matrix_sparse_block.hpp:47-59).  Used only to learn
+// the iteration order of the reference's std::unordered_map containers, which
+// sets the order of its Schur sums.
+struct ref_key {
+    size_t first, second;
+    bool operator==(const ref_key& o) const { return first == o.first && second == o.second; }
+};
+struct ref_hash {
+    std::size_t operator()(const ref_key& k) const {
+        return ((std::hash<size_t>()(k.first) ^ (std::hash<size_t>()(k.second) << 1)) >> 1);
+    }
+};
+using ref_map = std::unordered_map<ref_key, char, ref_hash>;
+
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -110,6 +126,21 @@ struct zsba_graph {
 
     zsba_stats stats{};
     zb::kernel_timer timer;
+
+    // Exact-order state: the keys of the reference's factor_graph::h_pl container,
+    // cleared and refilled by every make_hessian and kept across solves exactly as
+    // that member is (factor_graph.hpp:366, 654, 704-709).
+    ref_map hpl_mirror;
+    std::vector<ref_key> hpl_sequence;  // first-occurrence key order of the current structure
+    std::vector<int32_t> key_pair;      // pair index of hpl_sequence[i]
+    bool orders_ready = false;          // set 1 computed for the current structure
+    bool mirror_filled = false;         // the mirror has been filled for the current structure
+    int2* d_contrib_x[2] = {nullptr, nullptr};
+    int32_t* d_pose_pairs_x[2] = {nullptr, nullptr};
+    int32_t* d_lm_pairs_x[2] = {nullptr, nullptr};
+    std::vector<int2> contrib_fast;     // contributions grouped by block (host copy)
+    std::vector<int64_t> blk_start_h;
+    std::vector<int32_t> pose_pair_start_h, pose_pairs_h, lm_pair_start_h, pair_pose_h, pair_point_h;
 
     ~zsba_graph() {
         if (hip_ready) {
@@ -197,7 +228,10 @@ struct zsba_graph {
     int32_t build_structure();
     int32_t upload_values();
     int32_t prepare();
-    int32_t run_fp64(int32_t iterations, int32_t use_rel);
+    int32_t prepare_exact_orders();
+    void order_set_from_mirror(std::vector<int2>& contrib, std::vector<int32_t>& pose_pairs,
+                               std::vector<int32_t>& lm_pairs);
+    int32_t run_fp64(int32_t iterations, int32_t use_rel, bool exact);
     void print_trace();
 };
 
@@ -267,6 +301,30 @@ int32_t zsba_graph::build_structure() {
     }
     lm_pair_start[nl] = static_cast<int32_t>(pair_pose.size());
     const int32_t npair = static_cast<int32_t>(pair_pose.size());
+
+    // key sequence of h_pl (first occurrence in insertion order, factor_graph.hpp:664-709)
+    {
+        hpl_sequence.clear();
+        key_pair.clear();
+        std::vector<uint8_t> inserted(npair, 0);
+        for (int32_t e = 0; e < no; ++e) {
+            const int32_t k = e_pose[e], l = e_point[e];
+            if (pfix[k] || lfix[l]) continue;
+            const int32_t a = obs_pair[e_sorted[e]];
+            if (inserted[a]) continue;
+            inserted[a] = 1;
+            hpl_sequence.push_back(ref_key{static_cast<size_t>(k), static_cast<size_t>(l)});
+            key_pair.push_back(a);
+        }
+    }
+    // W blocks of each pose (for b_s)
+    std::vector<int32_t> pose_pair_start(np + 1, 0), pose_pairs(npair);
+    for (int32_t a = 0; a < npair; ++a) pose_pair_start[pair_pose[a] + 1]++;
+    for (int32_t i = 0; i < np; ++i) pose_pair_start[i + 1] += pose_pair_start[i];
+    {
+        std::vector<int32_t> f(pose_pair_start.begin(), pose_pair_start.end() - 1);
+        for (int32_t a = 0; a < npair; ++a) pose_pairs[f[pair_pose[a]]++] = a;
+    }
 
     // lower-triangle blocks of the reduced system and their contributions
     std::vector<int64_t> block_key;  // row * np + col
@@ -392,6 +450,15 @@ int32_t zsba_graph::build_structure() {
     w.S = mem.get<double>(static_cast<size_t>(n * n)); w.bs = mem.get<double>(n); w.dxp = mem.get<double>(n);
     w.dxl = mem.get<double>(3L * nl);
     w.partial = mem.get<double>(3 * zb::kPartials + 8);
+    w.cbuf = mem.get<double>(no);
+    w.sbuf = mem.get<double>(6L * np + 3L * nl);
+    w.Lt = mem.get<double>(static_cast<size_t>(n * n));
+    for (int set = 0; set < 2; ++set) {
+        d_contrib_x[set] = mem.get<int2>(static_cast<size_t>(ncontrib));
+        d_pose_pairs_x[set] = mem.get<int32_t>(npair);
+        d_lm_pairs_x[set] = mem.get<int32_t>(npair);
+    }
+    int32_t* d_pose_pair_start = mem.get<int32_t>(np + 1);
     w.chol_scale = mem.get<double>(4);
     w.chol_status = mem.get<int32_t>(4);
     w.lm = mem.get<lm_state>(1);
@@ -414,12 +481,107 @@ int32_t zsba_graph::build_structure() {
     p.blk_row = up(d_blk_row, blk_row); p.blk_col = up(d_blk_col, blk_col); p.blk_start = up(d_blk_start, blk_start);
     p.contrib = up(d_contrib, contrib); p.diag_blk = up(d_diag, diag_blk);
     p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
+    p.pose_pair_start = up(d_pose_pair_start, pose_pair_start);
+    for (int set = 0; set < 2; ++set) {
+        p.contrib_x[set] = d_contrib_x[set];
+        p.pose_pairs_x[set] = d_pose_pairs_x[set];
+        p.lm_pairs_x[set] = d_lm_pairs_x[set];
+    }
+    contrib_fast = contrib;
+    blk_start_h = blk_start;
+    pose_pair_start_h = pose_pair_start;
+    pose_pairs_h = pose_pairs;
+    lm_pair_start_h = lm_pair_start;
+    pair_pose_h = pair_pose;
+    pair_point_h = pair_point;
+    orders_ready = false;
+    mirror_filled = false;
     p.cams = up(d_cams, cam_table); p.loss_kind = up(d_lk, lk); p.loss_delta = up(d_ld, ld);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "structure upload");
     dp = p;
     structure_dirty = false;
     values_dirty = true;
+    return 0;
+}
+
+// Refill the mirror as make_hessian refills h_pl, then derive the orders the
+// reference sums in: Y = multiply(h_pl, h_ll^-1) is a fresh map reserved to
+// |h_pl| and filled in h_pl order (matrix_sparse_block_diagonal.hpp:107-122);
+// h_lp = get_transpose() likewise with swapped keys (matrix_sparse_block.hpp:132-139).
+// P(i,k) accumulates over Y in Y order, b_s over Y in Y order, the point
+// back-substitution over h_lp in h_lp order.
+void zsba_graph::order_set_from_mirror(std::vector<int2>& contrib, std::vector<int32_t>& pose_pairs,
+                                       std::vector<int32_t>& lm_pairs) {
+    hpl_mirror.clear();
+    for (const ref_key& k : hpl_sequence)
+        if (hpl_mirror.find(k) == hpl_mirror.end()) hpl_mirror[k] = 0;
+    std::unordered_map<int64_t, int32_t> pair_of;
+    pair_of.reserve(key_pair.size() * 2 + 1);
+    const int64_t nl = static_cast<int64_t>(point_slot_id.size());
+    for (size_t i = 0; i < hpl_sequence.size(); ++i)
+        pair_of[static_cast<int64_t>(hpl_sequence[i].first) * nl + static_cast<int64_t>(hpl_sequence[i].second)] = key_pair[i];
+    ref_map Y, LP;
+    Y.reserve(hpl_mirror.size());
+    LP.reserve(hpl_mirror.size());
+    for (auto it = hpl_mirror.begin(); it != hpl_mirror.end(); ++it) {
+        if (Y.find(it->first) == Y.end()) Y[it->first] = 0;
+        LP[ref_key{it->first.second, it->first.first}] = 0;
+    }
+    const size_t npair = key_pair.size();
+    std::vector<int32_t> rank_y(npair, 0), rank_lp(npair, 0);
+    int32_t r = 0;
+    for (auto it = Y.begin(); it != Y.end(); ++it, ++r)
+        rank_y[pair_of[static_cast<int64_t>(it->first.first) * nl + static_cast<int64_t>(it->first.second)]] = r;
+    r = 0;
+    for (auto it = LP.begin(); it != LP.end(); ++it, ++r)
+        rank_lp[pair_of[static_cast<int64_t>(it->first.second) * nl + static_cast<int64_t>(it->first.first)]] = r;
+    contrib = contrib_fast;
+    const size_t nblk = blk_start_h.size() - 1;
+    for (size_t b = 0; b < nblk; ++b)
+        std::stable_sort(contrib.begin() + blk_start_h[b], contrib.begin() + blk_start_h[b + 1],
+                         [&](const int2& u, const int2& v) { return rank_y[u.x] < rank_y[v.x]; });
+    pose_pairs = pose_pairs_h;
+    for (size_t i = 0; i + 1 < pose_pair_start_h.size(); ++i)
+        std::sort(pose_pairs.begin() + pose_pair_start_h[i], pose_pairs.begin() + pose_pair_start_h[i + 1],
+                  [&](int32_t u, int32_t v) { return rank_y[u] < rank_y[v]; });
+    lm_pairs.resize(npair);
+    for (size_t a = 0; a < npair; ++a) lm_pairs[a] = static_cast<int32_t>(a);
+    for (size_t l = 0; l + 1 < lm_pair_start_h.size(); ++l)
+        std::sort(lm_pairs.begin() + lm_pair_start_h[l], lm_pairs.begin() + lm_pair_start_h[l + 1],
+                  [&](int32_t u, int32_t v) { return rank_lp[u] < rank_lp[v]; });
+}
+
+// Orders for the first linearisation of this solve (set 0) and for the later
+// ones (set 1).  After one refill of the current key sequence the mirror's
+// bucket count no longer changes, so every later refill iterates identically.
+int32_t zsba_graph::prepare_exact_orders() {
+    std::vector<int2> c;
+    std::vector<int32_t> pp, lp;
+    hipError_t e;
+    auto upload = [&](int set) -> hipError_t {
+        hipError_t r = copy(d_contrib_x[set], c.data(), c.size() * sizeof(int2), hipMemcpyHostToDevice);
+        if (r != hipSuccess) return r;
+        r = copy(d_pose_pairs_x[set], pp.data(), pp.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+        if (r != hipSuccess) return r;
+        return copy(d_lm_pairs_x[set], lp.data(), lp.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    };
+    if (!orders_ready) {
+        order_set_from_mirror(c, pp, lp);  // fill #1 of this structure
+        if ((e = upload(0)) != hipSuccess) return hip_fail(e, "order upload");
+        order_set_from_mirror(c, pp, lp);  // fill #2: steady state
+        if ((e = upload(1)) != hipSuccess) return hip_fail(e, "order upload");
+        orders_ready = true;
+        mirror_filled = true;
+    } else if (mirror_filled) {
+        // set 0 of this solve equals the steady-state order
+        if ((e = copy(d_contrib_x[0], d_contrib_x[1], contrib_fast.size() * sizeof(int2), hipMemcpyDeviceToDevice)) != hipSuccess)
+            return hip_fail(e, "order copy");
+        if ((e = copy(d_pose_pairs_x[0], d_pose_pairs_x[1], pose_pairs_h.size() * sizeof(int32_t), hipMemcpyDeviceToDevice)) != hipSuccess)
+            return hip_fail(e, "order copy");
+        if ((e = copy(d_lm_pairs_x[0], d_lm_pairs_x[1], pair_pose_h.size() * sizeof(int32_t), hipMemcpyDeviceToDevice)) != hipSuccess)
+            return hip_fail(e, "order copy");
+    }
     return 0;
 }
 
@@ -464,8 +626,12 @@ void zsba_graph::print_trace() {
                  lm_pinned->solve_failures, lm_pinned->initial_chi2, lm_pinned->chi2, lm_pinned->lambda);
 }
 
-int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
+int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel, bool exact) {
     hipError_t e;
+    if (exact) {
+        const int32_t rc = prepare_exact_orders();
+        if (rc) return rc;
+    }
     timer.reset();
     const double t0 = now_ms();
     std::memset(lm_pinned, 0, sizeof(lm_state));
@@ -473,7 +639,7 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
     lm_pinned->use_relative = use_rel ? 1 : 0;
     if ((e = hipMemcpyAsync(w.lm, lm_pinned, sizeof(lm_state), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload controller");
-    zb::launch_cost64(dp, w, stream, 0, false);
+    zb::launch_cost64(dp, w, stream, 0, false, exact);
     zb::launch_linearise64(dp, w, stream, false);
     zb::launch_init_lm64(dp, w, stream);
     const int32_t max_attempts = iterations * 10;
@@ -482,11 +648,11 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel) {
     (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
     int32_t* done_pinned = &lm_pinned->done;
     for (int32_t a = 0; a < max_attempts; ++a) {
-        zb::launch_reduced_system64(dp, w, stream, nullptr);
-        zb::launch_cholesky_solve64(dp.n, w, stream, true);
+        zb::launch_reduced_system64(dp, w, stream, nullptr, exact);
+        zb::launch_cholesky_solve64(dp.n, w, stream, true, exact);
         zb::launch_after_solve64(w, stream);
-        zb::launch_backsub_update64(dp, w, stream);
-        zb::launch_cost64(dp, w, stream, 1, true);
+        zb::launch_backsub_update64(dp, w, stream, exact);
+        zb::launch_cost64(dp, w, stream, 1, true, exact);
         zb::launch_decide64(w, stream);
         zb::launch_rollback64(dp, w, stream);
         zb::launch_linearise64(dp, w, stream, true);
@@ -702,7 +868,7 @@ int32_t zsba_solve(zsba_graph* g, int32_t iterations, int32_t use_relative_conve
     if (!g) return ZSBA_EINVAL;
     g->stats = zsba_stats{};
     g->stats.precision = precision;
-    if (precision != ZSBA_PRECISION_FP64 && precision != ZSBA_PRECISION_MIXED)
+    if (precision != ZSBA_PRECISION_FP64 && precision != ZSBA_PRECISION_MIXED && precision != ZSBA_PRECISION_FP64_FAST)
         return g->fail(ZSBA_EINVAL, "unknown precision %d", precision);
     // factor_graph.hpp:494-499
     if (g->live_obs() == 0 || (g->live_poses() == 0 && g->live_points() == 0)) {
@@ -715,14 +881,14 @@ int32_t zsba_solve(zsba_graph* g, int32_t iterations, int32_t use_relative_conve
     g->stats.n_poses = g->dp.np;
     g->stats.n_points = g->dp.nl;
     if (precision == ZSBA_PRECISION_MIXED) return g->fail(ZSBA_EUNSUP, "mixed precision is not available yet");
-    return g->run_fp64(iterations, use_relative_convergence);
+    return g->run_fp64(iterations, use_relative_convergence, precision == ZSBA_PRECISION_FP64);
 }
 
 double zsba_current_chi(zsba_graph* g, int32_t recompute) {
     if (!g) return NAN;
     if (g->live_obs() == 0) return 0.0;
     if (g->prepare()) return NAN;
-    if (recompute) zb::launch_cost64(g->dp, g->w, g->stream, 3, false);
+    if (recompute) zb::launch_cost64(g->dp, g->w, g->stream, 3, false, true);
     else zb::launch_cost_stored64(g->dp, g->w, g->stream);
     double v = NAN;
     if (g->copy(&v, g->w.partial + zb::kPartials, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
@@ -853,6 +1019,28 @@ int32_t zsba_restore(zsba_graph* g) {
         return g->hip_fail(e, "restore");
     g->device_newer = true;
     return 0;
+}
+
+int32_t zsba_debug_math(int32_t fn, int32_t n, const double* in, double* out, int32_t on_device) {
+    if (n < 0 || (n > 0 && (!in || !out)) || fn < 0 || fn > 3) return ZSBA_EINVAL;
+    if (!on_device) {
+        for (int32_t i = 0; i < n; ++i)
+            out[i] = fn == 0 ? zb::cr_sin(in[i]) : fn == 1 ? zb::cr_cos(in[i]) : fn == 2 ? zb::cr_cube(in[i]) : std::sqrt(in[i]);
+        return 0;
+    }
+    double *d_in = nullptr, *d_out = nullptr;
+    if (hipMalloc(&d_in, sizeof(double) * (n + 1)) != hipSuccess) return ZSBA_ENOMEM;
+    if (hipMalloc(&d_out, sizeof(double) * (n + 1)) != hipSuccess) { (void)hipFree(d_in); return ZSBA_ENOMEM; }
+    int32_t rc = 0;
+    if (hipMemcpy(d_in, in, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) rc = ZSBA_EDEVICE;
+    if (!rc) {
+        zb::launch_math_probe(fn, n, d_in, d_out, nullptr);
+        if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = ZSBA_EDEVICE;
+    }
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    return rc;
 }
 
 int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations, int32_t use_rel, int32_t precision,

@@ -145,17 +145,74 @@ __global__ void __launch_bounds__(256) k_syrk(int n, double* A, int k0, int firs
         const int r = t / NB, c = t % NB;
         if (r0 + r >= n || c0 + c >= n) continue;
         if (I == J && c > r) continue;
-        double s = 0.0;
-#pragma unroll 8
-        for (int m = 0; m < NB; ++m) s = s + LI[r][m] * LJ[c][m];
+        // one subtraction per panel column, in column order: the sequence the
+        // reference's row-Crout loop applies to this element
         double* out = A + static_cast<int64_t>(r0 + r) * n + c0 + c;
-        *out = *out - s;
+        double v = *out;
+#pragma unroll 8
+        for (int m = 0; m < nb; ++m) v = v - LI[r][m] * LJ[c][m];
+        *out = v;
     }
+}
+
+__device__ __forceinline__ double readlane_d(double v, int j) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Lt[i][j] = L[j][i] for j > i (the strict upper triangle of L^T).
+__global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L, double* Lt, const int32_t* status,
+                                                         const lm_state* lm, int gated) {
+    if (skip(lm, gated, status)) return;
+    __shared__ double T[32][33];
+    const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;  // source tile rows bi.., cols bj..
+    if (bj > bi + 31) return;
+    for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int r = t / 32, c = t % 32;
+        T[r][c] = (bi + r < n && bj + c < n) ? L[static_cast<int64_t>(bi + r) * n + bj + c] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int r = t / 32, c = t % 32;  // destination row bj + r, column bi + c
+        if (bj + r < n && bi + c < n && bi + c > bj + r) Lt[static_cast<int64_t>(bj + r) * n + bi + c] = T[c][r];
+    }
+}
+
+// Backward substitution L^T x = y in the reference's order
+// (matrix_decomposition_cholesky.hpp:85-93): x_i -= L_ji x_j for j = i+1 .. n-1
+// ascending, then x_i /= L_ii.  One wave; the products are formed 64 at a time
+// in parallel and subtracted in order.
+__global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, const double* Lt, double* x,
+                                                       const int32_t* status, lm_state* lm, int gated) {
+    if (skip(lm, gated, status)) return;
+    const int lane = threadIdx.x;
+    for (int i = n - 1; i >= 0; --i) {
+        double v = x[i];
+        const double* u = Lt + static_cast<int64_t>(i) * n;
+        for (int j0 = i + 1; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            const double pj = (j < n) ? u[j] * x[j] : 0.0;
+            const int cnt = (n - j0) < 64 ? (n - j0) : 64;
+            if (cnt == 64) {
+#pragma unroll
+                for (int k = 0; k < 64; ++k) v = v - readlane_d(pj, k);
+            } else {
+                for (int k = 0; k < cnt; ++k) v = v - readlane_d(pj, k);
+            }
+        }
+        v = v / L[static_cast<int64_t>(i) * n + i];
+        if (lane == 0) x[i] = v;
+        __threadfence_block();
+        __syncthreads();
+    }
+    if (lane == 0) lm->solve_ok = 1;
 }
 
 // Forward and backward substitution in one workgroup; sets lm->solve_ok.
 __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* x, int32_t* status,
-                                               lm_state* lm, int gated) {
+                                               lm_state* lm, int gated, int forward_only) {
     if (gated && lm->done) return;
     if (*status != 0) {
         if (threadIdx.x == 0) lm->solve_ok = 0;
@@ -222,6 +279,7 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
         }
         __syncthreads();
     }
+    if (forward_only) return;
     // backward: L^T x = y
     for (int t = tiles - 1; t >= 0; --t) {
         const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
@@ -258,7 +316,7 @@ __global__ void k_empty_ok(lm_state* lm, int gated) {
 
 }  // namespace
 
-void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated) {
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact) {
     const int g = gated ? 1 : 0;
     if (n <= 0) {
         k_empty_ok<<<1, 1, 0, s>>>(w.lm, g);
@@ -280,7 +338,12 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     }
     ZB_TIME_END(w, s);
     ZB_TIME_BEGIN(w, K_TRSV, s);
-    k_trsv<<<1, 1024, 0, s>>>(n, w.S, w.bs, w.dxp, w.chol_status, w.lm, g);
+    k_trsv<<<1, 1024, 0, s>>>(n, w.S, w.bs, w.dxp, w.chol_status, w.lm, g, exact ? 1 : 0);
+    if (exact) {
+        const int tiles32 = (n + 31) / 32;
+        k_transpose_lower<<<dim3(tiles32, tiles32), 256, 0, s>>>(n, w.S, w.Lt, w.chol_status, w.lm, g);
+        k_backward_exact<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
+    }
     ZB_TIME_END(w, s);
 }
 

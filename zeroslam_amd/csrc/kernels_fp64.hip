@@ -59,7 +59,8 @@ __device__ double block_sum(double v, double* sh) {
 
 // ---- cost: chi2 = sum of robust costs (factor_graph.hpp:618-629) -------------
 __global__ void __launch_bounds__(kBlock) k_cost(dev_problem p, double* pose, double* point, double* resid,
-                                                 double* partial, const lm_state* lm, int gate, int recompute) {
+                                                 double* partial, const lm_state* lm, int gate, int recompute,
+                                                 double* terms) {
     if (gate == 1 && lm->done) return;
     if (gate == 2 && (lm->done || !lm->step_applied)) return;
     __shared__ double sh[kBlock];
@@ -84,10 +85,50 @@ __global__ void __launch_bounds__(kBlock) k_cost(dev_problem p, double* pose, do
         int32_t kind;
         double delta;
         loss_of(p, param, kind, delta);
-        acc = acc + robust_chi2(kind, delta, r, W);
+        const double c = robust_chi2(kind, delta, r, W);
+        if (terms) terms[p.obs_orig[s]] = c;
+        else acc = acc + c;
     }
+    if (terms) return;
     const double b = block_sum(acc, sh);
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+
+// Lane-uniform read of lane j's value.
+__device__ __forceinline__ double readlane_d(double v, int j) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Sequential sum a[0] + a[1] + ... from 0.0, in index order (one wave; the other
+// lanes only prefetch), as the reference's accumulation loops
+// (factor_graph.hpp:556-560, 624-628).
+__global__ void __launch_bounds__(64) k_seqsum(const double* a, int64_t n, lm_state* lm, int target, int gate,
+                                               double* out) {
+    if (gate == 1 && lm->done) return;
+    if (gate == 2 && (lm->done || !lm->step_applied)) return;
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    double v = (lane < n) ? a[lane] : 0.0;
+    for (int64_t i0 = 0; i0 < n; i0 += 64) {
+        const double next = (i0 + 64 + lane < n) ? a[i0 + 64 + lane] : 0.0;
+        if (n - i0 >= 64) {
+#pragma unroll
+            for (int j = 0; j < 64; ++j) acc = acc + readlane_d(v, j);
+        } else {
+            const int cnt = static_cast<int>(n - i0);
+            for (int j = 0; j < cnt; ++j) acc = acc + readlane_d(v, j);
+        }
+        v = next;
+    }
+    if (lane == 0) {
+        if (target == 0) lm->chi2 = acc;
+        else if (target == 1) lm->chi2_new = acc;
+        else if (target == 2) lm->scale = acc + 1e-3;
+        else if (out) *out = acc;
+    }
 }
 
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int n, double* out, lm_state* lm,
@@ -210,7 +251,10 @@ __global__ void __launch_bounds__(256) k_reduce_pose(dev_problem p, const double
     else bp[6L * wave + (lane - 21)] = acc;
 }
 
-__global__ void k_linearise_done(lm_state* lm) { lm->need_linearize = 0; }
+__global__ void k_linearise_done(lm_state* lm) {
+    lm->need_linearize = 0;
+    lm->lin_count += 1;
+}
 
 // ---- LM initialisation, factor_graph.hpp:511-522 -----------------------------
 // max_diagonal = max over all pose then point diagonal entries with
@@ -286,6 +330,7 @@ __global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const double* H
         lm->need_rollback = 0;
         lm->step_applied = 0;
         lm->solve_ok = 0;
+        lm->lin_count = 1;
         lm->done = (lm->iterations <= 0) ? 1 : 0;
     }
 }
@@ -405,6 +450,70 @@ __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, c
     }
 }
 
+__device__ __forceinline__ int order_set(const lm_state* lm) { return lm->lin_count <= 1 ? 0 : 1; }
+
+// Reduced camera system in the reference's order, one wave per block, lane e < 36
+// owning entry (e / 6, e % 6):  P = Y_a W_c^T for the first contribution, then
+// P = P + Y_a W_c^T in the order the reference's Y map iterates
+// (matrix_sparse_block.hpp:164-177); S = (H_pp + lambda I) - P on the diagonal,
+// S = -P off it (matrix_sparse_block_diagonal.hpp:129-140).
+__global__ void __launch_bounds__(256) k_schur_exact(dev_problem p, const double* Y, const double* Wbuf,
+                                                     const double* Hpp, double* S, const lm_state* lm) {
+    if (lm->done) return;
+    const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (blk >= p.nblk || lane >= 36) return;
+    const int row = p.blk_row[blk], col = p.blk_col[blk];
+    const int r = lane / 6, c = lane % 6;
+    const int2* contrib = p.contrib_x[order_set(lm)];
+    const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
+    double P = 0.0;
+    for (int64_t t = c0; t < c1; ++t) {
+        const int2 ab = contrib[t];
+        const double* Ya = Y + 18L * ab.x + 3 * r;
+        const double* Wc = Wbuf + 18L * ab.y + 3 * c;
+        double prod = 0;
+        prod += Ya[0] * Wc[0];
+        prod += Ya[1] * Wc[1];
+        prod += Ya[2] * Wc[2];
+        P = (t == c0) ? prod : P + prod;
+    }
+    const int64_t n = p.n;
+    double* out = S + (6L * row + r) * n + 6L * col + c;
+    if (row == col) {
+        if (c <= r) {
+            double base = Hpp[21L * row + r * (r + 1) / 2 + c];
+            if (r == c) base += lm->lambda;
+            *out = (c1 > c0) ? base - P : base;
+        }
+    } else {
+        *out = -P;
+    }
+}
+
+// b_s = b_p - Y b_l with Y b_l accumulated from zero in Y-map order
+// (matrix_sparse_block.hpp:184-196), one wave per pose, lanes 0..5.
+__global__ void __launch_bounds__(256) k_bs_exact(dev_problem p, const double* Y, const double* bp, const double* bl,
+                                                  double* bs, const lm_state* lm) {
+    if (lm->done) return;
+    const int pose = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (pose >= p.np || lane >= 6) return;
+    const int32_t* pairs = p.pose_pairs_x[order_set(lm)];
+    double yb = 0.0;
+    for (int t = p.pose_pair_start[pose]; t < p.pose_pair_start[pose + 1]; ++t) {
+        const int a = pairs[t];
+        const int l = p.pair_point[a];
+        const double* Ya = Y + 18L * a + 3 * lane;
+        double prod = 0;
+        prod += Ya[0] * bl[3L * l];
+        prod += Ya[1] * bl[3L * l + 1];
+        prod += Ya[2] * bl[3L * l + 2];
+        yb = yb + prod;
+    }
+    bs[6L * pose + lane] = bp[6L * pose + lane] - yb;
+}
+
 // ---- after the reduced solve: failure bookkeeping, factor_graph.hpp:542-551 ----
 __device__ void outer_end(lm_state* lm) {
     const double floor_chi2 = 1e-12;
@@ -445,7 +554,7 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
                                                            double* point_backup, const double* Wbuf, const double* Hinv,
                                                            const double* dxp, double* dxl, const double* bp,
                                                            const double* bl, double* partial, const lm_state* lm,
-                                                           int gated, int apply) {
+                                                           int gated, int apply, double* sterms) {
     if (gated && (lm->done || !lm->step_applied)) return;
     __shared__ double sh[kBlock];
     const double lam = lm->lambda;
@@ -455,8 +564,13 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
         double d[6];
 #pragma unroll
         for (int i = 0; i < 6; ++i) d[i] = dxp[6L * t + i];
+        if (sterms) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) acc = acc + d[i] * (lam * d[i] + bp[6L * t + i]);
+            for (int i = 0; i < 6; ++i) sterms[6L * t + i] = d[i] * (lam * d[i] + bp[6L * t + i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) acc = acc + d[i] * (lam * d[i] + bp[6L * t + i]);
+        }
         if (apply && !p.pose_fixed[t]) {
             double P[7];
             load7(pose + 7L * t, P);
@@ -469,7 +583,9 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
     } else if (t < p.np + p.nl) {
         const int l = t - p.np;
         double tt[3] = {0.0, 0.0, 0.0};
-        for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+        const int32_t* order = sterms ? p.lm_pairs_x[order_set(lm)] : nullptr;
+        for (int t2 = p.lm_pair_start[l]; t2 < p.lm_pair_start[l + 1]; ++t2) {
+            const int a = order ? order[t2] : t2;
             double Wa[18], lp[18], d[6], prod[3];
 #pragma unroll
             for (int i = 0; i < 18; ++i) Wa[i] = Wbuf[18L * a + i];
@@ -491,8 +607,13 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
         for (int i = 0; i < 3; ++i) dl[i] = 0.0 + prod[i];
 #pragma unroll
         for (int i = 0; i < 3; ++i) dxl[3L * l + i] = dl[i];
+        if (sterms) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) acc = acc + dl[i] * (lam * dl[i] + bl[3L * l + i]);
+            for (int i = 0; i < 3; ++i) sterms[6L * p.np + 3L * l + i] = dl[i] * (lam * dl[i] + bl[3L * l + i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc = acc + dl[i] * (lam * dl[i] + bl[3L * l + i]);
+        }
         if (apply && !p.point_fixed[l]) {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
@@ -502,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
         }
     }
     }
-    if (partial) {
+    if (partial && !sterms) {
         const double b = block_sum(acc, sh);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
@@ -516,7 +637,7 @@ __global__ void k_decide(lm_state* lm) {
     lm->rho = rho;
     bool good = false;
     if ((rho > 0) && isfinite(temp_chi)) {
-        double alpha = 1.0 - pow((2.0 * rho - 1.0), 3.0);
+        double alpha = 1.0 - cr_cube(2.0 * rho - 1.0);  // math::pow(x, 3.0), correctly rounded
         alpha = (2.0 / 3.0 < alpha) ? 2.0 / 3.0 : alpha;
         const double scale_factor = (1.0 / 3.0 < alpha) ? alpha : 1.0 / 3.0;
         lm->lambda *= scale_factor;
@@ -609,6 +730,13 @@ __global__ void __launch_bounds__(kBlock) k_inliers(dev_problem p, const double*
     flags[s] = inlier(kind, delta, r, W) ? 1 : 0;
 }
 
+__global__ void k_math_probe(int fn, int n, const double* in, double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = in[i];
+    out[i] = fn == 0 ? cr_sin(x) : fn == 1 ? cr_cos(x) : fn == 2 ? cr_cube(x) : fn == 3 ? sqrt(x) : x;
+}
+
 inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
 inline int cost_grid(const dev_problem& p) {
     const int g = grid_for(p.no, kBlock);
@@ -618,19 +746,28 @@ inline int cost_grid(const dev_problem& p) {
 }  // namespace
 
 // target: 0 -> lm->chi2, 1 -> lm->chi2_new (gated on an applied step), 3 -> partial[kPartials]
-void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step) {
+void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step, bool exact) {
     const int g = cost_grid(p);
     const int gate = only_if_step ? 2 : (target == 3 ? 0 : 1);
     ZB_TIME_BEGIN(w, K_COST, s);
-    k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1);
-    k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate);
+    if (exact) {
+        k_cost<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1, w.cbuf);
+        k_seqsum<<<1, 64, 0, s>>>(w.cbuf, p.no, w.lm, target, gate, w.partial + kPartials);
+    } else {
+        k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1, nullptr);
+        k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate);
+    }
     ZB_TIME_END(w, s);
 }
 
+// chi2 of the stored residuals (get_current_chi(false)), summed in insertion order.
 void launch_cost_stored64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
-    const int g = cost_grid(p);
-    k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, 0, 0);
-    k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, 3, 0);
+    k_cost<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, 0, 0, w.cbuf);
+    k_seqsum<<<1, 64, 0, s>>>(w.cbuf, p.no, w.lm, 3, 0, w.partial + kPartials);
+}
+
+void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t s) {
+    if (n > 0) k_math_probe<<<grid_for(n, kBlock), kBlock, 0, s>>>(fn, n, in, out);
 }
 
 void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
@@ -652,7 +789,8 @@ void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s) 
     k_init_lm<<<1, 1024, 0, s>>>(p, w.Hpp, w.Hll, w.lm);
 }
 
-void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override) {
+void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override,
+                             bool exact) {
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_INVERT, s);
         k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override);
@@ -671,8 +809,13 @@ void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStrea
     ZB_TIME_END(w, s);
     if (p.nblk > 0) {
         ZB_TIME_BEGIN(w, K_SCHUR, s);
-        k_schur<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
-                                                            lambda_override);
+        if (exact && !lambda_override) {
+            k_schur_exact<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.S, w.lm);
+            k_bs_exact<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.Y, w.bp, w.bl, w.bs, w.lm);
+        } else {
+            k_schur<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+                                                                lambda_override);
+        }
         ZB_TIME_END(w, s);
     }
 }
@@ -683,15 +826,16 @@ void launch_after_solve64(const dev_work64& w, hipStream_t s) {
     ZB_TIME_END(w, s);
 }
 
-void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool exact) {
     int g = grid_for(p.np + p.nl, kBlock);
     if (g > kPartials) g = kPartials;
     if (g < 1) g = 1;
     double* partial = w.partial + 2 * kPartials;
     ZB_TIME_BEGIN(w, K_BACKSUB, s);
     k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
-                                          w.bp, w.bl, partial, w.lm, 1, 1);
-    k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
+                                          w.bp, w.bl, partial, w.lm, 1, 1, exact ? w.sbuf : nullptr);
+    if (exact) k_seqsum<<<1, 64, 0, s>>>(w.sbuf, 6L * p.np + 3L * p.nl, w.lm, 2, 2, nullptr);
+    else k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
     ZB_TIME_END(w, s);
 }
 
@@ -700,7 +844,7 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
     if (g > kPartials) g = kPartials;
     if (g < 1) g = 1;
     k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
-                                          w.bp, w.bl, nullptr, w.lm, 0, 0);
+                                          w.bp, w.bl, nullptr, w.lm, 0, 0, nullptr);
 }
 
 void launch_decide64(const dev_work64& w, hipStream_t s) {

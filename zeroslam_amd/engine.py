@@ -15,6 +15,18 @@ from .problem import Problem
 
 FP64 = _lib.ZSBA_PRECISION_FP64
 MIXED = _lib.ZSBA_PRECISION_MIXED
+FP64_FAST = _lib.ZSBA_PRECISION_FP64_FAST
+
+
+def math_probe(fn: int, x, on_device: bool) -> np.ndarray:
+    """The engine's correctly rounded sin (0), cos (1), cube (2), sqrt (3)."""
+    lib = _lib.load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    rc = lib.zsba_debug_math(int(fn), len(x), ptr(x), ptr(out), int(bool(on_device)))
+    if rc < 0:
+        raise ZsbaError(rc, "math probe failed")
+    return out
 
 
 class ZsbaError(RuntimeError):
