@@ -74,7 +74,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64_fast", "mixed"])
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -83,12 +83,12 @@ def main() -> int:
               "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
         return 2
 
-    from zeroslam_amd.engine import FP64, MIXED, Graph
+    from zeroslam_amd.engine import FP64, FP64_FAST, MIXED, Graph
     from zeroslam_amd.problem import ITERATIONS, synth
 
     problem = synth(args.config)
     iters = ITERATIONS[args.config]
-    precision = FP64 if args.precision == "fp64" else MIXED
+    precision = {"fp64": FP64, "fp64_fast": FP64_FAST, "mixed": MIXED}[args.precision]
     g = Graph.from_problem(problem, timing=True)
     g.solve(0)          # structure analysis + upload (untimed)
     g.snapshot()
@@ -144,7 +144,7 @@ def main() -> int:
         "metric": "BA observation*iters/s, 100KF/20k pts: mixed vs FP64 GPU at equal final cost",
         "value": value, "unit": "obs*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if precision == FP64 else "f32+f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32+f64" if precision == MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"c{args.config}: {problem.n_poses} KF, {problem.n_points} pts, {problem.n_obs} obs, "
                                f"{iters} LM iterations", "precision": args.precision,
                    "lm_iterations": stats["iterations"], "attempts": stats["attempts"], "final_chi2": chi2},

@@ -97,6 +97,7 @@ struct dev_problem {
     const int32_t* lm_start = nullptr;   // [nl+1] observations of a point
     const int32_t* pose_start = nullptr; // [np+1] into pose_list
     const int32_t* pose_list = nullptr;  // [no] sorted indices of a pose's observations, insertion order
+    const int32_t* obs_pose_pos = nullptr; // [no] position of observation s in pose_list (pose-major slot)
     const int32_t* lm_pair_start = nullptr; // [nl+1] W blocks of a point (first-occurrence order)
     const int32_t* pair_pose = nullptr;  // [npair]
     const int32_t* pair_point = nullptr; // [npair]
@@ -142,7 +143,7 @@ struct dev_work64 {
     double* point = nullptr;        // [nl*3]
     double* point_backup = nullptr;
     double* residual = nullptr;     // [no*2] residuals of the last evaluation, sorted order
-    double* pose_terms = nullptr;   // [no*27] per-observation H_pp lower (21) + v_p (6)
+    double* pose_terms = nullptr;   // [no*27] per-observation H_pp lower (21) + v_p (6), pose-major order
     double* Hpp = nullptr;          // [np*21]
     double* bp = nullptr;           // [np*6]
     double* Hll = nullptr;          // [nl*9]
@@ -159,7 +160,7 @@ struct dev_work64 {
     double* sbuf = nullptr;         // [6np + 3nl] per-slot LM scale terms (exact sums)
     double* Lt = nullptr;           // [n*n] transposed factor (exact backward substitution)
     double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
-    int32_t* chol_status = nullptr; // [1] 0 ok, 1 decompose failed, 2 solve failed
+    int32_t* chol_status = nullptr; // [0] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
     kernel_timer* timer = nullptr;  // host object; null = no timing
 };

@@ -272,9 +272,14 @@ int32_t zsba_graph::build_structure() {
     std::vector<int32_t> pose_start(np + 1, 0), pose_list(no);
     for (int32_t e = 0; e < no; ++e) pose_start[e_pose[e] + 1]++;
     for (int32_t i = 0; i < np; ++i) pose_start[i + 1] += pose_start[i];
+    std::vector<int32_t> obs_pose_pos(no);
     {
         std::vector<int32_t> f(pose_start.begin(), pose_start.end() - 1);
-        for (int32_t e = 0; e < no; ++e) pose_list[f[e_pose[e]]++] = e_sorted[e];
+        for (int32_t e = 0; e < no; ++e) {
+            const int32_t t = f[e_pose[e]]++;
+            pose_list[t] = e_sorted[e];
+            obs_pose_pos[e_sorted[e]] = t;
+        }
     }
     // W blocks: distinct free (pose, point) pairs, first-occurrence order per point
     std::vector<int32_t> obs_pair(no, -1), lm_pair_start(nl + 1, 0), pair_pose, pair_point, pair_first;
@@ -459,6 +464,7 @@ int32_t zsba_graph::build_structure() {
         d_lm_pairs_x[set] = mem.get<int32_t>(npair);
     }
     int32_t* d_pose_pair_start = mem.get<int32_t>(np + 1);
+    int32_t* d_obs_pose_pos = mem.get<int32_t>(no);
     w.chol_scale = mem.get<double>(4);
     w.chol_status = mem.get<int32_t>(4);
     w.lm = mem.get<lm_state>(1);
@@ -482,6 +488,7 @@ int32_t zsba_graph::build_structure() {
     p.contrib = up(d_contrib, contrib); p.diag_blk = up(d_diag, diag_blk);
     p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
     p.pose_pair_start = up(d_pose_pair_start, pose_pair_start);
+    p.obs_pose_pos = up(d_obs_pose_pos, obs_pose_pos);
     for (int set = 0; set < 2; ++set) {
         p.contrib_x[set] = d_contrib_x[set];
         p.pose_pairs_x[set] = d_pose_pairs_x[set];
@@ -497,6 +504,8 @@ int32_t zsba_graph::build_structure() {
     orders_ready = false;
     mirror_filled = false;
     p.cams = up(d_cams, cam_table); p.loss_kind = up(d_lk, lk); p.loss_delta = up(d_ld, ld);
+    (void)hipMemsetAsync(w.chol_status, 0, 4 * sizeof(int32_t), stream);
+    (void)hipStreamSynchronize(stream);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "structure upload");
     dp = p;
@@ -671,6 +680,7 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel, bool exact) {
     if (e != hipSuccess) return hip_fail(e, "solve");
     if ((e = copy(lm_pinned, w.lm, sizeof(lm_state), hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "download controller");
+
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
     stats.solve_ms = now_ms() - t0;
     timer.collect();

@@ -20,6 +20,13 @@ namespace {
 
 constexpr int NB = 32;
 
+__device__ __forceinline__ double lane_value(double v, int j) {
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
 __device__ __forceinline__ bool skip(const lm_state* lm, int gated, const int32_t* status) {
     return (gated && lm->done) || *status != 0;
 }
@@ -45,49 +52,166 @@ __global__ void __launch_bounds__(1024) k_chol_begin(int n, const double* S, dou
     }
     if (threadIdx.x == 0) {
         eps[0] = sh[0] * 1e-14;
-        *status = 0;
+        status[0] = 0;
     }
 }
 
-__global__ void __launch_bounds__(256) k_potrf_diag(int n, double* A, int k0, const double* eps, int32_t* status,
-                                                    const lm_state* lm, int gated) {
-    if (skip(lm, gated, status)) return;
-    __shared__ double T[NB][NB + 1];
-    __shared__ int fail;
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        if (c <= r) T[r][c] = A[static_cast<int64_t>(k0 + r) * n + k0 + c];
+// ---- panel-lookahead factorisation -------------------------------------------
+// Lower-triangle tiles (I, J), I >= J, of 32 x 32.  Launch k (k = 0 .. T-1) does
+// two independent jobs that only read panel k-1, which launch k-1 finished:
+//   * column k: for every row tile I >= k, apply the panel k-1 update to (I, k),
+//     factor the diagonal tile (k, k) (each workgroup factors its own copy, so no
+//     workgroup waits for another) and solve (I, k) against it;
+//   * every tile (I, J) with J > k: apply the panel k-1 update.
+// Each element therefore receives a -= L_ij * L_kj for panel columns in
+// ascending order and its division / square root last: the operation sequence
+// of the reference's row-Crout loop.  Ordering comes from kernel boundaries only.
+
+// own -= L_I L_J^T over one 32-column panel, column by column (LDS operands).
+__device__ __forceinline__ void panel_update(double (*own)[NB + 1], const double (*LI)[NB + 1], const double (*LJ)[NB + 1],
+                                             int nr, int nc, bool lower_only) {
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+        const int r = idx / NB, c = idx % NB;
+        if (r >= nr || c >= nc || (lower_only && c > r)) continue;
+        double v = own[r][c];
+#pragma unroll 8
+        for (int m = 0; m < NB; ++m) v = v - LI[r][m] * LJ[c][m];
+        own[r][c] = v;
     }
-    if (tid == 0) fail = 0;
+}
+
+// Factorisation of a diagonal tile by wave 0 with row r in the registers of lane
+// r; column j is broadcast lane by lane.  Returns false (uniformly) when a pivot
+// is not greater than eps.
+__device__ bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* ok_sh) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        double row[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) row[c] = (lane < nr) ? D[lane][c] : 0.0;
+        int ok = 1;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < nr && ok) {
+                const double d = lane_value(row[j], j);
+                if (!(d > eps)) {
+                    ok = 0;
+                } else {
+                    const double ljj = sqrt(d);
+                    if (lane == j) row[j] = ljj;
+                    else if (lane > j) row[j] = row[j] / ljj;
+#pragma unroll
+                    for (int c = j + 1; c < NB; ++c) {
+                        if (c < nr) {
+                            const double lcj = lane_value(row[j], c);
+                            if (lane >= c) row[c] = row[c] - row[j] * lcj;
+                        }
+                    }
+                }
+            }
+        }
+        if (lane < nr) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) D[lane][c] = row[c];
+        }
+        if (lane == 0) *ok_sh = ok;
+    }
     __syncthreads();
-    const double e = eps[0];
-    for (int j = 0; j < nb; ++j) {
-        if (tid == 0) {
-            const double v = T[j][j];
-            if (!(v > e)) fail = 1;
-            else T[j][j] = sqrt(v);
-        }
-        __syncthreads();
-        if (fail) break;
-        for (int r = j + 1 + tid; r < nb; r += blockDim.x) T[r][j] = T[r][j] / T[j][j];
-        __syncthreads();
-        const int m = nb - j - 1;
-        for (int idx = tid; idx < m * m; idx += blockDim.x) {
-            const int r = j + 1 + idx / m, c = j + 1 + idx % m;
-            if (c <= r) T[r][c] = T[r][c] - T[r][j] * T[c][j];
-        }
-        __syncthreads();
+    return *ok_sh != 0;
+}
+
+__device__ __forceinline__ void load_tile(double (*dst)[NB + 1], const double* A, int n, int r0, int c0, int nr, int nc,
+                                          bool lower_only) {
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+        const int r = idx / NB, c = idx % NB;
+        dst[r][c] = (r < nr && c < nc && (!lower_only || c <= r)) ? A[static_cast<int64_t>(r0 + r) * n + c0 + c] : 0.0;
     }
-    if (fail) {
-        if (tid == 0) *status = 1;
+}
+__device__ __forceinline__ void store_tile(const double (*src)[NB + 1], double* A, int n, int r0, int c0, int nr, int nc,
+                                           bool lower_only) {
+    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
+        const int r = idx / NB, c = idx % NB;
+        if (r < nr && c < nc && (!lower_only || c <= r)) A[static_cast<int64_t>(r0 + r) * n + c0 + c] = src[r][c];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const double* eps_p, int32_t* status, int k,
+                                                   const lm_state* lm, int gated) {
+    if ((gated && lm->done) || status[0] != 0) return;
+    __shared__ double own[NB][NB + 1];
+    __shared__ double D[NB][NB + 1];
+    __shared__ double LI[NB][NB + 1];
+    __shared__ double LJ[NB][NB + 1];
+    __shared__ int ok_sh;
+    const int T = (n + NB - 1) / NB;
+    const int np_wg = T - k;  // column-k workgroups
+    const int k0 = k * NB, kp0 = (k - 1) * NB;
+    const int nk = (n - k0) < NB ? (n - k0) : NB;
+    if (static_cast<int>(blockIdx.x) < np_wg) {
+        const int I = k + blockIdx.x;
+        const int r0 = I * NB;
+        const int nr = (n - r0) < NB ? (n - r0) : NB;
+        // diagonal tile (k, k) with its panel k-1 update
+        load_tile(D, A, n, k0, k0, nk, nk, true);
+        if (k > 0) {
+            load_tile(LJ, A, n, k0, kp0, nk, NB, false);
+            __syncthreads();
+            panel_update(D, LJ, LJ, nk, nk, true);
+        }
+        __syncthreads();
+        if (!factor_diag_tile(D, nk, eps_p[0], &ok_sh)) {
+            if (threadIdx.x == 0) atomicCAS(status, 0, 1);
+            return;
+        }
+        if (I == k) {
+            store_tile(D, A, n, k0, k0, nk, nk, true);
+            return;
+        }
+        // row tile (I, k): panel k-1 update, then solve against L_kk^T
+        load_tile(own, A, n, r0, k0, nr, nk, false);
+        if (k > 0) {
+            load_tile(LI, A, n, r0, kp0, nr, NB, false);
+            __syncthreads();
+            panel_update(own, LI, LJ, nr, nk, false);
+        }
+        __syncthreads();
+        if (threadIdx.x < nr) {
+            double x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) x[c] = own[threadIdx.x][c];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < nk) {
+                    double v = x[c];
+#pragma unroll
+                    for (int m = 0; m < c; ++m) v = v - x[m] * D[c][m];
+                    x[c] = v / D[c][c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c) own[threadIdx.x][c] = x[c];
+        }
+        __syncthreads();
+        store_tile(own, A, n, r0, k0, nr, nk, false);
         return;
     }
-    for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        if (c <= r) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = T[r][c];
-    }
+    if (k == 0) return;
+    // trailing tile (I, J), J > k: panel k-1 update
+    const int64_t u = static_cast<int64_t>(blockIdx.x) - np_wg;
+    int64_t Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) / 2.0);
+    while ((Ip + 1) * (Ip + 2) / 2 <= u) ++Ip;
+    while (Ip * (Ip + 1) / 2 > u) --Ip;
+    const int64_t Jp = u - Ip * (Ip + 1) / 2;
+    const int I = k + 1 + static_cast<int>(Ip), J = k + 1 + static_cast<int>(Jp);
+    const int r0 = I * NB, c0 = J * NB;
+    const int nr = (n - r0) < NB ? (n - r0) : NB, nc = (n - c0) < NB ? (n - c0) : NB;
+    load_tile(own, A, n, r0, c0, nr, nc, I == J);
+    load_tile(LI, A, n, r0, kp0, nr, NB, false);
+    load_tile(LJ, A, n, c0, kp0, nc, NB, false);
+    __syncthreads();
+    panel_update(own, LI, LJ, nr, nc, I == J);
+    __syncthreads();
+    store_tile(own, A, n, r0, c0, nr, nc, I == J);
 }
 
 // Rows below the diagonal tile: L_rk = A_rk L_kk^-T.
@@ -182,8 +306,38 @@ __global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L,
 
 // Backward substitution L^T x = y in the reference's order
 // (matrix_decomposition_cholesky.hpp:85-93): x_i -= L_ji x_j for j = i+1 .. n-1
-// ascending, then x_i /= L_ii.  One wave; the products are formed 64 at a time
-// in parallel and subtracted in order.
+// ascending, then x_i /= L_ii.  One wave with x in LDS: while lane 0 runs the
+// ordered subtractions for row i, the wave has already staged row i-1 of L^T.
+constexpr int kBackLds = 6144;  // largest n handled from LDS
+__global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* L, const double* Lt, double* x,
+                                                           const int32_t* status, lm_state* lm, int gated) {
+    if (skip(lm, gated, status)) return;
+    __shared__ double xs[kBackLds];
+    __shared__ double prod[kBackLds];
+    const int lane = threadIdx.x;
+    for (int j = lane; j < n; j += 64) xs[j] = x[j];
+    __syncthreads();
+    for (int i = n - 1; i >= 0; --i) {
+        // products L_ji x_j (x_j final for j > i), formed in parallel
+        const double* u = Lt + static_cast<int64_t>(i) * n;
+        for (int j = i + 1 + lane; j < n; j += 64) prod[j] = u[j] * xs[j];
+        __syncthreads();
+        if (lane == 0) {
+            double v = xs[i];
+            int j = i + 1;
+            for (; j + 32 <= n; j += 32) {
+#pragma unroll
+                for (int k = 0; k < 32; ++k) v = v - prod[j + k];
+            }
+            for (; j < n; ++j) v = v - prod[j];
+            xs[i] = v / L[static_cast<int64_t>(i) * n + i];
+        }
+        __syncthreads();
+    }
+    for (int j = lane; j < n; j += 64) x[j] = xs[j];
+    if (lane == 0) lm->solve_ok = 1;
+}
+
 __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, const double* Lt, double* x,
                                                        const int32_t* status, lm_state* lm, int gated) {
     if (skip(lm, gated, status)) return;
@@ -324,17 +478,12 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     }
     ZB_TIME_BEGIN(w, K_CHOLESKY, s);
     k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g);
-    const int tiles = (n + NB - 1) / NB;
-    for (int t = 0; t < tiles; ++t) {
-        const int k0 = t * NB;
-        k_potrf_diag<<<1, 256, 0, s>>>(n, w.S, k0, w.chol_scale, w.chol_status, w.lm, g);
-        const int below = n - (k0 + NB);
-        if (below > 0) {
-            k_trsm<<<(below + 63) / 64, 64, 0, s>>>(n, w.S, k0, w.chol_status, w.lm, g);
-            const int64_t m = tiles - t - 1;
-            const int64_t pairs = m * (m + 1) / 2;
-            k_syrk<<<static_cast<unsigned>(pairs), 256, 0, s>>>(n, w.S, k0, t + 1, w.chol_status, w.lm, g);
-        }
+    const int64_t tiles = (n + NB - 1) / NB;
+    for (int64_t k = 0; k < tiles; ++k) {
+        const int64_t m = tiles - k - 1;
+        const int64_t grid = (tiles - k) + (k > 0 ? m * (m + 1) / 2 : 0);
+        k_chol_step<<<static_cast<unsigned>(grid), 256, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, static_cast<int>(k),
+                                                               w.lm, g);
     }
     ZB_TIME_END(w, s);
     ZB_TIME_BEGIN(w, K_TRSV, s);
@@ -342,7 +491,8 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     if (exact) {
         const int tiles32 = (n + 31) / 32;
         k_transpose_lower<<<dim3(tiles32, tiles32), 256, 0, s>>>(n, w.S, w.Lt, w.chol_status, w.lm, g);
-        k_backward_exact<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
+        if (n <= kBackLds) k_backward_exact_lds<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
+        else k_backward_exact<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
     }
     ZB_TIME_END(w, s);
 }

@@ -102,26 +102,49 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
-// Sequential sum a[0] + a[1] + ... from 0.0, in index order (one wave; the other
-// lanes only prefetch), as the reference's accumulation loops
-// (factor_graph.hpp:556-560, 624-628).
+// Sequential sum a[0] + a[1] + ... from 0.0, in index order, as the reference's
+// accumulation loops (factor_graph.hpp:556-560, 624-628).  One wave: all lanes
+// stream the next 1024 values into LDS while lane 0 adds the current 1024, so
+// only the dependent additions remain on the critical path.
+constexpr int kSeqChunk = 1024;
 __global__ void __launch_bounds__(64) k_seqsum(const double* a, int64_t n, lm_state* lm, int target, int gate,
                                                double* out) {
     if (gate == 1 && lm->done) return;
     if (gate == 2 && (lm->done || !lm->step_applied)) return;
+    __shared__ double buf[2][kSeqChunk];
     const int lane = threadIdx.x;
+    double reg[kSeqChunk / 64];
     double acc = 0.0;
-    double v = (lane < n) ? a[lane] : 0.0;
-    for (int64_t i0 = 0; i0 < n; i0 += 64) {
-        const double next = (i0 + 64 + lane < n) ? a[i0 + 64 + lane] : 0.0;
-        if (n - i0 >= 64) {
 #pragma unroll
-            for (int j = 0; j < 64; ++j) acc = acc + readlane_d(v, j);
-        } else {
-            const int cnt = static_cast<int>(n - i0);
-            for (int j = 0; j < cnt; ++j) acc = acc + readlane_d(v, j);
+    for (int k = 0; k < kSeqChunk / 64; ++k) {
+        const int64_t i = k * 64 + lane;
+        buf[0][k * 64 + lane] = (i < n) ? a[i] : 0.0;
+    }
+    for (int64_t base = 0, c = 0; base < n; base += kSeqChunk, ++c) {
+        const int64_t nbase = base + kSeqChunk;
+        if (nbase < n) {
+#pragma unroll
+            for (int k = 0; k < kSeqChunk / 64; ++k) {
+                const int64_t i = nbase + k * 64 + lane;
+                reg[k] = (i < n) ? a[i] : 0.0;
+            }
         }
-        v = next;
+        __syncthreads();
+        if (lane == 0) {
+            const double* b = buf[c & 1];
+            const int cnt = (n - base) < kSeqChunk ? static_cast<int>(n - base) : kSeqChunk;
+            if (cnt == kSeqChunk) {
+#pragma unroll 32
+                for (int k = 0; k < kSeqChunk; ++k) acc = acc + b[k];
+            } else {
+                for (int k = 0; k < cnt; ++k) acc = acc + b[k];
+            }
+        }
+        __syncthreads();
+        if (nbase < n) {
+#pragma unroll
+            for (int k = 0; k < kSeqChunk / 64; ++k) buf[(c + 1) & 1][k * 64 + lane] = reg[k];
+        }
     }
     if (lane == 0) {
         if (target == 0) lm->chi2 = acc;
@@ -183,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, cons
         if (!pfixed) {
             double H21[21], Wpl[18], v[6];
             pose_terms(Jp, Jl, Wr, W, r, drho, H21, Wpl, v);
-            double* out = pterms + 27L * s;
+            double* out = pterms + 27L * p.obs_pose_pos[s];
 #pragma unroll
             for (int i = 0; i < 21; ++i) out[i] = H21[i];
 #pragma unroll
@@ -225,27 +248,25 @@ __global__ void __launch_bounds__(256) k_reduce_pose(dev_problem p, const double
     if (wave >= p.np || lane >= 27) return;
     double acc = 0.0;
     if (!p.pose_fixed[wave]) {
+        // terms of the pose's observations are contiguous, in insertion order
         const int t0 = p.pose_start[wave], t1 = p.pose_start[wave + 1];
+        const double* src = pose_terms + lane;
+        const double sign = lane < 21 ? 1.0 : -1.0;  // H_pp += H;  b_p -= v
+        constexpr int U = 16;
         int t = t0;
-        if (lane < 21) {
-            for (; t + 4 <= t1; t += 4) {
-                const double a0 = pose_terms[27L * p.pose_list[t] + lane];
-                const double a1 = pose_terms[27L * p.pose_list[t + 1] + lane];
-                const double a2 = pose_terms[27L * p.pose_list[t + 2] + lane];
-                const double a3 = pose_terms[27L * p.pose_list[t + 3] + lane];
-                acc = acc + a0; acc = acc + a1; acc = acc + a2; acc = acc + a3;
+        for (; t + U <= t1; t += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[27L * (t + u)];
+            if (sign > 0) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc = acc + v[u];
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc = acc - v[u];
             }
-            for (; t < t1; ++t) acc = acc + pose_terms[27L * p.pose_list[t] + lane];
-        } else {
-            for (; t + 4 <= t1; t += 4) {
-                const double a0 = pose_terms[27L * p.pose_list[t] + lane];
-                const double a1 = pose_terms[27L * p.pose_list[t + 1] + lane];
-                const double a2 = pose_terms[27L * p.pose_list[t + 2] + lane];
-                const double a3 = pose_terms[27L * p.pose_list[t + 3] + lane];
-                acc = acc - a0; acc = acc - a1; acc = acc - a2; acc = acc - a3;
-            }
-            for (; t < t1; ++t) acc = acc - pose_terms[27L * p.pose_list[t] + lane];
         }
+        for (; t < t1; ++t) acc = sign > 0 ? acc + src[27L * t] : acc - src[27L * t];
     }
     if (lane < 21) Hpp[21L * wave + lane] = acc;
     else bp[6L * wave + (lane - 21)] = acc;
@@ -468,7 +489,28 @@ __global__ void __launch_bounds__(256) k_schur_exact(dev_problem p, const double
     const int2* contrib = p.contrib_x[order_set(lm)];
     const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
     double P = 0.0;
-    for (int64_t t = c0; t < c1; ++t) {
+    constexpr int U = 8;  // operands of U contributions are loaded before the ordered additions
+    int64_t t = c0;
+    for (; t + U <= c1; t += U) {
+        double y[U][3], wv[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int2 ab = contrib[t + u];
+            const double* Ya = Y + 18L * ab.x + 3 * r;
+            const double* Wc = Wbuf + 18L * ab.y + 3 * c;
+            y[u][0] = Ya[0]; y[u][1] = Ya[1]; y[u][2] = Ya[2];
+            wv[u][0] = Wc[0]; wv[u][1] = Wc[1]; wv[u][2] = Wc[2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double prod = 0;
+            prod += y[u][0] * wv[u][0];
+            prod += y[u][1] * wv[u][1];
+            prod += y[u][2] * wv[u][2];
+            P = (t + u == c0) ? prod : P + prod;
+        }
+    }
+    for (; t < c1; ++t) {
         const int2 ab = contrib[t];
         const double* Ya = Y + 18L * ab.x + 3 * r;
         const double* Wc = Wbuf + 18L * ab.y + 3 * c;
@@ -501,7 +543,29 @@ __global__ void __launch_bounds__(256) k_bs_exact(dev_problem p, const double* Y
     if (pose >= p.np || lane >= 6) return;
     const int32_t* pairs = p.pose_pairs_x[order_set(lm)];
     double yb = 0.0;
-    for (int t = p.pose_pair_start[pose]; t < p.pose_pair_start[pose + 1]; ++t) {
+    constexpr int U = 8;
+    const int t0 = p.pose_pair_start[pose], t1 = p.pose_pair_start[pose + 1];
+    int t = t0;
+    for (; t + U <= t1; t += U) {
+        double y[U][3], bv[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int a = pairs[t + u];
+            const int l = p.pair_point[a];
+            const double* Ya = Y + 18L * a + 3 * lane;
+            y[u][0] = Ya[0]; y[u][1] = Ya[1]; y[u][2] = Ya[2];
+            bv[u][0] = bl[3L * l]; bv[u][1] = bl[3L * l + 1]; bv[u][2] = bl[3L * l + 2];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            double prod = 0;
+            prod += y[u][0] * bv[u][0];
+            prod += y[u][1] * bv[u][1];
+            prod += y[u][2] * bv[u][2];
+            yb = yb + prod;
+        }
+    }
+    for (; t < t1; ++t) {
         const int a = pairs[t];
         const int l = p.pair_point[a];
         const double* Ya = Y + 18L * a + 3 * lane;
