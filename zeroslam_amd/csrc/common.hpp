@@ -163,6 +163,18 @@ struct dev_work64 {
     int32_t* chol_status = nullptr; // [0] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
     kernel_timer* timer = nullptr;  // host object; null = no timing
+    // binary32 buffers of the mixed path (same layouts as the binary64 ones)
+    float* pose_terms_f = nullptr;  // [no*27], pose-major
+    float* Hpp_f = nullptr;         // [np*21]
+    float* bp_f = nullptr;          // [np*6]
+    float* Hll_f = nullptr;         // [nl*9]
+    float* bl_f = nullptr;          // [nl*3]
+    float* W_f = nullptr;           // [npair*18]
+    float* Hinv_f = nullptr;        // [nl*9]
+    float* Y_f = nullptr;           // [npair*18]
+    double* S_orig = nullptr;       // [n*n] reduced system before factorisation (refinement)
+    double* rvec = nullptr;         // [n] refinement residual
+    double* corr = nullptr;         // [n] refinement correction
 };
 
 #define ZB_TIME_BEGIN(w, id, s) do { if ((w).timer) (w).timer->begin((id), (s)); } while (0)
@@ -188,11 +200,20 @@ void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out,
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s);
 void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
+// ---- mixed binary32 / binary64 kernels (kernels_fp32.hip) ----------------------
+void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
+void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override);
+void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, bool gated);
+void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool apply);
+
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from
 // w.bs.  Sets *w.chol_status and lm->solve_ok; skips everything when the
 // controller is done (gated) or a previous stage failed.
 void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact = false);
+// Forward + backward substitution with the factor in w.S: x = S^-1 b.
+void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
 
 // ---- diagnostics ---------------------------------------------------------------
 void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t s);
