@@ -68,14 +68,118 @@ def cpu_baseline(problem, iters: int) -> dict:
                       f"{dt:.2f} s single-threaded on {cpu_model()} (nproc {os.cpu_count()})"}
 
 
+def envelope_first_column(problem) -> np.ndarray:
+    """First structurally non-zero column of each row of the reduced system's lower
+    triangle: pose i couples to pose j < i when both are free and observe a common
+    free point; its 6 rows start at 6 * (smallest such j, or i)."""
+    n_p = problem.n_poses
+    first = np.arange(n_p)
+    pose, point = problem.obs_pose, problem.obs_point
+    keep = (~problem.pose_fixed.astype(bool)[pose]) & (~problem.point_fixed.astype(bool)[point])
+    pose, point = pose[keep], point[keep]
+    m = np.full(problem.n_points, n_p, dtype=np.int64)
+    np.minimum.at(m, point, pose)
+    np.minimum.at(first, pose, m[point])
+    return np.repeat(6 * first, 6)
+
+
+def cholesky_flops(problem) -> float:
+    """Multiply-subtract pairs (2 flops each) plus the divisions and square roots
+    that a Cholesky factorisation restricted to the envelope must perform."""
+    f = envelope_first_column(problem)
+    total = 0.0
+    for i in range(len(f)):
+        j = np.arange(f[i], i + 1)
+        total += 2.0 * np.sum(j - np.maximum(f[i], f[j])) + len(j)
+    return total
+
+
+def time_solves(g, precision, iters, steps, warmup):
+    """Restore + solve(iters) `steps` times after `warmup` untimed runs; returns
+    (obs*iters/s, ms per step, last stats, per-class kernel ms, launches)."""
+    for _ in range(warmup):
+        g.restore()
+        g.solve(iters, False, precision)
+    kernel_ms, kernel_launches = {}, {}
+    obs_iters = 0
+    stats = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.restore()
+        g.solve(iters, False, precision)
+        stats = g.stats()
+        obs_iters += stats["n_obs"] * stats["iterations"]
+        for k, v in stats["kernel_ms"].items():
+            kernel_ms[k] = kernel_ms.get(k, 0.0) + v
+            kernel_launches[k] = kernel_launches.get(k, 0) + stats["kernel_launches"][k]
+    elapsed = time.perf_counter() - t0
+    return obs_iters / elapsed, elapsed / steps * 1e3, stats, kernel_ms, kernel_launches
+
+
+def iterations_to_reach(g, precision, target, lo, hi):
+    """Smallest iteration budget k in [lo, hi] whose solve ends at chi2 <= target.
+    A solve with budget k runs the first k iterations of one with budget k + 1, so
+    the final chi2 does not increase with k and a bisection applies."""
+    def chi(k):
+        g.restore()
+        g.solve(k, False, precision)
+        return g.stats()["final_chi2"]
+    if chi(hi) > target:
+        return None
+    while lo < hi:
+        mid = (lo + hi) // 2
+        if chi(mid) <= target:
+            hi = mid
+        else:
+            lo = mid + 1
+    return lo
+
+
+def state_error(P, X, Pr, Xr):
+    return float(max(np.abs(P - Pr).max() / max(np.abs(Pr).max(), 1e-300),
+                     np.abs(X - Xr).max() / max(np.abs(Xr).max(), 1e-300)))
+
+
+def roofline_of(problem, precision_name, kernel_ms, kernel_launches):
+    s = 4 if precision_name == "mixed" else 8
+    n_pairs = int(problem.n_obs)  # one W block per (pose, point) pair; the workload has no repeats
+    B = algorithmic_bytes(problem.n_obs, problem.n_poses, problem.n_points, n_pairs, s)
+    avg = {k: kernel_ms[k] / kernel_launches[k] for k in kernel_ms if kernel_launches[k]}
+    groups = {
+        "linearisation": ["linearise", "pose_reduce"],
+        "cost": ["cost"],
+        "schur": ["invert", "form_Y", "fill", "schur"],
+        "backsub": ["backsub"],
+        "cholesky": ["cholesky", "trsv"],
+    }
+    total = {gname: sum(kernel_ms.get(k, 0.0) for k in ks) for gname, ks in groups.items()}
+    dominant = max(total, key=total.get)
+    if dominant == "cholesky":
+        n = 6 * problem.n_poses
+        dur = avg.get("cholesky", float("nan"))
+        flops = cholesky_flops(problem)
+        achieved = flops / (dur * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": "cholesky",
+                    "algorithmic_flops": flops, "avg_ms": dur, "n": n}
+    else:
+        dur = sum(avg.get(k, 0.0) for k in groups[dominant])
+        achieved = B[dominant] / (dur * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "algorithmic_bytes": B[dominant], "avg_ms": dur}
+    return roofline, {k: round(v, 5) for k, v in avg.items()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp64_fast", "mixed"])
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "fp64_fast", "mixed"],
+                    help="mixed (default): the headline, mixed vs FP64 at equal final cost")
+    ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -92,66 +196,60 @@ def main() -> int:
     g = Graph.from_problem(problem, timing=True)
     g.solve(0)          # structure analysis + upload (untimed)
     g.snapshot()
-    for _ in range(args.warmup):
-        g.restore()
-        g.solve(iters, False, precision)
-    kernel_ms = {}
-    kernel_launches = {}
-    obs_iters = 0
-    chi2 = None
-    stats = None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.restore()
-        g.solve(iters, False, precision)
-        stats = g.stats()
-        obs_iters += stats["n_obs"] * stats["iterations"]
-        chi2 = stats["final_chi2"]
-        for k, v in stats["kernel_ms"].items():
-            kernel_ms[k] = kernel_ms.get(k, 0.0) + v
-            kernel_launches[k] = kernel_launches.get(k, 0) + stats["kernel_launches"][k]
-    elapsed = time.perf_counter() - t0
-    value = obs_iters / elapsed
+    print(f"bench: c{args.config} obs={problem.n_obs} poses={problem.n_poses} points={problem.n_points}",
+          file=sys.stderr, flush=True)
 
-    # roofline of the dominant kernel class (live HIP-event durations)
-    s = 8
-    n_pairs = int(problem.n_obs)  # one W block per (pose, point) pair; the workload has no repeats
-    B = algorithmic_bytes(problem.n_obs, problem.n_poses, problem.n_points, n_pairs, s)
-    avg = {k: kernel_ms[k] / kernel_launches[k] for k in kernel_ms if kernel_launches[k]}
-    groups = {
-        "linearisation": ["linearise", "pose_reduce"],
-        "cost": ["cost"],
-        "schur": ["invert", "form_Y", "fill", "schur"],
-        "backsub": ["backsub"],
-        "cholesky": ["cholesky", "trsv"],
-    }
-    total = {gname: sum(kernel_ms.get(k, 0.0) for k in ks) for gname, ks in groups.items()}
-    dominant = max(total, key=total.get)
-    if dominant == "cholesky":
-        n = 6 * problem.n_poses
-        dur = avg.get("cholesky", float("nan")) + avg.get("trsv", 0.0)
-        flops = n ** 3 / 3 + 2 * n * n
-        achieved = flops / (dur * 1e-3) / 1e12
-        roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": "cholesky+trsv"}
+    # the plain FP64 GPU path (the reference's operation order): throughput and final cost
+    v64, ms64, st64, k64, l64 = time_solves(g, FP64, iters, args.steps, args.warmup)
+    chi64 = st64["final_chi2"]
+    P64, X64 = g.poses(), g.points()
+    print(f"bench: fp64 {v64:.4g} obs*iter/s, chi2 {chi64!r}", file=sys.stderr, flush=True)
+    extra = {"fp64": {"value": v64, "ms_per_step": ms64, "lm_iterations": st64["iterations"],
+                      "attempts": st64["attempts"], "final_chi2": chi64}}
+    if precision == FP64:
+        value, ms, stats, kms, kl = v64, ms64, st64, k64, l64
+        run_iters = iters
     else:
-        dur = sum(avg.get(k, 0.0) for k in groups[dominant])
-        achieved = B[dominant] / (dur * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
-                    "algorithmic_bytes": B[dominant], "avg_ms": dur}
+        vf, msf, stf, kf, lf = time_solves(g, FP64_FAST, iters, args.steps, args.warmup)
+        extra["fp64_fast"] = {"value": vf, "ms_per_step": msf, "lm_iterations": stf["iterations"],
+                              "final_chi2": stf["final_chi2"],
+                              "chi2_rel_vs_fp64": (stf["final_chi2"] - chi64) / chi64,
+                              "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64)}
+        print(f"bench: fp64_fast {vf:.4g} obs*iter/s", file=sys.stderr, flush=True)
+        if precision == FP64_FAST:
+            value, ms, stats, kms, kl = vf, msf, stf, kf, lf
+            run_iters = iters
+        else:
+            # mixed: charged the iterations it needs to reach the FP64 final cost
+            target = chi64 * (1.0 + 1e-6)
+            run_iters = iterations_to_reach(g, MIXED, target, 1, 4 * iters)
+            if run_iters is None:
+                print("bench: mixed precision did not reach the FP64 final cost within "
+                      f"{4 * iters} iterations", file=sys.stderr)
+                return 1
+            value, ms, stats, kms, kl = time_solves(g, MIXED, run_iters, args.steps, args.warmup)
+            extra["mixed"] = {"iterations_to_equal_cost": run_iters, "final_chi2": stats["final_chi2"],
+                              "chi2_rel_vs_fp64": (stats["final_chi2"] - chi64) / chi64,
+                              "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64),
+                              "speedup_vs_fp64": value / v64, "speedup_vs_fp64_fast": value / vf}
+            print(f"bench: mixed {value:.4g} obs*iter/s at {run_iters} iterations", file=sys.stderr, flush=True)
+
+    roofline, avg = roofline_of(problem, args.precision, kms, kl)
     out = {
         "metric": "BA observation*iters/s, 100KF/20k pts: mixed vs FP64 GPU at equal final cost",
         "value": value, "unit": "obs*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if precision == MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"c{args.config}: {problem.n_poses} KF, {problem.n_points} pts, {problem.n_obs} obs, "
-                               f"{iters} LM iterations", "precision": args.precision,
-                   "lm_iterations": stats["iterations"], "attempts": stats["attempts"], "final_chi2": chi2},
+                               f"FP64 reference {iters} LM iterations", "precision": args.precision,
+                   "lm_iterations": stats["iterations"], "iteration_budget": run_iters,
+                   "attempts": stats["attempts"], "final_chi2": stats["final_chi2"]},
         "roofline": roofline,
-        "kernel_avg_ms": {k: round(v, 5) for k, v in avg.items()},
+        "compare": extra,
+        "kernel_avg_ms": avg,
     }
     if not args.no_cpu:
+        print("bench: reference CPU optimiser sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(problem, args.cpu_iters)
     print(json.dumps(out))
     return 0

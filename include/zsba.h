@@ -68,9 +68,14 @@ extern "C" {
 typedef struct zsba_options {
     int32_t device;       /* HIP device ordinal (default 0)                          */
     int32_t verbose;      /* 1: print the reference's [INIT]/[GOOD]/[ BAD] lines     */
-    int32_t refine_steps; /* MIXED: refinement steps of the reduced solve (default 2)*/
+    int32_t refine_steps; /* MIXED: refinement steps of the reduced solve: 0 = default
+                             (2), negative = none                                      */
     int32_t timing;       /* 1: time every kernel launch with HIP events (stats)      */
-    int32_t reserved[4];
+    int32_t floor_exp;    /* MIXED: the binary32-block reduced system is factorised
+                             with its diagonal raised by 2^-floor_exp max|S_ii| (its
+                             rounding floor); the binary64 refinement uses the true
+                             damping.  0 = default, negative = no floor              */
+    int32_t reserved[3];
 } zsba_options;
 
 /* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */
@@ -85,6 +90,7 @@ typedef struct zsba_options {
 #define ZSBA_K_TRSV        8  /* forward / backward substitution                            */
 #define ZSBA_K_BACKSUB     9  /* point back-substitution, state update, LM scale            */
 #define ZSBA_K_CONTROL    10  /* LM controller, rollback and bookkeeping                    */
+#define ZSBA_K_REFINE     11  /* mixed path: binary64 residual of the reduced system        */
 #define ZSBA_K_COUNT      16
 
 /* Per-solve statistics (filled by zsba_get_stats after zsba_solve). */

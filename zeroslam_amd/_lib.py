@@ -15,14 +15,14 @@ ZSBA_PRECISION_FP64, ZSBA_PRECISION_MIXED, ZSBA_PRECISION_FP64_FAST = 0, 1, 2
 EDGE_EVAL_STRIDE = 27
 
 KERNEL_NAMES = ("linearise", "pose_reduce", "cost", "invert", "form_Y", "fill", "schur", "cholesky", "trsv",
-                "backsub", "control")
+                "backsub", "control", "refine")
 
 ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EDEVICE", -4: "ENOMEM", -5: "EUNSUP"}
 
 
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("refine_steps", C.c_int32),
-                ("timing", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("timing", C.c_int32), ("floor_exp", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):

@@ -3,8 +3,10 @@
 Every HIP translation unit is compiled for gfx950 only with ``hipcc`` and linked
 into one shared library with a plain C ABI (``include/zsba.h``).  The binary64
 kernels are compiled without floating-point contraction so that per-observation
-arithmetic is the reference's IEEE sequence; the binary32 kernels may contract.
-Packed-FP32 instruction selection is disabled for all device code.
+arithmetic is the reference's IEEE sequence.  The mixed path's binary32 kernels
+are compiled the same way: they evaluate each observation's residual and robust
+weight in binary64 in the reference's order.  Packed-FP32 instruction selection
+is disabled for all device code.
 
 Usage: ``python -m zeroslam_amd.build`` (also called by ``__graft_entry__.build``).
 """
@@ -37,7 +39,7 @@ UNITS = [
     ("synth.cpp", ["-ffp-contract=off", "-x", "c++"]),
     ("engine.hip", ["-ffp-contract=off"]),
     ("kernels_fp64.hip", ["-ffp-contract=off"]),
-    ("kernels_fp32.hip", ["-ffp-contract=fast"]),
+    ("kernels_fp32.hip", ["-ffp-contract=off"]),
     ("kernels_chol.hip", ["-ffp-contract=off"]),
 ]
 

@@ -240,12 +240,30 @@ ZB_HD void transpose(const T* a, T* o) {
         for (int c = 0; c < C; ++c) o[c * R + r] = a[r * C + c];
 }
 
-// Pose-vertex terms of make_hessian for one edge, factor_graph.hpp:680-696, 725:
-//   jtw = Jp^T Omega',  H_pp = jtw Jp (lower triangle, 21 entries row-major),
-//   W = jtw Jl (6x3),  v = ((drho Jp^T) Omega) r  (b_p -= v).
+// Gradient terms of one edge, factor_graph.hpp:680-689, 725:
+//   v = ((drho J^T) Omega) r  (b -= v), for the pose (6) and the point (3).
 template <class T>
-ZB_HD void pose_terms(const T Jp[12], const T Jl[6], const T Wr[4], const T W[4], const T r[2], T drho, T Hpp21[21],
-                      T Wpl[18], T v[6]) {
+ZB_HD void pose_grad(const T Jp[12], const T W[4], const T r[2], T drho, T v[6]) {
+    T JpT[12], dJ[12], t[12];
+    transpose<2, 6>(Jp, JpT);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) dJ[i] = JpT[i] * drho;
+    mul<6, 2, 2>(dJ, W, t);
+    mul<6, 2, 1>(t, r, v);
+}
+template <class T>
+ZB_HD void point_grad(const T Jl[6], const T W[4], const T r[2], T drho, T v[3]) {
+    T JlT[6], dJ[6], t[6];
+    transpose<2, 3>(Jl, JlT);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) dJ[i] = JlT[i] * drho;
+    mul<3, 2, 2>(dJ, W, t);
+    mul<3, 2, 1>(t, r, v);
+}
+
+// Hessian terms of one edge without the gradient.
+template <class T>
+ZB_HD void pose_hessian(const T Jp[12], const T Jl[6], const T Wr[4], T Hpp21[21], T Wpl[18]) {
     T JpT[12], jtw[12];
     transpose<2, 6>(Jp, JpT);
     mul<6, 2, 2>(JpT, Wr, jtw);
@@ -260,26 +278,31 @@ ZB_HD void pose_terms(const T Jp[12], const T Jl[6], const T Wr[4], const T W[4]
             Hpp21[idx++] = s;
         }
     mul<6, 2, 3>(jtw, Jl, Wpl);
-    T dJ[12], t[12];
-#pragma unroll
-    for (int i = 0; i < 12; ++i) dJ[i] = JpT[i] * drho;
-    mul<6, 2, 2>(dJ, W, t);
-    mul<6, 2, 1>(t, r, v);
+}
+template <class T>
+ZB_HD void point_hessian(const T Jl[6], const T Wr[4], T Hll[9]) {
+    T JlT[6], jtw[6];
+    transpose<2, 3>(Jl, JlT);
+    mul<3, 2, 2>(JlT, Wr, jtw);
+    mul<3, 2, 3>(jtw, Jl, Hll);
+}
+
+// Pose-vertex terms of make_hessian for one edge, factor_graph.hpp:680-696, 725:
+//   jtw = Jp^T Omega',  H_pp = jtw Jp (lower triangle, 21 entries row-major),
+//   W = jtw Jl (6x3),  v = ((drho Jp^T) Omega) r  (b_p -= v).
+template <class T>
+ZB_HD void pose_terms(const T Jp[12], const T Jl[6], const T Wr[4], const T W[4], const T r[2], T drho, T Hpp21[21],
+                      T Wpl[18], T v[6]) {
+    pose_hessian(Jp, Jl, Wr, Hpp21, Wpl);
+    pose_grad(Jp, W, r, drho, v);
 }
 
 // Point-vertex terms, factor_graph.hpp:680-689, 720, 725:
 //   H_ll = (Jl^T Omega') Jl (3x3),  v = ((drho Jl^T) Omega) r  (b_l -= v).
 template <class T>
 ZB_HD void point_terms(const T Jl[6], const T Wr[4], const T W[4], const T r[2], T drho, T Hll[9], T v[3]) {
-    T JlT[6], jtw[6];
-    transpose<2, 3>(Jl, JlT);
-    mul<3, 2, 2>(JlT, Wr, jtw);
-    mul<3, 2, 3>(jtw, Jl, Hll);
-    T dJ[6], t[6];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) dJ[i] = JlT[i] * drho;
-    mul<3, 2, 2>(dJ, W, t);
-    mul<3, 2, 1>(t, r, v);
+    point_hessian(Jl, Wr, Hll);
+    point_grad(Jl, W, r, drho, v);
 }
 
 // Closed-form 3x3 inverse with the singular-to-zero rule, matrix.hpp:749-774.

@@ -73,7 +73,7 @@ struct kernel_timer {
 
 enum kernel_class : int {
     K_LINEARISE = 0, K_POSE_REDUCE = 1, K_COST = 2, K_INVERT = 3, K_FORM_Y = 4, K_FILL = 5, K_SCHUR = 6,
-    K_CHOLESKY = 7, K_TRSV = 8, K_BACKSUB = 9, K_CONTROL = 10
+    K_CHOLESKY = 7, K_TRSV = 8, K_BACKSUB = 9, K_CONTROL = 10, K_REFINE = 11
 };
 
 // One W block (6x3 pose-point coupling) exists per distinct (pose, point) pair
@@ -114,6 +114,7 @@ struct dev_problem {
     // iteration order of the reference's hash-map containers.
     const int2* contrib_x[2] = {nullptr, nullptr};      // per block, ordered as the Y map
     const int32_t* pose_pair_start = nullptr;           // [np+1]
+    const int32_t* pose_pairs = nullptr;                // [npair] W blocks grouped by pose, ascending
     const int32_t* pose_pairs_x[2] = {nullptr, nullptr};// W blocks of a pose, ordered as the Y map
     const int32_t* lm_pairs_x[2] = {nullptr, nullptr};  // W blocks of a point, ordered as the h_lp map
     // vertices
@@ -163,18 +164,21 @@ struct dev_work64 {
     int32_t* chol_status = nullptr; // [0] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
     kernel_timer* timer = nullptr;  // host object; null = no timing
-    // binary32 buffers of the mixed path (same layouts as the binary64 ones)
-    float* pose_terms_f = nullptr;  // [no*27], pose-major
+    // mixed path: binary32 Hessian blocks, binary64 gradient
+    float* pose_terms_f = nullptr;  // [no*21] per-observation H_pp terms, pose-major
+    double* pose_grad = nullptr;    // [no*6] per-observation b_p terms, pose-major
     float* Hpp_f = nullptr;         // [np*21]
-    float* bp_f = nullptr;          // [np*6]
+    double* bp_m = nullptr;         // [np*6]
     float* Hll_f = nullptr;         // [nl*9]
-    float* bl_f = nullptr;          // [nl*3]
+    double* bl_m = nullptr;         // [nl*3]
     float* W_f = nullptr;           // [npair*18]
     float* Hinv_f = nullptr;        // [nl*9]
     float* Y_f = nullptr;           // [npair*18]
-    double* S_orig = nullptr;       // [n*n] reduced system before factorisation (refinement)
-    double* rvec = nullptr;         // [n] refinement residual
-    double* corr = nullptr;         // [n] refinement correction
+    // mixed-path refinement (pose_grad doubles as the per-observation slots)
+    double* zl = nullptr;           // [nl*3] point rows r_l of the binary64 residual
+    double* gl = nullptr;           // [nl*3] H_ll^-1 r_l
+    double* rvec = nullptr;         // [n] reduced residual r_p - W H_ll^-1 r_l
+    double* corr = nullptr;         // [n] pose correction
 };
 
 #define ZB_TIME_BEGIN(w, id, s) do { if ((w).timer) (w).timer->begin((id), (s)); } while (0)
@@ -200,18 +204,23 @@ void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out,
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s);
 void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
-// ---- mixed binary32 / binary64 kernels (kernels_fp32.hip) ----------------------
+// ---- mixed path: binary32 linearisation and Schur (kernels_fp32.hip) -----------
 void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
-void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override);
-void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, bool gated);
-void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool apply);
+// ---- mixed path: binary64 arithmetic over the binary32 blocks (kernels_fp64.hip)
+void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
+// dx_l for the solved dx_p, then `steps` rounds of refinement of (dx_p, dx_l)
+// against the binary64 normal equations; the factor of S must be in w.S.
+void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, const double* lambda_override);
+void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_backsub_only_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from
 // w.bs.  Sets *w.chol_status and lm->solve_ok; skips everything when the
 // controller is done (gated) or a previous stage failed.
-void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact = false);
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact = false,
+                             double floor_rel = 0.0);
 // Forward + backward substitution with the factor in w.S: x = S^-1 b.
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
 

@@ -231,8 +231,14 @@ struct zsba_graph {
     int32_t prepare_exact_orders();
     void order_set_from_mirror(std::vector<int2>& contrib, std::vector<int32_t>& pose_pairs,
                                std::vector<int32_t>& lm_pairs);
-    int32_t run_fp64(int32_t iterations, int32_t use_rel, bool exact);
+    // precision: ZSBA_PRECISION_FP64 (reference order), _FP64_FAST, _MIXED
+    int32_t run(int32_t iterations, int32_t use_rel, int32_t precision);
     void print_trace();
+    double mixed_floor() const {
+        const int e = opt.floor_exp == 0 ? kDefaultFloorExp : opt.floor_exp;
+        return e < 0 ? 0.0 : std::ldexp(1.0, -e);
+    }
+    static constexpr int kDefaultFloorExp = 24;
 };
 
 // ---- structure analysis -----------------------------------------------------
@@ -458,6 +464,13 @@ int32_t zsba_graph::build_structure() {
     w.cbuf = mem.get<double>(no);
     w.sbuf = mem.get<double>(6L * np + 3L * nl);
     w.Lt = mem.get<double>(static_cast<size_t>(n * n));
+    w.pose_terms_f = mem.get<float>(21L * no); w.pose_grad = mem.get<double>(6L * no);
+    w.Hpp_f = mem.get<float>(21L * np); w.bp_m = mem.get<double>(6L * np);
+    w.Hll_f = mem.get<float>(9L * nl); w.bl_m = mem.get<double>(3L * nl);
+    w.W_f = mem.get<float>(18L * npair); w.Hinv_f = mem.get<float>(9L * nl); w.Y_f = mem.get<float>(18L * npair);
+    w.zl = mem.get<double>(3L * nl); w.gl = mem.get<double>(3L * nl);
+    w.rvec = mem.get<double>(n); w.corr = mem.get<double>(n);
+    int32_t* d_pose_pairs = mem.get<int32_t>(npair);
     for (int set = 0; set < 2; ++set) {
         d_contrib_x[set] = mem.get<int2>(static_cast<size_t>(ncontrib));
         d_pose_pairs_x[set] = mem.get<int32_t>(npair);
@@ -488,6 +501,7 @@ int32_t zsba_graph::build_structure() {
     p.contrib = up(d_contrib, contrib); p.diag_blk = up(d_diag, diag_blk);
     p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
     p.pose_pair_start = up(d_pose_pair_start, pose_pair_start);
+    p.pose_pairs = up(d_pose_pairs, pose_pairs);
     p.obs_pose_pos = up(d_obs_pose_pos, obs_pose_pos);
     for (int set = 0; set < 2; ++set) {
         p.contrib_x[set] = d_contrib_x[set];
@@ -635,8 +649,10 @@ void zsba_graph::print_trace() {
                  lm_pinned->solve_failures, lm_pinned->initial_chi2, lm_pinned->chi2, lm_pinned->lambda);
 }
 
-int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel, bool exact) {
+int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) {
     hipError_t e;
+    const bool exact = precision == ZSBA_PRECISION_FP64;
+    const bool mixed = precision == ZSBA_PRECISION_MIXED;
     if (exact) {
         const int32_t rc = prepare_exact_orders();
         if (rc) return rc;
@@ -649,22 +665,36 @@ int32_t zsba_graph::run_fp64(int32_t iterations, int32_t use_rel, bool exact) {
     if ((e = hipMemcpyAsync(w.lm, lm_pinned, sizeof(lm_state), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload controller");
     zb::launch_cost64(dp, w, stream, 0, false, exact);
-    zb::launch_linearise64(dp, w, stream, false);
-    zb::launch_init_lm64(dp, w, stream);
+    if (mixed) {
+        zb::launch_linearise_mixed(dp, w, stream, false);
+        zb::launch_init_lm_mixed(dp, w, stream);
+    } else {
+        zb::launch_linearise64(dp, w, stream, false);
+        zb::launch_init_lm64(dp, w, stream);
+    }
     const int32_t max_attempts = iterations * 10;
     hipEvent_t ev[2];
     (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
     int32_t* done_pinned = &lm_pinned->done;
     for (int32_t a = 0; a < max_attempts; ++a) {
-        zb::launch_reduced_system64(dp, w, stream, nullptr, exact);
-        zb::launch_cholesky_solve64(dp.n, w, stream, true, exact);
-        zb::launch_after_solve64(w, stream);
-        zb::launch_backsub_update64(dp, w, stream, exact);
+        if (mixed) {
+            zb::launch_reduced_system_mixed(dp, w, stream, nullptr);
+            zb::launch_cholesky_solve64(dp.n, w, stream, true, false, mixed_floor());
+            zb::launch_refine_mixed(dp, w, stream, opt.refine_steps, nullptr);
+            zb::launch_after_solve64(w, stream);
+            zb::launch_backsub_update_mixed(dp, w, stream);
+        } else {
+            zb::launch_reduced_system64(dp, w, stream, nullptr, exact);
+            zb::launch_cholesky_solve64(dp.n, w, stream, true, exact);
+            zb::launch_after_solve64(w, stream);
+            zb::launch_backsub_update64(dp, w, stream, exact);
+        }
         zb::launch_cost64(dp, w, stream, 1, true, exact);
         zb::launch_decide64(w, stream);
         zb::launch_rollback64(dp, w, stream);
-        zb::launch_linearise64(dp, w, stream, true);
+        if (mixed) zb::launch_linearise_mixed(dp, w, stream, true);
+        else zb::launch_linearise64(dp, w, stream, true);
         zb::launch_end_attempt64(w, stream);
         (void)hipMemcpyAsync(done_pinned, &w.lm->done, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
         (void)hipEventRecord(ev[a & 1], stream);
@@ -710,7 +740,8 @@ zsba_graph* zsba_create(const zsba_options* options) {
     zsba_graph* g = new (std::nothrow) zsba_graph();
     if (!g) return nullptr;
     if (options) g->opt = *options;
-    if (g->opt.refine_steps <= 0) g->opt.refine_steps = 2;
+    if (g->opt.refine_steps == 0) g->opt.refine_steps = 2;
+    else if (g->opt.refine_steps < 0) g->opt.refine_steps = 0;
     return g;
 }
 
@@ -890,8 +921,7 @@ int32_t zsba_solve(zsba_graph* g, int32_t iterations, int32_t use_relative_conve
     g->stats.n_obs = g->dp.no;
     g->stats.n_poses = g->dp.np;
     g->stats.n_points = g->dp.nl;
-    if (precision == ZSBA_PRECISION_MIXED) return g->fail(ZSBA_EUNSUP, "mixed precision is not available yet");
-    return g->run_fp64(iterations, use_relative_convergence, precision == ZSBA_PRECISION_FP64);
+    return g->run(iterations, use_relative_convergence, precision);
 }
 
 double zsba_current_chi(zsba_graph* g, int32_t recompute) {
@@ -968,7 +998,9 @@ int32_t zsba_debug_eval_edges(zsba_graph* g, double* out) {
 int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precision, double* S, double* b_s, double* dx_pose,
                                   double* dx_point) {
     if (!g) return ZSBA_EINVAL;
-    if (precision != ZSBA_PRECISION_FP64) return g->fail(ZSBA_EUNSUP, "debug system: precision %d", precision);
+    if (precision != ZSBA_PRECISION_FP64 && precision != ZSBA_PRECISION_MIXED)
+        return g->fail(ZSBA_EUNSUP, "debug system: precision %d", precision);
+    const bool mixed = precision == ZSBA_PRECISION_MIXED;
     if (g->live_obs() == 0) return g->fail(ZSBA_ESTATE, "no observations");
     int32_t rc = g->prepare();
     if (rc) return rc;
@@ -979,15 +1011,25 @@ int32_t zsba_debug_reduced_system(zsba_graph* g, double lambda, int32_t precisio
     if ((e = g->copy(g->w.lm, &st, sizeof st, hipMemcpyHostToDevice)) != hipSuccess) return g->hip_fail(e, "lm");
     if ((e = g->copy(g->lambda_dev, &lambda, sizeof lambda, hipMemcpyHostToDevice)) != hipSuccess)
         return g->hip_fail(e, "lambda");
-    zb::launch_linearise64(g->dp, g->w, g->stream, false);
-    zb::launch_reduced_system64(g->dp, g->w, g->stream, g->lambda_dev);
+    if (mixed) {
+        zb::launch_linearise_mixed(g->dp, g->w, g->stream, false);
+        zb::launch_reduced_system_mixed(g->dp, g->w, g->stream, g->lambda_dev);
+    } else {
+        zb::launch_linearise64(g->dp, g->w, g->stream, false);
+        zb::launch_reduced_system64(g->dp, g->w, g->stream, g->lambda_dev);
+    }
     const int64_t n = g->dp.n;
     if (S && (e = hipMemcpyAsync(S, g->w.S, n * n * sizeof(double), hipMemcpyDeviceToHost, g->stream)) != hipSuccess)
         return g->hip_fail(e, "S");
     if (b_s && (e = hipMemcpyAsync(b_s, g->w.bs, n * sizeof(double), hipMemcpyDeviceToHost, g->stream)) != hipSuccess)
         return g->hip_fail(e, "b_s");
-    zb::launch_cholesky_solve64(g->dp.n, g->w, g->stream, false);
-    zb::launch_backsub_only64(g->dp, g->w, g->stream);
+    zb::launch_cholesky_solve64(g->dp.n, g->w, g->stream, false, false, mixed ? g->mixed_floor() : 0.0);
+    if (mixed) {
+        zb::launch_refine_mixed(g->dp, g->w, g->stream, g->opt.refine_steps, g->lambda_dev);
+        zb::launch_backsub_only_mixed(g->dp, g->w, g->stream);
+    } else {
+        zb::launch_backsub_only64(g->dp, g->w, g->stream);
+    }
     if ((e = hipStreamSynchronize(g->stream)) != hipSuccess) return g->hip_fail(e, "reduced system");
     lm_state out{};
     if ((e = g->copy(&out, g->w.lm, sizeof out, hipMemcpyDeviceToHost)) != hipSuccess) return g->hip_fail(e, "lm");

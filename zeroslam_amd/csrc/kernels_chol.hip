@@ -32,8 +32,9 @@ __device__ __forceinline__ bool skip(const lm_state* lm, int gated, const int32_
 }
 
 // eps = 1e-14 * max|S_ii| with math::max semantics (NaN entries never win).
-__global__ void __launch_bounds__(1024) k_chol_begin(int n, const double* S, double* eps, int32_t* status,
-                                                     const lm_state* lm, int gated) {
+// floor_rel > 0 first raises every diagonal entry by floor_rel * max|S_ii|.
+__global__ void __launch_bounds__(1024) k_chol_begin(int n, double* S, double* eps, int32_t* status,
+                                                     const lm_state* lm, int gated, double floor_rel) {
     if (gated && lm->done) return;
     __shared__ double sh[1024];
     double m = 0;
@@ -50,8 +51,13 @@ __global__ void __launch_bounds__(1024) k_chol_begin(int n, const double* S, dou
         }
         __syncthreads();
     }
+    const double mx = sh[0];
+    const double lift = floor_rel * mx;
+    if (floor_rel > 0.0) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) S[static_cast<int64_t>(i) * n + i] += lift;
+    }
     if (threadIdx.x == 0) {
-        eps[0] = sh[0] * 1e-14;
+        eps[0] = (mx + lift) * 1e-14;
         status[0] = 0;
     }
 }
@@ -175,20 +181,17 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
             panel_update(own, LI, LJ, nr, nk, false);
         }
         __syncthreads();
-        // Row solve against L_kk^T, column by column: element c receives
-        // -= x_m L_cm for m = 0 .. c-1 in order and then / L_cc, the reference's
-        // sequence, while the updates of different c are independent.
         if (threadIdx.x < nr) {
             double x[NB];
 #pragma unroll
             for (int c = 0; c < NB; ++c) x[c] = own[threadIdx.x][c];
 #pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                if (m < nk) {
-                    x[m] = x[m] / D[m][m];
+            for (int c = 0; c < NB; ++c) {
+                if (c < nk) {
+                    double v = x[c];
 #pragma unroll
-                    for (int c = m + 1; c < NB; ++c)
-                        if (c < nk) x[c] = x[c] - x[m] * D[c][m];
+                    for (int m = 0; m < c; ++m) v = v - x[m] * D[c][m];
+                    x[c] = v / D[c][c];
                 }
             }
 #pragma unroll
@@ -311,32 +314,19 @@ __global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L,
 // (matrix_decomposition_cholesky.hpp:85-93): x_i -= L_ji x_j for j = i+1 .. n-1
 // ascending, then x_i /= L_ii.  One wave with x in LDS: while lane 0 runs the
 // ordered subtractions for row i, the wave has already staged row i-1 of L^T.
-constexpr int kBackLds = 4096;  // largest n handled from LDS
+constexpr int kBackLds = 6144;  // largest n handled from LDS
 __global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* L, const double* Lt, double* x,
                                                            const int32_t* status, lm_state* lm, int gated) {
     if (skip(lm, gated, status)) return;
     __shared__ double xs[kBackLds];
     __shared__ double prod[kBackLds];
-    __shared__ double urow[kBackLds];
     const int lane = threadIdx.x;
-    constexpr int kPer = kBackLds / 64;
     for (int j = lane; j < n; j += 64) xs[j] = x[j];
-    {
-        const double* u = Lt + static_cast<int64_t>(n - 1) * n;
-        for (int j = n + lane; j < n; j += 64) urow[j] = u[j];
-    }
     __syncthreads();
     for (int i = n - 1; i >= 0; --i) {
         // products L_ji x_j (x_j final for j > i), formed in parallel
-        for (int j = i + 1 + lane; j < n; j += 64) prod[j] = urow[j] * xs[j];
-        // issue the loads of the next row of L^T; they complete during the chain below
-        double nxt[kPer];
-        const double* un = Lt + static_cast<int64_t>(i > 0 ? i - 1 : 0) * n;
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int j = i + q * 64 + lane;
-            nxt[q] = (i > 0 && j < n) ? un[j] : 0.0;
-        }
+        const double* u = Lt + static_cast<int64_t>(i) * n;
+        for (int j = i + 1 + lane; j < n; j += 64) prod[j] = u[j] * xs[j];
         __syncthreads();
         if (lane == 0) {
             double v = xs[i];
@@ -347,11 +337,6 @@ __global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* 
             }
             for (; j < n; ++j) v = v - prod[j];
             xs[i] = v / L[static_cast<int64_t>(i) * n + i];
-        }
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int j = i + q * 64 + lane;
-            if (i > 0 && j < n) urow[j] = nxt[q];
         }
         __syncthreads();
     }
@@ -491,14 +476,14 @@ __global__ void k_empty_ok(lm_state* lm, int gated) {
 
 }  // namespace
 
-void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact) {
+void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool gated, bool exact, double floor_rel) {
     const int g = gated ? 1 : 0;
     if (n <= 0) {
         k_empty_ok<<<1, 1, 0, s>>>(w.lm, g);
         return;
     }
     ZB_TIME_BEGIN(w, K_CHOLESKY, s);
-    k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g);
+    k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g, floor_rel);
     const int64_t tiles = (n + NB - 1) / NB;
     for (int64_t k = 0; k < tiles; ++k) {
         const int64_t m = tiles - k - 1;

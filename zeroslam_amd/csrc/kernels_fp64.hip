@@ -299,7 +299,8 @@ __device__ max_scan scan_combine(max_scan a, max_scan b) {
     return r;
 }
 
-__global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const double* Hpp, const double* Hll, lm_state* lm) {
+template <class TS>
+__global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const TS* Hpp, const TS* Hll, lm_state* lm) {
     __shared__ max_scan sh[1024];
     const int64_t total = 6L * p.np + 3L * p.nl;
     const int64_t per = (total + blockDim.x - 1) / blockDim.x;
@@ -309,10 +310,10 @@ __global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const double* H
         double h;
         if (i < 6L * p.np) {
             const int64_t pose = i / 6, k = i % 6;
-            h = Hpp[21 * pose + k * (k + 1) / 2 + k];
+            h = static_cast<double>(Hpp[21 * pose + k * (k + 1) / 2 + k]);
         } else {
             const int64_t j = i - 6L * p.np;
-            h = Hll[9 * (j / 3) + 4 * (j % 3)];
+            h = static_cast<double>(Hll[9 * (j / 3) + 4 * (j % 3)]);
         }
         const double a = fabs(h);
         if (isnan(a)) {
@@ -614,11 +615,14 @@ __global__ void k_after_solve(lm_state* lm) {
 // dx_l = H_ll^-1 (b_l - sum W^T dx_p) (factor_graph.hpp:799-800) and update
 // points (factor_graph.hpp:809-830).  Each thread also adds its slots to the LM
 // scale sum_i dx_i (lambda dx_i + b_i) (factor_graph.hpp:556-559).
+// TS is the storage type of the W and H_ll^-1 blocks (binary32 on the mixed
+// path); the gradient and the arithmetic are binary64 either way.
+template <class TS>
 __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double* pose, double* pose_backup, double* point,
-                                                           double* point_backup, const double* Wbuf, const double* Hinv,
+                                                           double* point_backup, const TS* Wbuf, const TS* Hinv,
                                                            const double* dxp, double* dxl, const double* bp,
                                                            const double* bl, double* partial, const lm_state* lm,
-                                                           int gated, int apply, double* sterms) {
+                                                           int gated, int apply, double* sterms, int given_dxl = 0) {
     if (gated && (lm->done || !lm->step_applied)) return;
     __shared__ double sh[kBlock];
     const double lam = lm->lambda;
@@ -646,13 +650,17 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
         }
     } else if (t < p.np + p.nl) {
         const int l = t - p.np;
+        double dl[3];
+        if (given_dxl) {
+            load3(dxl + 3L * l, dl);
+        } else {
         double tt[3] = {0.0, 0.0, 0.0};
         const int32_t* order = sterms ? p.lm_pairs_x[order_set(lm)] : nullptr;
         for (int t2 = p.lm_pair_start[l]; t2 < p.lm_pair_start[l + 1]; ++t2) {
             const int a = order ? order[t2] : t2;
             double Wa[18], lp[18], d[6], prod[3];
 #pragma unroll
-            for (int i = 0; i < 18; ++i) Wa[i] = Wbuf[18L * a + i];
+            for (int i = 0; i < 18; ++i) Wa[i] = static_cast<double>(Wbuf[18L * a + i]);
             transpose<6, 3>(Wa, lp);
             const int k = p.pair_pose[a];
 #pragma unroll
@@ -661,19 +669,21 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
 #pragma unroll
             for (int i = 0; i < 3; ++i) tt[i] = tt[i] + prod[i];
         }
-        double v[3], Hi[9], prod[3], dl[3];
+        double v[3], Hi[9], prod[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) v[i] = bl[3L * l + i] - tt[i];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) Hi[i] = Hinv[9L * l + i];
+        for (int i = 0; i < 9; ++i) Hi[i] = static_cast<double>(Hinv[9L * l + i]);
         mul<3, 3, 1>(Hi, v, prod);
 #pragma unroll
         for (int i = 0; i < 3; ++i) dl[i] = 0.0 + prod[i];
 #pragma unroll
         for (int i = 0; i < 3; ++i) dxl[3L * l + i] = dl[i];
+        }
         if (sterms) {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) sterms[6L * p.np + 3L * l + i] = dl[i] * (lam * dl[i] + bl[3L * l + i]);
+            for (int i = 0; i < 3; ++i)
+                sterms[6L * p.np + 3L * l + i] = dl[i] * (lam * dl[i] + bl[3L * l + i]);
         } else {
 #pragma unroll
             for (int i = 0; i < 3; ++i) acc = acc + dl[i] * (lam * dl[i] + bl[3L * l + i]);
@@ -690,6 +700,195 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
     if (partial && !sterms) {
         const double b = block_sum(acc, sh);
         if (threadIdx.x == 0) partial[blockIdx.x] = b;
+    }
+}
+
+// ---- iterative refinement of the mixed solve in binary64 ------------------------
+// The mixed path factorises a reduced system formed from binary32 blocks.  Its
+// step (dx_p, dx_l) is refined against the binary64 normal equations of the
+// current linearisation, applied without forming them: per observation the
+// Jacobians and robust weight are recomputed in binary64 (the linearisation's
+// sequence) and
+//   r_p,i = b_p,i - lambda dx_p,i - sum_obs J_p^T Omega' (J_p dx_p,i + J_l dx_l,l)
+//   r_l,l = b_l,l - lambda dx_l,l - sum_obs J_l^T Omega' (J_p dx_p,i + J_l dx_l,l)
+// (rows of fixed vertices are zero).  The correction solves the binary32-block
+// system for (r_p, r_l) by the same elimination:  rs = r_p - W H_ll^-1 r_l,
+// delta_p = S^-1 rs (the Cholesky factor of S), delta_l = H_ll^-1 (r_l - W^T delta_p).
+__device__ __forceinline__ bool refine_skip(const lm_state* lm, int gated, const int32_t* status) {
+    return (gated && lm->done) || *status != 0;
+}
+
+// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p(a)) for the solved dx_p.
+template <class TS>
+__global__ void __launch_bounds__(kBlock) k_point_step(dev_problem p, const TS* Wbuf, const TS* Hinv, const double* bl,
+                                                       const double* dxp, double* dxl, const lm_state* lm, int gated,
+                                                       const int32_t* status) {
+    if (refine_skip(lm, gated, status)) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p.nl) return;
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+        const int k = p.pair_pose[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * dxp[6L * k + r];
+            t[c] += v;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double o = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (bl[3L * l + c] - t[c]);
+        dxl[3L * l + r] = o;
+    }
+}
+
+// Per point: r_l and g_l = H_ll^-1 r_l; the pose-side products J_p^T Omega' u of
+// each observation go to its pose-major slot.
+template <class TS>
+__global__ void __launch_bounds__(kBlock) k_full_residual(dev_problem p, const double* pose, const double* point,
+                                                          const double* dxp, const double* dxl, const double* bl,
+                                                          const TS* Hinv, double* pslots, double* rl, double* gl,
+                                                          const lm_state* lm, const double* lambda_override,
+                                                          const int32_t* status) {
+    if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p.nl) return;
+    const bool lfixed = p.point_fixed[l] != 0;
+    const double lam = lambda_override ? *lambda_override : lm->lambda;
+    double X[3], dl[3];
+    load3(point + 3L * l, X);
+    load3(dxl + 3L * l, dl);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = p.lm_start[l]; s < p.lm_start[l + 1]; ++s) {
+        const int ip = p.obs_pose[s];
+        const bool pfixed = p.pose_fixed[ip] != 0;
+        if (pfixed && lfixed) continue;
+        const int32_t param = p.obs_param[s];
+        const double* K = cam_of(p, param);
+        double P[7], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+        load7(pose + 7L * ip, P);
+        pose_apply(P, X, Pc);
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        residual(Pc, uv, K, r);
+        pose_quat(P, q);
+        jacobians(Pc, q, K, Jp, Jl);
+        info_of(p, s, W);
+        int32_t kind;
+        double delta;
+        loss_of(p, param, kind, delta);
+        robust_info(kind, delta, W, r, drho, Wr);
+        double u[2] = {0.0, 0.0};
+        if (!pfixed) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) u[m] += Jp[6 * m + k] * dxp[6L * ip + k];
+        }
+        if (!lfixed) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) u[m] += Jl[3 * m + k] * dl[k];
+        }
+        const double wu[2] = {Wr[0] * u[0] + Wr[1] * u[1], Wr[2] * u[0] + Wr[3] * u[1]};
+        if (!pfixed) {
+            double* out = pslots + 6L * p.obs_pose_pos[s];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) out[k] = Jp[k] * wu[0] + Jp[6 + k] * wu[1];
+        }
+        if (!lfixed) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += Jl[k] * wu[0] + Jl[3 + k] * wu[1];
+        }
+    }
+    double rr[3] = {0.0, 0.0, 0.0};
+    if (!lfixed) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rr[k] = bl[3L * l + k] - lam * dl[k] - acc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rl[3L * l + k] = rr[k];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double o = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * rr[c];
+        gl[3L * l + r] = o;
+    }
+}
+
+// One wave per pose: rs_i = r_p,i - sum_{a of pose i} W_a g_l(a).
+template <class TS>
+__global__ void __launch_bounds__(256) k_pose_residual(dev_problem p, const double* pslots, const double* bp,
+                                                       const double* dxp, const TS* Wbuf, const double* gl, double* rs,
+                                                       const lm_state* lm, const double* lambda_override,
+                                                       const int32_t* status) {
+    if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (i >= p.np) return;
+    const bool fixed = p.pose_fixed[i] != 0;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (!fixed) {
+        for (int t = p.pose_start[i] + lane; t < p.pose_start[i + 1]; t += 64) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] += pslots[6L * t + k];
+        }
+        for (int t = p.pose_pair_start[i] + lane; t < p.pose_pair_start[i + 1]; t += 64) {
+            const int a = p.pose_pairs[t];
+            const int l = p.pair_point[a];
+            const double g0 = gl[3L * l], g1 = gl[3L * l + 1], g2 = gl[3L * l + 2];
+            const TS* Wa = Wbuf + 18L * a;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                acc[r] += static_cast<double>(Wa[3 * r]) * g0 + static_cast<double>(Wa[3 * r + 1]) * g1 +
+                          static_cast<double>(Wa[3 * r + 2]) * g2;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane < 6) {
+        const double lam = lambda_override ? *lambda_override : lm->lambda;
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a = (q == lane) ? acc[q] : a;
+        rs[6L * i + lane] = fixed ? 0.0 : bp[6L * i + lane] - lam * dxp[6L * i + lane] - a;
+    }
+}
+
+// dx_l += H_ll^-1 (r_l - sum_a W_a^T delta_p(a)) per point; dx_p += delta_p.
+template <class TS>
+__global__ void __launch_bounds__(kBlock) k_refine_apply(dev_problem p, const TS* Wbuf, const TS* Hinv, const double* rl,
+                                                         const double* corr, double* dxp, double* dxl, const lm_state* lm,
+                                                         int gated, const int32_t* status) {
+    if (refine_skip(lm, gated, status)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < p.nl) {
+        const int l = t;
+        double tt[3] = {0.0, 0.0, 0.0};
+        for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+            const int k = p.pair_pose[a];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                double v = 0.0;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * corr[6L * k + r];
+                tt[c] += v;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double o = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (rl[3L * l + c] - tt[c]);
+            dxl[3L * l + r] += o;
+        }
+    } else if (t < p.nl + p.n) {
+        dxp[t - p.nl] += corr[t - p.nl];
     }
 }
 
@@ -850,7 +1049,39 @@ void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s
 }
 
 void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
-    k_init_lm<<<1, 1024, 0, s>>>(p, w.Hpp, w.Hll, w.lm);
+    k_init_lm<double><<<1, 1024, 0, s>>>(p, w.Hpp, w.Hll, w.lm);
+}
+
+void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    k_init_lm<float><<<1, 1024, 0, s>>>(p, w.Hpp_f, w.Hll_f, w.lm);
+}
+
+void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, const double* lambda_override) {
+    if (p.n <= 0 && p.nl <= 0) return;
+    const int gated = lambda_override ? 0 : 1;
+    ZB_TIME_BEGIN(w, K_REFINE, s);
+    if (p.nl > 0)
+        k_point_step<float><<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.W_f, w.Hinv_f, w.bl_m, w.dxp, w.dxl, w.lm, gated,
+                                                                      w.chol_status);
+    ZB_TIME_END(w, s);
+    for (int k = 0; k < steps; ++k) {
+        ZB_TIME_BEGIN(w, K_REFINE, s);
+        if (p.nl > 0)
+            k_full_residual<float><<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.dxp, w.dxl, w.bl_m,
+                                                                             w.Hinv_f, w.pose_grad, w.zl, w.gl, w.lm,
+                                                                             lambda_override, w.chol_status);
+        if (p.np > 0)
+            k_pose_residual<float><<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_grad, w.bp_m, w.dxp, w.W_f, w.gl,
+                                                                             w.rvec, w.lm, lambda_override, w.chol_status);
+        ZB_TIME_END(w, s);
+        ZB_TIME_BEGIN(w, K_TRSV, s);
+        launch_cholesky_resolve64(p.n, w, w.rvec, w.corr, s, gated != 0);
+        ZB_TIME_END(w, s);
+        ZB_TIME_BEGIN(w, K_REFINE, s);
+        k_refine_apply<float><<<grid_for(static_cast<int64_t>(p.nl) + p.n, kBlock), kBlock, 0, s>>>(
+            p, w.W_f, w.Hinv_f, w.zl, w.corr, w.dxp, w.dxl, w.lm, gated, w.chol_status);
+        ZB_TIME_END(w, s);
+    }
 }
 
 void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override,
@@ -896,8 +1127,8 @@ void launch_backsub_update64(const dev_problem& p, const dev_work64& w, hipStrea
     if (g < 1) g = 1;
     double* partial = w.partial + 2 * kPartials;
     ZB_TIME_BEGIN(w, K_BACKSUB, s);
-    k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
-                                          w.bp, w.bl, partial, w.lm, 1, 1, exact ? w.sbuf : nullptr);
+    k_backsub_update<double><<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp,
+                                                  w.dxl, w.bp, w.bl, partial, w.lm, 1, 1, exact ? w.sbuf : nullptr);
     if (exact) k_seqsum<<<1, 64, 0, s>>>(w.sbuf, 6L * p.np + 3L * p.nl, w.lm, 2, 2, nullptr);
     else k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
     ZB_TIME_END(w, s);
@@ -907,8 +1138,28 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
     int g = grid_for(p.np + p.nl, kBlock);
     if (g > kPartials) g = kPartials;
     if (g < 1) g = 1;
-    k_backsub_update<<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp, w.dxl,
-                                          w.bp, w.bl, nullptr, w.lm, 0, 0, nullptr);
+    k_backsub_update<double><<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp,
+                                                  w.dxl, w.bp, w.bl, nullptr, w.lm, 0, 0, nullptr);
+}
+
+void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    int g = grid_for(p.np + p.nl, kBlock);
+    if (g > kPartials) g = kPartials;
+    if (g < 1) g = 1;
+    double* partial = w.partial + 2 * kPartials;
+    ZB_TIME_BEGIN(w, K_BACKSUB, s);
+    k_backsub_update<float><<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W_f, w.Hinv_f, w.dxp,
+                                                 w.dxl, w.bp_m, w.bl_m, partial, w.lm, 1, 1, nullptr, 1);
+    k_sum_partials<<<1, 1024, 0, s>>>(partial, g, nullptr, w.lm, 2, 2);
+    ZB_TIME_END(w, s);
+}
+
+void launch_backsub_only_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    int g = grid_for(p.np + p.nl, kBlock);
+    if (g > kPartials) g = kPartials;
+    if (g < 1) g = 1;
+    k_backsub_update<float><<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W_f, w.Hinv_f, w.dxp,
+                                                 w.dxl, w.bp_m, w.bl_m, nullptr, w.lm, 0, 0, nullptr, 1);
 }
 
 void launch_decide64(const dev_work64& w, hipStream_t s) {
