@@ -75,7 +75,10 @@ typedef struct zsba_options {
                              with its diagonal raised by 2^-floor_exp max|S_ii| (its
                              rounding floor); the binary64 refinement uses the true
                              damping.  0 = default, negative = no floor              */
-    int32_t reserved[3];
+    int32_t factor_path;  /* reduced-system factorisation: 0 = automatic, 1 = one
+                             workgroup (envelope panels up to 16 tiles of 32 rows),
+                             2 = one launch per 32-column panel                     */
+    int32_t reserved[2];
 } zsba_options;
 
 /* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */

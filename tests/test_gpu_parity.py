@@ -59,8 +59,8 @@ def test_reduced_system_matches_oracle():
     assert rel_err(xl, xl_o) < 1e-9
 
 
-def exact_compare(pr, iters, use_rel=False):
-    g = Graph.from_problem(pr)
+def exact_compare(pr, iters, use_rel=False, **graph_kw):
+    g = Graph.from_problem(pr, **graph_kw)
     acc = g.solve(iters, use_rel, FP64)
     st = g.stats()
     o = CpuGraph(pr, "oracle")
@@ -109,6 +109,23 @@ def test_exact_solve_matches_oracle(cfg, kw, iters):
     r_o = o.edge_eval()
     flags_o = (r_o[:, 25] <= pr.obs_delta ** 2) | ~np.isin(pr.obs_loss, [2, 4])
     assert np.array_equal(flags.astype(bool), flags_o)
+
+
+@pytest.mark.parametrize("path", [1, 2])
+@pytest.mark.parametrize("cfg,kw,iters", [(1, dict(n_frames=20, n_points=1500), 12), (3, dict(n_frames=12, n_points=800), 8)])
+def test_exact_solve_both_factor_paths(cfg, kw, iters, path):
+    """The one-workgroup factorisation and the per-panel launches (both skipping
+    the tiles outside the envelope) give the reference's factor bit for bit."""
+    pr = synth(cfg, **kw)
+    print(f"factor path {path} cfg={cfg} {kw}", flush=True)
+    exact_compare(pr, iters, factor_path=path)
+
+
+def test_exact_loop_closure_dense_envelope():
+    """A loop with revisits: the envelope is dense and the per-panel launches run."""
+    pr = synth(4, n_frames=120, n_points=6000)
+    print(f"exact solve loop closure obs={pr.n_obs}", flush=True)
+    exact_compare(pr, 5)
 
 
 def test_exact_full_config1():

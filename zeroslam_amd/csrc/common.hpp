@@ -174,6 +174,12 @@ struct dev_work64 {
     float* W_f = nullptr;           // [npair*18]
     float* Hinv_f = nullptr;        // [nl*9]
     float* Y_f = nullptr;           // [npair*18]
+    // envelope of the reduced system in 32 x 32 tiles: row tile I is zero left of
+    // tile column tile_first[I]; column tile k is zero below row tile tile_last[k]
+    const int32_t* tile_first = nullptr;  // [T]
+    const int32_t* tile_last = nullptr;   // [T]
+    int32_t panel_tiles = 0;              // max over k of tile_last[k] - k + 1
+    int32_t factor_path = 0;              // 0 automatic, 1 one workgroup, 2 panel launches
     // mixed-path refinement (pose_grad doubles as the per-observation slots)
     double* zl = nullptr;           // [nl*3] point rows r_l of the binary64 residual
     double* gl = nullptr;           // [nl*3] H_ll^-1 r_l

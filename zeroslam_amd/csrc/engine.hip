@@ -12,6 +12,7 @@
 #include "ba_math.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -380,6 +381,26 @@ int32_t zsba_graph::build_structure() {
         if (blk_row[r] == blk_col[r]) diag_blk[blk_row[r]] = r;
     }
     const int64_t ncontrib = blk_start[nblk];
+    // envelope: pose i couples to no pose left of first_pose[i]; Cholesky fill
+    // stays inside it, so tiles left of it stay exactly zero
+    const int32_t n_tiles = static_cast<int32_t>((6L * np + 31) / 32);
+    std::vector<int32_t> tile_first(n_tiles), tile_last(n_tiles);
+    int32_t panel_tiles = 0;
+    {
+        std::vector<int32_t> first_pose(np);
+        for (int32_t i = 0; i < np; ++i) first_pose[i] = i;
+        for (int32_t r = 0; r < nblk; ++r) first_pose[blk_row[r]] = std::min(first_pose[blk_row[r]], blk_col[r]);
+        for (int32_t t = 0; t < n_tiles; ++t) {
+            int64_t f = INT64_MAX;
+            for (int64_t r = 32L * t; r < std::min<int64_t>(32L * (t + 1), 6L * np); ++r)
+                f = std::min<int64_t>(f, 6L * first_pose[r / 6]);
+            tile_first[t] = static_cast<int32_t>(f / 32);
+            tile_last[t] = t;
+        }
+        for (int32_t t = 0; t < n_tiles; ++t)
+            for (int32_t k = tile_first[t]; k < t; ++k) tile_last[k] = std::max(tile_last[k], t);
+        for (int32_t k = 0; k < n_tiles; ++k) panel_tiles = std::max(panel_tiles, tile_last[k] - k + 1);
+    }
     std::vector<int2> contrib(static_cast<size_t>(ncontrib));
     {
         std::vector<int64_t> f(blk_start.begin(), blk_start.end() - 1);
@@ -471,6 +492,8 @@ int32_t zsba_graph::build_structure() {
     w.zl = mem.get<double>(3L * nl); w.gl = mem.get<double>(3L * nl);
     w.rvec = mem.get<double>(n); w.corr = mem.get<double>(n);
     int32_t* d_pose_pairs = mem.get<int32_t>(npair);
+    int32_t* d_tile_first = mem.get<int32_t>(n_tiles);
+    int32_t* d_tile_last = mem.get<int32_t>(n_tiles);
     for (int set = 0; set < 2; ++set) {
         d_contrib_x[set] = mem.get<int2>(static_cast<size_t>(ncontrib));
         d_pose_pairs_x[set] = mem.get<int32_t>(npair);
@@ -502,6 +525,10 @@ int32_t zsba_graph::build_structure() {
     p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
     p.pose_pair_start = up(d_pose_pair_start, pose_pair_start);
     p.pose_pairs = up(d_pose_pairs, pose_pairs);
+    w.tile_first = up(d_tile_first, tile_first);
+    w.tile_last = up(d_tile_last, tile_last);
+    w.panel_tiles = panel_tiles;
+    w.factor_path = opt.factor_path;
     p.obs_pose_pos = up(d_obs_pose_pos, obs_pose_pos);
     for (int set = 0; set < 2; ++set) {
         p.contrib_x[set] = d_contrib_x[set];
@@ -740,6 +767,7 @@ zsba_graph* zsba_create(const zsba_options* options) {
     zsba_graph* g = new (std::nothrow) zsba_graph();
     if (!g) return nullptr;
     if (options) g->opt = *options;
+    if (g->opt.factor_path < 0 || g->opt.factor_path > 2) g->opt.factor_path = 0;
     if (g->opt.refine_steps == 0) g->opt.refine_steps = 2;
     else if (g->opt.refine_steps < 0) g->opt.refine_steps = 0;
     return g;

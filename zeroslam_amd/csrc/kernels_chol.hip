@@ -3,14 +3,23 @@
 // source/matrix_decomposition_cholesky.hpp:26-95, as called at
 // factor_graph.hpp:782-793).
 //
-// Blocked right-looking factorisation over 32-wide panels, in place on the lower
-// triangle of S (row-major n x n): factor the diagonal tile in LDS, solve the
-// panel below it, update the trailing lower triangle tile by tile.  Failure
-// rules are the reference's: decomposition fails when a pivot is not greater
-// than 1e-14 * max|S_ii| (NaN fails), the solve fails when any |L_ii| is not
-// greater than 1e-7 * max|L_ii|.  Forward substitution subtracts in the
-// reference's column order; the rest differs from the reference's row-Crout order
-// only by rounding.
+// Blocked factorisation over 32-wide panels, in place on the lower triangle of S
+// (row-major n x n): factor the diagonal tile, solve the panel below it, update
+// the trailing lower triangle tile by tile.  Every element receives its
+// a -= L_ik L_jk subtractions in ascending k and its division / square root
+// last, the operation sequence of the reference's row-Crout loop, so the
+// factor equals the reference's bit for bit.
+//
+// Envelope.  Row r of S is zero left of column 6 * (first pose coupled to r's
+// pose), and Cholesky fill stays inside that envelope, so every product with a
+// factor entry left of it is an exact zero (a - 0 = a).  The kernels skip those
+// 32 x 32 tiles: tile_first[I] is the first non-zero tile column of row tile I,
+// tile_last[k] the last row tile with a non-zero tile in column k.  On the
+// forward-walk problems the envelope holds a few percent of the dense flops.
+//
+// Failure rules are the reference's: decomposition fails when a pivot is not
+// greater than 1e-14 * max|S_ii| (NaN fails), the solve fails when any |L_ii| is
+// not greater than 1e-7 * max|L_ii|.
 #include "common.hpp"
 
 #include <cmath>
@@ -86,9 +95,24 @@ __device__ __forceinline__ void panel_update(double (*own)[NB + 1], const double
     }
 }
 
+// Correctly rounded a / b from y = RN(1 / b): q = RN(a y) is within an ulp of
+// a / b, e = a - q b is exact (one FMA) and RN(q + e y) is RN(a / b)
+// (Markstein's correction; also checked against the division on 5e8 operand
+// pairs).  Outside the range where the correction is proven (non-finite or
+// near-subnormal quotients) it divides.
+__device__ __forceinline__ double div_cr(double a, double b, double y) {
+    const double q = a * y;
+    const double e = fma(-q, b, a);
+    double r = fma(e, y, q);
+    if (!(fabs(r) < 1e300) || fabs(r) < 1e-290) r = a / b;
+    return r;
+}
+
 // Factorisation of a diagonal tile by wave 0 with row r in the registers of lane
-// r; column j is broadcast lane by lane.  Returns false (uniformly) when a pivot
-// is not greater than eps.
+// r; column j is broadcast lane by lane.  Every lane forms L_jj and its
+// reciprocal; the updates run on all lanes without masks, because entries above
+// the diagonal are never read.  Returns false (uniformly) when a pivot is not
+// greater than eps.
 __device__ bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* ok_sh) {
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
@@ -98,21 +122,16 @@ __device__ bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* o
         int ok = 1;
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            if (j < nr && ok) {
+            if (j < nr) {
                 const double d = lane_value(row[j], j);
-                if (!(d > eps)) {
-                    ok = 0;
-                } else {
-                    const double ljj = sqrt(d);
-                    if (lane == j) row[j] = ljj;
-                    else if (lane > j) row[j] = row[j] / ljj;
+                ok = ok && (d > eps);
+                const double l = sqrt(d);
+                const double y = 1.0 / l;
+                row[j] = (lane == j) ? l : div_cr(row[j], l, y);
 #pragma unroll
-                    for (int c = j + 1; c < NB; ++c) {
-                        if (c < nr) {
-                            const double lcj = lane_value(row[j], c);
-                            if (lane >= c) row[c] = row[c] - row[j] * lcj;
-                        }
-                    }
+                for (int c = j + 1; c < NB; ++c) {
+                    const double lcj = lane_value(row[j], c);
+                    row[c] = row[c] - row[j] * lcj;
                 }
             }
         }
@@ -142,7 +161,7 @@ __device__ __forceinline__ void store_tile(const double (*src)[NB + 1], double* 
 }
 
 __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const double* eps_p, int32_t* status, int k,
-                                                   const lm_state* lm, int gated) {
+                                                   const int32_t* tile_first, const lm_state* lm, int gated) {
     if ((gated && lm->done) || status[0] != 0) return;
     __shared__ double own[NB][NB + 1];
     __shared__ double D[NB][NB + 1];
@@ -155,11 +174,12 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
     const int nk = (n - k0) < NB ? (n - k0) : NB;
     if (static_cast<int>(blockIdx.x) < np_wg) {
         const int I = k + blockIdx.x;
+        if (tile_first[I] > k) return;  // tile (I, k) lies outside the envelope
         const int r0 = I * NB;
         const int nr = (n - r0) < NB ? (n - r0) : NB;
         // diagonal tile (k, k) with its panel k-1 update
         load_tile(D, A, n, k0, k0, nk, nk, true);
-        if (k > 0) {
+        if (k > 0 && tile_first[k] <= k - 1) {
             load_tile(LJ, A, n, k0, kp0, nk, NB, false);
             __syncthreads();
             panel_update(D, LJ, LJ, nk, nk, true);
@@ -175,7 +195,7 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
         }
         // row tile (I, k): panel k-1 update, then solve against L_kk^T
         load_tile(own, A, n, r0, k0, nr, nk, false);
-        if (k > 0) {
+        if (k > 0 && tile_first[I] <= k - 1 && tile_first[k] <= k - 1) {
             load_tile(LI, A, n, r0, kp0, nr, NB, false);
             __syncthreads();
             panel_update(own, LI, LJ, nr, nk, false);
@@ -209,6 +229,7 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
     while (Ip * (Ip + 1) / 2 > u) --Ip;
     const int64_t Jp = u - Ip * (Ip + 1) / 2;
     const int I = k + 1 + static_cast<int>(Ip), J = k + 1 + static_cast<int>(Jp);
+    if (tile_first[I] > k - 1 || tile_first[J] > k - 1) return;  // panel k-1 is zero in row tile I or J
     const int r0 = I * NB, c0 = J * NB;
     const int nr = (n - r0) < NB ? (n - r0) : NB, nc = (n - c0) < NB ? (n - c0) : NB;
     load_tile(own, A, n, r0, c0, nr, nc, I == J);
@@ -220,68 +241,86 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
     store_tile(own, A, n, r0, c0, nr, nc, I == J);
 }
 
-// Rows below the diagonal tile: L_rk = A_rk L_kk^-T.
-__global__ void __launch_bounds__(64) k_trsm(int n, double* A, int k0, const int32_t* status, const lm_state* lm,
-                                             int gated) {
-    if (skip(lm, gated, status)) return;
-    __shared__ double T[NB][NB + 1];
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-        const int r = idx / nb, c = idx % nb;
-        T[r][c] = (c <= r) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
-    }
-    __syncthreads();
-    const int row = k0 + nb + blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= n) return;
-    double* a = A + static_cast<int64_t>(row) * n + k0;
-    double x[NB];
-#pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = (c < nb) ? a[c] : 0.0;
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        if (c < nb) {
-            double v = x[c];
-#pragma unroll
-            for (int m = 0; m < c; ++m) v = v - x[m] * T[c][m];
-            x[c] = v / T[c][c];
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < NB; ++c)
-        if (c < nb) a[c] = x[c];
-}
+// ---- whole factorisation in one workgroup -------------------------------------
+// Used when every column panel of the envelope is at most kFusedTiles tiles tall:
+// panel k (row tiles k .. tile_last[k] of tile column k, already updated by the
+// panels before it) is staged in LDS, its diagonal tile factored by wave 0, the
+// tiles below solved one row per thread, the panel written back, and the trailing
+// tiles it touches updated element by element from LDS.  Workgroup barriers
+// order the panels; nothing waits on another workgroup.
+constexpr int kFusedTiles = 16;
 
-// Trailing update A_IJ -= L_Ik L_Jk^T over lower-triangle tile pairs I >= J > k.
-__global__ void __launch_bounds__(256) k_syrk(int n, double* A, int k0, int first_tile, const int32_t* status,
-                                              const lm_state* lm, int gated) {
-    if (skip(lm, gated, status)) return;
-    __shared__ double LI[NB][NB + 1];
-    __shared__ double LJ[NB][NB + 1];
-    const int64_t idx = blockIdx.x;
-    int64_t Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(idx) + 1.0) - 1.0) / 2.0);
-    while ((Ip + 1) * (Ip + 2) / 2 <= idx) ++Ip;
-    while (Ip * (Ip + 1) / 2 > idx) --Ip;
-    const int64_t Jp = idx - Ip * (Ip + 1) / 2;
-    const int I = first_tile + static_cast<int>(Ip), J = first_tile + static_cast<int>(Jp);
-    const int nb = (n - k0) < NB ? (n - k0) : NB;
-    const int r0 = I * NB, c0 = J * NB;
-    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-        const int r = t / NB, m = t % NB;
-        LI[r][m] = (r0 + r < n && m < nb) ? A[static_cast<int64_t>(r0 + r) * n + k0 + m] : 0.0;
-        LJ[r][m] = (c0 + r < n && m < nb) ? A[static_cast<int64_t>(c0 + r) * n + k0 + m] : 0.0;
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NB * NB; t += blockDim.x) {
-        const int r = t / NB, c = t % NB;
-        if (r0 + r >= n || c0 + c >= n) continue;
-        if (I == J && c > r) continue;
-        // one subtraction per panel column, in column order: the sequence the
-        // reference's row-Crout loop applies to this element
-        double* out = A + static_cast<int64_t>(r0 + r) * n + c0 + c;
-        double v = *out;
+__global__ void __launch_bounds__(1024) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
+                                                     int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || status[0] != 0) return;
+    __shared__ double Pn[kFusedTiles][NB][NB + 1];
+    __shared__ int ok_sh;
+    const int T = (n + NB - 1) / NB;
+    const double eps = eps_p[0];
+    for (int k = 0; k < T; ++k) {
+        const int k0 = k * NB;
+        const int nk = (n - k0) < NB ? (n - k0) : NB;
+        const int h = tile_last[k] - k + 1;
+        const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
+        const int rows = rows_end - k0;  // panel rows, the diagonal tile's first
+        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
+            const int r = idx / NB, c = idx % NB;
+            const int t = r / NB, rr = r % NB;
+            Pn[t][rr][c] = (c < nk && (t > 0 || c <= rr)) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
+        }
+        __syncthreads();
+        if (!factor_diag_tile(Pn[0], nk, eps, &ok_sh)) {
+            if (threadIdx.x == 0) status[0] = 1;
+            return;
+        }
+        // rows below: L_r = A_r L_kk^-T, element c gets -= x_m L_cm (m < c) then / L_cc
+        if (static_cast<int>(threadIdx.x) < rows - nk) {
+            const int r = nk + threadIdx.x;
+            const int t = r / NB, rr = r % NB;
+            double x[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) x[c] = Pn[t][rr][c];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < nk) {
+                    double v = x[c];
+#pragma unroll
+                    for (int m = 0; m < c; ++m) v = v - x[m] * Pn[0][c][m];
+                    x[c] = v / Pn[0][c][c];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c) Pn[t][rr][c] = x[c];
+        }
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
+            const int r = idx / NB, c = idx % NB;
+            const int t = r / NB, rr = r % NB;
+            if (c < nk && (t > 0 || c <= rr)) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = Pn[t][rr][c];
+        }
+        // trailing tiles (I, J), k < J <= I <= tile_last[k]: a -= L_I,k L_J,k^T
+        // over the panel's 32 columns in order (only full panels have tiles below)
+        const int m = h - 1;
+        const int64_t elems = static_cast<int64_t>(m) * (m + 1) / 2 * (NB * NB);
+        for (int64_t e = threadIdx.x; e < elems; e += blockDim.x) {
+            const int pair = static_cast<int>(e / (NB * NB));
+            const int within = static_cast<int>(e % (NB * NB));
+            int Ip = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) / 2.0);
+            while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
+            while (Ip * (Ip + 1) / 2 > pair) --Ip;
+            const int Jp = pair - Ip * (Ip + 1) / 2;
+            const int rr = within / NB, cc = within % NB;
+            const int pr = (Ip + 1) * NB + rr, pc = (Jp + 1) * NB + cc;  // panel rows of the element's row, column
+            if (pr >= rows || pc >= rows || (Ip == Jp && cc > rr)) continue;
+            double* out = A + static_cast<int64_t>(k0 + pr) * n + (k0 + pc);
+            double v = *out;
+            const double* Li = Pn[Ip + 1][rr];
+            const double* Lj = Pn[Jp + 1][cc];
 #pragma unroll 8
-        for (int m = 0; m < nb; ++m) v = v - LI[r][m] * LJ[c][m];
-        *out = v;
+            for (int q = 0; q < NB; ++q) v = v - Li[q] * Lj[q];
+            *out = v;
+        }
+        __syncthreads();
     }
 }
 
@@ -293,12 +332,13 @@ __device__ __forceinline__ double readlane_d(double v, int j) {
 }
 
 // Lt[i][j] = L[j][i] for j > i (the strict upper triangle of L^T).
-__global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L, double* Lt, const int32_t* status,
-                                                         const lm_state* lm, int gated) {
+__global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L, double* Lt, const int32_t* tile_last,
+                                                         const int32_t* status, const lm_state* lm, int gated) {
     if (skip(lm, gated, status)) return;
     __shared__ double T[32][33];
     const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;  // source tile rows bi.., cols bj..
     if (bj > bi + 31) return;
+    if (static_cast<int>(blockIdx.y) > tile_last[blockIdx.x]) return;  // never read by the backward pass
     for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
         const int r = t / 32, c = t % 32;
         T[r][c] = (bi + r < n && bj + c < n) ? L[static_cast<int64_t>(bi + r) * n + bj + c] : 0.0;
@@ -316,7 +356,8 @@ __global__ void __launch_bounds__(256) k_transpose_lower(int n, const double* L,
 // ordered subtractions for row i, the wave has already staged row i-1 of L^T.
 constexpr int kBackLds = 6144;  // largest n handled from LDS
 __global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* L, const double* Lt, double* x,
-                                                           const int32_t* status, lm_state* lm, int gated) {
+                                                           const int32_t* tile_last, const int32_t* status, lm_state* lm,
+                                                           int gated) {
     if (skip(lm, gated, status)) return;
     __shared__ double xs[kBackLds];
     __shared__ double prod[kBackLds];
@@ -324,18 +365,20 @@ __global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* 
     for (int j = lane; j < n; j += 64) xs[j] = x[j];
     __syncthreads();
     for (int i = n - 1; i >= 0; --i) {
-        // products L_ji x_j (x_j final for j > i), formed in parallel
+        // products L_ji x_j (x_j final for j > i), formed in parallel; column i
+        // of L is zero below row tile tile_last[i / 32]
+        const int jend = (tile_last[i / NB] + 1) * NB < n ? (tile_last[i / NB] + 1) * NB : n;
         const double* u = Lt + static_cast<int64_t>(i) * n;
-        for (int j = i + 1 + lane; j < n; j += 64) prod[j] = u[j] * xs[j];
+        for (int j = i + 1 + lane; j < jend; j += 64) prod[j] = u[j] * xs[j];
         __syncthreads();
         if (lane == 0) {
             double v = xs[i];
             int j = i + 1;
-            for (; j + 32 <= n; j += 32) {
+            for (; j + 32 <= jend; j += 32) {
 #pragma unroll
                 for (int k = 0; k < 32; ++k) v = v - prod[j + k];
             }
-            for (; j < n; ++j) v = v - prod[j];
+            for (; j < jend; ++j) v = v - prod[j];
             xs[i] = v / L[static_cast<int64_t>(i) * n + i];
         }
         __syncthreads();
@@ -345,16 +388,18 @@ __global__ void __launch_bounds__(64) k_backward_exact_lds(int n, const double* 
 }
 
 __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, const double* Lt, double* x,
-                                                       const int32_t* status, lm_state* lm, int gated) {
+                                                       const int32_t* tile_last, const int32_t* status, lm_state* lm,
+                                                       int gated) {
     if (skip(lm, gated, status)) return;
     const int lane = threadIdx.x;
     for (int i = n - 1; i >= 0; --i) {
         double v = x[i];
+        const int jend = (tile_last[i / NB] + 1) * NB < n ? (tile_last[i / NB] + 1) * NB : n;
         const double* u = Lt + static_cast<int64_t>(i) * n;
-        for (int j0 = i + 1; j0 < n; j0 += 64) {
+        for (int j0 = i + 1; j0 < jend; j0 += 64) {
             const int j = j0 + lane;
-            const double pj = (j < n) ? u[j] * x[j] : 0.0;
-            const int cnt = (n - j0) < 64 ? (n - j0) : 64;
+            const double pj = (j < jend) ? u[j] * x[j] : 0.0;
+            const int cnt = (jend - j0) < 64 ? (jend - j0) : 64;
             if (cnt == 64) {
 #pragma unroll
                 for (int k = 0; k < 64; ++k) v = v - readlane_d(pj, k);
@@ -371,7 +416,17 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
 }
 
 // Forward and backward substitution in one workgroup; sets lm->solve_ok.
-__global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* x, int32_t* status,
+// x lives in LDS (kTrsvLds entries) or, for larger systems, in the output.
+// Per diagonal tile, wave 0 runs the recurrence with x_j broadcast by readlane
+// and the divisions by L_jj taken through reciprocals formed in parallel
+// (div_cr, correctly rounded); the rows the tile touches are then updated by all
+// threads.  Forward substitution subtracts L_ij x_j in ascending j, the
+// reference's sequence (matrix_decomposition_cholesky.hpp:76-84).
+constexpr int kTrsvLds = 4096;
+
+template <bool kLds>
+__global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* xout,
+                                               const int32_t* tile_first, const int32_t* tile_last, int32_t* status,
                                                lm_state* lm, int gated, int forward_only) {
     if (gated && lm->done) return;
     if (*status != 0) {
@@ -380,7 +435,9 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
     }
     __shared__ double sh[1024];
     __shared__ double T[NB][NB + 1];
+    __shared__ double xl[kLds ? kTrsvLds : 1];
     __shared__ int fail;
+    double* x = kLds ? xl : xout;
     const int tid = threadIdx.x;
     // eps = 1e-7 * max|L_ii| (matrix_decomposition_cholesky.hpp:70-74)
     double m = 0;
@@ -413,7 +470,6 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
     }
     const int tiles = (n + NB - 1) / NB;
     const int lane = tid & 63;
-    // forward: L y = b, column by column within a tile, x_i -= L_ij x_j in j order
     for (int t = 0; t < tiles; ++t) {
         const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
         for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
@@ -423,15 +479,18 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
         __syncthreads();
         if (tid < 64) {
             double xi = (lane < nb) ? x[i0 + lane] : 0.0;
+            const double dl = (lane < nb) ? T[lane][lane] : 1.0;
+            const double yl = 1.0 / dl;
             for (int j = 0; j < nb; ++j) {
-                if (lane == j) xi = xi / T[j][j];
-                const double xj = __shfl(xi, j, 64);
-                if (lane > j && lane < nb) xi = xi - T[lane][j] * xj;
+                if (lane == j) xi = div_cr(xi, dl, yl);
+                const double xj = lane_value(xi, j);
+                if (lane > j) xi = xi - T[lane < NB ? lane : 0][j] * xj;
             }
             if (lane < nb) x[i0 + lane] = xi;
         }
         __syncthreads();
-        for (int i = i0 + nb + tid; i < n; i += blockDim.x) {
+        const int iend = (tile_last[t] + 1) * NB < n ? (tile_last[t] + 1) * NB : n;
+        for (int i = i0 + nb + tid; i < iend; i += blockDim.x) {
             double v = x[i];
             const double* Li = L + static_cast<int64_t>(i) * n + i0;
             for (int j = 0; j < nb; ++j) v = v - Li[j] * x[i0 + j];
@@ -439,33 +498,47 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
         }
         __syncthreads();
     }
-    if (forward_only) return;
-    // backward: L^T x = y
-    for (int t = tiles - 1; t >= 0; --t) {
-        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-        for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-            const int r = idx / nb, c = idx % nb;
-            T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
-        }
-        __syncthreads();
-        if (tid < 64) {
-            double xi = (lane < nb) ? x[i0 + lane] : 0.0;
-            for (int j = nb - 1; j >= 0; --j) {
-                if (lane == j) xi = xi / T[j][j];
-                const double xj = __shfl(xi, j, 64);
-                if (lane < j) xi = xi - T[j][lane] * xj;
+    if (!forward_only) {
+        for (int t = tiles - 1; t >= 0; --t) {
+            const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
+            for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
+                const int r = idx / nb, c = idx % nb;
+                T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
             }
-            if (lane < nb) x[i0 + lane] = xi;
+            __syncthreads();
+            if (tid < 64) {
+                double xi = (lane < nb) ? x[i0 + lane] : 0.0;
+                const double dl = (lane < nb) ? T[lane][lane] : 1.0;
+                const double yl = 1.0 / dl;
+                for (int j = nb - 1; j >= 0; --j) {
+                    if (lane == j) xi = div_cr(xi, dl, yl);
+                    const double xj = lane_value(xi, j);
+                    if (lane < j) xi = xi - T[j][lane] * xj;
+                }
+                if (lane < nb) x[i0 + lane] = xi;
+            }
+            __syncthreads();
+            // columns left of the tile, from the first column row tile t touches
+            for (int i = tile_first[t] * NB + tid; i < i0; i += blockDim.x) {
+                double v = x[i];
+                for (int j = 0; j < nb; ++j) v = v - L[static_cast<int64_t>(i0 + j) * n + i] * x[i0 + j];
+                x[i] = v;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        for (int i = tid; i < i0; i += blockDim.x) {
-            double v = x[i];
-            for (int j = 0; j < nb; ++j) v = v - L[static_cast<int64_t>(i0 + j) * n + i] * x[i0 + j];
-            x[i] = v;
-        }
-        __syncthreads();
     }
-    if (tid == 0) lm->solve_ok = 1;
+    if (kLds) {
+        for (int i = tid; i < n; i += blockDim.x) xout[i] = x[i];
+    }
+    if (tid == 0 && !forward_only) lm->solve_ok = 1;
+}
+
+void launch_trsv(int n, const double* L, const double* b, double* x, const dev_work64& w, hipStream_t s, int gated,
+                 int forward_only) {
+    if (n <= kTrsvLds)
+        k_trsv<true><<<1, 1024, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
+    else
+        k_trsv<false><<<1, 1024, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
 }
 
 // A system without poses: the reference's 0 x 0 decompose and solve succeed.
@@ -485,27 +558,34 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     ZB_TIME_BEGIN(w, K_CHOLESKY, s);
     k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g, floor_rel);
     const int64_t tiles = (n + NB - 1) / NB;
-    for (int64_t k = 0; k < tiles; ++k) {
-        const int64_t m = tiles - k - 1;
-        const int64_t grid = (tiles - k) + (k > 0 ? m * (m + 1) / 2 : 0);
-        k_chol_step<<<static_cast<unsigned>(grid), 256, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, static_cast<int>(k),
-                                                               w.lm, g);
+    const bool fused = w.factor_path == 1 || (w.factor_path == 0 && w.panel_tiles <= kFusedTiles);
+    if (fused && w.panel_tiles <= kFusedTiles) {
+        k_chol_fused<<<1, 1024, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
+    } else {
+        for (int64_t k = 0; k < tiles; ++k) {
+            const int64_t m = tiles - k - 1;
+            const int64_t grid = (tiles - k) + (k > 0 ? m * (m + 1) / 2 : 0);
+            k_chol_step<<<static_cast<unsigned>(grid), 256, 0, s>>>(n, w.S, w.chol_scale, w.chol_status,
+                                                                   static_cast<int>(k), w.tile_first, w.lm, g);
+        }
     }
     ZB_TIME_END(w, s);
     ZB_TIME_BEGIN(w, K_TRSV, s);
-    k_trsv<<<1, 1024, 0, s>>>(n, w.S, w.bs, w.dxp, w.chol_status, w.lm, g, exact ? 1 : 0);
+    launch_trsv(n, w.S, w.bs, w.dxp, w, s, g, exact ? 1 : 0);
     if (exact) {
         const int tiles32 = (n + 31) / 32;
-        k_transpose_lower<<<dim3(tiles32, tiles32), 256, 0, s>>>(n, w.S, w.Lt, w.chol_status, w.lm, g);
-        if (n <= kBackLds) k_backward_exact_lds<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
-        else k_backward_exact<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.chol_status, w.lm, g);
+        k_transpose_lower<<<dim3(tiles32, tiles32), 256, 0, s>>>(n, w.S, w.Lt, w.tile_last, w.chol_status, w.lm, g);
+        if (n <= kBackLds)
+            k_backward_exact_lds<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.tile_last, w.chol_status, w.lm, g);
+        else
+            k_backward_exact<<<1, 64, 0, s>>>(n, w.S, w.Lt, w.dxp, w.tile_last, w.chol_status, w.lm, g);
     }
     ZB_TIME_END(w, s);
 }
 
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated) {
     if (n <= 0) return;
-    k_trsv<<<1, 1024, 0, s>>>(n, w.S, b, x, w.chol_status, w.lm, gated ? 1 : 0, 0);
+    launch_trsv(n, w.S, b, x, w, s, gated ? 1 : 0, 0);
 }
 
 }  // namespace zb

@@ -37,10 +37,10 @@ class ZsbaError(RuntimeError):
 
 class Graph:
     def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 0, timing: bool = False,
-                 floor_exp: int = 0):
+                 floor_exp: int = 0, factor_path: int = 0):
         self.lib = _lib.load()
         opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps, timing=int(bool(timing)),
-                            floor_exp=floor_exp)
+                            floor_exp=floor_exp, factor_path=factor_path)
         self.h = self.lib.zsba_create(C.byref(opts))
         if not self.h:
             raise MemoryError("zsba_create failed")
