@@ -194,12 +194,38 @@ int32_t zsba_snapshot(zsba_graph* graph);
 int32_t zsba_restore(zsba_graph* graph);
 
 /* ---- batched local BA (no reference counterpart: many independent graphs) -----
- * Solves every graph with the same iteration count, convergence rule and
- * precision in one device-side launch sequence.  accepted[i] receives what
- * zsba_solve would have returned for graphs[i]. */
+ * A batch solves many independent graphs (local BA windows) with the same
+ * iteration count, convergence rule and precision.  Every window whose reduced
+ * system has at most 256 rows (42 poses) is solved by one workgroup of a single
+ * persistent launch that runs the window's whole LM loop; the launch is captured
+ * in a hipGraph and replayed by later zsba_batch_solve calls.  The workgroup size
+ * is chosen by timing the candidates on the first solve of each batch shape.
+ * Larger windows, and ZSBA_PRECISION_FP64 (the reference's operation order), are
+ * solved one by one with zsba_solve.  accepted[i] receives the accepted-step
+ * count of graphs[i]; per-graph zsba_get_stats reflect the batch solve.  The
+ * graphs must all use one device and stay alive for the life of the batch. */
+typedef struct zsba_batch zsba_batch;
+typedef struct zsba_batch_info {
+    int32_t windows;          /* graphs solved by the batched launch                 */
+    int32_t individual;       /* graphs solved one by one                            */
+    int32_t threads;          /* workgroup size per window                           */
+    int32_t graph_launches;   /* hipGraph launches by this solve                     */
+    float tune_ms[3];         /* autotuner trial times for 256, 512, 1024 threads    */
+    float pad;
+    double autotune_wall_ms;  /* time spent autotuning in this solve (0 when cached) */
+    double kernel_wall_ms;    /* batched launch, host-measured, synchronised         */
+    double solve_ms;          /* whole zsba_batch_solve call                         */
+    double reserved[4];
+} zsba_batch_info;
+zsba_batch* zsba_batch_create(zsba_graph* const* graphs, int32_t count);
+void        zsba_batch_destroy(zsba_batch* batch);
+const char* zsba_batch_last_error(const zsba_batch* batch);
+int32_t     zsba_batch_solve(zsba_batch* batch, int32_t iterations, int32_t use_relative_convergence,
+                             int32_t precision, int32_t* accepted /* count */);
+int32_t     zsba_batch_get_info(const zsba_batch* batch, zsba_batch_info* info);
+/* One-shot form: create, solve, destroy. */
 int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations,
-                         int32_t use_relative_convergence, int32_t precision,
-                         int32_t* accepted /* count */);
+                         int32_t use_relative_convergence, int32_t precision, int32_t* accepted);
 
 /* ---- implicit differentiation (no reference counterpart) ----------------------
  * At the converged state of the last FP64 solve, propagate dL/d(points) (and

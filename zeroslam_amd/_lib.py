@@ -51,6 +51,18 @@ class SynthParams(C.Structure):
                 ("reserved", C.c_int32 * 4)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("windows", C.c_int32), ("individual", C.c_int32), ("threads", C.c_int32),
+                ("graph_launches", C.c_int32), ("tune_ms", C.c_float * 3), ("pad", C.c_float),
+                ("autotune_wall_ms", C.c_double), ("kernel_wall_ms", C.c_double), ("solve_ms", C.c_double),
+                ("reserved", C.c_double * 4)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "pad", "tune_ms")}
+        d["tune_ms"] = list(self.tune_ms)
+        return d
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -86,6 +98,11 @@ _SIGNATURES = {
     "zsba_snapshot": (C.c_int32, [_P]),
     "zsba_restore": (C.c_int32, [_P]),
     "zsba_solve_batch": (C.c_int32, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_int32, C.c_int32, _I32]),
+    "zsba_batch_create": (_P, [C.POINTER(_P), C.c_int32]),
+    "zsba_batch_destroy": (None, [_P]),
+    "zsba_batch_last_error": (C.c_char_p, [_P]),
+    "zsba_batch_solve": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, _I32]),
+    "zsba_batch_get_info": (C.c_int32, [_P, C.POINTER(BatchInfo)]),
     "zsba_adjoint": (C.c_int32, [_P, _D, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),
     "zsba_debug_reduced_system": (C.c_int32, [_P, C.c_double, C.c_int32, _D, _D, _D, _D]),

@@ -41,6 +41,7 @@ UNITS = [
     ("kernels_fp64.hip", ["-ffp-contract=off"]),
     ("kernels_fp32.hip", ["-ffp-contract=off"]),
     ("kernels_chol.hip", ["-ffp-contract=off"]),
+    ("kernels_batch.hip", ["-ffp-contract=off"]),
 ]
 
 

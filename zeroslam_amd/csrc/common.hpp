@@ -221,6 +221,19 @@ void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t 
 void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_backsub_only_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
+// ---- batched local BA (kernels_batch.hip) --------------------------------------
+// One workgroup per window solves the window's whole LM loop.  Windows are
+// limited to reduced systems of kBatchMaxN rows (42 poses).
+constexpr int kBatchMaxN = 256;
+struct batch_window {
+    dev_problem p;
+    dev_work64 w;
+    int32_t iterations = 0, use_relative = 0, refine_steps = 0, pad = 0;
+    double floor_rel = 0.0;
+};
+// threads per window: 256, 512 or 1024; out[i] receives window i's final controller.
+void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out, hipStream_t s);
+
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from
 // w.bs.  Sets *w.chol_status and lm->solve_ok; skips everything when the

@@ -12,6 +12,9 @@
 #include "ba_math.hpp"
 
 #include <algorithm>
+#include <tuple>
+#include <mutex>
+#include <map>
 #include <cstdint>
 #include <array>
 #include <chrono>
@@ -758,6 +761,276 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     return lm_pinned->success;
 }
 
+// ---- batched local BA -----------------------------------------------------------
+// One persistent launch solves every window whose reduced system fits one
+// workgroup (zb::kBatchMaxN rows); larger windows and the reference-order FP64
+// precision go through the per-graph solve.  The launch is captured once into a
+// hipGraph and replayed by later solves of the same batch; the window table it
+// reads is refreshed before each replay.  The workgroup size is chosen per batch
+// shape by timing the candidates on the batch itself (states restored after each
+// trial) and remembered for the process.
+namespace {
+struct tune_key {
+    int32_t count, max_n, obs_bucket, mixed;
+    bool operator<(const tune_key& o) const {
+        return std::tie(count, max_n, obs_bucket, mixed) < std::tie(o.count, o.max_n, o.obs_bucket, o.mixed);
+    }
+};
+std::mutex tune_mutex;
+std::map<tune_key, int32_t> tune_cache;
+}  // namespace
+
+struct zsba_batch {
+    std::vector<zsba_graph*> graphs;
+    std::string err;
+    zsba_batch_info info{};
+    hipStream_t stream = nullptr;
+    int device = -1;
+    zb::batch_window* d_wins = nullptr;
+    lm_state* d_out = nullptr;
+    lm_state* h_out = nullptr;
+    int32_t capacity = 0;
+    std::vector<zb::batch_window> h_wins;
+    std::vector<int32_t> win_graph;  // window -> graph index
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t graph_threads = 0, graph_mixed = -1, graph_count = -1;
+
+    ~zsba_batch() { release(); }
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (d_wins) (void)hipFree(d_wins);
+        if (d_out) (void)hipFree(d_out);
+        if (h_out) (void)hipHostFree(h_out);
+        if (stream) (void)hipStreamDestroy(stream);
+        d_wins = nullptr;
+        d_out = nullptr;
+        h_out = nullptr;
+        stream = nullptr;
+        capacity = 0;
+    }
+    int32_t fail(int32_t code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int32_t hip_fail(hipError_t e, const char* what) {
+        return fail(ZSBA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+    }
+
+    int32_t ensure(int32_t count, int dev) {
+        hipError_t e;
+        if (device != dev) {
+            release();
+            device = dev;
+        }
+        if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+        if (!stream && (e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess)
+            return hip_fail(e, "hipStreamCreate");
+        if (count > capacity) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            if (d_wins) (void)hipFree(d_wins);
+            if (d_out) (void)hipFree(d_out);
+            if (h_out) (void)hipHostFree(h_out);
+            d_wins = nullptr;
+            d_out = nullptr;
+            h_out = nullptr;
+            if ((e = hipMalloc(&d_wins, sizeof(zb::batch_window) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipMalloc(&d_out, sizeof(lm_state) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_out), sizeof(lm_state) * count, hipHostMallocDefault)) !=
+                hipSuccess)
+                return hip_fail(e, "hipHostMalloc");
+            capacity = count;
+        }
+        return 0;
+    }
+
+    // One launch of the batch kernel for the current window table.
+    int32_t launch(int32_t threads, bool mixed) {
+        hipError_t e;
+        const int32_t count = static_cast<int32_t>(h_wins.size());
+        if (!exec || graph_threads != threads || graph_mixed != (mixed ? 1 : 0) || graph_count != count) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return hip_fail(e, "hipStreamBeginCapture");
+            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, stream);
+            if ((e = hipStreamEndCapture(stream, &graph)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
+                return hip_fail(e, "hipGraphInstantiate");
+            graph_threads = threads;
+            graph_mixed = mixed ? 1 : 0;
+            graph_count = count;
+        }
+        if ((e = hipGraphLaunch(exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        info.graph_launches += 1;
+        return 0;
+    }
+
+    // Times every candidate workgroup size once on this batch, restoring the window
+    // states after each trial, and returns the fastest.
+    int32_t autotune(bool mixed, int32_t& best) {
+        hipError_t e;
+        std::vector<double*> saved;
+        for (int32_t k : win_graph) {
+            zsba_graph* g = graphs[k];
+            const size_t bytes = sizeof(double) * (7L * g->dp.np + 3L * g->dp.nl);
+            double* buf = nullptr;
+            if ((e = hipMalloc(&buf, bytes)) != hipSuccess) {
+                for (double* s2 : saved) (void)hipFree(s2);
+                return hip_fail(e, "hipMalloc");
+            }
+            (void)hipMemcpyAsync(buf, g->w.pose, sizeof(double) * 7L * g->dp.np, hipMemcpyDeviceToDevice, stream);
+            (void)hipMemcpyAsync(buf + 7L * g->dp.np, g->w.point, sizeof(double) * 3L * g->dp.nl, hipMemcpyDeviceToDevice,
+                                 stream);
+            saved.push_back(buf);
+        }
+        hipEvent_t t0, t1;
+        (void)hipEventCreate(&t0);
+        (void)hipEventCreate(&t1);
+        float best_ms = 0.f;
+        best = 0;
+        int32_t rc = 0;
+        for (int32_t threads : {256, 512, 1024}) {
+            (void)hipEventRecord(t0, stream);
+            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, stream);
+            (void)hipEventRecord(t1, stream);
+            if ((e = hipEventSynchronize(t1)) != hipSuccess) {
+                rc = hip_fail(e, "autotune");
+                break;
+            }
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            info.tune_ms[threads == 256 ? 0 : threads == 512 ? 1 : 2] = ms;
+            if (best == 0 || ms < best_ms) {
+                best = threads;
+                best_ms = ms;
+            }
+            for (size_t i = 0; i < win_graph.size(); ++i) {
+                zsba_graph* g = graphs[win_graph[i]];
+                (void)hipMemcpyAsync(g->w.pose, saved[i], sizeof(double) * 7L * g->dp.np, hipMemcpyDeviceToDevice, stream);
+                (void)hipMemcpyAsync(g->w.point, saved[i] + 7L * g->dp.np, sizeof(double) * 3L * g->dp.nl,
+                                     hipMemcpyDeviceToDevice, stream);
+            }
+        }
+        (void)hipStreamSynchronize(stream);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        for (double* buf : saved) (void)hipFree(buf);
+        return rc;
+    }
+
+    int32_t solve(int32_t iterations, int32_t use_rel, int32_t precision, int32_t* accepted) {
+        const int32_t count = static_cast<int32_t>(graphs.size());
+        if (count > 0 && !accepted) return fail(ZSBA_EINVAL, "accepted must not be NULL");
+        if (precision != ZSBA_PRECISION_FP64 && precision != ZSBA_PRECISION_MIXED && precision != ZSBA_PRECISION_FP64_FAST)
+            return fail(ZSBA_EINVAL, "unknown precision " + std::to_string(precision));
+        info = zsba_batch_info{};
+        const double t_start = now_ms();
+        h_wins.clear();
+        win_graph.clear();
+        int dev = -1;
+        for (int32_t k = 0; k < count; ++k) {
+            zsba_graph* g = graphs[k];
+            accepted[k] = 0;
+            g->stats = zsba_stats{};
+            g->stats.precision = precision;
+            if (g->live_obs() == 0 || (g->live_poses() == 0 && g->live_points() == 0)) continue;
+            int32_t rc = g->prepare();
+            if (rc) return fail(rc, "graph " + std::to_string(k) + ": " + g->err);
+            g->stats.n_obs = g->dp.no;
+            g->stats.n_poses = g->dp.np;
+            g->stats.n_points = g->dp.nl;
+            const bool fits = g->dp.n <= zb::kBatchMaxN && g->w.panel_tiles <= zb::kBatchMaxN / 32;
+            if (precision == ZSBA_PRECISION_FP64 || !fits) {
+                rc = g->run(iterations, use_rel, precision);
+                if (rc < 0) return fail(rc, "graph " + std::to_string(k) + ": " + g->err);
+                accepted[k] = rc;
+                info.individual += 1;
+                continue;
+            }
+            if (dev >= 0 && g->opt.device != dev) return fail(ZSBA_EINVAL, "batched graphs must share one device");
+            dev = g->opt.device;
+            zb::batch_window bw;
+            bw.p = g->dp;
+            bw.w = g->w;
+            bw.w.timer = nullptr;
+            bw.iterations = iterations;
+            bw.use_relative = use_rel ? 1 : 0;
+            bw.refine_steps = g->opt.refine_steps;
+            bw.floor_rel = precision == ZSBA_PRECISION_MIXED ? g->mixed_floor() : 0.0;
+            h_wins.push_back(bw);
+            win_graph.push_back(k);
+        }
+        const int32_t nw = static_cast<int32_t>(h_wins.size());
+        info.windows = nw;
+        if (nw == 0) {
+            info.solve_ms = now_ms() - t_start;
+            return 0;
+        }
+        int32_t rc = ensure(nw, dev);
+        if (rc) return rc;
+        hipError_t e;
+        if ((e = hipMemcpyAsync(d_wins, h_wins.data(), sizeof(zb::batch_window) * nw, hipMemcpyHostToDevice, stream)) !=
+            hipSuccess)
+            return hip_fail(e, "upload windows");
+        const bool mixed = precision == ZSBA_PRECISION_MIXED;
+        int32_t max_n = 0;
+        int64_t obs = 0;
+        for (const zb::batch_window& bw : h_wins) {
+            max_n = std::max(max_n, bw.p.n);
+            obs += bw.p.no;
+        }
+        const tune_key key{nw, max_n, static_cast<int32_t>(std::log2(static_cast<double>(obs) + 1.0)), mixed ? 1 : 0};
+        int32_t threads = 0;
+        {
+            std::lock_guard<std::mutex> lock(tune_mutex);
+            auto it = tune_cache.find(key);
+            if (it != tune_cache.end()) threads = it->second;
+        }
+        if (threads == 0) {
+            const double t = now_ms();
+            if ((rc = autotune(mixed, threads))) return rc;
+            info.autotune_wall_ms = now_ms() - t;
+            std::lock_guard<std::mutex> lock(tune_mutex);
+            tune_cache[key] = threads;
+        }
+        info.threads = threads;
+        const double t0 = now_ms();
+        if ((rc = launch(threads, mixed))) return rc;
+        if ((e = hipMemcpyAsync(h_out, d_out, sizeof(lm_state) * nw, hipMemcpyDeviceToHost, stream)) != hipSuccess)
+            return hip_fail(e, "download controllers");
+        if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batch solve");
+        info.kernel_wall_ms = now_ms() - t0;
+        for (int32_t i = 0; i < nw; ++i) {
+            zsba_graph* g = graphs[win_graph[i]];
+            const lm_state& s = h_out[i];
+            *g->lm_pinned = s;
+            g->stats.iterations = s.executed_iterations;
+            g->stats.attempts = s.attempts;
+            g->stats.accepted = s.success;
+            g->stats.rejected = s.rejected;
+            g->stats.solve_failures = s.solve_failures;
+            g->stats.initial_chi2 = s.initial_chi2;
+            g->stats.final_chi2 = s.chi2;
+            g->stats.lambda = s.lambda;
+            g->stats.solve_ms = info.kernel_wall_ms;
+            g->device_newer = true;
+            accepted[win_graph[i]] = s.success;
+        }
+        info.solve_ms = now_ms() - t_start;
+        return 0;
+    }
+};
+
 // ---- C ABI ------------------------------------------------------------------
 extern "C" {
 
@@ -1123,15 +1396,44 @@ int32_t zsba_debug_math(int32_t fn, int32_t n, const double* in, double* out, in
     return rc;
 }
 
+zsba_batch* zsba_batch_create(zsba_graph* const* graphs, int32_t count) {
+    if (!graphs || count < 0) return nullptr;
+    zsba_batch* b = new (std::nothrow) zsba_batch();
+    if (!b) return nullptr;
+    b->graphs.assign(graphs, graphs + count);
+    for (zsba_graph* g : b->graphs) {
+        if (!g) {
+            delete b;
+            return nullptr;
+        }
+    }
+    return b;
+}
+
+void zsba_batch_destroy(zsba_batch* b) { delete b; }
+
+const char* zsba_batch_last_error(const zsba_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+int32_t zsba_batch_solve(zsba_batch* b, int32_t iterations, int32_t use_rel, int32_t precision, int32_t* accepted) {
+    if (!b) return ZSBA_EINVAL;
+    return b->solve(iterations, use_rel, precision, accepted);
+}
+
+int32_t zsba_batch_get_info(const zsba_batch* b, zsba_batch_info* info) {
+    if (!b || !info) return ZSBA_EINVAL;
+    *info = b->info;
+    return 0;
+}
+
 int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations, int32_t use_rel, int32_t precision,
                          int32_t* accepted) {
     if (!graphs || count < 0 || (count > 0 && !accepted)) return ZSBA_EINVAL;
-    for (int32_t i = 0; i < count; ++i) {
-        const int32_t r = zsba_solve(graphs[i], iterations, use_rel, precision);
-        if (r < 0) return r;
-        accepted[i] = r;
-    }
-    return 0;
+    zsba_batch* b = zsba_batch_create(graphs, count);
+    if (!b) return ZSBA_EINVAL;
+    const int32_t rc = b->solve(iterations, use_rel, precision, accepted);
+    if (rc < 0 && count > 0) graphs[0]->err = b->err;
+    delete b;
+    return rc;
 }
 
 int32_t zsba_adjoint(zsba_graph* g, const double*, const double*, double*, double*) {

@@ -1,0 +1,212 @@
+// Batched local bundle adjustment: one workgroup per window runs that window's
+// whole Levenberg-Marquardt solve (factor_graph::solve, factor_graph.hpp:489-615),
+// the same phases zsba_solve enqueues as separate launches: cost, linearisation,
+// damping and point inverses, Schur complement, envelope Cholesky, triangular
+// solves, binary64 refinement (mixed), update, accept / reject, rollback.
+// Workgroup barriers separate the phases; windows never wait on each other, so
+// one launch of `count` workgroups solves the batch and a window that finishes
+// early frees its compute unit.  Each window uses the device buffers of its own
+// graph handle; the controller is the shared lm_control.hpp code.
+#include "ba_math.hpp"
+#include "chol_device.hpp"
+#include "common.hpp"
+#include "lm_control.hpp"
+#include "mixed_items.hpp"
+
+namespace zb {
+namespace {
+
+constexpr int kBatchTiles = kBatchMaxN / wg::kTile;
+
+// Deterministic block-wide sum (fixed tree).
+__device__ double block_sum(double v, double* sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = sh[t] + sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Hessian-block buffers of a window for storage type TS.
+template <class TS> struct blocks;
+template <> struct blocks<float> {
+    float *pterms, *Hpp, *Hll, *W, *Hinv, *Y;
+    __device__ explicit blocks(const dev_work64& w)
+        : pterms(w.pose_terms_f), Hpp(w.Hpp_f), Hll(w.Hll_f), W(w.W_f), Hinv(w.Hinv_f), Y(w.Y_f) {}
+};
+template <> struct blocks<double> {
+    double *pterms, *Hpp, *Hll, *W, *Hinv, *Y;
+    __device__ explicit blocks(const dev_work64& w)
+        : pterms(w.pose_terms), Hpp(w.Hpp), Hll(w.Hll), W(w.W), Hinv(w.Hinv), Y(w.Y) {}
+};
+
+template <class TS>
+__device__ void linearise(const dev_problem& p, const dev_work64& w, const blocks<TS>& B) {
+    for (int l = threadIdx.x; l < p.nl; l += blockDim.x)
+        mixed::lin_point(p, w.pose, w.point, B.pterms, w.pose_grad, B.Hll, w.bl_m, B.W, l);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+    for (int i = threadIdx.x >> 6; i < p.np; i += nwaves) mixed::reduce_pose_lane(p, B.pterms, w.pose_grad, B.Hpp, w.bp_m, i, lane);
+    __syncthreads();
+}
+
+__device__ double cost(const dev_problem& p, const dev_work64& w, double* sh) {
+    double acc = 0.0;
+    for (int s = threadIdx.x; s < p.no; s += blockDim.x) acc = acc + mixed::cost_item(p, w.pose, w.point, w.residual, s);
+    return block_sum(acc, sh);
+}
+
+template <class TS, int kThreads>
+__global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wins, lm_state* out) {
+    const batch_window& bw = wins[blockIdx.x];
+    const dev_problem& p = bw.p;
+    const dev_work64& w = bw.w;
+    const blocks<TS> B(w);
+    lm_state* lm = w.lm;
+    __shared__ double sh[kThreads];
+    __shared__ max_scan ms[kThreads];
+    __shared__ double Pn[kBatchTiles][wg::kTile][wg::kTile + 1];
+    __shared__ double T[wg::kTile][wg::kTile + 1];
+    __shared__ double xs[kBatchMaxN];
+    __shared__ int flag;
+    const int tid = threadIdx.x, lane = tid & 63, nwaves = blockDim.x >> 6;
+    const int n = p.n;
+    const bool mixed_path = sizeof(TS) == 4;
+
+    if (tid == 0) {
+        *lm = lm_state{};
+        lm->iterations = bw.iterations;
+        lm->use_relative = bw.use_relative;
+    }
+    __syncthreads();
+    {
+        const double c = cost(p, w, sh);
+        if (tid == 0) lm->chi2 = c;
+    }
+    linearise(p, w, B);
+    {
+        // max_diagonal over pose then point diagonal entries, in order
+        const int64_t total = 6L * p.np + 3L * p.nl;
+        const int64_t per = (total + blockDim.x - 1) / blockDim.x;
+        const int64_t i0 = tid * per, i1 = (i0 + per < total) ? i0 + per : total;
+        max_scan m = {0.0, 0, 0, 1};
+        for (int64_t i = i0; i < i1; ++i) {
+            const double h = (i < 6L * p.np)
+                                 ? static_cast<double>(B.Hpp[21 * (i / 6) + (i % 6) * (i % 6 + 1) / 2 + i % 6])
+                                 : static_cast<double>(B.Hll[9 * ((i - 6L * p.np) / 3) + 4 * ((i - 6L * p.np) % 3)]);
+            m = scan_push(m, h);
+        }
+        ms[tid] = m;
+        __syncthreads();
+        if (tid == 0) {
+            max_scan acc = {0.0, 0, 0, 1};
+            for (unsigned t = 0; t < blockDim.x; ++t) acc = scan_combine(acc, ms[t]);
+            lm_init(lm, scan_max_diagonal(acc));
+        }
+        __syncthreads();
+    }
+    const int max_attempts = 11 * bw.iterations + 1;
+    for (int attempt = 0; attempt < max_attempts; ++attempt) {
+        if (lm->done) break;
+        const double lam = lm->lambda;
+        // reduced camera system
+        for (int l = tid; l < p.nl; l += blockDim.x) mixed::damp_invert_item(p, B.Hll, B.Hinv, lam, l);
+        for (int64_t i = tid; i < static_cast<int64_t>(n) * n; i += blockDim.x) w.S[i] = 0.0;
+        __syncthreads();
+        for (int a = tid; a < p.npair; a += blockDim.x) mixed::form_Y_item(p, B.W, B.Hinv, B.Y, a);
+        __syncthreads();
+        for (int blk = tid >> 6; blk < p.nblk; blk += nwaves)
+            mixed::schur_block(p, B.Y, B.W, B.Hpp, w.bp_m, w.bl_m, w.S, w.bs, lam, blk, lane);
+        __syncthreads();
+        // factorisation and solve
+        double mx = 0.0;
+        for (int i = tid; i < n; i += blockDim.x) {
+            const double a = fabs(w.S[static_cast<int64_t>(i) * n + i]);
+            mx = (mx < a) ? a : mx;
+        }
+        mx = wg::block_max(mx, sh);
+        const double lift = bw.floor_rel * mx;
+        if (bw.floor_rel > 0.0)
+            for (int i = tid; i < n; i += blockDim.x) w.S[static_cast<int64_t>(i) * n + i] += lift;
+        __syncthreads();
+        bool ok = wg::fused_factor<kBatchTiles>(n, w.S, w.tile_last, (mx + lift) * 1e-14, Pn, &flag);
+        if (ok) ok = wg::triangular_solve(n, w.S, w.bs, xs, w.tile_first, w.tile_last, T, sh, &flag, false) == 0;
+        if (ok) {
+            for (int i = tid; i < n; i += blockDim.x) w.dxp[i] = xs[i];
+            __syncthreads();
+            for (int l = tid; l < p.nl; l += blockDim.x) mixed::point_step_item(p, B.W, B.Hinv, w.bl_m, w.dxp, w.dxl, l);
+            __syncthreads();
+            const int steps = mixed_path ? bw.refine_steps : 0;
+            for (int k = 0; k < steps; ++k) {
+                for (int l = tid; l < p.nl; l += blockDim.x)
+                    mixed::full_residual_item(p, w.pose, w.point, w.dxp, w.dxl, w.bl_m, B.Hinv, w.pose_grad, w.zl, w.gl,
+                                              lam, l);
+                __syncthreads();
+                for (int i = tid >> 6; i < p.np; i += nwaves)
+                    mixed::pose_residual_wave(p, w.pose_grad, w.bp_m, w.dxp, B.W, w.gl, w.rvec, lam, i, lane);
+                __syncthreads();
+                if (wg::triangular_solve(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, T, sh, &flag, false) != 0) {
+                    ok = false;
+                    break;
+                }
+                for (int l = tid; l < p.nl; l += blockDim.x) mixed::refine_point_item(p, B.W, B.Hinv, w.zl, xs, w.dxl, l);
+                for (int i = tid; i < n; i += blockDim.x) w.dxp[i] += xs[i];
+                __syncthreads();
+            }
+        }
+        if (tid == 0) {
+            lm->solve_ok = ok ? 1 : 0;
+            lm_after_solve(lm);
+        }
+        __syncthreads();
+        if (lm->step_applied) {
+            double acc = 0.0;
+            for (int t = tid; t < p.np + p.nl; t += blockDim.x)
+                acc = acc + mixed::apply_step_item(p, w.pose, w.pose_backup, w.point, w.point_backup, w.dxp, w.dxl, w.bp_m,
+                                                   w.bl_m, lam, t);
+            const double scale = block_sum(acc, sh);
+            const double c = cost(p, w, sh);
+            if (tid == 0) {
+                lm->scale = scale + 1e-3;  // factor_graph.hpp:556-560
+                lm->chi2_new = c;
+                lm_decide(lm);
+            }
+            __syncthreads();
+            if (lm->need_rollback) {
+                for (int t = tid; t < p.np + p.nl; t += blockDim.x)
+                    mixed::rollback_item(p, w.pose, w.pose_backup, w.point, w.point_backup, t);
+            }
+            if (!lm->done && lm->need_linearize) {
+                __syncthreads();
+                linearise(p, w, B);
+                if (tid == 0) lm_linearise_done(lm);
+            }
+        }
+        if (tid == 0) lm_end_attempt(lm);
+        __syncthreads();
+    }
+    if (tid == 0) out[blockIdx.x] = *lm;
+}
+
+}  // namespace
+
+
+void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out, hipStream_t s) {
+    if (count <= 0) return;
+    if (mixed_path) {
+        if (threads == 256) k_batch_solve<float, 256><<<count, 256, 0, s>>>(wins, out);
+        else if (threads == 512) k_batch_solve<float, 512><<<count, 512, 0, s>>>(wins, out);
+        else k_batch_solve<float, 1024><<<count, 1024, 0, s>>>(wins, out);
+    } else {
+        if (threads == 256) k_batch_solve<double, 256><<<count, 256, 0, s>>>(wins, out);
+        else if (threads == 512) k_batch_solve<double, 512><<<count, 512, 0, s>>>(wins, out);
+        else k_batch_solve<double, 1024><<<count, 1024, 0, s>>>(wins, out);
+    }
+}
+
+}  // namespace zb

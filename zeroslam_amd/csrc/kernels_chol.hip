@@ -20,6 +20,7 @@
 // Failure rules are the reference's: decomposition fails when a pivot is not
 // greater than 1e-14 * max|S_ii| (NaN fails), the solve fails when any |L_ii| is
 // not greater than 1e-7 * max|L_ii|.
+#include "chol_device.hpp"
 #include "common.hpp"
 
 #include <cmath>
@@ -27,14 +28,9 @@
 namespace zb {
 namespace {
 
-constexpr int NB = 32;
-
-__device__ __forceinline__ double lane_value(double v, int j) {
-    const long long bits = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits), j);
-    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), j);
-    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
+constexpr int NB = wg::kTile;
+using wg::factor_diag_tile;
+using wg::lane_value;
 
 __device__ __forceinline__ bool skip(const lm_state* lm, int gated, const int32_t* status) {
     return (gated && lm->done) || *status != 0;
@@ -93,56 +89,6 @@ __device__ __forceinline__ void panel_update(double (*own)[NB + 1], const double
         for (int m = 0; m < NB; ++m) v = v - LI[r][m] * LJ[c][m];
         own[r][c] = v;
     }
-}
-
-// Correctly rounded a / b from y = RN(1 / b): q = RN(a y) is within an ulp of
-// a / b, e = a - q b is exact (one FMA) and RN(q + e y) is RN(a / b)
-// (Markstein's correction; also checked against the division on 5e8 operand
-// pairs).  Outside the range where the correction is proven (non-finite or
-// near-subnormal quotients) it divides.
-__device__ __forceinline__ double div_cr(double a, double b, double y) {
-    const double q = a * y;
-    const double e = fma(-q, b, a);
-    double r = fma(e, y, q);
-    if (!(fabs(r) < 1e300) || fabs(r) < 1e-290) r = a / b;
-    return r;
-}
-
-// Factorisation of a diagonal tile by wave 0 with row r in the registers of lane
-// r; column j is broadcast lane by lane.  Every lane forms L_jj and its
-// reciprocal; the updates run on all lanes without masks, because entries above
-// the diagonal are never read.  Returns false (uniformly) when a pivot is not
-// greater than eps.
-__device__ bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* ok_sh) {
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        double row[NB];
-#pragma unroll
-        for (int c = 0; c < NB; ++c) row[c] = (lane < nr) ? D[lane][c] : 0.0;
-        int ok = 1;
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < nr) {
-                const double d = lane_value(row[j], j);
-                ok = ok && (d > eps);
-                const double l = sqrt(d);
-                const double y = 1.0 / l;
-                row[j] = (lane == j) ? l : div_cr(row[j], l, y);
-#pragma unroll
-                for (int c = j + 1; c < NB; ++c) {
-                    const double lcj = lane_value(row[j], c);
-                    row[c] = row[c] - row[j] * lcj;
-                }
-            }
-        }
-        if (lane < nr) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) D[lane][c] = row[c];
-        }
-        if (lane == 0) *ok_sh = ok;
-    }
-    __syncthreads();
-    return *ok_sh != 0;
 }
 
 __device__ __forceinline__ void load_tile(double (*dst)[NB + 1], const double* A, int n, int r0, int c0, int nr, int nc,
@@ -242,12 +188,8 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
 }
 
 // ---- whole factorisation in one workgroup -------------------------------------
-// Used when every column panel of the envelope is at most kFusedTiles tiles tall:
-// panel k (row tiles k .. tile_last[k] of tile column k, already updated by the
-// panels before it) is staged in LDS, its diagonal tile factored by wave 0, the
-// tiles below solved one row per thread, the panel written back, and the trailing
-// tiles it touches updated element by element from LDS.  Workgroup barriers
-// order the panels; nothing waits on another workgroup.
+// Used when every column panel of the envelope is at most kFusedTiles tiles tall
+// (wg::fused_factor).
 constexpr int kFusedTiles = 16;
 
 __global__ void __launch_bounds__(1024) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
@@ -255,73 +197,7 @@ __global__ void __launch_bounds__(1024) k_chol_fused(int n, double* A, const int
     if ((gated && lm->done) || status[0] != 0) return;
     __shared__ double Pn[kFusedTiles][NB][NB + 1];
     __shared__ int ok_sh;
-    const int T = (n + NB - 1) / NB;
-    const double eps = eps_p[0];
-    for (int k = 0; k < T; ++k) {
-        const int k0 = k * NB;
-        const int nk = (n - k0) < NB ? (n - k0) : NB;
-        const int h = tile_last[k] - k + 1;
-        const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
-        const int rows = rows_end - k0;  // panel rows, the diagonal tile's first
-        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
-            const int r = idx / NB, c = idx % NB;
-            const int t = r / NB, rr = r % NB;
-            Pn[t][rr][c] = (c < nk && (t > 0 || c <= rr)) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
-        }
-        __syncthreads();
-        if (!factor_diag_tile(Pn[0], nk, eps, &ok_sh)) {
-            if (threadIdx.x == 0) status[0] = 1;
-            return;
-        }
-        // rows below: L_r = A_r L_kk^-T, element c gets -= x_m L_cm (m < c) then / L_cc
-        if (static_cast<int>(threadIdx.x) < rows - nk) {
-            const int r = nk + threadIdx.x;
-            const int t = r / NB, rr = r % NB;
-            double x[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) x[c] = Pn[t][rr][c];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                if (c < nk) {
-                    double v = x[c];
-#pragma unroll
-                    for (int m = 0; m < c; ++m) v = v - x[m] * Pn[0][c][m];
-                    x[c] = v / Pn[0][c][c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NB; ++c) Pn[t][rr][c] = x[c];
-        }
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
-            const int r = idx / NB, c = idx % NB;
-            const int t = r / NB, rr = r % NB;
-            if (c < nk && (t > 0 || c <= rr)) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = Pn[t][rr][c];
-        }
-        // trailing tiles (I, J), k < J <= I <= tile_last[k]: a -= L_I,k L_J,k^T
-        // over the panel's 32 columns in order (only full panels have tiles below)
-        const int m = h - 1;
-        const int64_t elems = static_cast<int64_t>(m) * (m + 1) / 2 * (NB * NB);
-        for (int64_t e = threadIdx.x; e < elems; e += blockDim.x) {
-            const int pair = static_cast<int>(e / (NB * NB));
-            const int within = static_cast<int>(e % (NB * NB));
-            int Ip = static_cast<int>((sqrt(8.0 * pair + 1.0) - 1.0) / 2.0);
-            while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
-            while (Ip * (Ip + 1) / 2 > pair) --Ip;
-            const int Jp = pair - Ip * (Ip + 1) / 2;
-            const int rr = within / NB, cc = within % NB;
-            const int pr = (Ip + 1) * NB + rr, pc = (Jp + 1) * NB + cc;  // panel rows of the element's row, column
-            if (pr >= rows || pc >= rows || (Ip == Jp && cc > rr)) continue;
-            double* out = A + static_cast<int64_t>(k0 + pr) * n + (k0 + pc);
-            double v = *out;
-            const double* Li = Pn[Ip + 1][rr];
-            const double* Lj = Pn[Jp + 1][cc];
-#pragma unroll 8
-            for (int q = 0; q < NB; ++q) v = v - Li[q] * Lj[q];
-            *out = v;
-        }
-        __syncthreads();
-    }
+    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], Pn, &ok_sh) && threadIdx.x == 0) status[0] = 1;
 }
 
 __device__ __forceinline__ double readlane_d(double v, int j) {
@@ -415,13 +291,8 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
     if (lane == 0) lm->solve_ok = 1;
 }
 
-// Forward and backward substitution in one workgroup; sets lm->solve_ok.
-// x lives in LDS (kTrsvLds entries) or, for larger systems, in the output.
-// Per diagonal tile, wave 0 runs the recurrence with x_j broadcast by readlane
-// and the divisions by L_jj taken through reciprocals formed in parallel
-// (div_cr, correctly rounded); the rows the tile touches are then updated by all
-// threads.  Forward substitution subtracts L_ij x_j in ascending j, the
-// reference's sequence (matrix_decomposition_cholesky.hpp:76-84).
+// Forward and backward substitution in one workgroup (wg::triangular_solve);
+// sets lm->solve_ok.  x lives in LDS for n <= kTrsvLds, otherwise in the output.
 constexpr int kTrsvLds = 4096;
 
 template <bool kLds>
@@ -436,101 +307,20 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
     __shared__ double sh[1024];
     __shared__ double T[NB][NB + 1];
     __shared__ double xl[kLds ? kTrsvLds : 1];
-    __shared__ int fail;
+    __shared__ int flag;
     double* x = kLds ? xl : xout;
-    const int tid = threadIdx.x;
-    // eps = 1e-7 * max|L_ii| (matrix_decomposition_cholesky.hpp:70-74)
-    double m = 0;
-    for (int i = tid; i < n; i += blockDim.x) {
-        const double a = fabs(L[static_cast<int64_t>(i) * n + i]);
-        m = (m < a) ? a : m;
-    }
-    sh[tid] = m;
-    if (tid == 0) fail = 0;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            const double a = sh[tid + w];
-            sh[tid] = (sh[tid] < a) ? a : sh[tid];
-        }
-        __syncthreads();
-    }
-    const double eps = sh[0] * 1e-7;
-    for (int i = tid; i < n; i += blockDim.x) {
-        if (!(fabs(L[static_cast<int64_t>(i) * n + i]) > eps)) fail = 1;
-        x[i] = b[i];
-    }
-    __syncthreads();
-    if (fail) {
-        if (tid == 0) {
-            *status = 2;
+    const int rc = wg::triangular_solve(n, L, b, x, tile_first, tile_last, T, sh, &flag, forward_only != 0);
+    if (rc != 0) {
+        if (threadIdx.x == 0) {
+            *status = rc;
             lm->solve_ok = 0;
         }
         return;
     }
-    const int tiles = (n + NB - 1) / NB;
-    const int lane = tid & 63;
-    for (int t = 0; t < tiles; ++t) {
-        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-        for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-            const int r = idx / nb, c = idx % nb;
-            T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
-        }
-        __syncthreads();
-        if (tid < 64) {
-            double xi = (lane < nb) ? x[i0 + lane] : 0.0;
-            const double dl = (lane < nb) ? T[lane][lane] : 1.0;
-            const double yl = 1.0 / dl;
-            for (int j = 0; j < nb; ++j) {
-                if (lane == j) xi = div_cr(xi, dl, yl);
-                const double xj = lane_value(xi, j);
-                if (lane > j) xi = xi - T[lane < NB ? lane : 0][j] * xj;
-            }
-            if (lane < nb) x[i0 + lane] = xi;
-        }
-        __syncthreads();
-        const int iend = (tile_last[t] + 1) * NB < n ? (tile_last[t] + 1) * NB : n;
-        for (int i = i0 + nb + tid; i < iend; i += blockDim.x) {
-            double v = x[i];
-            const double* Li = L + static_cast<int64_t>(i) * n + i0;
-            for (int j = 0; j < nb; ++j) v = v - Li[j] * x[i0 + j];
-            x[i] = v;
-        }
-        __syncthreads();
-    }
-    if (!forward_only) {
-        for (int t = tiles - 1; t >= 0; --t) {
-            const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-            for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-                const int r = idx / nb, c = idx % nb;
-                T[r][c] = (c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : 0.0;
-            }
-            __syncthreads();
-            if (tid < 64) {
-                double xi = (lane < nb) ? x[i0 + lane] : 0.0;
-                const double dl = (lane < nb) ? T[lane][lane] : 1.0;
-                const double yl = 1.0 / dl;
-                for (int j = nb - 1; j >= 0; --j) {
-                    if (lane == j) xi = div_cr(xi, dl, yl);
-                    const double xj = lane_value(xi, j);
-                    if (lane < j) xi = xi - T[j][lane] * xj;
-                }
-                if (lane < nb) x[i0 + lane] = xi;
-            }
-            __syncthreads();
-            // columns left of the tile, from the first column row tile t touches
-            for (int i = tile_first[t] * NB + tid; i < i0; i += blockDim.x) {
-                double v = x[i];
-                for (int j = 0; j < nb; ++j) v = v - L[static_cast<int64_t>(i0 + j) * n + i] * x[i0 + j];
-                x[i] = v;
-            }
-            __syncthreads();
-        }
-    }
     if (kLds) {
-        for (int i = tid; i < n; i += blockDim.x) xout[i] = x[i];
+        for (int i = threadIdx.x; i < n; i += blockDim.x) xout[i] = x[i];
     }
-    if (tid == 0 && !forward_only) lm->solve_ok = 1;
+    if (threadIdx.x == 0 && !forward_only) lm->solve_ok = 1;
 }
 
 void launch_trsv(int n, const double* L, const double* b, double* x, const dev_work64& w, hipStream_t s, int gated,

@@ -13,6 +13,8 @@
 // applied, ...), so the host can enqueue attempts without reading anything back.
 #include "ba_math.hpp"
 #include "common.hpp"
+#include "lm_control.hpp"
+#include "mixed_items.hpp"
 
 #include <cmath>
 
@@ -272,33 +274,9 @@ __global__ void __launch_bounds__(256) k_reduce_pose(dev_problem p, const double
     else bp[6L * wave + (lane - 21)] = acc;
 }
 
-__global__ void k_linearise_done(lm_state* lm) {
-    lm->need_linearize = 0;
-    lm->lin_count += 1;
-}
+__global__ void k_linearise_done(lm_state* lm) { lm_linearise_done(lm); }
 
 // ---- LM initialisation, factor_graph.hpp:511-522 -----------------------------
-// max_diagonal = max over all pose then point diagonal entries with
-// std::max(|h|, m) semantics (a NaN is kept only until the next entry).
-struct max_scan {
-    double tail;   // max of the entries after the last NaN
-    int has_nan;
-    int last_nan;  // the final entry was NaN
-    int empty;
-};
-__device__ max_scan scan_combine(max_scan a, max_scan b) {
-    if (b.empty) return a;
-    if (a.empty) return b;
-    if (b.has_nan) return b;
-    max_scan r;
-    r.has_nan = a.has_nan;
-    r.last_nan = 0;
-    r.empty = 0;
-    r.tail = (a.tail < b.tail) ? b.tail : a.tail;
-    if (a.last_nan) r.tail = b.tail;
-    return r;
-}
-
 template <class TS>
 __global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const TS* Hpp, const TS* Hll, lm_state* lm) {
     __shared__ max_scan sh[1024];
@@ -315,45 +293,14 @@ __global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const TS* Hpp, 
             const int64_t j = i - 6L * p.np;
             h = static_cast<double>(Hll[9 * (j / 3) + 4 * (j % 3)]);
         }
-        const double a = fabs(h);
-        if (isnan(a)) {
-            m.has_nan = 1; m.last_nan = 1; m.empty = 0; m.tail = 0.0;
-        } else {
-            if (m.empty || m.last_nan) { m.tail = a; } else { m.tail = (a < m.tail) ? m.tail : a; }
-            m.last_nan = 0; m.empty = 0;
-        }
+        m = scan_push(m, h);
     }
     sh[threadIdx.x] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         max_scan acc = {0.0, 0, 0, 1};
         for (unsigned t = 0; t < blockDim.x; ++t) acc = scan_combine(acc, sh[t]);
-        // std::max(|h|, m) starting from m = 0: a final NaN leaves NaN, otherwise
-        // the max of the entries after the last NaN (and of 0 when none).
-        double max_diagonal;
-        if (acc.empty) max_diagonal = 0.0;
-        else if (acc.last_nan) max_diagonal = NAN;
-        else max_diagonal = acc.has_nan ? acc.tail : ((acc.tail < 0.0) ? 0.0 : acc.tail);
-        max_diagonal = (max_diagonal < 5e10) ? max_diagonal : 5e10;  // std::min(5e10, m)
-        const double tau = 1e-5;
-        lm->lambda = tau * max_diagonal;
-        lm->nu = 2.0;
-        lm->legacy_threshold = 1e-10 * lm->chi2;
-        lm->initial_chi2 = lm->chi2;
-        lm->last_chi2 = lm->chi2;
-        lm->iter = 0;
-        lm->failures = 0;
-        lm->success = 0;
-        lm->attempts = 0;
-        lm->rejected = 0;
-        lm->solve_failures = 0;
-        lm->executed_iterations = 0;
-        lm->need_linearize = 0;
-        lm->need_rollback = 0;
-        lm->step_applied = 0;
-        lm->solve_ok = 0;
-        lm->lin_count = 1;
-        lm->done = (lm->iterations <= 0) ? 1 : 0;
+        lm_init(lm, scan_max_diagonal(acc));
     }
 }
 
@@ -579,36 +526,7 @@ __global__ void __launch_bounds__(256) k_bs_exact(dev_problem p, const double* Y
     bs[6L * pose + lane] = bp[6L * pose + lane] - yb;
 }
 
-// ---- after the reduced solve: failure bookkeeping, factor_graph.hpp:542-551 ----
-__device__ void outer_end(lm_state* lm) {
-    const double floor_chi2 = 1e-12;
-    const double m = (lm->last_chi2 < floor_chi2) ? floor_chi2 : lm->last_chi2;  // std::max(last, 1e-12)
-    const double threshold = lm->use_relative ? (1e-6 * m) : lm->legacy_threshold;
-    const bool stop = (lm->failures >= 10) || ((lm->last_chi2 - lm->chi2) < threshold) || !isfinite(lm->lambda);
-    lm->executed_iterations += 1;
-    lm->iter += 1;
-    if (stop || lm->iter >= lm->iterations) {
-        lm->done = 1;
-    } else {
-        lm->last_chi2 = lm->chi2;
-        lm->failures = 0;
-    }
-}
-
-__global__ void k_after_solve(lm_state* lm) {
-    if (lm->done) return;
-    lm->attempts += 1;
-    if (lm->solve_ok) {
-        lm->step_applied = 1;
-        return;
-    }
-    lm->step_applied = 0;
-    lm->solve_failures += 1;
-    lm->lambda *= lm->nu;
-    lm->nu *= 2.0;
-    lm->failures += 1;
-    if (!isfinite(lm->lambda) || lm->failures >= 10) outer_end(lm);
-}
+__global__ void k_after_solve(lm_state* lm) { lm_after_solve(lm); }
 
 // ---- back-substitution + state update + LM scale -------------------------------
 // Threads [0, np) update poses (T <- exp(dx) T), threads [np, np + nl) solve
@@ -703,51 +621,20 @@ __global__ void __launch_bounds__(kBlock) k_backsub_update(dev_problem p, double
     }
 }
 
-// ---- iterative refinement of the mixed solve in binary64 ------------------------
-// The mixed path factorises a reduced system formed from binary32 blocks.  Its
-// step (dx_p, dx_l) is refined against the binary64 normal equations of the
-// current linearisation, applied without forming them: per observation the
-// Jacobians and robust weight are recomputed in binary64 (the linearisation's
-// sequence) and
-//   r_p,i = b_p,i - lambda dx_p,i - sum_obs J_p^T Omega' (J_p dx_p,i + J_l dx_l,l)
-//   r_l,l = b_l,l - lambda dx_l,l - sum_obs J_l^T Omega' (J_p dx_p,i + J_l dx_l,l)
-// (rows of fixed vertices are zero).  The correction solves the binary32-block
-// system for (r_p, r_l) by the same elimination:  rs = r_p - W H_ll^-1 r_l,
-// delta_p = S^-1 rs (the Cholesky factor of S), delta_l = H_ll^-1 (r_l - W^T delta_p).
+// ---- iterative refinement of the mixed solve in binary64 (mixed_items.hpp) ----
 __device__ __forceinline__ bool refine_skip(const lm_state* lm, int gated, const int32_t* status) {
     return (gated && lm->done) || *status != 0;
 }
 
-// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p(a)) for the solved dx_p.
 template <class TS>
 __global__ void __launch_bounds__(kBlock) k_point_step(dev_problem p, const TS* Wbuf, const TS* Hinv, const double* bl,
                                                        const double* dxp, double* dxl, const lm_state* lm, int gated,
                                                        const int32_t* status) {
     if (refine_skip(lm, gated, status)) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= p.nl) return;
-    double t[3] = {0.0, 0.0, 0.0};
-    for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
-        const int k = p.pair_pose[a];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            double v = 0.0;
-#pragma unroll
-            for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * dxp[6L * k + r];
-            t[c] += v;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        double o = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (bl[3L * l + c] - t[c]);
-        dxl[3L * l + r] = o;
-    }
+    if (l < p.nl) mixed::point_step_item(p, Wbuf, Hinv, bl, dxp, dxl, l);
 }
 
-// Per point: r_l and g_l = H_ll^-1 r_l; the pose-side products J_p^T Omega' u of
-// each observation go to its pose-major slot.
 template <class TS>
 __global__ void __launch_bounds__(kBlock) k_full_residual(dev_problem p, const double* pose, const double* point,
                                                           const double* dxp, const double* dxl, const double* bl,
@@ -756,72 +643,12 @@ __global__ void __launch_bounds__(kBlock) k_full_residual(dev_problem p, const d
                                                           const int32_t* status) {
     if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= p.nl) return;
-    const bool lfixed = p.point_fixed[l] != 0;
-    const double lam = lambda_override ? *lambda_override : lm->lambda;
-    double X[3], dl[3];
-    load3(point + 3L * l, X);
-    load3(dxl + 3L * l, dl);
-    double acc[3] = {0.0, 0.0, 0.0};
-    for (int s = p.lm_start[l]; s < p.lm_start[l + 1]; ++s) {
-        const int ip = p.obs_pose[s];
-        const bool pfixed = p.pose_fixed[ip] != 0;
-        if (pfixed && lfixed) continue;
-        const int32_t param = p.obs_param[s];
-        const double* K = cam_of(p, param);
-        double P[7], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
-        load7(pose + 7L * ip, P);
-        pose_apply(P, X, Pc);
-        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
-        residual(Pc, uv, K, r);
-        pose_quat(P, q);
-        jacobians(Pc, q, K, Jp, Jl);
-        info_of(p, s, W);
-        int32_t kind;
-        double delta;
-        loss_of(p, param, kind, delta);
-        robust_info(kind, delta, W, r, drho, Wr);
-        double u[2] = {0.0, 0.0};
-        if (!pfixed) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m)
-#pragma unroll
-                for (int k = 0; k < 6; ++k) u[m] += Jp[6 * m + k] * dxp[6L * ip + k];
-        }
-        if (!lfixed) {
-#pragma unroll
-            for (int m = 0; m < 2; ++m)
-#pragma unroll
-                for (int k = 0; k < 3; ++k) u[m] += Jl[3 * m + k] * dl[k];
-        }
-        const double wu[2] = {Wr[0] * u[0] + Wr[1] * u[1], Wr[2] * u[0] + Wr[3] * u[1]};
-        if (!pfixed) {
-            double* out = pslots + 6L * p.obs_pose_pos[s];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) out[k] = Jp[k] * wu[0] + Jp[6 + k] * wu[1];
-        }
-        if (!lfixed) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) acc[k] += Jl[k] * wu[0] + Jl[3 + k] * wu[1];
-        }
-    }
-    double rr[3] = {0.0, 0.0, 0.0};
-    if (!lfixed) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) rr[k] = bl[3L * l + k] - lam * dl[k] - acc[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) rl[3L * l + k] = rr[k];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        double o = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * rr[c];
-        gl[3L * l + r] = o;
-    }
+    if (l < p.nl)
+        mixed::full_residual_item(p, pose, point, dxp, dxl, bl, Hinv, pslots, rl, gl,
+                                  lambda_override ? *lambda_override : lm->lambda, l);
 }
 
-// One wave per pose: rs_i = r_p,i - sum_{a of pose i} W_a g_l(a).
+// One wave per pose.
 template <class TS>
 __global__ void __launch_bounds__(256) k_pose_residual(dev_problem p, const double* pslots, const double* bp,
                                                        const double* dxp, const TS* Wbuf, const double* gl, double* rs,
@@ -829,103 +656,23 @@ __global__ void __launch_bounds__(256) k_pose_residual(dev_problem p, const doub
                                                        const int32_t* status) {
     if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (i >= p.np) return;
-    const bool fixed = p.pose_fixed[i] != 0;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (!fixed) {
-        for (int t = p.pose_start[i] + lane; t < p.pose_start[i + 1]; t += 64) {
-#pragma unroll
-            for (int k = 0; k < 6; ++k) acc[k] += pslots[6L * t + k];
-        }
-        for (int t = p.pose_pair_start[i] + lane; t < p.pose_pair_start[i + 1]; t += 64) {
-            const int a = p.pose_pairs[t];
-            const int l = p.pair_point[a];
-            const double g0 = gl[3L * l], g1 = gl[3L * l + 1], g2 = gl[3L * l + 2];
-            const TS* Wa = Wbuf + 18L * a;
-#pragma unroll
-            for (int r = 0; r < 6; ++r)
-                acc[r] += static_cast<double>(Wa[3 * r]) * g0 + static_cast<double>(Wa[3 * r + 1]) * g1 +
-                          static_cast<double>(Wa[3 * r + 2]) * g2;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 6; ++r) acc[r] = wave_sum(acc[r]);
-    if (lane < 6) {
-        const double lam = lambda_override ? *lambda_override : lm->lambda;
-        double a = 0.0;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) a = (q == lane) ? acc[q] : a;
-        rs[6L * i + lane] = fixed ? 0.0 : bp[6L * i + lane] - lam * dxp[6L * i + lane] - a;
-    }
+    if (i < p.np)
+        mixed::pose_residual_wave(p, pslots, bp, dxp, Wbuf, gl, rs, lambda_override ? *lambda_override : lm->lambda, i,
+                                  threadIdx.x & 63);
 }
 
-// dx_l += H_ll^-1 (r_l - sum_a W_a^T delta_p(a)) per point; dx_p += delta_p.
+// Threads [0, nl): dx_l += H_ll^-1 (r_l - W^T delta_p); threads [nl, nl + n): dx_p += delta_p.
 template <class TS>
 __global__ void __launch_bounds__(kBlock) k_refine_apply(dev_problem p, const TS* Wbuf, const TS* Hinv, const double* rl,
                                                          const double* corr, double* dxp, double* dxl, const lm_state* lm,
                                                          int gated, const int32_t* status) {
     if (refine_skip(lm, gated, status)) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < p.nl) {
-        const int l = t;
-        double tt[3] = {0.0, 0.0, 0.0};
-        for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
-            const int k = p.pair_pose[a];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double v = 0.0;
-#pragma unroll
-                for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * corr[6L * k + r];
-                tt[c] += v;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-            double o = 0.0;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (rl[3L * l + c] - tt[c]);
-            dxl[3L * l + r] += o;
-        }
-    } else if (t < p.nl + p.n) {
-        dxp[t - p.nl] += corr[t - p.nl];
-    }
+    if (t < p.nl) mixed::refine_point_item(p, Wbuf, Hinv, rl, corr, dxl, t);
+    else if (t < p.nl + p.n) dxp[t - p.nl] += corr[t - p.nl];
 }
 
-// ---- accept / reject, factor_graph.hpp:566-605 ---------------------------------
-__global__ void k_decide(lm_state* lm) {
-    if (lm->done || !lm->step_applied) return;
-    const double temp_chi = lm->chi2_new;
-    const double rho = (lm->chi2 - temp_chi) / lm->scale;
-    lm->rho = rho;
-    bool good = false;
-    if ((rho > 0) && isfinite(temp_chi)) {
-        double alpha = 1.0 - cr_cube(2.0 * rho - 1.0);  // math::pow(x, 3.0), correctly rounded
-        alpha = (2.0 / 3.0 < alpha) ? 2.0 / 3.0 : alpha;
-        const double scale_factor = (1.0 / 3.0 < alpha) ? alpha : 1.0 / 3.0;
-        lm->lambda *= scale_factor;
-        lm->nu = 2.0;
-        lm->chi2 = temp_chi;
-        good = true;
-        lm->success += 1;
-    } else {
-        lm->lambda *= lm->nu;
-        lm->nu *= 2.0;
-        lm->rejected += 1;
-    }
-    if (!isfinite(lm->lambda)) {  // the rejected step stays applied (factor_graph.hpp:592-594)
-        outer_end(lm);
-        return;
-    }
-    if (good) {
-        outer_end(lm);
-        if (!lm->done) lm->need_linearize = 1;
-        return;
-    }
-    lm->need_rollback = 1;
-    lm->failures += 1;
-    if (lm->failures >= 10) outer_end(lm);
-}
+__global__ void k_decide(lm_state* lm) { lm_decide(lm); }
 
 __global__ void __launch_bounds__(kBlock) k_rollback(dev_problem p, double* pose, const double* pose_backup, double* point,
                                                      const double* point_backup, const lm_state* lm) {
@@ -941,11 +688,7 @@ __global__ void __launch_bounds__(kBlock) k_rollback(dev_problem p, double* pose
     }
 }
 
-__global__ void k_end_attempt(lm_state* lm) {
-    lm->need_rollback = 0;
-    lm->step_applied = 0;
-    lm->solve_ok = 0;
-}
+__global__ void k_end_attempt(lm_state* lm) { lm_end_attempt(lm); }
 
 // ---- diagnostics ---------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_eval_edges(dev_problem p, const double* pose, const double* point, double* out) {

@@ -1,0 +1,485 @@
+// Per-item device functions of the mixed-precision solve, shared by the
+// per-graph kernels (kernels_fp32.hip, kernels_fp64.hip) and the batched
+// per-window solver (kernels_batch.hip).  TS is the storage type of the Hessian
+// blocks (binary32 on the mixed path, binary64 for a binary64 batch); residuals,
+// robust weights, gradients, the reduced system and all state updates are
+// binary64.  Each function handles one point, pose, pair or reduced-system block;
+// the caller maps items to threads or waves.
+//
+// Buffer layouts: pose-side terms of observation s in pose-major slot
+// obs_pose_pos[s]: 21 H_pp terms (TS) and 6 gradient terms (binary64); H_pp 21
+// lower-triangle entries per pose; H_ll and H_ll^-1 9 per point; W and Y 18 per
+// (pose, point) pair, row-major 6 x 3.
+#pragma once
+
+#include "ba_math.hpp"
+#include "common.hpp"
+
+namespace zb {
+namespace mixed {
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+template <class T> __device__ __forceinline__ T wave_sum(T v);
+template <> __device__ __forceinline__ float wave_sum<float>(float v) { return wave_sum_f(v); }
+template <> __device__ __forceinline__ double wave_sum<double>(double v) { return wave_sum_d(v); }
+
+__device__ __forceinline__ void obs_weight(const dev_problem& p, int s, double W[4]) {
+    if (p.identity_info) {
+        W[0] = 1.0; W[1] = 0.0; W[2] = 0.0; W[3] = 1.0;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) W[i] = p.obs_info[4L * s + i];
+    }
+}
+
+// Robust cost of observation s (edge_base::robust_chi2 of the recomputed
+// residual, factor_graph.hpp:221-228, 990-1009); the residual is stored.
+__device__ inline double cost_item(const dev_problem& p, const double* pose, const double* point, double* resid, int s) {
+    const int32_t param = p.obs_param[s];
+    double P[7], X[3], Pc[3], r[2];
+    const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+#pragma unroll
+    for (int i = 0; i < 7; ++i) P[i] = pose[7L * p.obs_pose[s] + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) X[i] = point[3L * p.obs_point[s] + i];
+    pose_apply(P, X, Pc);
+    residual(Pc, uv, p.cams + 4 * (param & 0xFFFF), r);
+    resid[2L * s] = r[0];
+    resid[2L * s + 1] = r[1];
+    double W[4];
+    obs_weight(p, s, W);
+    const int li = param >> 16;
+    return robust_chi2(p.loss_kind[li], p.loss_delta[li], r, W);
+}
+
+// make_hessian (factor_graph.hpp:648-732) for the observations of point l, in
+// insertion order: H_ll and the W blocks accumulate in TS, b_l in binary64; the
+// pose-side terms go to pose-major slots.  Residual, robust weight and Jacobians
+// are binary64 in the reference's order; the Hessian products use the Jacobians
+// rounded to TS.
+template <class TS>
+__device__ inline void lin_point(const dev_problem& p, const double* pose, const double* point, TS* pterms, double* pgrad,
+                                 TS* Hll, double* bl, TS* Wbuf, int l) {
+    const bool lfixed = p.point_fixed[l] != 0;
+    const double X[3] = {point[3L * l], point[3L * l + 1], point[3L * l + 2]};
+    TS H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double b[3] = {0, 0, 0};
+    const int s0 = p.lm_start[l], s1 = p.lm_start[l + 1];
+    for (int s = s0; s < s1; ++s) {
+        const int ip = p.obs_pose[s];
+        const bool pfixed = p.pose_fixed[ip] != 0;
+        if (pfixed && lfixed) continue;
+        const int32_t param = p.obs_param[s];
+        const double* K = p.cams + 4 * (param & 0xFFFF);
+        double P[7], Pc[3], q[4];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) P[i] = pose[7L * ip + i];
+        pose_apply(P, X, Pc);
+        pose_quat(P, q);
+        double r[2], W[4], Wr[4], drho, Jp[12], Jl[6];
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        residual(Pc, uv, K, r);
+        obs_weight(p, s, W);
+        const int li = param >> 16;
+        robust_info(p.loss_kind[li], p.loss_delta[li], W, r, drho, Wr);
+        jacobians(Pc, q, K, Jp, Jl);
+        TS Jpf[12], Jlf[6], Wrf[4];
+#pragma unroll
+        for (int i = 0; i < 12; ++i) Jpf[i] = static_cast<TS>(Jp[i]);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) Jlf[i] = static_cast<TS>(Jl[i]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Wrf[i] = static_cast<TS>(Wr[i]);
+        if (!pfixed) {
+            TS H21[21], Wpl[18];
+            double v[6];
+            pose_hessian(Jpf, Jlf, Wrf, H21, Wpl);
+            pose_grad(Jp, W, r, drho, v);
+            const int64_t slot = p.obs_pose_pos[s];
+#pragma unroll
+            for (int i = 0; i < 21; ++i) pterms[21 * slot + i] = H21[i];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) pgrad[6 * slot + i] = v[i];
+            if (!lfixed) {
+                const int pr = p.obs_pair[s];
+                TS* Wp = Wbuf + 18L * pr;
+                if (p.pair_first[pr] == s) {
+#pragma unroll
+                    for (int i = 0; i < 18; ++i) Wp[i] = Wpl[i];
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 18; ++i) Wp[i] = Wp[i] + Wpl[i];
+                }
+            }
+        }
+        if (!lfixed) {
+            TS Hl[9];
+            double v[3];
+            point_hessian(Jlf, Wrf, Hl);
+            point_grad(Jl, W, r, drho, v);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) H[i] = H[i] + Hl[i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) b[i] = b[i] - v[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hll[9L * l + i] = H[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bl[3L * l + i] = b[i];
+}
+
+// Lane j < 27 of the wave for pose i: H_pp term j (j < 21, TS) or gradient term
+// j - 21 (binary64) summed over the pose's slots (H_pp += H; b_p -= v).
+template <class TS>
+__device__ inline void reduce_pose_lane(const dev_problem& p, const TS* pterms, const double* pgrad, TS* Hpp, double* bp,
+                                        int i, int lane) {
+    const bool fixed = p.pose_fixed[i] != 0;
+    const int t0 = p.pose_start[i], t1 = fixed ? t0 : p.pose_start[i + 1];
+    constexpr int U = 16;
+    if (lane < 21) {
+        const TS* src = pterms + lane;
+        TS part[4] = {0, 0, 0, 0};
+        int t = t0;
+        for (; t + U <= t1; t += U) {
+            TS v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[21L * (t + u)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) part[u & 3] += v[u];
+        }
+        for (; t < t1; ++t) part[0] += src[21L * t];
+        Hpp[21L * i + lane] = (part[0] + part[1]) + (part[2] + part[3]);
+    } else if (lane < 27) {
+        const double* src = pgrad + (lane - 21);
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        int t = t0;
+        for (; t + U <= t1; t += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[6L * (t + u)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) part[u & 3] += v[u];
+        }
+        for (; t < t1; ++t) part[0] += src[6L * t];
+        bp[6L * i + (lane - 21)] = -((part[0] + part[1]) + (part[2] + part[3]));
+    }
+}
+
+// Damped point block and its inverse (factor_graph.hpp:742-765).
+template <class TS>
+__device__ inline void damp_invert_item(const dev_problem& p, const TS* Hll, TS* Hinv, double lambda, int l) {
+    const TS lam = static_cast<TS>(lambda);
+    TS H[9], inv[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = Hll[9L * l + i];
+    H[0] += lam; H[4] += lam; H[8] += lam;
+    if (p.point_fixed[l]) {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) inv[i] = TS(0);
+    } else {
+        invert3(H, inv);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hinv[9L * l + i] = inv[i];
+}
+
+// Y = W H_ll^-1 for pair a (matrix_sparse_block_diagonal.hpp:107-122).
+template <class TS>
+__device__ inline void form_Y_item(const dev_problem& p, const TS* Wbuf, const TS* Hinv, TS* Y, int a) {
+    TS Wa[18], Hi[9], o[18];
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Wa[i] = Wbuf[18L * a + i];
+    const int l = p.pair_point[a];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Hi[i] = Hinv[9L * l + i];
+    mul<6, 3, 3>(Wa, Hi, o);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) Y[18L * a + i] = o[i];
+}
+
+// One wave per 6x6 lower-triangle block of the reduced camera system: the
+// products Y_a W_c^T accumulate in TS per lane and across the wave, Y b_l in
+// binary64; the block is assembled in binary64:
+//   S_ik = [i == k] (H_pp,i + lambda I) - sum Y W^T,  b_s,i = b_p,i - sum Y b_l
+// (factor_graph.hpp:768-781, matrix_sparse_block.hpp:158-196).
+template <class TS>
+__device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* Wbuf, const TS* Hpp, const double* bp,
+                                   const double* bl, double* S, double* bs, double lam, int blk, int lane) {
+    const int row = p.blk_row[blk], col = p.blk_col[blk];
+    const bool diag = row == col;
+    TS acc[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) acc[i] = TS(0);
+    double accb[6] = {0, 0, 0, 0, 0, 0};
+    const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
+    for (int64_t c = c0 + lane; c < c1; c += 64) {
+        const int2 ab = p.contrib[c];
+        TS Ya[18], Wc[18];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc)
+                acc[r * 6 + cc] += Ya[r * 3] * Wc[cc * 3] + Ya[r * 3 + 1] * Wc[cc * 3 + 1] + Ya[r * 3 + 2] * Wc[cc * 3 + 2];
+        if (diag) {
+            const int l = p.pair_point[ab.x];
+            const double b0 = bl[3L * l], b1 = bl[3L * l + 1], b2 = bl[3L * l + 2];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                accb[r] += static_cast<double>(Ya[r * 3]) * b0 + static_cast<double>(Ya[r * 3 + 1]) * b1 +
+                           static_cast<double>(Ya[r * 3 + 2]) * b2;
+        }
+    }
+    const int64_t n = p.n;
+    double* Sblk = S + (6L * row) * n + 6L * col;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) {
+        const TS tot = wave_sum<TS>(acc[i]);
+        const int r = i / 6, c = i % 6;
+        if (lane == 0) {
+            if (diag) {
+                if (c <= r) {
+                    double base = static_cast<double>(Hpp[21L * row + r * (r + 1) / 2 + c]);
+                    if (r == c) base += lam;
+                    Sblk[r * n + c] = base - static_cast<double>(tot);
+                }
+            } else {
+                Sblk[r * n + c] = -static_cast<double>(tot);
+            }
+        }
+    }
+    if (diag) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            const double tot = wave_sum_d(accb[i]);
+            if (lane == 0) bs[6L * row + i] = bp[6L * row + i] - tot;
+        }
+    }
+}
+
+// ---- iterative refinement of the mixed solve in binary64 ------------------------
+// The step (dx_p, dx_l) solved with the TS-block reduced system is refined
+// against the binary64 normal equations of the current linearisation, applied
+// without forming them: per observation the Jacobians and robust weight are
+// recomputed in binary64 (the linearisation's sequence) and
+//   r_p,i = b_p,i - lambda dx_p,i - sum_obs J_p^T Omega' (J_p dx_p,i + J_l dx_l,l)
+//   r_l,l = b_l,l - lambda dx_l,l - sum_obs J_l^T Omega' (J_p dx_p,i + J_l dx_l,l)
+// (rows of fixed vertices are zero).  The correction solves the TS-block system
+// for (r_p, r_l) by the same elimination:  rs = r_p - W H_ll^-1 r_l,
+// delta_p = S^-1 rs (the Cholesky factor of S), delta_l = H_ll^-1 (r_l - W^T delta_p).
+
+// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p(a)) for point l.
+template <class TS>
+__device__ inline void point_step_item(const dev_problem& p, const TS* Wbuf, const TS* Hinv, const double* bl,
+                                       const double* dxp, double* dxl, int l) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+        const int k = p.pair_pose[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * dxp[6L * k + r];
+            t[c] += v;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double o = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (bl[3L * l + c] - t[c]);
+        dxl[3L * l + r] = o;
+    }
+}
+
+// Point l: r_l and g_l = H_ll^-1 r_l; the pose-side products J_p^T Omega' u of
+// each observation go to its pose-major slot.
+template <class TS>
+__device__ inline void full_residual_item(const dev_problem& p, const double* pose, const double* point, const double* dxp,
+                                          const double* dxl, const double* bl, const TS* Hinv, double* pslots, double* rl,
+                                          double* gl, double lam, int l) {
+    const bool lfixed = p.point_fixed[l] != 0;
+    const double X[3] = {point[3L * l], point[3L * l + 1], point[3L * l + 2]};
+    const double dl[3] = {dxl[3L * l], dxl[3L * l + 1], dxl[3L * l + 2]};
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = p.lm_start[l]; s < p.lm_start[l + 1]; ++s) {
+        const int ip = p.obs_pose[s];
+        const bool pfixed = p.pose_fixed[ip] != 0;
+        if (pfixed && lfixed) continue;
+        const int32_t param = p.obs_param[s];
+        const double* K = p.cams + 4 * (param & 0xFFFF);
+        double P[7], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) P[i] = pose[7L * ip + i];
+        pose_apply(P, X, Pc);
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        residual(Pc, uv, K, r);
+        pose_quat(P, q);
+        jacobians(Pc, q, K, Jp, Jl);
+        obs_weight(p, s, W);
+        const int li = param >> 16;
+        robust_info(p.loss_kind[li], p.loss_delta[li], W, r, drho, Wr);
+        double u[2] = {0.0, 0.0};
+        if (!pfixed) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int k = 0; k < 6; ++k) u[m] += Jp[6 * m + k] * dxp[6L * ip + k];
+        }
+        if (!lfixed) {
+#pragma unroll
+            for (int m = 0; m < 2; ++m)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) u[m] += Jl[3 * m + k] * dl[k];
+        }
+        const double wu[2] = {Wr[0] * u[0] + Wr[1] * u[1], Wr[2] * u[0] + Wr[3] * u[1]};
+        if (!pfixed) {
+            double* out = pslots + 6L * p.obs_pose_pos[s];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) out[k] = Jp[k] * wu[0] + Jp[6 + k] * wu[1];
+        }
+        if (!lfixed) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[k] += Jl[k] * wu[0] + Jl[3 + k] * wu[1];
+        }
+    }
+    double rr[3] = {0.0, 0.0, 0.0};
+    if (!lfixed) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) rr[k] = bl[3L * l + k] - lam * dl[k] - acc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rl[3L * l + k] = rr[k];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double o = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * rr[c];
+        gl[3L * l + r] = o;
+    }
+}
+
+// Wave for pose i: rs_i = r_p,i - sum_{a of pose i} W_a g_l(a).
+template <class TS>
+__device__ inline void pose_residual_wave(const dev_problem& p, const double* pslots, const double* bp, const double* dxp,
+                                          const TS* Wbuf, const double* gl, double* rs, double lam, int i, int lane) {
+    const bool fixed = p.pose_fixed[i] != 0;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (!fixed) {
+        for (int t = p.pose_start[i] + lane; t < p.pose_start[i + 1]; t += 64) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc[k] += pslots[6L * t + k];
+        }
+        for (int t = p.pose_pair_start[i] + lane; t < p.pose_pair_start[i + 1]; t += 64) {
+            const int a = p.pose_pairs[t];
+            const int l = p.pair_point[a];
+            const double g0 = gl[3L * l], g1 = gl[3L * l + 1], g2 = gl[3L * l + 2];
+            const TS* Wa = Wbuf + 18L * a;
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                acc[r] += static_cast<double>(Wa[3 * r]) * g0 + static_cast<double>(Wa[3 * r + 1]) * g1 +
+                          static_cast<double>(Wa[3 * r + 2]) * g2;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[r] = wave_sum_d(acc[r]);
+    if (lane < 6) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a = (q == lane) ? acc[q] : a;
+        rs[6L * i + lane] = fixed ? 0.0 : bp[6L * i + lane] - lam * dxp[6L * i + lane] - a;
+    }
+}
+
+// dx_l += H_ll^-1 (r_l - sum_a W_a^T delta_p(a)) for point l.
+template <class TS>
+__device__ inline void refine_point_item(const dev_problem& p, const TS* Wbuf, const TS* Hinv, const double* rl,
+                                         const double* corr, double* dxl, int l) {
+    double tt[3] = {0.0, 0.0, 0.0};
+    for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
+        const int k = p.pair_pose[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = 0.0;
+#pragma unroll
+            for (int r = 0; r < 6; ++r) v += static_cast<double>(Wbuf[18L * a + 3 * r + c]) * corr[6L * k + r];
+            tt[c] += v;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        double o = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (rl[3L * l + c] - tt[c]);
+        dxl[3L * l + r] += o;
+    }
+}
+
+// ---- state update with the solved step, factor_graph.hpp:556-560, 809-830 ------
+// Slot t < np: pose t (T <- exp(dx) T); slot np + l: point l (X += dx).  Returns
+// the slot's LM scale terms sum_i dx_i (lambda dx_i + b_i).
+__device__ inline double apply_step_item(const dev_problem& p, double* pose, double* pose_backup, double* point,
+                                         double* point_backup, const double* dxp, const double* dxl, const double* bp,
+                                         const double* bl, double lam, int t) {
+    double acc = 0.0;
+    if (t < p.np) {
+        double d[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) d[i] = dxp[6L * t + i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) acc = acc + d[i] * (lam * d[i] + bp[6L * t + i]);
+        if (!p.pose_fixed[t]) {
+            double P[7];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) P[i] = pose[7L * t + i];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) pose_backup[7L * t + i] = P[i];
+            pose_plus(P, d);
+#pragma unroll
+            for (int i = 0; i < 7; ++i) pose[7L * t + i] = P[i];
+        }
+    } else if (t < p.np + p.nl) {
+        const int l = t - p.np;
+        double dl[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) dl[i] = dxl[3L * l + i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc = acc + dl[i] * (lam * dl[i] + bl[3L * l + i]);
+        if (!p.point_fixed[l]) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                point_backup[3L * l + i] = point[3L * l + i];
+                point[3L * l + i] = point[3L * l + i] + dl[i];
+            }
+        }
+    }
+    return acc;
+}
+
+__device__ inline void rollback_item(const dev_problem& p, double* pose, const double* pose_backup, double* point,
+                                     const double* point_backup, int t) {
+    if (t < p.np) {
+        if (!p.pose_fixed[t])
+            for (int i = 0; i < 7; ++i) pose[7L * t + i] = pose_backup[7L * t + i];
+    } else if (t < p.np + p.nl) {
+        const int l = t - p.np;
+        if (!p.point_fixed[l])
+            for (int i = 0; i < 3; ++i) point[3L * l + i] = point_backup[3L * l + i];
+    }
+}
+
+}  // namespace mixed
+}  // namespace zb

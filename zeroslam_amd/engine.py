@@ -211,6 +211,44 @@ class Graph:
         return d_obs, d_K
 
 
+class Batch:
+    """Many independent graphs (local BA windows) solved together (zsba_batch_*).
+
+    Windows that fit one workgroup run in one persistent, hipGraph-replayed launch;
+    the graphs must outlive the batch."""
+
+    def __init__(self, graphs: list[Graph]):
+        self.lib = _lib.load()
+        self.graphs = list(graphs)
+        arr = (C.c_void_p * len(self.graphs))(*[g.h for g in self.graphs])
+        self.h = self.lib.zsba_batch_create(arr, len(self.graphs))
+        if not self.h:
+            raise ValueError("zsba_batch_create failed")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.zsba_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def solve(self, iterations: int, use_relative_convergence: bool = False, precision: int = MIXED) -> np.ndarray:
+        acc = np.zeros(len(self.graphs), np.int32)
+        rc = self.lib.zsba_batch_solve(self.h, int(iterations), int(bool(use_relative_convergence)), int(precision),
+                                       ptr(acc, C.c_int32))
+        if rc < 0:
+            raise ZsbaError(rc, self.lib.zsba_batch_last_error(self.h).decode())
+        return acc
+
+    def info(self) -> dict:
+        inf = _lib.BatchInfo()
+        rc = self.lib.zsba_batch_get_info(self.h, C.byref(inf))
+        if rc < 0:
+            raise ZsbaError(rc, "batch info")
+        return inf.as_dict()
+
+
 def solve_batch(graphs: list[Graph], iterations: int, use_relative_convergence: bool = False,
                 precision: int = MIXED) -> np.ndarray:
     lib = _lib.load()
