@@ -215,6 +215,10 @@ typedef struct zsba_batch_info {
     double autotune_wall_ms;  /* time spent autotuning in this solve (0 when cached) */
     double kernel_wall_ms;    /* batched launch, host-measured, synchronised         */
     double solve_ms;          /* whole zsba_batch_solve call                         */
+    double phase_ms[8];       /* mean per-window device time by phase: cost,
+                                 linearisation, damping + inverses + Y, Schur,
+                                 factorisation, triangular solve, refinement,
+                                 update + controller                                 */
     double reserved[4];
 } zsba_batch_info;
 zsba_batch* zsba_batch_create(zsba_graph* const* graphs, int32_t count);

@@ -55,11 +55,14 @@ class BatchInfo(C.Structure):
     _fields_ = [("windows", C.c_int32), ("individual", C.c_int32), ("threads", C.c_int32),
                 ("graph_launches", C.c_int32), ("tune_ms", C.c_float * 3), ("pad", C.c_float),
                 ("autotune_wall_ms", C.c_double), ("kernel_wall_ms", C.c_double), ("solve_ms", C.c_double),
-                ("reserved", C.c_double * 4)]
+                ("phase_ms", C.c_double * 8), ("reserved", C.c_double * 4)]
+
+    PHASES = ("cost", "linearise", "invert", "schur", "factor", "solve", "refine", "update")
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "pad", "tune_ms")}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "pad", "tune_ms", "phase_ms")}
         d["tune_ms"] = list(self.tune_ms)
+        d["phase_ms"] = {name: self.phase_ms[i] for i, name in enumerate(self.PHASES)}
         return d
 
 

@@ -231,8 +231,15 @@ struct batch_window {
     int32_t iterations = 0, use_relative = 0, refine_steps = 0, pad = 0;
     double floor_rel = 0.0;
 };
-// threads per window: 256, 512 or 1024; out[i] receives window i's final controller.
-void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out, hipStream_t s);
+// Phases timed inside the batched solver.
+enum batch_phase : int {
+    PH_COST = 0, PH_LINEARISE = 1, PH_INVERT = 2, PH_SCHUR = 3, PH_FACTOR = 4, PH_SOLVE = 5, PH_REFINE = 6, PH_UPDATE = 7
+};
+constexpr int kBatchPhases = 8;
+// threads per window: 256, 512 or 1024; out[i] receives window i's final
+// controller, phase_ticks[i * kBatchPhases + k] its time in phase k (100 MHz ticks).
+void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out,
+                        int64_t* phase_ticks, hipStream_t s);
 
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from

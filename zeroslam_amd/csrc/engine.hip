@@ -789,6 +789,8 @@ struct zsba_batch {
     zb::batch_window* d_wins = nullptr;
     lm_state* d_out = nullptr;
     lm_state* h_out = nullptr;
+    int64_t* d_ticks = nullptr;
+    int64_t* h_ticks = nullptr;
     int32_t capacity = 0;
     std::vector<zb::batch_window> h_wins;
     std::vector<int32_t> win_graph;  // window -> graph index
@@ -806,10 +808,14 @@ struct zsba_batch {
         if (d_wins) (void)hipFree(d_wins);
         if (d_out) (void)hipFree(d_out);
         if (h_out) (void)hipHostFree(h_out);
+        if (d_ticks) (void)hipFree(d_ticks);
+        if (h_ticks) (void)hipHostFree(h_ticks);
         if (stream) (void)hipStreamDestroy(stream);
         d_wins = nullptr;
         d_out = nullptr;
         h_out = nullptr;
+        d_ticks = nullptr;
+        h_ticks = nullptr;
         stream = nullptr;
         capacity = 0;
     }
@@ -838,9 +844,17 @@ struct zsba_batch {
             if (d_wins) (void)hipFree(d_wins);
             if (d_out) (void)hipFree(d_out);
             if (h_out) (void)hipHostFree(h_out);
+            if (d_ticks) (void)hipFree(d_ticks);
+            if (h_ticks) (void)hipHostFree(h_ticks);
             d_wins = nullptr;
             d_out = nullptr;
             h_out = nullptr;
+            d_ticks = nullptr;
+            h_ticks = nullptr;
+            const size_t tick_bytes = sizeof(int64_t) * zb::kBatchPhases * count;
+            if ((e = hipMalloc(&d_ticks, tick_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_ticks), tick_bytes, hipHostMallocDefault)) != hipSuccess)
+                return hip_fail(e, "hipHostMalloc");
             if ((e = hipMalloc(&d_wins, sizeof(zb::batch_window) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipMalloc(&d_out, sizeof(lm_state) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_out), sizeof(lm_state) * count, hipHostMallocDefault)) !=
@@ -862,7 +876,7 @@ struct zsba_batch {
             graph = nullptr;
             if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return hip_fail(e, "hipStreamBeginCapture");
-            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, stream);
+            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, d_ticks, stream);
             if ((e = hipStreamEndCapture(stream, &graph)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
                 return hip_fail(e, "hipGraphInstantiate");
@@ -901,7 +915,7 @@ struct zsba_batch {
         int32_t rc = 0;
         for (int32_t threads : {256, 512, 1024}) {
             (void)hipEventRecord(t0, stream);
-            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, stream);
+            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, d_ticks, stream);
             (void)hipEventRecord(t1, stream);
             if ((e = hipEventSynchronize(t1)) != hipSuccess) {
                 rc = hip_fail(e, "autotune");
@@ -1008,6 +1022,9 @@ struct zsba_batch {
         if ((rc = launch(threads, mixed))) return rc;
         if ((e = hipMemcpyAsync(h_out, d_out, sizeof(lm_state) * nw, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download controllers");
+        if ((e = hipMemcpyAsync(h_ticks, d_ticks, sizeof(int64_t) * zb::kBatchPhases * nw, hipMemcpyDeviceToHost,
+                                stream)) != hipSuccess)
+            return hip_fail(e, "download phase clocks");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batch solve");
         info.kernel_wall_ms = now_ms() - t0;
         for (int32_t i = 0; i < nw; ++i) {
@@ -1025,6 +1042,12 @@ struct zsba_batch {
             g->stats.solve_ms = info.kernel_wall_ms;
             g->device_newer = true;
             accepted[win_graph[i]] = s.success;
+        }
+        // mean per-window phase times (the device clock runs at 100 MHz)
+        for (int k = 0; k < zb::kBatchPhases; ++k) {
+            double sum = 0.0;
+            for (int32_t i = 0; i < nw; ++i) sum += static_cast<double>(h_ticks[i * zb::kBatchPhases + k]);
+            info.phase_ms[k] = sum / nw * 1e-5;
         }
         info.solve_ms = now_ms() - t_start;
         return 0;

@@ -61,8 +61,27 @@ __device__ double cost(const dev_problem& p, const dev_work64& w, double* sh) {
     return block_sum(acc, sh);
 }
 
+// Phase clocks (thread 0, after the phase's closing barrier), summed per window
+// into phase_ticks[window * kBatchPhases + phase] in 100 MHz ticks.
+struct phase_clock {
+    int64_t* out;
+    int64_t last;
+    int64_t acc[kBatchPhases];
+    __device__ explicit phase_clock(int64_t* o) : out(o), last(wall_clock64()) {
+        for (int i = 0; i < kBatchPhases; ++i) acc[i] = 0;
+    }
+    __device__ void mark(int phase) {
+        const int64_t t = wall_clock64();
+        acc[phase] += t - last;
+        last = t;
+    }
+    __device__ void flush() {
+        for (int i = 0; i < kBatchPhases; ++i) out[i] = acc[i];
+    }
+};
+
 template <class TS, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wins, lm_state* out) {
+__global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wins, lm_state* out, int64_t* phase_ticks) {
     const batch_window& bw = wins[blockIdx.x];
     const dev_problem& p = bw.p;
     const dev_work64& w = bw.w;
@@ -77,6 +96,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
     const int tid = threadIdx.x, lane = tid & 63, nwaves = blockDim.x >> 6;
     const int n = p.n;
     const bool mixed_path = sizeof(TS) == 4;
+    phase_clock clk(phase_ticks + static_cast<int64_t>(blockIdx.x) * kBatchPhases);
 
     if (tid == 0) {
         *lm = lm_state{};
@@ -88,7 +108,9 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         const double c = cost(p, w, sh);
         if (tid == 0) lm->chi2 = c;
     }
+    clk.mark(PH_COST);
     linearise(p, w, B);
+    clk.mark(PH_LINEARISE);
     {
         // max_diagonal over pose then point diagonal entries, in order
         const int64_t total = 6L * p.np + 3L * p.nl;
@@ -120,9 +142,11 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         __syncthreads();
         for (int a = tid; a < p.npair; a += blockDim.x) mixed::form_Y_item(p, B.W, B.Hinv, B.Y, a);
         __syncthreads();
+        clk.mark(PH_INVERT);
         for (int blk = tid >> 6; blk < p.nblk; blk += nwaves)
             mixed::schur_block(p, B.Y, B.W, B.Hpp, w.bp_m, w.bl_m, w.S, w.bs, lam, blk, lane);
         __syncthreads();
+        clk.mark(PH_SCHUR);
         // factorisation and solve
         double mx = 0.0;
         for (int i = tid; i < n; i += blockDim.x) {
@@ -135,12 +159,15 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
             for (int i = tid; i < n; i += blockDim.x) w.S[static_cast<int64_t>(i) * n + i] += lift;
         __syncthreads();
         bool ok = wg::fused_factor<kBatchTiles>(n, w.S, w.tile_last, (mx + lift) * 1e-14, Pn, &flag);
+        clk.mark(PH_FACTOR);
         if (ok) ok = wg::triangular_solve(n, w.S, w.bs, xs, w.tile_first, w.tile_last, T, sh, &flag, false) == 0;
+        clk.mark(PH_SOLVE);
         if (ok) {
             for (int i = tid; i < n; i += blockDim.x) w.dxp[i] = xs[i];
             __syncthreads();
             for (int l = tid; l < p.nl; l += blockDim.x) mixed::point_step_item(p, B.W, B.Hinv, w.bl_m, w.dxp, w.dxl, l);
             __syncthreads();
+            clk.mark(PH_UPDATE);
             const int steps = mixed_path ? bw.refine_steps : 0;
             for (int k = 0; k < steps; ++k) {
                 for (int l = tid; l < p.nl; l += blockDim.x)
@@ -158,6 +185,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                 for (int i = tid; i < n; i += blockDim.x) w.dxp[i] += xs[i];
                 __syncthreads();
             }
+            clk.mark(PH_REFINE);
         }
         if (tid == 0) {
             lm->solve_ok = ok ? 1 : 0;
@@ -170,7 +198,9 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                 acc = acc + mixed::apply_step_item(p, w.pose, w.pose_backup, w.point, w.point_backup, w.dxp, w.dxl, w.bp_m,
                                                    w.bl_m, lam, t);
             const double scale = block_sum(acc, sh);
+            clk.mark(PH_UPDATE);
             const double c = cost(p, w, sh);
+            clk.mark(PH_COST);
             if (tid == 0) {
                 lm->scale = scale + 1e-3;  // factor_graph.hpp:556-560
                 lm->chi2_new = c;
@@ -183,29 +213,36 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
             }
             if (!lm->done && lm->need_linearize) {
                 __syncthreads();
+                clk.mark(PH_UPDATE);
                 linearise(p, w, B);
+                clk.mark(PH_LINEARISE);
                 if (tid == 0) lm_linearise_done(lm);
             }
         }
         if (tid == 0) lm_end_attempt(lm);
         __syncthreads();
+        clk.mark(PH_UPDATE);
     }
-    if (tid == 0) out[blockIdx.x] = *lm;
+    if (tid == 0) {
+        out[blockIdx.x] = *lm;
+        clk.flush();
+    }
 }
 
 }  // namespace
 
 
-void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out, hipStream_t s) {
+void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out,
+                        int64_t* phase_ticks, hipStream_t s) {
     if (count <= 0) return;
     if (mixed_path) {
-        if (threads == 256) k_batch_solve<float, 256><<<count, 256, 0, s>>>(wins, out);
-        else if (threads == 512) k_batch_solve<float, 512><<<count, 512, 0, s>>>(wins, out);
-        else k_batch_solve<float, 1024><<<count, 1024, 0, s>>>(wins, out);
+        if (threads == 256) k_batch_solve<float, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks);
+        else if (threads == 512) k_batch_solve<float, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks);
+        else k_batch_solve<float, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks);
     } else {
-        if (threads == 256) k_batch_solve<double, 256><<<count, 256, 0, s>>>(wins, out);
-        else if (threads == 512) k_batch_solve<double, 512><<<count, 512, 0, s>>>(wins, out);
-        else k_batch_solve<double, 1024><<<count, 1024, 0, s>>>(wins, out);
+        if (threads == 256) k_batch_solve<double, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks);
+        else if (threads == 512) k_batch_solve<double, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks);
+        else k_batch_solve<double, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks);
     }
 }
 
