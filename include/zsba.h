@@ -232,13 +232,32 @@ int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t itera
                          int32_t use_relative_convergence, int32_t precision, int32_t* accepted);
 
 /* ---- implicit differentiation (no reference counterpart) ----------------------
- * At the converged state of the last FP64 solve, propagate dL/d(points) (and
- * optionally dL/d(pose local increments)) to the observations and the camera
- * intrinsics through the optimality condition of the final normal equations.
- * Requires one camera for all observations.  Outputs are indexed by id. */
+ * zsba_gauss_newton applies `steps` undamped Gauss-Newton steps unconditionally
+ * (lambda = 0; no cost test), polishing a converged solve to the stationary
+ * point of the binary64 cost.  Returns 0 or ZSBA_ESTATE when the normal
+ * equations are not positive definite.
+ *
+ * zsba_adjoint: at a stationary state x* (after zsba_solve to convergence and
+ * zsba_gauss_newton), propagate dL/d(points) (and optionally dL/d(pose local
+ * increments, rotation then translation as in vertex_pose::plus,
+ * factor_graph.hpp:947-960)) to every observation's pixel coordinates and to the
+ * pinhole intrinsics (fx, fy, cx, cy), by the implicit function theorem on the
+ * optimality condition g(x*, theta) = 0 of the robust cost:
+ *   dL/dtheta = -lambda^T dg/dtheta,   H lambda = dL/dx,
+ * with H the exact Hessian (forward-mode derivatives of the reference's
+ * gradient arithmetic) solved by conjugate gradients preconditioned with the
+ * Gauss-Newton reduced system's Cholesky factor.  dL_dK is the derivative for
+ * intrinsics applied to every observation's camera.  Outputs are indexed by id
+ * (NULL outputs are skipped).  Returns the conjugate-gradient iterations. */
+int32_t zsba_gauss_newton(zsba_graph* graph, int32_t steps);
 int32_t zsba_adjoint(zsba_graph* graph, const double* dL_dpoints /* n_points_added*3 */,
                      const double* dL_dposes /* n_poses_added*6 or NULL */,
-                     double* dL_dobs /* n_obs_added*2 */, double dL_dK[4]);
+                     double* dL_dobs /* n_obs_added*2 or NULL */, double dL_dK[4] /* or NULL */);
+
+/* Adjoint operators on a vector ordered [live poses 6 each | live points 3 each]
+ * in slot order: Hv = exact Hessian product, Minv_v = Gauss-Newton Schur
+ * preconditioner applied (either output may be NULL). */
+int32_t zsba_debug_adjoint_ops(zsba_graph* graph, const double* v, double* Hv, double* Minv_v);
 
 /* ---- diagnostics used by the parity tests -------------------------------------
  * Per-observation FP64 evaluation on the device at the current state, in

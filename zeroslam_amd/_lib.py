@@ -107,6 +107,8 @@ _SIGNATURES = {
     "zsba_batch_solve": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, _I32]),
     "zsba_batch_get_info": (C.c_int32, [_P, C.POINTER(BatchInfo)]),
     "zsba_adjoint": (C.c_int32, [_P, _D, _D, _D, _D]),
+    "zsba_gauss_newton": (C.c_int32, [_P, C.c_int32]),
+    "zsba_debug_adjoint_ops": (C.c_int32, [_P, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),
     "zsba_debug_reduced_system": (C.c_int32, [_P, C.c_double, C.c_int32, _D, _D, _D, _D]),
     "zsba_debug_math": (C.c_int32, [C.c_int32, C.c_int32, _D, _D, C.c_int32]),

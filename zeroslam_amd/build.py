@@ -42,6 +42,7 @@ UNITS = [
     ("kernels_fp32.hip", ["-ffp-contract=off"]),
     ("kernels_chol.hip", ["-ffp-contract=off"]),
     ("kernels_batch.hip", ["-ffp-contract=off"]),
+    ("kernels_adjoint.hip", ["-ffp-contract=off"]),
 ]
 
 

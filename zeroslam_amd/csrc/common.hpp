@@ -174,6 +174,11 @@ struct dev_work64 {
     float* W_f = nullptr;           // [npair*18]
     float* Hinv_f = nullptr;        // [nl*9]
     float* Y_f = nullptr;           // [npair*18]
+    // adjoint (conjugate gradients): 6 vectors of 6 np + 3 nl, reduction scratch
+    double* adj = nullptr;
+    double* adj_partial = nullptr;  // [513]
+    double* adj_obs = nullptr;      // [2 no] dL/duv by compacted observation
+    uint8_t* adj_frozen = nullptr;  // [nl] point held constant by the adjoint
     // envelope of the reduced system in 32 x 32 tiles: row tile I is zero left of
     // tile column tile_first[I]; column tile k is zero below row tile tile_last[k]
     const int32_t* tile_first = nullptr;  // [T]
@@ -209,6 +214,8 @@ void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s)
 void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s);
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s);
 void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s);
+// Point step dx_l for the solved dx_p and the update of every free vertex (no LM gating).
+void launch_gn_apply64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
 // ---- mixed path: binary32 linearisation and Schur (kernels_fp32.hip) -----------
 void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
@@ -220,6 +227,22 @@ void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t
 void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, const double* lambda_override);
 void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_backsub_only_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);
+
+// ---- adjoint of the converged solve (kernels_adjoint.hip) ----------------------
+// Vectors are [poses 6 np | points 3 nl].  out = d/d eps g(x + eps v, K + eps dK)
+// (v or dK may be null): H v, or dg/dK along dK.
+void launch_grad_tangent(const dev_problem& p, const dev_work64& w, const double* v, const double* dK, double* out,
+                         hipStream_t s);
+// z = M^-1 r with the Gauss-Newton reduced-system factor in w.S and H_ll^-1 in w.Hinv.
+void launch_adjoint_precond(const dev_problem& p, const dev_work64& w, const double* r, double* z, hipStream_t s);
+// Marks the points the adjoint holds constant (fixed, or singular undamped block:
+// w.Hinv of the undamped linearisation is zero) and zeroes their rows of v.
+void launch_adjoint_frozen(const dev_problem& p, const dev_work64& w, hipStream_t s);
+void launch_zero_frozen_rows(const dev_problem& p, const dev_work64& w, double* v, hipStream_t s);
+void launch_dot(const double* a, const double* b, int64_t n, double* partial /*512*/, double* out, hipStream_t s);
+void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
+// dL/duv per observation, indexed by compacted observation e (2 per observation).
+void launch_obs_grad(const dev_problem& p, const dev_work64& w, const double* lam, double* dobs_by_e, hipStream_t s);
 
 // ---- batched local BA (kernels_batch.hip) --------------------------------------
 // One workgroup per window solves the window's whole LM loop.  Windows are

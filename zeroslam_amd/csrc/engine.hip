@@ -237,6 +237,9 @@ struct zsba_graph {
                                std::vector<int32_t>& lm_pairs);
     // precision: ZSBA_PRECISION_FP64 (reference order), _FP64_FAST, _MIXED
     int32_t run(int32_t iterations, int32_t use_rel, int32_t precision);
+    int32_t adjoint(const double* dL_dpoints, const double* dL_dposes, double* dL_dobs, double* dL_dK);
+    int32_t gauss_newton(int32_t steps);
+    double adjoint_residual = 0.0;  // relative CG residual of the last adjoint
     void print_trace();
     double mixed_floor() const {
         const int e = opt.floor_exp == 0 ? kDefaultFloorExp : opt.floor_exp;
@@ -495,6 +498,10 @@ int32_t zsba_graph::build_structure() {
     w.zl = mem.get<double>(3L * nl); w.gl = mem.get<double>(3L * nl);
     w.rvec = mem.get<double>(n); w.corr = mem.get<double>(n);
     int32_t* d_pose_pairs = mem.get<int32_t>(npair);
+    w.adj = mem.get<double>(6 * (6L * np + 3L * nl));
+    w.adj_obs = mem.get<double>(2L * no);
+    w.adj_frozen = mem.get<uint8_t>(nl);
+    w.adj_partial = mem.get<double>(513);
     int32_t* d_tile_first = mem.get<int32_t>(n_tiles);
     int32_t* d_tile_last = mem.get<int32_t>(n_tiles);
     for (int set = 0; set < 2; ++set) {
@@ -759,6 +766,143 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     if (opt.verbose) print_trace();
     device_newer = true;
     return lm_pinned->success;
+}
+
+// ---- undamped Gauss-Newton polish ------------------------------------------------
+// steps x (linearise, reduced system with lambda = 0 and a 1e-12 max|S_ii|
+// diagonal floor for the empty rows of fixed poses, factorise, update), every
+// step applied.  Converges quadratically to the stationary point the adjoint
+// differentiates.
+int32_t zsba_graph::gauss_newton(int32_t steps) {
+    hipError_t e;
+    const double zero = 0.0;
+    if ((e = copy(lambda_dev, &zero, sizeof zero, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "lambda");
+    for (int32_t k = 0; k < steps; ++k) {
+        zb::launch_linearise64(dp, w, stream, false);
+        zb::launch_reduced_system64(dp, w, stream, lambda_dev, false);
+        zb::launch_cholesky_solve64(dp.n, w, stream, false, false, 1e-12);
+        int32_t status = 0;
+        if ((e = copy(&status, w.chol_status, sizeof status, hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "status");
+        if (status != 0) return fail(ZSBA_ESTATE, "Gauss-Newton step %d: the normal equations are not positive definite", k);
+        zb::launch_gn_apply64(dp, w, stream);
+    }
+    if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "Gauss-Newton");
+    device_newer = true;
+    return 0;
+}
+
+// ---- adjoint of the converged solve --------------------------------------------
+// H lambda = dL/dx by conjugate gradients with the Gauss-Newton Schur factor as
+// preconditioner (kernels_adjoint.hip), then dL/duv = -lambda^T dg/duv and
+// dL/dK = -lambda^T dg/dK.  Returns the CG iterations.
+int32_t zsba_graph::adjoint(const double* dL_dpoints, const double* dL_dposes, double* dL_dobs, double* dL_dK) {
+    hipError_t e;
+    const int64_t np = dp.np, nl = dp.nl, N = 6 * np + 3 * nl;
+    double* x = w.adj;  // lambda
+    double* r = x + N;
+    double* z = r + N;
+    double* pv = z + N;
+    double* Ap = pv + N;
+    double* b = Ap + N;
+    // right-hand side in slot order (rows of fixed vertices zero)
+    std::vector<double> hb(N, 0.0);
+    for (int64_t i = 0; i < np; ++i) {
+        const int32_t id = pose_slot_id[i];
+        if (dL_dposes && !pose_fixed[id])
+            for (int k = 0; k < 6; ++k) hb[6 * i + k] = dL_dposes[6L * id + k];
+    }
+    for (int64_t j = 0; j < nl; ++j) {
+        const int32_t id = point_slot_id[j];
+        if (dL_dpoints && !point_fixed[id])
+            for (int k = 0; k < 3; ++k) hb[6 * np + 3 * j + k] = dL_dpoints[3L * id + k];
+    }
+    if ((e = copy(b, hb.data(), N * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "rhs");
+    // Gauss-Newton preconditioner at x*: undamped blocks, reduced system with a
+    // 1e-10 max|S_ii| diagonal floor (fixed poses have empty rows)
+    const double zero = 0.0;
+    if ((e = copy(lambda_dev, &zero, sizeof zero, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "lambda");
+    zb::launch_linearise64(dp, w, stream, false);
+    zb::launch_reduced_system64(dp, w, stream, lambda_dev, false);
+    zb::launch_cholesky_solve64(dp.n, w, stream, false, false, 1e-10);
+    zb::launch_adjoint_frozen(dp, w, stream);
+    zb::launch_zero_frozen_rows(dp, w, b, stream);
+    int32_t status = 0;
+    if ((e = copy(&status, w.chol_status, sizeof status, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "status");
+    if (status != 0) return fail(ZSBA_ESTATE, "the Gauss-Newton system at the current state is not positive definite");
+    auto dot = [&](const double* a, const double* c, double& out) -> hipError_t {
+        zb::launch_dot(a, c, N, w.adj_partial, w.adj_partial + 512, stream);
+        return copy(&out, w.adj_partial + 512, sizeof(double), hipMemcpyDeviceToHost);
+    };
+    // CG: x = 0, r = b, z = M^-1 r, p = z
+    (void)hipMemsetAsync(x, 0, N * sizeof(double), stream);
+    (void)hipMemcpyAsync(r, b, N * sizeof(double), hipMemcpyDeviceToDevice, stream);
+    zb::launch_adjoint_precond(dp, w, r, z, stream);
+    (void)hipMemcpyAsync(pv, z, N * sizeof(double), hipMemcpyDeviceToDevice, stream);
+    double rz = 0.0, bb = 0.0;
+    if ((e = dot(r, z, rz)) != hipSuccess || (e = dot(b, b, bb)) != hipSuccess) return hip_fail(e, "cg");
+    // Stop at a relative residual of 1e-12 or when it has not improved for 10
+    // iterations (the rounding floor, about eps cond(H)); fail above 1e-8.
+    int32_t it = 0;
+    const int32_t max_it = 300;
+    double best = 1.0;
+    int32_t since_best = 0;
+    adjoint_residual = 0.0;
+    if (bb > 0.0) {
+        for (it = 1; it <= max_it; ++it) {
+            zb::launch_grad_tangent(dp, w, pv, nullptr, Ap, stream);
+            double pAp = 0.0;
+            if ((e = dot(pv, Ap, pAp)) != hipSuccess) return hip_fail(e, "cg");
+            if (!(pAp > 0.0)) return fail(ZSBA_ESTATE, "the Hessian at the current state is not positive definite");
+            const double alpha = rz / pAp;
+            zb::launch_axpby(alpha, pv, 1.0, x, N, stream);
+            zb::launch_axpby(-alpha, Ap, 1.0, r, N, stream);
+            double rr = 0.0;
+            if ((e = dot(r, r, rr)) != hipSuccess) return hip_fail(e, "cg");
+            const double rel = std::sqrt(rr / bb);
+            if (opt.verbose) std::fprintf(stderr, "[zsba] adjoint CG %d: relative residual %.3e\n", it, rel);
+            adjoint_residual = rel;
+            if (rel < best) {
+                best = rel;
+                since_best = 0;
+            } else if (++since_best >= 10) {
+                break;
+            }
+            if (rel <= 1e-12) break;
+            zb::launch_adjoint_precond(dp, w, r, z, stream);
+            double rz_new = 0.0;
+            if ((e = dot(r, z, rz_new)) != hipSuccess) return hip_fail(e, "cg");
+            const double beta = rz_new / rz;
+            rz = rz_new;
+            zb::launch_axpby(1.0, z, beta, pv, N, stream);
+        }
+        if (it > max_it) it = max_it;
+        if (!(adjoint_residual <= 1e-8))
+            return fail(ZSBA_ESTATE, "conjugate gradients stopped at relative residual %.3e after %d iterations",
+                        adjoint_residual, it);
+    }
+    // dL/dK_j = -lambda . dg/dK_j
+    if (dL_dK) {
+        for (int j = 0; j < 4; ++j) {
+            double dK[4] = {0.0, 0.0, 0.0, 0.0};
+            dK[j] = 1.0;
+            zb::launch_grad_tangent(dp, w, nullptr, dK, Ap, stream);
+            double t = 0.0;
+            if ((e = dot(x, Ap, t)) != hipSuccess) return hip_fail(e, "dK");
+            dL_dK[j] = -t;
+        }
+    }
+    if (dL_dobs) {
+        zb::launch_obs_grad(dp, w, x, w.adj_obs, stream);
+        std::vector<double> d(2L * dp.no);
+        if ((e = copy(d.data(), w.adj_obs, d.size() * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "dL/duv");
+        for (int32_t k = 0; k < dp.no; ++k) {
+            dL_dobs[2L * obs_e_id[k]] = d[2L * k];
+            dL_dobs[2L * obs_e_id[k] + 1] = d[2L * k + 1];
+        }
+    }
+    return it;
 }
 
 // ---- batched local BA -----------------------------------------------------------
@@ -1459,9 +1603,52 @@ int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t itera
     return rc;
 }
 
-int32_t zsba_adjoint(zsba_graph* g, const double*, const double*, double*, double*) {
+int32_t zsba_debug_adjoint_ops(zsba_graph* g, const double* v, double* Hv, double* Minv_v) {
+    if (!g || !v) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return g->fail(ZSBA_ESTATE, "no observations");
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    hipError_t e;
+    const int64_t N = 6L * g->dp.np + 3L * g->dp.nl;
+    double* dv = g->w.adj;
+    double* out = dv + N;
+    if ((e = g->copy(dv, v, N * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return g->hip_fail(e, "v");
+    {
+        const double zero = 0.0;
+        if ((e = g->copy(g->lambda_dev, &zero, sizeof zero, hipMemcpyHostToDevice)) != hipSuccess)
+            return g->hip_fail(e, "lambda");
+        zb::launch_linearise64(g->dp, g->w, g->stream, false);
+        zb::launch_reduced_system64(g->dp, g->w, g->stream, g->lambda_dev, false);
+        zb::launch_cholesky_solve64(g->dp.n, g->w, g->stream, false, false, 1e-10);
+        zb::launch_adjoint_frozen(g->dp, g->w, g->stream);
+    }
+    if (Hv) {
+        zb::launch_grad_tangent(g->dp, g->w, dv, nullptr, out, g->stream);
+        if ((e = g->copy(Hv, out, N * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess) return g->hip_fail(e, "Hv");
+    }
+    if (Minv_v) {
+        zb::launch_adjoint_precond(g->dp, g->w, dv, out, g->stream);
+        if ((e = g->copy(Minv_v, out, N * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return g->hip_fail(e, "Minv v");
+    }
+    return 0;
+}
+
+int32_t zsba_gauss_newton(zsba_graph* g, int32_t steps) {
+    if (!g || steps < 0) return ZSBA_EINVAL;
+    if (g->live_obs() == 0 || steps == 0) return 0;
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    return g->gauss_newton(steps);
+}
+
+int32_t zsba_adjoint(zsba_graph* g, const double* dL_dpoints, const double* dL_dposes, double* dL_dobs, double* dL_dK) {
     if (!g) return ZSBA_EINVAL;
-    return g->fail(ZSBA_EUNSUP, "adjoint is not available yet");
+    if (!dL_dpoints && !dL_dposes) return g->fail(ZSBA_EINVAL, "no loss gradient given");
+    if (g->live_obs() == 0) return g->fail(ZSBA_ESTATE, "no observations");
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    return g->adjoint(dL_dpoints, dL_dposes, dL_dobs, dL_dK);
 }
 
 }  // extern "C"

@@ -885,6 +885,14 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
                                                   w.dxl, w.bp, w.bl, nullptr, w.lm, 0, 0, nullptr);
 }
 
+void launch_gn_apply64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    int g = grid_for(p.np + p.nl, kBlock);
+    if (g > kPartials) g = kPartials;
+    if (g < 1) g = 1;
+    k_backsub_update<double><<<g, kBlock, 0, s>>>(p, w.pose, w.pose_backup, w.point, w.point_backup, w.W, w.Hinv, w.dxp,
+                                                  w.dxl, w.bp, w.bl, nullptr, w.lm, 0, 1, nullptr);
+}
+
 void launch_backsub_update_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s) {
     int g = grid_for(p.np + p.nl, kBlock);
     if (g > kPartials) g = kPartials;

@@ -201,14 +201,47 @@ class Graph:
                                                             ptr(xp), ptr(xl)))
         return bool(ok), S, bs, xp, xl
 
-    def adjoint(self, dL_dpoints, dL_dposes=None):
+    def adjoint(self, dL_dpoints, dL_dposes=None, return_iterations: bool = False):
+        """Gradient of a loss L(x*) of the converged state with respect to every
+        observation's pixel coordinates (by observation id) and to the pinhole
+        intrinsics (fx, fy, cx, cy) shared by the observations, by implicit
+        differentiation (zsba_adjoint).  dL_dpoints: (n_points, 3) by point id;
+        dL_dposes: (n_poses, 6) in the pose retraction's local coordinates
+        (rotation, translation) or None."""
         n_p, n_l, n_o = self.counts()
         dL_dpoints = np.ascontiguousarray(dL_dpoints, dtype=np.float64).reshape(n_l, 3)
         dL_dposes = None if dL_dposes is None else np.ascontiguousarray(dL_dposes, dtype=np.float64).reshape(n_p, 6)
         d_obs = np.zeros((n_o, 2))
         d_K = np.zeros(4)
-        self._check(self.lib.zsba_adjoint(self.h, ptr(dL_dpoints), ptr(dL_dposes), ptr(d_obs), ptr(d_K)))
-        return d_obs, d_K
+        it = self._check(self.lib.zsba_adjoint(self.h, ptr(dL_dpoints), ptr(dL_dposes), ptr(d_obs), ptr(d_K)))
+        return (d_obs, d_K, it) if return_iterations else (d_obs, d_K)
+
+    def adjoint_ops(self, v):
+        """(H v, M^-1 v) for v ordered [live poses 6 each | live points 3 each]."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        Hv = np.zeros_like(v)
+        Mv = np.zeros_like(v)
+        self._check(self.lib.zsba_debug_adjoint_ops(self.h, ptr(v), ptr(Hv), ptr(Mv)))
+        return Hv, Mv
+
+    def gauss_newton(self, steps: int = 4) -> None:
+        """Undamped Gauss-Newton steps, every one applied (zsba_gauss_newton)."""
+        self._check(self.lib.zsba_gauss_newton(self.h, int(steps)))
+
+    def solve_to_convergence(self, precision: int = FP64_FAST, max_rounds: int = 20, iterations: int = 50,
+                             polish: int = 4) -> int:
+        """Repeat solve(iterations) until a solve accepts no step (the LM restarts
+        with a fresh damping each round), then `polish` undamped Gauss-Newton
+        steps; returns the total accepted LM steps."""
+        total = 0
+        for _ in range(max_rounds):
+            acc = self.solve(iterations, False, precision)
+            total += acc
+            if acc == 0:
+                break
+        if polish:
+            self.gauss_newton(polish)
+        return total
 
 
 class Batch:
