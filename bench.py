@@ -140,7 +140,36 @@ def state_error(P, X, Pr, Xr):
                      np.abs(X - Xr).max() / max(np.abs(Xr).max(), 1e-300)))
 
 
-def roofline_of(problem, precision_name, kernel_ms, kernel_launches):
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_mixed_hbm_traffic.md")
+GROUP_KERNELS = {
+    "cholesky": ["k_chol_begin", "k_chol_fused"],
+    "linearisation": ["k_lin_points_f", "k_reduce_pose_f"],
+    "schur": ["k_damp_invert_f", "k_form_Y_f", "k_fill_zero_d", "k_schur_f"],
+    "cost": ["k_cost"],
+    "backsub": ["k_backsub_update<float>"],
+}
+
+
+def measured_traffic(group: str):
+    """HBM bytes per launch of a kernel group, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of the mixed 100-keyframe solve (calibrated
+    against a copy kernel); None when the summary does not cover the group."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None
+    table = {}
+    for line in lines:
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic":
+            table[parts[1]] = float(parts[2]) + float(parts[3])
+    names = GROUP_KERNELS.get(group, [])
+    if not names or any(k not in table for k in names):
+        return None
+    return sum(table[k] for k in names)
+
+
+def roofline_of(problem, precision_name, kernel_ms, kernel_launches, config=1):
     s = 4 if precision_name == "mixed" else 8
     n_pairs = int(problem.n_obs)  # one W block per (pose, point) pair; the workload has no repeats
     B = algorithmic_bytes(problem.n_obs, problem.n_poses, problem.n_points, n_pairs, s)
@@ -154,19 +183,20 @@ def roofline_of(problem, precision_name, kernel_ms, kernel_launches):
     }
     total = {gname: sum(kernel_ms.get(k, 0.0) for k in ks) for gname, ks in groups.items()}
     dominant = max(total, key=total.get)
+    traffic = measured_traffic(dominant) if (precision_name == "mixed" and config == 1) else None
     if dominant == "cholesky":
         n = 6 * problem.n_poses
         dur = avg.get("cholesky", float("nan"))
         flops = cholesky_flops(problem)
         achieved = flops / (dur * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None, "kernel": "cholesky",
+                    "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic, "kernel": "cholesky",
                     "algorithmic_flops": flops, "avg_ms": dur, "n": n}
     else:
         dur = sum(avg.get(k, 0.0) for k in groups[dominant])
         achieved = B[dominant] / (dur * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dominant,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dominant,
                     "algorithmic_bytes": B[dominant], "avg_ms": dur}
     return roofline, {k: round(v, 5) for k, v in avg.items()}
 
@@ -263,6 +293,8 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "fp64_fast", "mixed"],
                     help="mixed (default): the headline, mixed vs FP64 at equal final cost")
+    ap.add_argument("--single", action="store_true",
+                    help="time only --precision at the workload's iteration count (no FP64 comparison; profiling)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -285,6 +317,14 @@ def main() -> int:
     g.snapshot()
     print(f"bench: c{args.config} obs={problem.n_obs} poses={problem.n_poses} points={problem.n_points}",
           file=sys.stderr, flush=True)
+
+    if args.single:
+        value, ms, stats, kms, kl = time_solves(g, precision, iters, args.steps, args.warmup)
+        roofline, avg = roofline_of(problem, args.precision, kms, kl, args.config)
+        print(json.dumps({"metric": "BA observation*iters/s (single precision run)", "value": value,
+                          "unit": "obs*iter/s", "ms_per_step": ms, "precision": args.precision,
+                          "roofline": roofline, "kernel_avg_ms": avg}))
+        return 0
 
     # the plain FP64 GPU path (the reference's operation order): throughput and final cost
     v64, ms64, st64, k64, l64 = time_solves(g, FP64, iters, args.steps, args.warmup)
@@ -321,7 +361,7 @@ def main() -> int:
                               "speedup_vs_fp64": value / v64, "speedup_vs_fp64_fast": value / vf}
             print(f"bench: mixed {value:.4g} obs*iter/s at {run_iters} iterations", file=sys.stderr, flush=True)
 
-    roofline, avg = roofline_of(problem, args.precision, kms, kl)
+    roofline, avg = roofline_of(problem, args.precision, kms, kl, args.config)
     out = {
         "metric": "BA observation*iters/s, 100KF/20k pts: mixed vs FP64 GPU at equal final cost",
         "value": value, "unit": "obs*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
