@@ -27,6 +27,15 @@ __device__ __forceinline__ double lane_value(double v, int j) {
     return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
+// The lane index, hidden from loop-invariant code motion: values derived from it
+// inside a loop (lane masks, per-lane constants) are then formed where they are
+// used instead of being hoisted out and kept live, which spills registers.
+__device__ __forceinline__ int opaque_lane() {
+    int lane = static_cast<int>(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));
+    return lane;
+}
+
 // Block-wide maximum of one value per thread (NaN never wins).
 __device__ inline double block_max(double v, double* sh) {
     const int t = threadIdx.x;
@@ -59,41 +68,140 @@ __device__ __forceinline__ double div_cr(double a, double b, double y) {
     return r;
 }
 
-// Factorisation of a diagonal tile by wave 0 with row r in the registers of lane
-// r; column j is broadcast lane by lane.  Every lane forms L_jj and its
-// reciprocal; the updates run on all lanes without masks, because entries above
-// the diagonal are never read.  Returns false (uniformly) when a pivot is not
+// div_cr without its fallback, for straight-line code: sets bad where the
+// fallback would have divided, so that the caller can redo the work with div_cr.
+__device__ __forceinline__ double div_cr_nofb(double a, double b, double y, bool& bad) {
+    const double q = a * y;
+    const double e = fma(-q, b, a);
+    const double r = fma(e, y, q);
+    const double ar = fabs(r);
+    bad |= (a != 0.0) & ((!(ar < 1e300)) | (ar < 1e-290));
+    return (a == 0.0) ? q : r;
+}
+
+// Row-Crout factorisation of a 32 x 32 tile held one row per lane (lane r holds
+// row r; rows at or past nr are identity rows, which leave the others untouched:
+// their entries in real columns are zero and real rows are zero in theirs; lanes
+// 32..63 repeat rows 0..31 and are never read).  Column j: every lane reads the
+// pivot A_jj, forms L_jj = sqrt, its reciprocal and its own L_rj, then subtracts
+// L_rj L_cj from its entries c > j.  L_{j+1,j}, which the next pivot needs, comes
+// from lane j + 1 by readlane; the other L_cj are broadcast through col[j] in
+// LDS, so no column's broadcasts pile up in scalar registers.  No branches: bad
+// is set where a division needs div_cr's fallback.  recips[j] receives
+// RN(1 / L_jj).  Returns false (uniformly) when a pivot of a real row is not
 // greater than eps.
-__device__ inline bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* ok_sh) {
+__device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double eps, double* recips, bool& bad,
+                                            double (*col)[64]) {
+    const int lane = opaque_lane();
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const double d = lane_value(row[j], j);
+        ok &= (j >= nr) | (d > eps);
+        const double l = sqrt(d);
+        const double y = 1.0 / l;
+        const double q = div_cr_nofb(row[j], l, y, bad);
+        row[j] = (lane == j) ? l : q;
+        recips[j] = y;
+        if (j + 1 < NB) {
+            col[j][lane] = row[j];
+            row[j + 1] = row[j + 1] - row[j] * lane_value(row[j], j + 1);
+#pragma unroll
+            for (int c = j + 2; c < NB; ++c) row[c] = row[c] - row[j] * col[j][c];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return ok;
+}
+
+// The same factorisation in place in LDS with div_cr (fallback included), one
+// row per lane of wave 0: the rare path when factor_rows reports bad.
+__device__ __noinline__ bool factor_tile_lds(double (*D)[NB + 1], int nr, double eps, double* recips) {
+    const int lane = threadIdx.x & 63;
+    bool ok = true;
+    for (int j = 0; j < nr; ++j) {
+        const double d = D[j][j];
+        ok = ok && (d > eps);
+        const double l = sqrt(d);
+        const double y = 1.0 / l;
+        if (lane == j) {
+            D[j][j] = l;
+            recips[j] = y;
+        } else if (lane > j && lane < nr) {
+            D[lane][j] = div_cr(D[lane][j], l, y);
+        }
+        if (lane > j && lane < nr) {
+            for (int c = j + 1; c <= lane; ++c) D[lane][c] = D[lane][c] - D[lane][j] * D[c][j];
+        }
+    }
+    return ok;
+}
+
+// Factorisation of a diagonal tile (in LDS, lower triangle, zero above) by wave
+// 0; rows at or past nr are overwritten with identity rows.  recips[j] receives
+// RN(1 / L_jj); col is NB x 64 doubles of LDS scratch.  Returns false
+// (uniformly) when a pivot is not greater than eps.
+__device__ __forceinline__ bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps, int* ok_sh, double* recips,
+                                                 double (*col)[64]) {
     if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
+        const int lane = opaque_lane();
+        const int lr = lane & (NB - 1);
+        if (nr < NB && lane >= nr && lane < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) D[lane][c] = 0.0;
+            D[lane][lane] = 1.0;
+        }
         double row[NB];
 #pragma unroll
-        for (int c = 0; c < NB; ++c) row[c] = (lane < nr) ? D[lane][c] : 0.0;
-        int ok = 1;
+        for (int c = 0; c < NB; ++c) row[c] = D[lr][c];
+        bool bad = false;
+        bool ok = factor_rows(row, nr, eps, recips, bad, col);
+        if (!__any(bad)) {
+            if (lane < nr) {
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            if (j < nr) {
-                const double d = lane_value(row[j], j);
-                ok = ok && (d > eps);
-                const double l = sqrt(d);
-                const double y = 1.0 / l;
-                row[j] = (lane == j) ? l : div_cr(row[j], l, y);
-#pragma unroll
-                for (int c = j + 1; c < NB; ++c) {
-                    const double lcj = lane_value(row[j], c);
-                    row[c] = row[c] - row[j] * lcj;
-                }
+                for (int c = 0; c < NB; ++c) D[lane][c] = row[c];
             }
+        } else {
+            ok = factor_tile_lds(D, nr, eps, recips);
         }
-        if (lane < nr) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) D[lane][c] = row[c];
-        }
-        if (lane == 0) *ok_sh = ok;
+        if (lane == 0) *ok_sh = ok ? 1 : 0;
     }
     __syncthreads();
     return *ok_sh != 0;
+}
+
+// Row i of a panel below the diagonal tile: L_ic = (A_ic - sum_{m<c} L_im L_cm) / L_cc
+// in ascending c, subtractions in ascending m (the row-Crout order), divisions by
+// the reciprocals of the diagonal (div_cr, correctly rounded).  The row is solved
+// in place in LDS; only full 32-column panels have rows below their diagonal
+// tile.
+__device__ __noinline__ void solve_panel_row_slow(double* row, const double (*D)[NB + 1]) {
+    for (int c = 0; c < NB; ++c) {
+        double v = row[c];
+        for (int m = 0; m < c; ++m) v = v - row[m] * D[c][m];
+        row[c] = v / D[c][c];
+    }
+}
+
+__device__ __forceinline__ void solve_panel_row(double* row, const double (*D)[NB + 1], const double* recips) {
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = row[c];
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        double v = x[c];
+#pragma unroll
+        for (int m = 0; m < c; ++m) v = v - x[m] * D[c][m];
+        x[c] = div_cr_nofb(v, D[c][c], recips[c], bad);
+        __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of later columns from being hoisted
+    }
+    if (!bad) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c) row[c] = x[c];
+    } else {
+        solve_panel_row_slow(row, D);
+    }
 }
 
 // ---- whole factorisation in one workgroup -------------------------------------
@@ -106,66 +214,78 @@ __device__ inline bool factor_diag_tile(double (*D)[NB + 1], int nr, double eps,
 // every column panel of the envelope must be at most kTiles tiles tall.
 template <int kTiles>
 __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double eps, double (*Pn)[NB][NB + 1],
-                             int* ok_sh) {
+                             int* ok_sh, double* recips, double (*col)[64]) {
     const int T = (n + NB - 1) / NB;
+    const int tid = threadIdx.x;
     for (int k = 0; k < T; ++k) {
         const int k0 = k * NB;
         const int nk = (n - k0) < NB ? (n - k0) : NB;
         const int h = tile_last[k] - k + 1;
         const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
         const int rows = rows_end - k0;  // panel rows, the diagonal tile's first
-        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
+#pragma unroll 4
+        for (int idx = tid; idx < rows * NB; idx += blockDim.x) {
             const int r = idx / NB, c = idx % NB;
             const int t = r / NB, rr = r % NB;
             Pn[t][rr][c] = (c < nk && (t > 0 || c <= rr)) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
         }
         __syncthreads();
-        if (!factor_diag_tile(Pn[0], nk, eps, ok_sh)) return false;
+        if (!factor_diag_tile(Pn[0], nk, eps, ok_sh, recips, col)) return false;
         // rows below: L_r = A_r L_kk^-T, element c gets -= x_m L_cm (m < c) then / L_cc
-        if (static_cast<int>(threadIdx.x) < rows - nk) {
-            const int r = nk + threadIdx.x;
+        if (tid < rows - nk) {
+            const int r = nk + tid;
             const int t = r / NB, rr = r % NB;
-            double x[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) x[c] = Pn[t][rr][c];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                if (c < nk) {
-                    double v = x[c];
-#pragma unroll
-                    for (int m = 0; m < c; ++m) v = v - x[m] * Pn[0][c][m];
-                    x[c] = v / Pn[0][c][c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NB; ++c) Pn[t][rr][c] = x[c];
+            solve_panel_row(Pn[t][rr], Pn[0], recips);
         }
         __syncthreads();
-        for (int idx = threadIdx.x; idx < rows * NB; idx += blockDim.x) {
+#pragma unroll 4
+        for (int idx = tid; idx < rows * NB; idx += blockDim.x) {
             const int r = idx / NB, c = idx % NB;
             const int t = r / NB, rr = r % NB;
             if (c < nk && (t > 0 || c <= rr)) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = Pn[t][rr][c];
         }
         // trailing tiles (I, J), k < J <= I <= tile_last[k]: a -= L_I,k L_J,k^T
-        // over the panel's 32 columns in order (only full panels have tiles below)
+        // over the panel's 32 columns in order (only full panels have tiles below).
+        // Each thread loads kChunk elements before updating them.
+        constexpr int kChunk = 4;
         const int m = h - 1;
         const int64_t elems = static_cast<int64_t>(m) * (m + 1) / 2 * (NB * NB);
-        for (int64_t e = threadIdx.x; e < elems; e += blockDim.x) {
-            const int pair = static_cast<int>(e / (NB * NB));
-            const int within = static_cast<int>(e % (NB * NB));
-            int Ip = 0;
-            while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
-            const int Jp = pair - Ip * (Ip + 1) / 2;
-            const int rr = within / NB, cc = within % NB;
-            const int pr = (Ip + 1) * NB + rr, pc = (Jp + 1) * NB + cc;  // panel rows of the element's row, column
-            if (pr >= rows || pc >= rows || (Ip == Jp && cc > rr)) continue;
-            double* out = A + static_cast<int64_t>(k0 + pr) * n + (k0 + pc);
-            double v = *out;
-            const double* Li = Pn[Ip + 1][rr];
-            const double* Lj = Pn[Jp + 1][cc];
+        const double* P = &Pn[0][0][0];
+        for (int64_t e0 = tid; e0 < elems; e0 += static_cast<int64_t>(kChunk) * blockDim.x) {
+            int64_t off[kChunk];
+            int li[kChunk], lj[kChunk];
+            double v[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                const int64_t e = e0 + static_cast<int64_t>(u) * blockDim.x;
+                off[u] = -1;
+                li[u] = lj[u] = 0;
+                if (e < elems) {
+                    const int pair = static_cast<int>(e / (NB * NB));
+                    const int within = static_cast<int>(e % (NB * NB));
+                    int Ip = 0;
+                    while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
+                    const int Jp = pair - Ip * (Ip + 1) / 2;
+                    const int rr = within / NB, cc = within % NB;
+                    const int pr = (Ip + 1) * NB + rr, pc = (Jp + 1) * NB + cc;  // panel row, column
+                    if (pr < rows && pc < rows && (Ip != Jp || cc <= rr)) {
+                        off[u] = static_cast<int64_t>(k0 + pr) * n + (k0 + pc);
+                        li[u] = ((Ip + 1) * NB + rr) * (NB + 1);
+                        lj[u] = ((Jp + 1) * NB + cc) * (NB + 1);
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) v[u] = (off[u] >= 0) ? A[off[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kChunk; ++u) {
+                if (off[u] >= 0) {
+                    double a = v[u];
 #pragma unroll 8
-            for (int q = 0; q < NB; ++q) v = v - Li[q] * Lj[q];
-            *out = v;
+                    for (int q = 0; q < NB; ++q) a = a - P[li[u] + q] * P[lj[u] + q];
+                    A[off[u]] = a;
+                }
+            }
         }
         __syncthreads();
     }
@@ -254,6 +374,175 @@ __device__ inline int triangular_solve(int n, const double* L, const double* b, 
             double v = x[i];
             for (int j = 0; j < nb; ++j) v = v - L[static_cast<int64_t>(i0 + j) * n + i] * x[i0 + j];
             x[i] = v;
+        }
+        __syncthreads();
+    }
+    return 0;
+}
+
+// ---- pipelined substitution for the reassociating precisions -----------------
+// Left-looking by 32-row tiles with one workgroup barrier per tile.  While wave
+// 0 runs tile t's recurrence (and the product of tile t's solution with the one
+// factor tile that couples it to the next tile), the other waves form the
+// contributions of every earlier, already final unknown to the next tile and
+// stage the next tile's two factor tiles in LDS.  The sums' association is fixed
+// (lane-strided column sums reduced by a butterfly for the forward pass, kSlots
+// row slots summed in order for the backward pass), so the result does not depend
+// on the workgroup size.  Scratch: 6 LDS tiles (two diagonal, two coupling tiles,
+// two tiles of partial sums); blockDim.x is at least 128.
+constexpr int kSlots = 30;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+// dst = tile (r0.., c0..) of L, zero outside nr x nc (and above the diagonal
+// when lower), by threads first, first + stride, ...
+__device__ __forceinline__ void stage_tile(double (*dst)[NB + 1], const double* L, int n, int r0, int c0, int nr,
+                                           int nc, bool lower, int first, int stride) {
+    for (int idx = first; idx < NB * NB; idx += stride) {
+        const int r = idx >> 5, c = idx & (NB - 1);
+        dst[r][c] = (r < nr && c < nc && (!lower || c <= r)) ? L[static_cast<int64_t>(r0 + r) * n + c0 + c] : 0.0;
+    }
+}
+
+__device__ __forceinline__ int triangular_solve_fast(int n, const double* L, const double* b, double* x,
+                                            const int32_t* tile_first, const int32_t* tile_last,
+                                            double (*buf)[NB][NB + 1], double* sh, int* flag) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nwk = static_cast<int>(blockDim.x >> 6) - 1;  // worker waves
+    double m = 0;
+    for (int i = tid; i < n; i += blockDim.x) {
+        const double a = fabs(L[static_cast<int64_t>(i) * n + i]);
+        m = (m < a) ? a : m;
+    }
+    const double mx = block_max(m, sh);
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    const double eps = mx * 1e-7;
+    for (int i = tid; i < n; i += blockDim.x) {
+        if (!(fabs(L[static_cast<int64_t>(i) * n + i]) > eps)) *flag = 1;
+        x[i] = b[i];
+    }
+    const int T = (n + NB - 1) / NB;
+    auto rows_of = [n](int t) { return (n - t * NB) < NB ? (n - t * NB) : NB; };
+    double* part = &buf[4][0][0];  // forward [2][NB]; backward [2][kSlots][NB]
+    stage_tile(buf[0], L, n, 0, 0, rows_of(0), rows_of(0), true, tid, blockDim.x);
+    if (T > 1 && tile_first[1] <= 0) stage_tile(buf[2], L, n, NB, 0, rows_of(1), NB, false, tid, blockDim.x);
+    __syncthreads();
+    if (*flag) return 2;
+
+    // forward: L y = b
+    double own = 0.0;  // wave 0: the coupling tile's product for the next tile
+    for (int t = 0; t < T; ++t) {
+        const int p = t & 1, i0 = t * NB, nb = rows_of(t);
+        if (wave == 0) {
+            const int lane = opaque_lane(), lr = lane & (NB - 1);
+            const double (*D)[NB + 1] = buf[p];
+            double rowv[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) rowv[c] = (lane < NB && c < lane) ? D[lr][c] : 0.0;
+            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
+            double xi = 0.0;
+            if (lane < nb) xi = x[i0 + lane] - (((t > 0) ? part[p * NB + lane] : 0.0) + own);
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                xi = (lane == j) ? xi * yl : xi;
+                xi = fma(-rowv[j], lane_value(xi, j), xi);
+            }
+            if (lane < nb) x[i0 + lane] = xi;
+            own = 0.0;
+            if (t + 1 < T && tile_first[t + 1] <= t) {
+                const double (*O)[NB + 1] = buf[2 + p];
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+                for (int c = 0; c < NB; c += 4) {
+                    a0 = fma(O[lr][c], lane_value(xi, c), a0);
+                    a1 = fma(O[lr][c + 1], lane_value(xi, c + 1), a1);
+                    a2 = fma(O[lr][c + 2], lane_value(xi, c + 2), a2);
+                    a3 = fma(O[lr][c + 3], lane_value(xi, c + 3), a3);
+                }
+                own = (a0 + a1) + (a2 + a3);
+            }
+        } else if (t + 1 < T) {
+            const int u = t + 1, iu = u * NB, nbu = rows_of(u), c0 = tile_first[u] * NB;
+            for (int r = wave - 1; r < nbu; r += nwk) {
+                const double* Lr = L + static_cast<int64_t>(iu + r) * n;
+                double acc = 0.0;
+                for (int j = c0 + lane; j < i0; j += 64) acc = fma(Lr[j], x[j], acc);
+                acc = wave_sum(acc);
+                if (lane == 0) part[(u & 1) * NB + r] = acc;
+            }
+            const int first = tid - 64, stride = nwk * 64;
+            stage_tile(buf[u & 1], L, n, iu, iu, nbu, nbu, true, first, stride);
+            if (u + 1 < T && tile_first[u + 1] <= u)
+                stage_tile(buf[2 + (u & 1)], L, n, iu + NB, iu, rows_of(u + 1), NB, false, first, stride);
+        }
+        __syncthreads();
+    }
+
+    // backward: L^T x = y
+    {
+        const int t = T - 1;
+        stage_tile(buf[t & 1], L, n, t * NB, t * NB, rows_of(t), rows_of(t), true, tid, blockDim.x);
+        if (t >= 1 && tile_last[t - 1] >= t)
+            stage_tile(buf[2 + (t & 1)], L, n, t * NB, (t - 1) * NB, rows_of(t), NB, false, tid, blockDim.x);
+    }
+    __syncthreads();
+    own = 0.0;
+    for (int t = T - 1; t >= 0; --t) {
+        const int p = t & 1, i0 = t * NB, nb = rows_of(t);
+        if (wave == 0) {
+            const int lane = opaque_lane(), lr = lane & (NB - 1);
+            const double (*D)[NB + 1] = buf[p];
+            double colv[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) colv[j] = (lane < NB && j > lane) ? D[j][lr] : 0.0;
+            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
+            double xc = 0.0;
+            if (lane < nb) {
+                double red = 0.0;
+                if (t < T - 1) {
+                    const double* ps = part + p * kSlots * NB + lane;
+                    for (int s = 0; s < kSlots; ++s) red += ps[s * NB];
+                }
+                xc = x[i0 + lane] - (red + own);
+            }
+#pragma unroll
+            for (int j = NB - 1; j >= 0; --j) {
+                xc = (lane == j) ? xc * yl : xc;
+                xc = fma(-colv[j], lane_value(xc, j), xc);
+            }
+            if (lane < nb) x[i0 + lane] = xc;
+            own = 0.0;
+            if (t >= 1 && tile_last[t - 1] >= t) {
+                const double (*O)[NB + 1] = buf[2 + p];
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+                for (int j = 0; j < NB; j += 4) {
+                    a0 = fma(O[j][lr], lane_value(xc, j), a0);
+                    a1 = fma(O[j + 1][lr], lane_value(xc, j + 1), a1);
+                    a2 = fma(O[j + 2][lr], lane_value(xc, j + 2), a2);
+                    a3 = fma(O[j + 3][lr], lane_value(xc, j + 3), a3);
+                }
+                own = (a0 + a1) + (a2 + a3);
+            }
+        } else if (t >= 1) {
+            const int u = t - 1, iu = u * NB;
+            const int jbeg = i0 + NB;
+            const int jend = (tile_last[u] + 1) * NB < n ? (tile_last[u] + 1) * NB : n;
+            const int c = lane & (NB - 1);
+            for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
+                double acc = 0.0;
+                for (int j = jbeg + s; j < jend; j += kSlots) acc = fma(L[static_cast<int64_t>(j) * n + iu + c], x[j], acc);
+                part[(u & 1) * kSlots * NB + s * NB + c] = acc;
+            }
+            const int first = tid - 64, stride = nwk * 64;
+            stage_tile(buf[u & 1], L, n, iu, iu, NB, NB, true, first, stride);
+            if (u >= 1 && tile_last[u - 1] >= u)
+                stage_tile(buf[2 + (u & 1)], L, n, iu, iu - NB, NB, NB, false, first, stride);
         }
         __syncthreads();
     }

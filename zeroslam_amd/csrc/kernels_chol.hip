@@ -113,6 +113,8 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
     __shared__ double D[NB][NB + 1];
     __shared__ double LI[NB][NB + 1];
     __shared__ double LJ[NB][NB + 1];
+    __shared__ double recips[NB];
+    __shared__ double col[NB][64];
     __shared__ int ok_sh;
     const int T = (n + NB - 1) / NB;
     const int np_wg = T - k;  // column-k workgroups
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
             panel_update(D, LJ, LJ, nk, nk, true);
         }
         __syncthreads();
-        if (!factor_diag_tile(D, nk, eps_p[0], &ok_sh)) {
+        if (!factor_diag_tile(D, nk, eps_p[0], &ok_sh, recips, col)) {
             if (threadIdx.x == 0) atomicCAS(status, 0, 1);
             return;
         }
@@ -147,21 +149,8 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
             panel_update(own, LI, LJ, nr, nk, false);
         }
         __syncthreads();
-        if (threadIdx.x < nr) {
-            double x[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) x[c] = own[threadIdx.x][c];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) {
-                if (c < nk) {
-                    double v = x[c];
-#pragma unroll
-                    for (int m = 0; m < c; ++m) v = v - x[m] * D[c][m];
-                    x[c] = v / D[c][c];
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NB; ++c) own[threadIdx.x][c] = x[c];
+        if (threadIdx.x < nr) {  // I > k, so column k is a full panel
+            wg::solve_panel_row(own[threadIdx.x], D, recips);
         }
         __syncthreads();
         store_tile(own, A, n, r0, k0, nr, nk, false);
@@ -196,8 +185,11 @@ __global__ void __launch_bounds__(1024) k_chol_fused(int n, double* A, const int
                                                      int32_t* status, const lm_state* lm, int gated) {
     if ((gated && lm->done) || status[0] != 0) return;
     __shared__ double Pn[kFusedTiles][NB][NB + 1];
+    __shared__ double recips[NB];
+    __shared__ double col[NB][64];
     __shared__ int ok_sh;
-    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], Pn, &ok_sh) && threadIdx.x == 0) status[0] = 1;
+    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], Pn, &ok_sh, recips, col) && threadIdx.x == 0)
+        status[0] = 1;
 }
 
 __device__ __forceinline__ double readlane_d(double v, int j) {
@@ -291,9 +283,12 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
     if (lane == 0) lm->solve_ok = 1;
 }
 
-// Forward and backward substitution in one workgroup (wg::triangular_solve);
-// sets lm->solve_ok.  x lives in LDS for n <= kTrsvLds, otherwise in the output.
-constexpr int kTrsvLds = 4096;
+// Substitution in one workgroup; sets lm->solve_ok after a full solve.
+// forward_only: the forward pass in the reference's order (wg::triangular_solve,
+// the binary64 parity path); otherwise both passes pipelined
+// (wg::triangular_solve_fast).  x lives in LDS for n <= kTrsvLds, otherwise in
+// the output.
+constexpr int kTrsvLds = 8192;
 
 template <bool kLds>
 __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* xout,
@@ -305,11 +300,12 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
         return;
     }
     __shared__ double sh[1024];
-    __shared__ double T[NB][NB + 1];
+    __shared__ double buf[6][NB][NB + 1];
     __shared__ double xl[kLds ? kTrsvLds : 1];
     __shared__ int flag;
     double* x = kLds ? xl : xout;
-    const int rc = wg::triangular_solve(n, L, b, x, tile_first, tile_last, T, sh, &flag, forward_only != 0);
+    const int rc = forward_only ? wg::triangular_solve(n, L, b, x, tile_first, tile_last, buf[0], sh, &flag, true)
+                                : wg::triangular_solve_fast(n, L, b, x, tile_first, tile_last, buf, sh, &flag);
     if (rc != 0) {
         if (threadIdx.x == 0) {
             *status = rc;
