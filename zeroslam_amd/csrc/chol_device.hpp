@@ -86,7 +86,8 @@ __device__ __forceinline__ double div_cr_nofb(double a, double b, double y, bool
 // pivot A_jj, forms L_jj = sqrt, its reciprocal and its own L_rj, then subtracts
 // L_rj L_cj from its entries c > j.  L_{j+1,j}, which the next pivot needs, comes
 // from lane j + 1 by readlane; the other L_cj are broadcast through col[j] in
-// LDS, so no column's broadcasts pile up in scalar registers.  No branches: bad
+// LDS (col[j][c] = L_cj for c >= j: the factor transposed), so no column's
+// broadcasts pile up in scalar registers.  No branches: bad
 // is set where a division needs div_cr's fallback.  recips[j] receives
 // RN(1 / L_jj).  Returns false (uniformly) when a pivot of a real row is not
 // greater than eps.
@@ -103,11 +104,14 @@ __device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double ep
         const double q = div_cr_nofb(row[j], l, y, bad);
         row[j] = (lane == j) ? l : q;
         recips[j] = y;
+        col[j][lane] = row[j];
         if (j + 1 < NB) {
-            col[j][lane] = row[j];
             row[j + 1] = row[j + 1] - row[j] * lane_value(row[j], j + 1);
+            double t[NB];  // the column's broadcasts, loaded together before any use
 #pragma unroll
-            for (int c = j + 2; c < NB; ++c) row[c] = row[c] - row[j] * col[j][c];
+            for (int c = j + 2; c < NB; ++c) t[c] = col[j][c];
+#pragma unroll
+            for (int c = j + 2; c < NB; ++c) row[c] = row[c] - row[j] * t[c];
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -205,87 +209,160 @@ __device__ __forceinline__ void solve_panel_row(double* row, const double (*D)[N
 }
 
 // ---- whole factorisation in one workgroup -------------------------------------
-// Panel k (row tiles k .. tile_last[k] of tile column k, already updated by the
-// panels before it) is staged in LDS, its diagonal tile factored by wave 0, the
-// tiles below solved one row per thread, the panel written back, and the trailing
-// tiles it touches updated element by element from LDS.  Workgroup barriers
-// order the panels; nothing waits on another workgroup.
-// Returns false (uniformly) when a pivot fails.  Pn holds kTiles staged tiles;
-// every column panel of the envelope must be at most kTiles tiles tall.
+// Per 32-column panel k (rows k0 .. the last row tile of the envelope in column
+// k, already updated by the panels before it):
+//   1. wave 0 loads the diagonal tile into registers and factors it
+//      (factor_rows: L_kk transposed in col, reciprocals of the diagonal in
+//      recips), writing L_kk back; meanwhile every other thread loads the row
+//      below the tile it will solve;
+//   2. each row below is solved right-looking against col: x_m = a_m / L_mm
+//      (div_cr), then a_c -= x_m L_cm for c > m, so each element receives its
+//      subtractions in ascending m and its division last; the row goes back to
+//      memory and, transposed, into LDS (PT, column q of the panel tile contiguous
+//      in the row);
+//   3. the trailing tiles (I, J), k < J <= I, are updated in memory from PT,
+//      each thread holding a 2 x 4 block: a -= L_Iq L_Jq for q ascending.
+// Workgroup barriers order the phases; nothing waits on another workgroup.
+// Returns false (uniformly) when a pivot fails.  PT holds kTiles - 1 transposed
+// tiles; every column panel of the envelope must be at most kTiles tiles tall.
+constexpr int kPT = NB + 2;  // PT row stride: 16-byte aligned rows
+
+// A row below the diagonal tile whose fast solve needed div_cr's fallback:
+// solved again in memory with divisions.
+__device__ __noinline__ void solve_row_slow(double* arow, const double (*col)[64]) {
+    for (int c = 0; c < NB; ++c) {
+        double v = arow[c];
+        for (int m = 0; m < c; ++m) v = v - arow[m] * col[m][c];
+        arow[c] = v / col[c][c];
+    }
+}
+
 template <int kTiles>
-__device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double eps, double (*Pn)[NB][NB + 1],
+__device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double eps, double (*PT)[NB][kPT],
                              int* ok_sh, double* recips, double (*col)[64]) {
     const int T = (n + NB - 1) / NB;
     const int tid = threadIdx.x;
     for (int k = 0; k < T; ++k) {
         const int k0 = k * NB;
         const int nk = (n - k0) < NB ? (n - k0) : NB;
-        const int h = tile_last[k] - k + 1;
         const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
-        const int rows = rows_end - k0;  // panel rows, the diagonal tile's first
-#pragma unroll 4
-        for (int idx = tid; idx < rows * NB; idx += blockDim.x) {
-            const int r = idx / NB, c = idx % NB;
-            const int t = r / NB, rr = r % NB;
-            Pn[t][rr][c] = (c < nk && (t > 0 || c <= rr)) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
-        }
-        __syncthreads();
-        if (!factor_diag_tile(Pn[0], nk, eps, ok_sh, recips, col)) return false;
-        // rows below: L_r = A_r L_kk^-T, element c gets -= x_m L_cm (m < c) then / L_cc
-        if (tid < rows - nk) {
-            const int r = nk + tid;
-            const int t = r / NB, rr = r % NB;
-            solve_panel_row(Pn[t][rr], Pn[0], recips);
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int idx = tid; idx < rows * NB; idx += blockDim.x) {
-            const int r = idx / NB, c = idx % NB;
-            const int t = r / NB, rr = r % NB;
-            if (c < nk && (t > 0 || c <= rr)) A[static_cast<int64_t>(k0 + r) * n + k0 + c] = Pn[t][rr][c];
-        }
-        // trailing tiles (I, J), k < J <= I <= tile_last[k]: a -= L_I,k L_J,k^T
-        // over the panel's 32 columns in order (only full panels have tiles below).
-        // Each thread loads kChunk elements before updating them.
-        constexpr int kChunk = 4;
-        const int m = h - 1;
-        const int64_t elems = static_cast<int64_t>(m) * (m + 1) / 2 * (NB * NB);
-        const double* P = &Pn[0][0][0];
-        for (int64_t e0 = tid; e0 < elems; e0 += static_cast<int64_t>(kChunk) * blockDim.x) {
-            int64_t off[kChunk];
-            int li[kChunk], lj[kChunk];
-            double v[kChunk];
+        const int nbelow = rows_end - k0 - nk;  // rows below the diagonal tile (full panels only)
+        const int rt = tid - 64;                // row-solve thread index (waves 1..)
+        double v[NB];
+        if (tid < 64) {
+            const int lane = opaque_lane();
+            const int lr = lane & (NB - 1);
+            const bool real = lr < nk;
+            const double* src = A + static_cast<int64_t>(k0 + (real ? lr : 0)) * n + k0;
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
-                const int64_t e = e0 + static_cast<int64_t>(u) * blockDim.x;
-                off[u] = -1;
-                li[u] = lj[u] = 0;
-                if (e < elems) {
-                    const int pair = static_cast<int>(e / (NB * NB));
-                    const int within = static_cast<int>(e % (NB * NB));
-                    int Ip = 0;
-                    while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
-                    const int Jp = pair - Ip * (Ip + 1) / 2;
-                    const int rr = within / NB, cc = within % NB;
-                    const int pr = (Ip + 1) * NB + rr, pc = (Jp + 1) * NB + cc;  // panel row, column
-                    if (pr < rows && pc < rows && (Ip != Jp || cc <= rr)) {
-                        off[u] = static_cast<int64_t>(k0 + pr) * n + (k0 + pc);
-                        li[u] = ((Ip + 1) * NB + rr) * (NB + 1);
-                        lj[u] = ((Jp + 1) * NB + cc) * (NB + 1);
+            for (int c = 0; c < NB; ++c) {
+                double a = 0.0;
+                if (real && c <= lr) a = src[c];
+                v[c] = real ? a : ((c == lr) ? 1.0 : 0.0);
+            }
+            bool bad = false;
+            bool ok = factor_rows(v, nk, eps, recips, bad, col);
+            if (__any(bad)) {
+                // rare: redo the tile in LDS with divisions, from the unfactored rows
+                double (*D)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&PT[0][0][0]);
+                if (lane < NB) {
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) D[lane][c] = (real && c <= lr) ? src[c] : ((c == lr) ? 1.0 : 0.0);
+                }
+                ok = factor_tile_lds(D, nk, eps, recips);
+                if (lane < NB) {
+#pragma unroll
+                    for (int c = 0; c < NB; ++c) {
+                        v[c] = D[lane][c];
+                        col[c][lane] = D[lane][c];
                     }
                 }
             }
+            if (lane < nk) {
+                double* dst = A + static_cast<int64_t>(k0 + lane) * n + k0;
 #pragma unroll
-            for (int u = 0; u < kChunk; ++u) v[u] = (off[u] >= 0) ? A[off[u]] : 0.0;
-#pragma unroll
-            for (int u = 0; u < kChunk; ++u) {
-                if (off[u] >= 0) {
-                    double a = v[u];
-#pragma unroll 8
-                    for (int q = 0; q < NB; ++q) a = a - P[li[u] + q] * P[lj[u] + q];
-                    A[off[u]] = a;
-                }
+                for (int c = 0; c < NB; ++c)
+                    if (c <= lane) dst[c] = v[c];
             }
+            if (lane == 0) *ok_sh = ok ? 1 : 0;
+        } else if (rt < nbelow) {
+            const double* src = A + static_cast<int64_t>(k0 + nk + rt) * n + k0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) v[c] = src[c];
+        }
+        __syncthreads();
+        if (*ok_sh == 0) return false;
+        if (tid >= 64 && rt < nbelow) {
+            bool bad = false;
+#pragma unroll
+            for (int m = 0; m < NB; ++m) {
+                double t[NB];  // column m of L_kk, loaded together before any use
+#pragma unroll
+                for (int c = m; c < NB; ++c) t[c] = col[m][c];
+                v[m] = div_cr_nofb(v[m], t[m], recips[m], bad);
+#pragma unroll
+                for (int c = m + 1; c < NB; ++c) v[c] = v[c] - v[m] * t[c];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            double* arow = A + static_cast<int64_t>(k0 + nk + rt) * n + k0;
+            if (bad) {
+                solve_row_slow(arow, col);
+#pragma unroll
+                for (int c = 0; c < NB; ++c) v[c] = arow[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NB; ++c) arow[c] = v[c];
+            }
+            const int t = rt >> 5, rr = rt & (NB - 1);
+#pragma unroll
+            for (int c = 0; c < NB; ++c) PT[t][c][rr] = v[c];
+        }
+        __syncthreads();
+        // trailing tiles (I, J) = (k + 1 + Ip, k + 1 + Jp), 0 <= Jp <= Ip < m: 128
+        // threads per tile, a 2 x 4 block each
+        const int m = (nbelow + NB - 1) / NB;
+        const int blocks = m * (m + 1) / 2 * 128;
+        for (int bidx = tid; bidx < blocks; bidx += blockDim.x) {
+            const int pair = bidx >> 7, b = bidx & 127;
+            int Ip = 0;
+            while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
+            const int Jp = pair - Ip * (Ip + 1) / 2;
+            const int r0 = (b >> 3) * 2, c0 = (b & 7) * 4;
+            if (Ip == Jp && c0 > r0 + 1) continue;  // above the diagonal
+            const int gr = k0 + NB * (1 + Ip) + r0, gc = k0 + NB * (1 + Jp) + c0;
+            double acc[2][4];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const bool in = (Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end;
+                    acc[a][c] = in ? A[static_cast<int64_t>(gr + a) * n + gc + c] : 0.0;
+                }
+            const double (*LI)[kPT] = PT[Ip];
+            const double (*LJ)[kPT] = PT[Jp];
+#pragma unroll 1
+            for (int q0 = 0; q0 < NB; q0 += 4) {
+                double li[4][2], lj[4][4];  // four steps' operands, loaded before any use
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    li[u][0] = LI[q0 + u][r0];
+                    li[u][1] = LI[q0 + u][r0 + 1];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) lj[u][c] = LJ[q0 + u][c0 + c];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) acc[a][c] = acc[a][c] - li[u][a] * lj[u][c];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if ((Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end)
+                        A[static_cast<int64_t>(gr + a) * n + gc + c] = acc[a][c];
         }
         __syncthreads();
     }
@@ -342,9 +419,21 @@ __device__ inline int triangular_solve(int n, const double* L, const double* b, 
         __syncthreads();
         const int iend = (tile_last[t] + 1) * NB < n ? (tile_last[t] + 1) * NB : n;
         for (int i = i0 + nb + tid; i < iend; i += blockDim.x) {
-            double v = x[i];
+            // only full tiles have rows below them (nb == NB); operands loaded before any use
             const double* Li = L + static_cast<int64_t>(i) * n + i0;
-            for (int j = 0; j < nb; ++j) v = v - Li[j] * x[i0 + j];
+            double v = x[i];
+#pragma unroll
+            for (int h = 0; h < NB; h += NB / 2) {
+                double lv[NB / 2], xv[NB / 2];
+#pragma unroll
+                for (int j = 0; j < NB / 2; ++j) {
+                    lv[j] = Li[h + j];
+                    xv[j] = x[i0 + h + j];
+                }
+#pragma unroll
+                for (int j = 0; j < NB / 2; ++j) v = v - lv[j] * xv[j];
+                __builtin_amdgcn_sched_barrier(0);
+            }
             x[i] = v;
         }
         __syncthreads();
@@ -381,38 +470,85 @@ __device__ inline int triangular_solve(int n, const double* L, const double* b, 
 }
 
 // ---- pipelined substitution for the reassociating precisions -----------------
-// Left-looking by 32-row tiles with one workgroup barrier per tile.  While wave
+// Left-looking by 32-row tiles with one workgroup barrier per tile, each tile's
+// rows scaled by their reciprocal pivots (unit-diagonal recurrences).  While wave
 // 0 runs tile t's recurrence (and the product of tile t's solution with the one
 // factor tile that couples it to the next tile), the other waves form the
 // contributions of every earlier, already final unknown to the next tile and
 // stage the next tile's two factor tiles in LDS.  The sums' association is fixed
-// (lane-strided column sums reduced by a butterfly for the forward pass, kSlots
-// row slots summed in order for the backward pass), so the result does not depend
-// on the workgroup size.  Scratch: 6 LDS tiles (two diagonal, two coupling tiles,
-// two tiles of partial sums); blockDim.x is at least 128.
+// (kSlots interleaved slots of the earlier unknowns, each summed in ascending
+// order, then the slots summed in order by wave 0), so the result does not
+// depend on the workgroup size.  Scratch: 6 LDS tiles (two diagonal, two
+// coupling tiles, two tiles of partial sums); blockDim.x == kThreads >= 128.
 constexpr int kSlots = 30;
 
-__device__ __forceinline__ double wave_sum(double v) {
+// Tile (r0.., c0..) of L, zero outside nr x nc (and above the diagonal when
+// lower), staged into dst by threads first, first + stride, ...: stage_load
+// issues the loads into v, stage_store writes them (kPer >= NB * NB / stride).
+template <int kPer>
+__device__ __forceinline__ void stage_load(double (&v)[kPer], const double* L, int n, int r0, int c0, int nr, int nc,
+                                           bool lower, int first, int stride) {
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-    return v;
-}
-
-// dst = tile (r0.., c0..) of L, zero outside nr x nc (and above the diagonal
-// when lower), by threads first, first + stride, ...
-__device__ __forceinline__ void stage_tile(double (*dst)[NB + 1], const double* L, int n, int r0, int c0, int nr,
-                                           int nc, bool lower, int first, int stride) {
-    for (int idx = first; idx < NB * NB; idx += stride) {
+    for (int u = 0; u < kPer; ++u) {
+        const int idx = first + u * stride;
         const int r = idx >> 5, c = idx & (NB - 1);
-        dst[r][c] = (r < nr && c < nc && (!lower || c <= r)) ? L[static_cast<int64_t>(r0 + r) * n + c0 + c] : 0.0;
+        v[u] = (idx < NB * NB && r < nr && c < nc && (!lower || c <= r)) ? L[static_cast<int64_t>(r0 + r) * n + c0 + c]
+                                                                           : 0.0;
     }
 }
+template <int kPer>
+__device__ __forceinline__ void stage_store(double (*dst)[NB + 1], const double (&v)[kPer], int first, int stride) {
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+        const int idx = first + u * stride;
+        if (idx < NB * NB) dst[idx >> 5][idx & (NB - 1)] = v[u];
+    }
+}
+__device__ __forceinline__ void stage_tile(double (*dst)[NB + 1], const double* L, int n, int r0, int c0, int nr,
+                                           int nc, bool lower, int first, int stride) {
+    constexpr int kMax = NB * NB / 64;  // stride is at least one wave
+    double v[kMax];
+    stage_load(v, L, n, r0, c0, nr, nc, lower, first, stride);
+    stage_store(dst, v, first, stride);
+}
 
+// Slot s of the partial sums for the next tile: sum over j = j0 + s, j0 + s +
+// kSlots, ... < j1 of a(j) * x[j], in ascending j, four loads in flight.
+template <class F>
+__device__ __forceinline__ double slot_sum(F a, const double* x, int j0, int j1) {
+    double acc = 0.0;
+    int j = j0;
+    for (; j + 3 * kSlots < j1; j += 4 * kSlots) {
+        const double l0 = a(j), l1 = a(j + kSlots), l2 = a(j + 2 * kSlots), l3 = a(j + 3 * kSlots);
+        const double x0 = x[j], x1 = x[j + kSlots], x2 = x[j + 2 * kSlots], x3 = x[j + 3 * kSlots];
+        acc = fma(l0, x0, acc);
+        acc = fma(l1, x1, acc);
+        acc = fma(l2, x2, acc);
+        acc = fma(l3, x3, acc);
+    }
+    for (; j < j1; j += kSlots) acc = fma(a(j), x[j], acc);
+    return acc;
+}
+
+// Sum of the kSlots partial sums of one unknown (stride NB), in a fixed order.
+__device__ __forceinline__ double slot_total(const double* ps) {
+    double pv[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) pv[s] = ps[s * NB];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) a[s & 3] += pv[s];
+    return (a[0] + a[1]) + (a[2] + a[3]);
+}
+
+template <int kThreads>
 __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, const double* b, double* x,
-                                            const int32_t* tile_first, const int32_t* tile_last,
-                                            double (*buf)[NB][NB + 1], double* sh, int* flag) {
+                                                     const int32_t* tile_first, const int32_t* tile_last,
+                                                     double (*buf)[NB][NB + 1], double* sh, int* flag) {
+    static_assert(kThreads >= 128 && kThreads % 64 == 0, "one wave for the recurrences, at least one worker wave");
+    constexpr int nwk = kThreads / 64 - 1;                           // worker waves
+    constexpr int kPer = (NB * NB + nwk * 64 - 1) / (nwk * 64);      // staged elements per worker thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nwk = static_cast<int>(blockDim.x >> 6) - 1;  // worker waves
     double m = 0;
     for (int i = tid; i < n; i += blockDim.x) {
         const double a = fabs(L[static_cast<int64_t>(i) * n + i]);
@@ -441,44 +577,57 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
         if (wave == 0) {
             const int lane = opaque_lane(), lr = lane & (NB - 1);
             const double (*D)[NB + 1] = buf[p];
+            double xi = 0.0;
+            if (lane < nb) {
+                double red = 0.0;
+                if (t > 0) red = slot_total(part + p * kSlots * NB + lane);
+                xi = x[i0 + lane] - (red + own);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            // unit-diagonal form: row i scaled by 1 / L_ii, so that step j is one
+            // broadcast and one fma
+            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
             double rowv[NB];
 #pragma unroll
-            for (int c = 0; c < NB; ++c) rowv[c] = (lane < NB && c < lane) ? D[lr][c] : 0.0;
-            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
-            double xi = 0.0;
-            if (lane < nb) xi = x[i0 + lane] - (((t > 0) ? part[p * NB + lane] : 0.0) + own);
+            for (int c = 0; c < NB; ++c) rowv[c] = D[lr][c] * ((c < lane) ? yl : 0.0);  // loads not predicated
+            xi *= yl;
 #pragma unroll
-            for (int j = 0; j < NB; ++j) {
-                xi = (lane == j) ? xi * yl : xi;
-                xi = fma(-rowv[j], lane_value(xi, j), xi);
-            }
+            for (int j = 0; j < NB; ++j) xi = fma(-rowv[j], lane_value(xi, j), xi);
             if (lane < nb) x[i0 + lane] = xi;
             own = 0.0;
             if (t + 1 < T && tile_first[t + 1] <= t) {
                 const double (*O)[NB + 1] = buf[2 + p];
+                double ov[NB];
+#pragma unroll
+                for (int c = 0; c < NB; ++c) ov[c] = O[lr][c];
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
                 for (int c = 0; c < NB; c += 4) {
-                    a0 = fma(O[lr][c], lane_value(xi, c), a0);
-                    a1 = fma(O[lr][c + 1], lane_value(xi, c + 1), a1);
-                    a2 = fma(O[lr][c + 2], lane_value(xi, c + 2), a2);
-                    a3 = fma(O[lr][c + 3], lane_value(xi, c + 3), a3);
+                    a0 = fma(ov[c], lane_value(xi, c), a0);
+                    a1 = fma(ov[c + 1], lane_value(xi, c + 1), a1);
+                    a2 = fma(ov[c + 2], lane_value(xi, c + 2), a2);
+                    a3 = fma(ov[c + 3], lane_value(xi, c + 3), a3);
+                    if ((c & 7) == 4) __builtin_amdgcn_sched_barrier(0);  // bound the broadcasts in flight
                 }
                 own = (a0 + a1) + (a2 + a3);
             }
         } else if (t + 1 < T) {
             const int u = t + 1, iu = u * NB, nbu = rows_of(u), c0 = tile_first[u] * NB;
-            for (int r = wave - 1; r < nbu; r += nwk) {
-                const double* Lr = L + static_cast<int64_t>(iu + r) * n;
-                double acc = 0.0;
-                for (int j = c0 + lane; j < i0; j += 64) acc = fma(Lr[j], x[j], acc);
-                acc = wave_sum(acc);
-                if (lane == 0) part[(u & 1) * NB + r] = acc;
-            }
             const int first = tid - 64, stride = nwk * 64;
-            stage_tile(buf[u & 1], L, n, iu, iu, nbu, nbu, true, first, stride);
-            if (u + 1 < T && tile_first[u + 1] <= u)
-                stage_tile(buf[2 + (u & 1)], L, n, iu + NB, iu, rows_of(u + 1), NB, false, first, stride);
+            // the next tile's factor tiles, loaded first (they do not depend on x)
+            double sd[kPer], so[kPer];
+            const bool has_o = u + 1 < T && tile_first[u + 1] <= u;
+            stage_load(sd, L, n, iu, iu, nbu, nbu, true, first, stride);
+            if (has_o) stage_load(so, L, n, iu + NB, iu, rows_of(u + 1), NB, false, first, stride);
+            // slot s, row r: columns c0 + s, c0 + s + kSlots, ... left of this tile
+            const int r = lane & (NB - 1);
+            const double* Lr = L + static_cast<int64_t>(iu + (r < nbu ? r : 0)) * n;
+            for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
+                const double acc = (r < nbu) ? slot_sum([Lr](int j) { return Lr[j]; }, x, c0 + s, i0) : 0.0;
+                part[(u & 1) * kSlots * NB + s * NB + r] = acc;
+            }
+            stage_store(buf[u & 1], sd, first, stride);
+            if (has_o) stage_store(buf[2 + (u & 1)], so, first, stride);
         }
         __syncthreads();
     }
@@ -497,35 +646,35 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
         if (wave == 0) {
             const int lane = opaque_lane(), lr = lane & (NB - 1);
             const double (*D)[NB + 1] = buf[p];
-            double colv[NB];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) colv[j] = (lane < NB && j > lane) ? D[j][lr] : 0.0;
-            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
             double xc = 0.0;
             if (lane < nb) {
                 double red = 0.0;
-                if (t < T - 1) {
-                    const double* ps = part + p * kSlots * NB + lane;
-                    for (int s = 0; s < kSlots; ++s) red += ps[s * NB];
-                }
+                if (t < T - 1) red = slot_total(part + p * kSlots * NB + lane);
                 xc = x[i0 + lane] - (red + own);
             }
+            __builtin_amdgcn_sched_barrier(0);
+            const double yl = 1.0 / ((lane < nb) ? D[lr][lr] : 1.0);
+            double colv[NB];
 #pragma unroll
-            for (int j = NB - 1; j >= 0; --j) {
-                xc = (lane == j) ? xc * yl : xc;
-                xc = fma(-colv[j], lane_value(xc, j), xc);
-            }
+            for (int j = 0; j < NB; ++j) colv[j] = D[j][lr] * ((j > lane) ? yl : 0.0);  // loads not predicated
+            xc *= yl;
+#pragma unroll
+            for (int j = NB - 1; j >= 0; --j) xc = fma(-colv[j], lane_value(xc, j), xc);
             if (lane < nb) x[i0 + lane] = xc;
             own = 0.0;
             if (t >= 1 && tile_last[t - 1] >= t) {
                 const double (*O)[NB + 1] = buf[2 + p];
+                double ov[NB];
+#pragma unroll
+                for (int j = 0; j < NB; ++j) ov[j] = O[j][lr];
                 double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
 #pragma unroll
                 for (int j = 0; j < NB; j += 4) {
-                    a0 = fma(O[j][lr], lane_value(xc, j), a0);
-                    a1 = fma(O[j + 1][lr], lane_value(xc, j + 1), a1);
-                    a2 = fma(O[j + 2][lr], lane_value(xc, j + 2), a2);
-                    a3 = fma(O[j + 3][lr], lane_value(xc, j + 3), a3);
+                    a0 = fma(ov[j], lane_value(xc, j), a0);
+                    a1 = fma(ov[j + 1], lane_value(xc, j + 1), a1);
+                    a2 = fma(ov[j + 2], lane_value(xc, j + 2), a2);
+                    a3 = fma(ov[j + 3], lane_value(xc, j + 3), a3);
+                    if ((j & 7) == 4) __builtin_amdgcn_sched_barrier(0);  // bound the broadcasts in flight
                 }
                 own = (a0 + a1) + (a2 + a3);
             }
@@ -533,16 +682,19 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
             const int u = t - 1, iu = u * NB;
             const int jbeg = i0 + NB;
             const int jend = (tile_last[u] + 1) * NB < n ? (tile_last[u] + 1) * NB : n;
+            const int first = tid - 64, stride = nwk * 64;
+            double sd[kPer], so[kPer];
+            const bool has_o = u >= 1 && tile_last[u - 1] >= u;
+            stage_load(sd, L, n, iu, iu, NB, NB, true, first, stride);
+            if (has_o) stage_load(so, L, n, iu, iu - NB, NB, NB, false, first, stride);
             const int c = lane & (NB - 1);
+            const double* Lc = L + iu + c;
             for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
-                double acc = 0.0;
-                for (int j = jbeg + s; j < jend; j += kSlots) acc = fma(L[static_cast<int64_t>(j) * n + iu + c], x[j], acc);
+                const double acc = slot_sum([Lc, n](int j) { return Lc[static_cast<int64_t>(j) * n]; }, x, jbeg + s, jend);
                 part[(u & 1) * kSlots * NB + s * NB + c] = acc;
             }
-            const int first = tid - 64, stride = nwk * 64;
-            stage_tile(buf[u & 1], L, n, iu, iu, NB, NB, true, first, stride);
-            if (u >= 1 && tile_last[u - 1] >= u)
-                stage_tile(buf[2 + (u & 1)], L, n, iu, iu - NB, NB, NB, false, first, stride);
+            stage_store(buf[u & 1], sd, first, stride);
+            if (has_o) stage_store(buf[2 + (u & 1)], so, first, stride);
         }
         __syncthreads();
     }

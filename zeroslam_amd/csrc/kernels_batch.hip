@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
     lm_state* lm = w.lm;
     __shared__ double sh[kThreads];
     __shared__ max_scan ms[kThreads];
-    __shared__ double Pn[kBatchTiles][wg::kTile][wg::kTile + 1];
+    __shared__ double PT[kBatchTiles - 1][wg::kTile][wg::kPT];  // factor panels, then the solve's tiles
+    static_assert((kBatchTiles - 1) * wg::kPT >= 6 * (wg::kTile + 1), "solve scratch");
+    auto solve_buf = reinterpret_cast<double (*)[wg::kTile][wg::kTile + 1]>(&PT[0][0][0]);
     __shared__ double recips[wg::kTile];
     __shared__ double col[wg::kTile][64];
     __shared__ double xs[kBatchMaxN];
@@ -159,9 +161,9 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         if (bw.floor_rel > 0.0)
             for (int i = tid; i < n; i += blockDim.x) w.S[static_cast<int64_t>(i) * n + i] += lift;
         __syncthreads();
-        bool ok = wg::fused_factor<kBatchTiles>(n, w.S, w.tile_last, (mx + lift) * 1e-14, Pn, &flag, recips, col);
+        bool ok = wg::fused_factor<kBatchTiles>(n, w.S, w.tile_last, (mx + lift) * 1e-14, PT, &flag, recips, col);
         clk.mark(PH_FACTOR);
-        if (ok) ok = wg::triangular_solve_fast(n, w.S, w.bs, xs, w.tile_first, w.tile_last, Pn, sh, &flag) == 0;
+        if (ok) ok = wg::triangular_solve_fast<kThreads>(n, w.S, w.bs, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) == 0;
         clk.mark(PH_SOLVE);
         if (ok) {
             for (int i = tid; i < n; i += blockDim.x) w.dxp[i] = xs[i];
@@ -178,7 +180,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                 for (int i = tid >> 6; i < p.np; i += nwaves)
                     mixed::pose_residual_wave(p, w.pose_grad, w.bp_m, w.dxp, B.W, w.gl, w.rvec, lam, i, lane);
                 __syncthreads();
-                if (wg::triangular_solve_fast(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, Pn, sh, &flag) != 0) {
+                if (wg::triangular_solve_fast<kThreads>(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) != 0) {
                     ok = false;
                     break;
                 }

@@ -181,14 +181,14 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
 // (wg::fused_factor).
 constexpr int kFusedTiles = 16;
 
-__global__ void __launch_bounds__(1024) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
+__global__ void __launch_bounds__(512) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
                                                      int32_t* status, const lm_state* lm, int gated) {
     if ((gated && lm->done) || status[0] != 0) return;
-    __shared__ double Pn[kFusedTiles][NB][NB + 1];
-    __shared__ double recips[NB];
     __shared__ double col[NB][64];
+    __shared__ double recips[NB];
+    __shared__ double PT[kFusedTiles - 1][NB][wg::kPT];
     __shared__ int ok_sh;
-    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], Pn, &ok_sh, recips, col) && threadIdx.x == 0)
+    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], PT, &ok_sh, recips, col) && threadIdx.x == 0)
         status[0] = 1;
 }
 
@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
 constexpr int kTrsvLds = 8192;
 
 template <bool kLds>
-__global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const double* b, double* xout,
+__global__ void __launch_bounds__(512) k_trsv(int n, const double* L, const double* b, double* xout,
                                                const int32_t* tile_first, const int32_t* tile_last, int32_t* status,
                                                lm_state* lm, int gated, int forward_only) {
     if (gated && lm->done) return;
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
     __shared__ int flag;
     double* x = kLds ? xl : xout;
     const int rc = forward_only ? wg::triangular_solve(n, L, b, x, tile_first, tile_last, buf[0], sh, &flag, true)
-                                : wg::triangular_solve_fast(n, L, b, x, tile_first, tile_last, buf, sh, &flag);
+                                : wg::triangular_solve_fast<512>(n, L, b, x, tile_first, tile_last, buf, sh, &flag);
     if (rc != 0) {
         if (threadIdx.x == 0) {
             *status = rc;
@@ -322,9 +322,9 @@ __global__ void __launch_bounds__(1024) k_trsv(int n, const double* L, const dou
 void launch_trsv(int n, const double* L, const double* b, double* x, const dev_work64& w, hipStream_t s, int gated,
                  int forward_only) {
     if (n <= kTrsvLds)
-        k_trsv<true><<<1, 1024, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
+        k_trsv<true><<<1, 512, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
     else
-        k_trsv<false><<<1, 1024, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
+        k_trsv<false><<<1, 512, 0, s>>>(n, L, b, x, w.tile_first, w.tile_last, w.chol_status, w.lm, gated, forward_only);
 }
 
 // A system without poses: the reference's 0 x 0 decompose and solve succeed.
@@ -346,7 +346,7 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     const int64_t tiles = (n + NB - 1) / NB;
     const bool fused = w.factor_path == 1 || (w.factor_path == 0 && w.panel_tiles <= kFusedTiles);
     if (fused && w.panel_tiles <= kFusedTiles) {
-        k_chol_fused<<<1, 1024, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
+        k_chol_fused<<<1, 512, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
     } else {
         for (int64_t k = 0; k < tiles; ++k) {
             const int64_t m = tiles - k - 1;
