@@ -118,6 +118,37 @@ __device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double ep
     return ok;
 }
 
+// factor_rows for the reassociating precisions: 1 / sqrt(A_jj) from the hardware
+// estimate and two Newton steps, L_jj = A_jj / sqrt(A_jj), L_rj = A_rj / L_jj as a
+// product with that reciprocal, updates as fused multiply-adds.  recips[j]
+// receives the reciprocal.
+__device__ __forceinline__ bool factor_rows_fast(double (&row)[NB], int nr, double eps, double* recips,
+                                                 double (*col)[64]) {
+    const int lane = opaque_lane();
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const double d = lane_value(row[j], j);
+        ok &= (j >= nr) | (d > eps);
+        double y = __builtin_amdgcn_rsq(d);
+        y = y * fma(-0.5 * d * y, y, 1.5);
+        y = y * fma(-0.5 * d * y, y, 1.5);
+        row[j] = (lane == j) ? d * y : row[j] * y;
+        recips[j] = y;
+        col[j][lane] = row[j];
+        if (j + 1 < NB) {
+            row[j + 1] = fma(-row[j], lane_value(row[j], j + 1), row[j + 1]);
+            double t[NB];
+#pragma unroll
+            for (int c = j + 2; c < NB; ++c) t[c] = col[j][c];
+#pragma unroll
+            for (int c = j + 2; c < NB; ++c) row[c] = fma(-row[j], t[c], row[c]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    return ok;
+}
+
 // The same factorisation in place in LDS with div_cr (fallback included), one
 // row per lane of wave 0: the rare path when factor_rows reports bad.
 __device__ __noinline__ bool factor_tile_lds(double (*D)[NB + 1], int nr, double eps, double* recips) {
@@ -223,6 +254,9 @@ __device__ __forceinline__ void solve_panel_row(double* row, const double (*D)[N
 //   3. the trailing tiles (I, J), k < J <= I, are updated in memory from PT,
 //      each thread holding a 2 x 4 block: a -= L_Iq L_Jq for q ascending.
 // Workgroup barriers order the phases; nothing waits on another workgroup.
+// kFast (the reassociating precisions): reciprocal square roots, products with
+// reciprocals and fused multiply-adds instead of the reference's operation
+// sequence (factor_rows_fast).
 // Returns false (uniformly) when a pivot fails.  PT holds kTiles - 1 transposed
 // tiles; every column panel of the envelope must be at most kTiles tiles tall.
 constexpr int kPT = NB + 2;  // PT row stride: 16-byte aligned rows
@@ -237,7 +271,7 @@ __device__ __noinline__ void solve_row_slow(double* arow, const double (*col)[64
     }
 }
 
-template <int kTiles>
+template <int kTiles, bool kFast>
 __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double eps, double (*PT)[NB][kPT],
                              int* ok_sh, double* recips, double (*col)[64]) {
     const int T = (n + NB - 1) / NB;
@@ -261,8 +295,8 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
                 v[c] = real ? a : ((c == lr) ? 1.0 : 0.0);
             }
             bool bad = false;
-            bool ok = factor_rows(v, nk, eps, recips, bad, col);
-            if (__any(bad)) {
+            bool ok = kFast ? factor_rows_fast(v, nk, eps, recips, col) : factor_rows(v, nk, eps, recips, bad, col);
+            if (!kFast && __any(bad)) {
                 // rare: redo the tile in LDS with divisions, from the unfactored rows
                 double (*D)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&PT[0][0][0]);
                 if (lane < NB) {
@@ -299,9 +333,15 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
                 double t[NB];  // column m of L_kk, loaded together before any use
 #pragma unroll
                 for (int c = m; c < NB; ++c) t[c] = col[m][c];
-                v[m] = div_cr_nofb(v[m], t[m], recips[m], bad);
+                if (kFast) {
+                    v[m] = v[m] * recips[m];
 #pragma unroll
-                for (int c = m + 1; c < NB; ++c) v[c] = v[c] - v[m] * t[c];
+                    for (int c = m + 1; c < NB; ++c) v[c] = fma(-v[m], t[c], v[c]);
+                } else {
+                    v[m] = div_cr_nofb(v[m], t[m], recips[m], bad);
+#pragma unroll
+                    for (int c = m + 1; c < NB; ++c) v[c] = v[c] - v[m] * t[c];
+                }
                 __builtin_amdgcn_sched_barrier(0);
             }
             double* arow = A + static_cast<int64_t>(k0 + nk + rt) * n + k0;
@@ -355,7 +395,8 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
 #pragma unroll
                     for (int a = 0; a < 2; ++a)
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) acc[a][c] = acc[a][c] - li[u][a] * lj[u][c];
+                        for (int c = 0; c < 4; ++c)
+                            acc[a][c] = kFast ? fma(-li[u][a], lj[u][c], acc[a][c]) : acc[a][c] - li[u][a] * lj[u][c];
             }
 #pragma unroll
             for (int a = 0; a < 2; ++a)

@@ -190,12 +190,17 @@ struct dev_work64 {
     double* gl = nullptr;           // [nl*3] H_ll^-1 r_l
     double* rvec = nullptr;         // [n] reduced residual r_p - W H_ll^-1 r_l
     double* corr = nullptr;         // [n] pose correction
+    double* pose_part = nullptr;    // [np*kPoseChunks*27] per-chunk partial sums of the per-pose reductions
 };
 
 #define ZB_TIME_BEGIN(w, id, s) do { if ((w).timer) (w).timer->begin((id), (s)); } while (0)
 #define ZB_TIME_END(w, s) do { if ((w).timer) (w).timer->end((s)); } while (0)
 
 constexpr int kPartials = 1024;  // reduction partial slots
+// The mixed path's per-pose sums over a pose's observations (and pairs) are split
+// into kPoseChunks contiguous chunks, one wave each, whose partial sums are then
+// added in chunk order: a fixed association, independent of the launch shape.
+constexpr int kPoseChunks = 16;
 
 // ---- binary64 kernels (kernels_fp64.hip) ----------------------------------
 // `exact`: reproduce the reference's summation orders (sequential sums over all

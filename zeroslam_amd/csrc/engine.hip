@@ -497,6 +497,7 @@ int32_t zsba_graph::build_structure() {
     w.W_f = mem.get<float>(18L * npair); w.Hinv_f = mem.get<float>(9L * nl); w.Y_f = mem.get<float>(18L * npair);
     w.zl = mem.get<double>(3L * nl); w.gl = mem.get<double>(3L * nl);
     w.rvec = mem.get<double>(n); w.corr = mem.get<double>(n);
+    w.pose_part = mem.get<double>(static_cast<size_t>(np) * zb::kPoseChunks * 27);
     int32_t* d_pose_pairs = mem.get<int32_t>(npair);
     w.adj = mem.get<double>(6 * (6L * np + 3L * nl));
     w.adj_obs = mem.get<double>(2L * no);

@@ -51,7 +51,11 @@ __device__ void linearise(const dev_problem& p, const dev_work64& w, const block
         mixed::lin_point(p, w.pose, w.point, B.pterms, w.pose_grad, B.Hll, w.bl_m, B.W, l);
     __syncthreads();
     const int lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
-    for (int i = threadIdx.x >> 6; i < p.np; i += nwaves) mixed::reduce_pose_lane(p, B.pterms, w.pose_grad, B.Hpp, w.bp_m, i, lane);
+    for (int ic = threadIdx.x >> 6; ic < p.np * kPoseChunks; ic += nwaves)
+        mixed::reduce_pose_chunk(p, B.pterms, w.pose_grad, w.pose_part, ic / kPoseChunks, ic % kPoseChunks, lane);
+    __syncthreads();
+    for (int ij = threadIdx.x; ij < p.np * 27; ij += blockDim.x)
+        mixed::reduce_pose_finish(w.pose_part, B.Hpp, w.bp_m, ij / 27, ij % 27);
     __syncthreads();
 }
 
@@ -161,7 +165,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         if (bw.floor_rel > 0.0)
             for (int i = tid; i < n; i += blockDim.x) w.S[static_cast<int64_t>(i) * n + i] += lift;
         __syncthreads();
-        bool ok = wg::fused_factor<kBatchTiles>(n, w.S, w.tile_last, (mx + lift) * 1e-14, PT, &flag, recips, col);
+        bool ok = wg::fused_factor<kBatchTiles, true>(n, w.S, w.tile_last, (mx + lift) * 1e-14, PT, &flag, recips, col);
         clk.mark(PH_FACTOR);
         if (ok) ok = wg::triangular_solve_fast<kThreads>(n, w.S, w.bs, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) == 0;
         clk.mark(PH_SOLVE);
@@ -177,8 +181,12 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                     mixed::full_residual_item(p, w.pose, w.point, w.dxp, w.dxl, w.bl_m, B.Hinv, w.pose_grad, w.zl, w.gl,
                                               lam, l);
                 __syncthreads();
-                for (int i = tid >> 6; i < p.np; i += nwaves)
-                    mixed::pose_residual_wave(p, w.pose_grad, w.bp_m, w.dxp, B.W, w.gl, w.rvec, lam, i, lane);
+                for (int ic = tid >> 6; ic < p.np * kPoseChunks; ic += nwaves)
+                    mixed::pose_residual_chunk(p, w.pose_grad, B.W, w.gl, w.pose_part, ic / kPoseChunks,
+                                               ic % kPoseChunks, lane);
+                __syncthreads();
+                for (int ir = tid; ir < p.np * 6; ir += blockDim.x)
+                    mixed::pose_residual_finish(p, w.pose_part, w.bp_m, w.dxp, w.rvec, lam, ir / 6, ir % 6);
                 __syncthreads();
                 if (wg::triangular_solve_fast<kThreads>(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) != 0) {
                     ok = false;

@@ -181,14 +181,15 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
 // (wg::fused_factor).
 constexpr int kFusedTiles = 16;
 
+template <bool kFast>
 __global__ void __launch_bounds__(512) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
-                                                     int32_t* status, const lm_state* lm, int gated) {
+                                                    int32_t* status, const lm_state* lm, int gated) {
     if ((gated && lm->done) || status[0] != 0) return;
     __shared__ double col[NB][64];
     __shared__ double recips[NB];
     __shared__ double PT[kFusedTiles - 1][NB][wg::kPT];
     __shared__ int ok_sh;
-    if (!wg::fused_factor<kFusedTiles>(n, A, tile_last, eps_p[0], PT, &ok_sh, recips, col) && threadIdx.x == 0)
+    if (!wg::fused_factor<kFusedTiles, kFast>(n, A, tile_last, eps_p[0], PT, &ok_sh, recips, col) && threadIdx.x == 0)
         status[0] = 1;
 }
 
@@ -346,7 +347,10 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     const int64_t tiles = (n + NB - 1) / NB;
     const bool fused = w.factor_path == 1 || (w.factor_path == 0 && w.panel_tiles <= kFusedTiles);
     if (fused && w.panel_tiles <= kFusedTiles) {
-        k_chol_fused<<<1, 512, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
+        if (exact)
+            k_chol_fused<false><<<1, 512, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
+        else
+            k_chol_fused<true><<<1, 512, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
     } else {
         for (int64_t k = 0; k < tiles; ++k) {
             const int64_t m = tiles - k - 1;

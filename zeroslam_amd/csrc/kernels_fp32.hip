@@ -44,12 +44,20 @@ __global__ void __launch_bounds__(kBlock) k_lin_points_f(dev_problem p, const do
     if (l < p.nl) mixed::lin_point(p, pose, point, pterms, pgrad, Hll, bl, Wbuf, l);
 }
 
-// One wave per pose.
-__global__ void __launch_bounds__(256) k_reduce_pose_f(dev_problem p, const float* pterms, const double* pgrad,
-                                                       float* Hpp, double* bp, const lm_state* lm, int gated) {
+// One wave per (pose, chunk), then one thread per (pose, term).
+__global__ void __launch_bounds__(256) k_reduce_pose_part_f(dev_problem p, const float* pterms, const double* pgrad,
+                                                            double* part, const lm_state* lm, int gated) {
     if (gated && (lm->done || !lm->need_linearize)) return;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (wave < p.np) mixed::reduce_pose_lane(p, pterms, pgrad, Hpp, bp, wave, threadIdx.x & 63);
+    if (wave < p.np * kPoseChunks)
+        mixed::reduce_pose_chunk(p, pterms, pgrad, part, wave / kPoseChunks, wave % kPoseChunks, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_pose_f(dev_problem p, const double* part, float* Hpp, double* bp,
+                                                       const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < p.np * 27) mixed::reduce_pose_finish(part, Hpp, bp, t / 27, t % 27);
 }
 
 __global__ void k_linearise_done_f(lm_state* lm) { lm_linearise_done(lm); }
@@ -97,8 +105,9 @@ void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream
     }
     if (p.np > 0) {
         ZB_TIME_BEGIN(w, K_POSE_REDUCE, s);
-        k_reduce_pose_f<<<grid_of(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms_f, w.pose_grad, w.Hpp_f, w.bp_m, w.lm,
-                                                                  gated ? 1 : 0);
+        k_reduce_pose_part_f<<<grid_of(64L * p.np * kPoseChunks, 256), 256, 0, s>>>(p, w.pose_terms_f, w.pose_grad,
+                                                                                     w.pose_part, w.lm, gated ? 1 : 0);
+        k_reduce_pose_f<<<grid_of(27L * p.np, 256), 256, 0, s>>>(p, w.pose_part, w.Hpp_f, w.bp_m, w.lm, gated ? 1 : 0);
         ZB_TIME_END(w, s);
     }
     if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);

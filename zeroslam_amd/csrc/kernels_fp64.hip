@@ -648,17 +648,24 @@ __global__ void __launch_bounds__(kBlock) k_full_residual(dev_problem p, const d
                                   lambda_override ? *lambda_override : lm->lambda, l);
 }
 
-// One wave per pose.
+// One wave per (pose, chunk), then one thread per (pose, row).
 template <class TS>
-__global__ void __launch_bounds__(256) k_pose_residual(dev_problem p, const double* pslots, const double* bp,
-                                                       const double* dxp, const TS* Wbuf, const double* gl, double* rs,
-                                                       const lm_state* lm, const double* lambda_override,
-                                                       const int32_t* status) {
+__global__ void __launch_bounds__(256) k_pose_residual_part(dev_problem p, const double* pslots, const TS* Wbuf,
+                                                            const double* gl, double* part, const lm_state* lm,
+                                                            const double* lambda_override, const int32_t* status) {
     if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
-    const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (i < p.np)
-        mixed::pose_residual_wave(p, pslots, bp, dxp, Wbuf, gl, rs, lambda_override ? *lambda_override : lm->lambda, i,
-                                  threadIdx.x & 63);
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (wave < p.np * kPoseChunks)
+        mixed::pose_residual_chunk(p, pslots, Wbuf, gl, part, wave / kPoseChunks, wave % kPoseChunks, threadIdx.x & 63);
+}
+
+__global__ void __launch_bounds__(256) k_pose_residual(dev_problem p, const double* part, const double* bp,
+                                                       const double* dxp, double* rs, const lm_state* lm,
+                                                       const double* lambda_override, const int32_t* status) {
+    if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < p.np * 6)
+        mixed::pose_residual_finish(p, part, bp, dxp, rs, lambda_override ? *lambda_override : lm->lambda, t / 6, t % 6);
 }
 
 // Threads [0, nl): dx_l += H_ll^-1 (r_l - W^T delta_p); threads [nl, nl + n): dx_p += delta_p.
@@ -813,9 +820,12 @@ void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t 
             k_full_residual<float><<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.dxp, w.dxl, w.bl_m,
                                                                              w.Hinv_f, w.pose_grad, w.zl, w.gl, w.lm,
                                                                              lambda_override, w.chol_status);
-        if (p.np > 0)
-            k_pose_residual<float><<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_grad, w.bp_m, w.dxp, w.W_f, w.gl,
-                                                                             w.rvec, w.lm, lambda_override, w.chol_status);
+        if (p.np > 0) {
+            k_pose_residual_part<float><<<grid_for(64L * p.np * kPoseChunks, 256), 256, 0, s>>>(
+                p, w.pose_grad, w.W_f, w.gl, w.pose_part, w.lm, lambda_override, w.chol_status);
+            k_pose_residual<<<grid_for(6L * p.np, 256), 256, 0, s>>>(p, w.pose_part, w.bp_m, w.dxp, w.rvec, w.lm,
+                                                                     lambda_override, w.chol_status);
+        }
         ZB_TIME_END(w, s);
         ZB_TIME_BEGIN(w, K_TRSV, s);
         launch_cholesky_resolve64(p.n, w, w.rvec, w.corr, s, gated != 0);

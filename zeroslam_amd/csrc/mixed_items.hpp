@@ -138,41 +138,66 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
     for (int i = 0; i < 3; ++i) bl[3L * l + i] = b[i];
 }
 
-// Lane j < 27 of the wave for pose i: H_pp term j (j < 21, TS) or gradient term
-// j - 21 (binary64) summed over the pose's slots (H_pp += H; b_p -= v).
+// Chunk c of [t0, t1) split into kPoseChunks contiguous pieces.
+__device__ __forceinline__ void chunk_range(int t0, int t1, int c, int& c0, int& c1) {
+    const int64_t len = t1 - t0;
+    c0 = t0 + static_cast<int>(len * c / kPoseChunks);
+    c1 = t0 + static_cast<int>(len * (c + 1) / kPoseChunks);
+}
+
+// Wave for (pose i, chunk c): lane j < 27 sums H_pp term j (j < 21) or gradient
+// term j - 21 over the chunk's slots into part[(i kPoseChunks + c) 27 + j].
 template <class TS>
-__device__ inline void reduce_pose_lane(const dev_problem& p, const TS* pterms, const double* pgrad, TS* Hpp, double* bp,
-                                        int i, int lane) {
+__device__ inline void reduce_pose_chunk(const dev_problem& p, const TS* pterms, const double* pgrad, double* part,
+                                         int i, int c, int lane) {
     const bool fixed = p.pose_fixed[i] != 0;
-    const int t0 = p.pose_start[i], t1 = fixed ? t0 : p.pose_start[i + 1];
-    constexpr int U = 16;
+    int t0, t1;
+    chunk_range(p.pose_start[i], fixed ? p.pose_start[i] : p.pose_start[i + 1], c, t0, t1);
+    constexpr int U = 8;
+    double* out = part + (static_cast<int64_t>(i) * kPoseChunks + c) * 27;
     if (lane < 21) {
         const TS* src = pterms + lane;
-        TS part[4] = {0, 0, 0, 0};
+        TS acc[4] = {0, 0, 0, 0};
         int t = t0;
         for (; t + U <= t1; t += U) {
             TS v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = src[21L * (t + u)];
 #pragma unroll
-            for (int u = 0; u < U; ++u) part[u & 3] += v[u];
+            for (int u = 0; u < U; ++u) acc[u & 3] += v[u];
         }
-        for (; t < t1; ++t) part[0] += src[21L * t];
-        Hpp[21L * i + lane] = (part[0] + part[1]) + (part[2] + part[3]);
+        for (; t < t1; ++t) acc[0] += src[21L * t];
+        out[lane] = static_cast<double>((acc[0] + acc[1]) + (acc[2] + acc[3]));
     } else if (lane < 27) {
         const double* src = pgrad + (lane - 21);
-        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
         int t = t0;
         for (; t + U <= t1; t += U) {
             double v[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) v[u] = src[6L * (t + u)];
 #pragma unroll
-            for (int u = 0; u < U; ++u) part[u & 3] += v[u];
+            for (int u = 0; u < U; ++u) acc[u & 3] += v[u];
         }
-        for (; t < t1; ++t) part[0] += src[6L * t];
-        bp[6L * i + (lane - 21)] = -((part[0] + part[1]) + (part[2] + part[3]));
+        for (; t < t1; ++t) acc[0] += src[6L * t];
+        out[lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
+}
+
+// Thread for (pose i, term j < 27): the chunk partials added in order; H_pp term
+// (TS) or b_p = -(sum of gradient terms).
+template <class TS>
+__device__ inline void reduce_pose_finish(const double* part, TS* Hpp, double* bp, int i, int j) {
+    const double* src = part + static_cast<int64_t>(i) * kPoseChunks * 27 + j;
+    double v[kPoseChunks];
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) v[c] = src[27 * c];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) a[c & 3] += v[c];
+    const double sum = (a[0] + a[1]) + (a[2] + a[3]);
+    if (j < 21) Hpp[21L * i + j] = static_cast<TS>(sum);
+    else bp[6L * i + (j - 21)] = -sum;
 }
 
 // Damped point block and its inverse (factor_graph.hpp:742-765).
@@ -372,18 +397,22 @@ __device__ inline void full_residual_item(const dev_problem& p, const double* po
     }
 }
 
-// Wave for pose i: rs_i = r_p,i - sum_{a of pose i} W_a g_l(a).
+// Wave for (pose i, chunk c): the chunk's share of sum_t slots_t + sum_a W_a g_l(a)
+// into part[(i kPoseChunks + c) 6 + r].
 template <class TS>
-__device__ inline void pose_residual_wave(const dev_problem& p, const double* pslots, const double* bp, const double* dxp,
-                                          const TS* Wbuf, const double* gl, double* rs, double lam, int i, int lane) {
+__device__ inline void pose_residual_chunk(const dev_problem& p, const double* pslots, const TS* Wbuf, const double* gl,
+                                           double* part, int i, int c, int lane) {
     const bool fixed = p.pose_fixed[i] != 0;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     if (!fixed) {
-        for (int t = p.pose_start[i] + lane; t < p.pose_start[i + 1]; t += 64) {
+        int t0, t1, a0, a1;
+        chunk_range(p.pose_start[i], p.pose_start[i + 1], c, t0, t1);
+        chunk_range(p.pose_pair_start[i], p.pose_pair_start[i + 1], c, a0, a1);
+        for (int t = t0 + lane; t < t1; t += 64) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc[k] += pslots[6L * t + k];
         }
-        for (int t = p.pose_pair_start[i] + lane; t < p.pose_pair_start[i + 1]; t += 64) {
+        for (int t = a0 + lane; t < a1; t += 64) {
             const int a = p.pose_pairs[t];
             const int l = p.pair_point[a];
             const double g0 = gl[3L * l], g1 = gl[3L * l + 1], g2 = gl[3L * l + 2];
@@ -400,8 +429,22 @@ __device__ inline void pose_residual_wave(const dev_problem& p, const double* ps
         double a = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) a = (q == lane) ? acc[q] : a;
-        rs[6L * i + lane] = fixed ? 0.0 : bp[6L * i + lane] - lam * dxp[6L * i + lane] - a;
+        part[(static_cast<int64_t>(i) * kPoseChunks + c) * 6 + lane] = a;
     }
+}
+
+// Thread for (pose i, row r < 6): rs = r_p - lambda dx_p - (chunk partials in order).
+__device__ inline void pose_residual_finish(const dev_problem& p, const double* part, const double* bp,
+                                            const double* dxp, double* rs, double lam, int i, int r) {
+    const double* src = part + static_cast<int64_t>(i) * kPoseChunks * 6 + r;
+    double v[kPoseChunks];
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) v[c] = src[6 * c];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) a[c & 3] += v[c];
+    const double sum = (a[0] + a[1]) + (a[2] + a[3]);
+    rs[6L * i + r] = p.pose_fixed[i] ? 0.0 : bp[6L * i + r] - lam * dxp[6L * i + r] - sum;
 }
 
 // dx_l += H_ll^-1 (r_l - sum_a W_a^T delta_p(a)) for point l.
