@@ -147,6 +147,64 @@ def state_error(P, X, Pr, Xr):
                      np.abs(X - Xr).max() / max(np.abs(Xr).max(), 1e-300)))
 
 
+def equal_cost(g, problem, iters, steps, warmup, chi64, ms64, P64, X64):
+    """Mixed against the FP64 (reference order) and FP64-fast paths at equal cost.
+    The level is the FP64 cost after `iters` iterations; when the mixed run settles
+    in a local minimum above it (the robust problem has many nearby minima and
+    rounding decides which one a run ends in), the level is the mixed run's
+    converged cost, which the FP64 runs reach first.  Each path is charged the
+    wall time of its smallest iteration budget that reaches the level."""
+    from zeroslam_amd.engine import FP64, FP64_FAST, MIXED
+    level = chi64 * (1.0 + 1e-6)
+    level_name = f"FP64 cost after {iters} iterations"
+    k64, ms64_level = iters, ms64
+    kmix = iterations_to_reach(g, MIXED, level, 1, 4 * iters)
+    if kmix is None:
+        g.restore()
+        g.solve(4 * iters, False, MIXED)
+        cmix = g.stats()["final_chi2"]
+        level = cmix * (1.0 + 1e-6)
+        level_name = "converged mixed cost (a local minimum above the FP64 cost)"
+        kmix = iterations_to_reach(g, MIXED, level, 1, 4 * iters)
+        k64 = iterations_to_reach(g, FP64, level, 1, iters)
+        ms64_level = time_solves(g, FP64, k64, steps, warmup)[1]
+        print(f"bench: mixed settles at chi2 {cmix!r} (FP64 {chi64!r}); comparing at that level",
+              file=sys.stderr, flush=True)
+    kfast = iterations_to_reach(g, FP64_FAST, level, 1, 4 * iters)
+    msf_level = time_solves(g, FP64_FAST, kfast, steps, warmup)[1] if kfast else None
+    vraw, ms, stats, kms, kl = time_solves(g, MIXED, kmix, steps, warmup)
+    cmp = {"equal_cost_level": level_name, "level_chi2": level,
+           "iterations_mixed": kmix, "iterations_fp64": k64, "iterations_fp64_fast": kfast,
+           "ms_mixed": ms, "ms_fp64": ms64_level, "ms_fp64_fast": msf_level,
+           "mixed_obs_iters_per_s": vraw, "final_chi2": stats["final_chi2"],
+           "chi2_rel_vs_fp64": (stats["final_chi2"] - chi64) / chi64,
+           "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64),
+           "speedup_vs_fp64": ms64_level / ms,
+           "speedup_vs_fp64_fast": (msf_level / ms) if msf_level else None}
+    print(f"bench: mixed reaches the level in {kmix} iterations, {ms:.2f} ms (FP64 {k64} iterations, "
+          f"{ms64_level:.2f} ms; FP64-fast {kfast} iterations, {msf_level if msf_level else float('nan'):.2f} ms)",
+          file=sys.stderr, flush=True)
+    return cmp, (vraw, ms, stats, kms, kl), k64
+
+
+def seed_compare(seed, config, iters):
+    """The equal-cost comparison on another instance of the same workload
+    (generator seed `seed`), two timed solves each."""
+    from zeroslam_amd.engine import FP64, Graph
+    from zeroslam_amd.problem import synth
+    pr = synth(config, seed=seed)
+    g = Graph.from_problem(pr)
+    g.solve(0)
+    g.snapshot()
+    _, ms64, st64, _, _ = time_solves(g, FP64, iters, 2, 1)
+    cmp, _, _ = equal_cost(g, pr, iters, 2, 1, st64["final_chi2"], ms64, g.poses(), g.points())
+    keep = ("equal_cost_level", "iterations_mixed", "iterations_fp64", "iterations_fp64_fast", "ms_mixed", "ms_fp64",
+            "ms_fp64_fast", "chi2_rel_vs_fp64", "speedup_vs_fp64", "speedup_vs_fp64_fast")
+    out = {"seed": seed, "n_obs": pr.n_obs}
+    out.update({k: cmp[k] for k in keep})
+    return out
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_mixed_hbm_traffic.md")
 GROUP_KERNELS = {
     "cholesky": ["k_chol_begin", "k_chol_fused"],
@@ -304,10 +362,12 @@ def main() -> int:
                     help="time only --precision at the workload's iteration count (no FP64 comparison; profiling)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seeds", type=int, default=4,
+                    help="repeat the equal-cost comparison on this many other instances of the workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+              "(multi-GPU sharding is not implemented)", file=sys.stderr)
         return 2
 
     if args.config == 2:
@@ -354,43 +414,14 @@ def main() -> int:
             value, ms, stats, kms, kl = vf, msf, stf, kf, lf
             run_iters = iters
         else:
-            # mixed, charged the time it needs to reach the FP64 final cost.  The
-            # robust (Huber) problem has many nearby local minima and which one an
-            # LM run settles in depends on rounding; when the mixed run settles
-            # above the FP64 cost, both are compared at the cost level they both
-            # reach (the mixed run's converged cost) instead.
-            level = chi64 * (1.0 + 1e-6)
-            level_name = f"FP64 cost after {iters} iterations"
-            k64, ms64_level = iters, ms64
-            kmix = iterations_to_reach(g, MIXED, level, 1, 4 * iters)
-            if kmix is None:
-                g.restore()
-                g.solve(4 * iters, False, MIXED)
-                cmix = g.stats()["final_chi2"]
-                level = cmix * (1.0 + 1e-6)
-                level_name = "converged mixed cost (a local minimum above the FP64 cost)"
-                kmix = iterations_to_reach(g, MIXED, level, 1, 4 * iters)
-                k64 = iterations_to_reach(g, FP64, level, 1, iters)
-                ms64_level = time_solves(g, FP64, k64, args.steps, args.warmup)[1]
-                print(f"bench: mixed settles at chi2 {cmix!r} (FP64 {chi64!r}); comparing at that level",
-                      file=sys.stderr, flush=True)
-            kfast = iterations_to_reach(g, FP64_FAST, level, 1, 4 * iters)
-            msf_level = time_solves(g, FP64_FAST, kfast, args.steps, args.warmup)[1] if kfast else None
-            vraw, ms, stats, kms, kl = time_solves(g, MIXED, kmix, args.steps, args.warmup)
-            run_iters = kmix
+            cmp, (vraw, ms, stats, kms, kl), k64 = equal_cost(g, problem, iters, args.steps, args.warmup, chi64, ms64,
+                                                              P64, X64)
+            run_iters = cmp["iterations_mixed"]
             # observation*iterations of the FP64 run to the same cost, per second of the mixed run
             value = problem.n_obs * k64 / (ms * 1e-3)
-            extra["mixed"] = {"equal_cost_level": level_name, "level_chi2": level,
-                              "iterations_mixed": kmix, "iterations_fp64": k64, "iterations_fp64_fast": kfast,
-                              "ms_mixed": ms, "ms_fp64": ms64_level, "ms_fp64_fast": msf_level,
-                              "mixed_obs_iters_per_s": vraw, "final_chi2": stats["final_chi2"],
-                              "chi2_rel_vs_fp64": (stats["final_chi2"] - chi64) / chi64,
-                              "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64),
-                              "speedup_vs_fp64": ms64_level / ms,
-                              "speedup_vs_fp64_fast": (msf_level / ms) if msf_level else None}
-            print(f"bench: mixed reaches the level in {kmix} iterations, {ms:.2f} ms "
-                  f"(FP64 {k64} iterations, {ms64_level:.2f} ms; FP64-fast {kfast} iterations, "
-                  f"{msf_level if msf_level else float('nan'):.2f} ms)", file=sys.stderr, flush=True)
+            extra["mixed"] = cmp
+            if args.seeds > 0:
+                extra["other_seeds"] = [seed_compare(seed, args.config, iters) for seed in range(2, 2 + args.seeds)]
 
     roofline, avg = roofline_of(problem, args.precision, kms, kl, args.config)
     out = {

@@ -371,25 +371,46 @@ __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, c
     for (int i = 0; i < 36; ++i) acc[i] = 0.0;
     double accb[6] = {0, 0, 0, 0, 0, 0};
     const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
-    for (int64_t c = c0 + lane; c < c1; c += 64) {
+    // contributions c, c + 64 of this lane: two in flight (loads first), added in order
+    auto fetch = [&](int64_t c, double (&Ya)[18], double (&Wc)[18], double (&b3)[3]) {
         const int2 ab = p.contrib[c];
-        double Ya[18], Wc[18], lp[18], prod[36];
 #pragma unroll
         for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
 #pragma unroll
         for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
+        b3[0] = b3[1] = b3[2] = 0.0;
+        if (diag) {
+            const int l = p.pair_point[ab.x];
+            b3[0] = bl[3L * l];
+            b3[1] = bl[3L * l + 1];
+            b3[2] = bl[3L * l + 2];
+        }
+    };
+    auto add = [&](const double (&Ya)[18], const double (&Wc)[18], const double (&b3)[3]) {
+        double lp[18], prod[36];
         transpose<6, 3>(Wc, lp);
         mul<6, 3, 6>(Ya, lp, prod);
 #pragma unroll
         for (int i = 0; i < 36; ++i) acc[i] = acc[i] + prod[i];
         if (diag) {
-            const int l = p.pair_point[ab.x];
-            const double b3[3] = {bl[3L * l], bl[3L * l + 1], bl[3L * l + 2]};
             double yb[6];
             mul<6, 3, 1>(Ya, b3, yb);
 #pragma unroll
             for (int i = 0; i < 6; ++i) accb[i] = accb[i] + yb[i];
         }
+    };
+    int64_t c = c0 + lane;
+    for (; c + 64 < c1; c += 128) {
+        double Ya0[18], Wc0[18], Ya1[18], Wc1[18], b0[3], b1[3];
+        fetch(c, Ya0, Wc0, b0);
+        fetch(c + 64, Ya1, Wc1, b1);
+        add(Ya0, Wc0, b0);
+        add(Ya1, Wc1, b1);
+    }
+    if (c < c1) {
+        double Ya0[18], Wc0[18], b0[3];
+        fetch(c, Ya0, Wc0, b0);
+        add(Ya0, Wc0, b0);
     }
     const double lam = lambda_override ? *lambda_override : lm->lambda;
     const int64_t n = p.n;

@@ -247,26 +247,48 @@ __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* 
     for (int i = 0; i < 36; ++i) acc[i] = TS(0);
     double accb[6] = {0, 0, 0, 0, 0, 0};
     const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
-    for (int64_t c = c0 + lane; c < c1; c += 64) {
-        const int2 ab = p.contrib[c];
-        TS Ya[18], Wc[18];
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
+    // contributions c, c + 64 of this lane: two in flight (loads first), added in order
+    auto add = [&](const TS (&Ya)[18], const TS (&Wc)[18], const double (&bv)[3]) {
 #pragma unroll
         for (int r = 0; r < 6; ++r)
 #pragma unroll
             for (int cc = 0; cc < 6; ++cc)
                 acc[r * 6 + cc] += Ya[r * 3] * Wc[cc * 3] + Ya[r * 3 + 1] * Wc[cc * 3 + 1] + Ya[r * 3 + 2] * Wc[cc * 3 + 2];
         if (diag) {
-            const int l = p.pair_point[ab.x];
-            const double b0 = bl[3L * l], b1 = bl[3L * l + 1], b2 = bl[3L * l + 2];
 #pragma unroll
             for (int r = 0; r < 6; ++r)
-                accb[r] += static_cast<double>(Ya[r * 3]) * b0 + static_cast<double>(Ya[r * 3 + 1]) * b1 +
-                           static_cast<double>(Ya[r * 3 + 2]) * b2;
+                accb[r] += static_cast<double>(Ya[r * 3]) * bv[0] + static_cast<double>(Ya[r * 3 + 1]) * bv[1] +
+                           static_cast<double>(Ya[r * 3 + 2]) * bv[2];
         }
+    };
+    auto fetch = [&](int64_t c, TS (&Ya)[18], TS (&Wc)[18], double (&bv)[3]) {
+        const int2 ab = p.contrib[c];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
+        bv[0] = bv[1] = bv[2] = 0.0;
+        if (diag) {
+            const int l = p.pair_point[ab.x];
+            bv[0] = bl[3L * l];
+            bv[1] = bl[3L * l + 1];
+            bv[2] = bl[3L * l + 2];
+        }
+    };
+    int64_t c = c0 + lane;
+    for (; c + 64 < c1; c += 128) {
+        TS Ya0[18], Wc0[18], Ya1[18], Wc1[18];
+        double b0[3], b1[3];
+        fetch(c, Ya0, Wc0, b0);
+        fetch(c + 64, Ya1, Wc1, b1);
+        add(Ya0, Wc0, b0);
+        add(Ya1, Wc1, b1);
+    }
+    if (c < c1) {
+        TS Ya0[18], Wc0[18];
+        double b0[3];
+        fetch(c, Ya0, Wc0, b0);
+        add(Ya0, Wc0, b0);
     }
     const int64_t n = p.n;
     double* Sblk = S + (6L * row) * n + 6L * col;
