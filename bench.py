@@ -379,7 +379,10 @@ def main() -> int:
     problem = synth(args.config)
     iters = ITERATIONS[args.config]
     precision = {"fp64": FP64, "fp64_fast": FP64_FAST, "mixed": MIXED}[args.precision]
-    g = Graph.from_problem(problem, timing=True)
+    # HIP events time the factorisation and solve classes inside the timed region
+    # (the dominant group; every class with --single).  Events around every class
+    # would add about a fifth to the solve time.
+    g = Graph.from_problem(problem, timing=True, timing_classes=None if args.single else ["cholesky", "trsv"])
     g.solve(0)          # structure analysis + upload (untimed)
     g.snapshot()
     print(f"bench: c{args.config} obs={problem.n_obs} poses={problem.n_poses} points={problem.n_points}",

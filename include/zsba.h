@@ -70,7 +70,7 @@ typedef struct zsba_options {
     int32_t verbose;      /* 1: print the reference's [INIT]/[GOOD]/[ BAD] lines     */
     int32_t refine_steps; /* MIXED: refinement steps of the reduced solve: 0 = default
                              (2), negative = none                                      */
-    int32_t timing;       /* 1: time every kernel launch with HIP events (stats)      */
+    int32_t timing;       /* 1: time kernel launches with HIP events (stats)          */
     int32_t floor_exp;    /* MIXED: the binary32-block reduced system is factorised
                              with its diagonal raised by 2^-floor_exp max|S_ii| (its
                              rounding floor); the binary64 refinement uses the true
@@ -78,7 +78,9 @@ typedef struct zsba_options {
     int32_t factor_path;  /* reduced-system factorisation: 0 = automatic, 1 = one
                              workgroup (envelope panels up to 16 tiles of 32 rows),
                              2 = one launch per 32-column panel                     */
-    int32_t reserved[2];
+    int32_t timing_classes; /* with timing: bit mask of the kernel classes timed
+                             (bit ZSBA_K_*); 0 = every class                        */
+    int32_t reserved[1];
 } zsba_options;
 
 /* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */

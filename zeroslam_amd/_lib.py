@@ -23,7 +23,7 @@ ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EDEVICE", -4: "ENOMEM", -5: "EUNSUP"}
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("refine_steps", C.c_int32),
                 ("timing", C.c_int32), ("floor_exp", C.c_int32), ("factor_path", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("timing_classes", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class Stats(C.Structure):

@@ -23,6 +23,8 @@ struct kernel_timer {
         int id;
     };
     bool on = false;
+    uint32_t mask = ~0u;  // classes timed (bit per class id)
+    bool open = false;    // a begin() recorded and awaiting its end()
     std::vector<record> pending;
     std::vector<hipEvent_t> pool;
     double ms[kClasses] = {};
@@ -39,14 +41,16 @@ struct kernel_timer {
         return e;
     }
     void begin(int id, hipStream_t s) {
-        if (!on) return;
+        if (!on || !((mask >> id) & 1u)) return;
         record r{take(), take(), id};
         (void)hipEventRecord(r.start, s);
         pending.push_back(r);
+        open = true;
     }
     void end(hipStream_t s) {
-        if (!on || pending.empty()) return;
+        if (!on || !open) return;
         (void)hipEventRecord(pending.back().stop, s);
+        open = false;
     }
     // Call after the stream has been synchronised.
     void collect() {

@@ -479,6 +479,7 @@ int32_t zsba_graph::build_structure() {
     w = dev_work64{};
     w.timer = opt.timing ? &timer : nullptr;
     timer.on = opt.timing != 0;
+    timer.mask = opt.timing_classes ? static_cast<uint32_t>(opt.timing_classes) : ~0u;
     w.pose = mem.get<double>(7L * np); w.pose_backup = mem.get<double>(7L * np);
     w.point = mem.get<double>(3L * nl); w.point_backup = mem.get<double>(3L * nl);
     w.residual = mem.get<double>(2L * no); w.pose_terms = mem.get<double>(27L * no);

@@ -84,6 +84,22 @@ struct phase_clock {
     }
 };
 
+// The workgroup factorisation and solve as calls of their own: their register
+// demand (a 32-wide row per lane) then does not add to what the megakernel keeps
+// live around them.
+__device__ __noinline__ bool batch_factor(int n, double* S, const int32_t* tile_last, double eps,
+                                          double (*PT)[wg::kTile][wg::kPT], int* flag, double* recips,
+                                          double (*col)[64]) {
+    return wg::fused_factor<kBatchTiles, true>(n, S, tile_last, eps, PT, flag, recips, col);
+}
+
+template <int kThreads>
+__device__ __noinline__ int batch_solve(int n, const double* S, const double* b, double* x, const int32_t* tile_first,
+                                        const int32_t* tile_last, double (*buf)[wg::kTile][wg::kTile + 1], double* sh,
+                                        int* flag) {
+    return wg::triangular_solve_fast<kThreads>(n, S, b, x, tile_first, tile_last, buf, sh, flag);
+}
+
 template <class TS, int kThreads>
 __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wins, lm_state* out, int64_t* phase_ticks) {
     const batch_window& bw = wins[blockIdx.x];
@@ -165,9 +181,9 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         if (bw.floor_rel > 0.0)
             for (int i = tid; i < n; i += blockDim.x) w.S[static_cast<int64_t>(i) * n + i] += lift;
         __syncthreads();
-        bool ok = wg::fused_factor<kBatchTiles, true>(n, w.S, w.tile_last, (mx + lift) * 1e-14, PT, &flag, recips, col);
+        bool ok = batch_factor(n, w.S, w.tile_last, (mx + lift) * 1e-14, PT, &flag, recips, col);
         clk.mark(PH_FACTOR);
-        if (ok) ok = wg::triangular_solve_fast<kThreads>(n, w.S, w.bs, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) == 0;
+        if (ok) ok = batch_solve<kThreads>(n, w.S, w.bs, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) == 0;
         clk.mark(PH_SOLVE);
         if (ok) {
             for (int i = tid; i < n; i += blockDim.x) w.dxp[i] = xs[i];
@@ -188,7 +204,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                 for (int ir = tid; ir < p.np * 6; ir += blockDim.x)
                     mixed::pose_residual_finish(p, w.pose_part, w.bp_m, w.dxp, w.rvec, lam, ir / 6, ir % 6);
                 __syncthreads();
-                if (wg::triangular_solve_fast<kThreads>(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) != 0) {
+                if (batch_solve<kThreads>(n, w.S, w.rvec, xs, w.tile_first, w.tile_last, solve_buf, sh, &flag) != 0) {
                     ok = false;
                     break;
                 }

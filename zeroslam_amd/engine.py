@@ -37,10 +37,16 @@ class ZsbaError(RuntimeError):
 
 class Graph:
     def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 0, timing: bool = False,
-                 floor_exp: int = 0, factor_path: int = 0):
+                 floor_exp: int = 0, factor_path: int = 0, timing_classes=None):
+        """timing: time kernel classes with HIP events (stats()["kernel_ms"]);
+        timing_classes: names from KERNEL_NAMES to time (None: all)."""
         self.lib = _lib.load()
+        mask = 0
+        if timing_classes:
+            for name in timing_classes:
+                mask |= 1 << _lib.KERNEL_NAMES.index(name)
         opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps, timing=int(bool(timing)),
-                            floor_exp=floor_exp, factor_path=factor_path)
+                            floor_exp=floor_exp, factor_path=factor_path, timing_classes=mask)
         self.h = self.lib.zsba_create(C.byref(opts))
         if not self.h:
             raise MemoryError("zsba_create failed")
