@@ -212,7 +212,9 @@ constexpr int kPoseChunks = 16;
 // fixed-shape parallel reductions.
 void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target /*0 chi2, 1 chi2_new*/,
                    bool only_if_step, bool exact = false);
-void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
+// exact: per-pose sums in insertion order (the reference's); otherwise in the
+// fixed chunks of the mixed path.
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact = true);
 void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override,
                              bool exact = false);

@@ -708,7 +708,7 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
         zb::launch_linearise_mixed(dp, w, stream, false);
         zb::launch_init_lm_mixed(dp, w, stream);
     } else {
-        zb::launch_linearise64(dp, w, stream, false);
+        zb::launch_linearise64(dp, w, stream, false, exact);
         zb::launch_init_lm64(dp, w, stream);
     }
     const int32_t max_attempts = iterations * 10;
@@ -733,7 +733,7 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
         zb::launch_decide64(w, stream);
         zb::launch_rollback64(dp, w, stream);
         if (mixed) zb::launch_linearise_mixed(dp, w, stream, true);
-        else zb::launch_linearise64(dp, w, stream, true);
+        else zb::launch_linearise64(dp, w, stream, true, exact);
         zb::launch_end_attempt64(w, stream);
         (void)hipMemcpyAsync(done_pinned, &w.lm->done, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
         (void)hipEventRecord(ev[a & 1], stream);

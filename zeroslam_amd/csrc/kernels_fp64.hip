@@ -274,6 +274,53 @@ __global__ void __launch_bounds__(256) k_reduce_pose(dev_problem p, const double
     else bp[6L * wave + (lane - 21)] = acc;
 }
 
+// The same sums in kPoseChunks fixed chunks (reassociating precision): one wave
+// per (pose, chunk), lane j < 27 summing term j, then one thread per (pose, term)
+// adding the chunks in order.
+__global__ void __launch_bounds__(256) k_reduce_pose_part(dev_problem p, const double* pose_terms, double* part,
+                                                          const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wave >= p.np * kPoseChunks || lane >= 27) return;
+    const int i = wave / kPoseChunks, c = wave % kPoseChunks;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (!p.pose_fixed[i]) {
+        int t0, t1;
+        mixed::chunk_range(p.pose_start[i], p.pose_start[i + 1], c, t0, t1);
+        const double* src = pose_terms + lane;
+        constexpr int U = 8;
+        int t = t0;
+        for (; t + U <= t1; t += U) {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = src[27L * (t + u)];
+#pragma unroll
+            for (int u = 0; u < U; ++u) acc[u & 3] += v[u];
+        }
+        for (; t < t1; ++t) acc[0] += src[27L * t];
+    }
+    part[static_cast<int64_t>(wave) * 27 + lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+__global__ void __launch_bounds__(256) k_reduce_pose_chunks(dev_problem p, const double* part, double* Hpp,
+                                                            double* bp, const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= p.np * 27) return;
+    const int i = t / 27, j = t % 27;
+    const double* src = part + static_cast<int64_t>(i) * kPoseChunks * 27 + j;
+    double v[kPoseChunks];
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) v[c] = src[27 * c];
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < kPoseChunks; ++c) a[c & 3] += v[c];
+    const double sum = (a[0] + a[1]) + (a[2] + a[3]);
+    if (j < 21) Hpp[21L * i + j] = sum;  // H_pp += H
+    else bp[6L * i + (j - 21)] = -sum;   // b_p -= v
+}
+
 __global__ void k_linearise_done(lm_state* lm) { lm_linearise_done(lm); }
 
 // ---- LM initialisation, factor_graph.hpp:511-522 -----------------------------
@@ -804,7 +851,7 @@ void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t
     if (n > 0) k_math_probe<<<grid_for(n, kBlock), kBlock, 0, s>>>(fn, n, in, out);
 }
 
-void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact) {
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_linearise_points<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms, w.Hll, w.bl, w.W,
@@ -813,7 +860,14 @@ void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s
     }
     if (p.np > 0) {
         ZB_TIME_BEGIN(w, K_POSE_REDUCE, s);
-        k_reduce_pose<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms, w.Hpp, w.bp, w.lm, gated ? 1 : 0);
+        if (exact) {
+            k_reduce_pose<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.pose_terms, w.Hpp, w.bp, w.lm, gated ? 1 : 0);
+        } else {
+            k_reduce_pose_part<<<grid_for(64L * p.np * kPoseChunks, 256), 256, 0, s>>>(p, w.pose_terms, w.pose_part,
+                                                                                     w.lm, gated ? 1 : 0);
+            k_reduce_pose_chunks<<<grid_for(27L * p.np, 256), 256, 0, s>>>(p, w.pose_part, w.Hpp, w.bp, w.lm,
+                                                                          gated ? 1 : 0);
+        }
         ZB_TIME_END(w, s);
     }
     if (gated) k_linearise_done<<<1, 1, 0, s>>>(w.lm);
