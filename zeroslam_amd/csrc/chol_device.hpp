@@ -91,6 +91,7 @@ __device__ __forceinline__ double div_cr_nofb(double a, double b, double y, bool
 // is set where a division needs div_cr's fallback.  recips[j] receives
 // RN(1 / L_jj).  Returns false (uniformly) when a pivot of a real row is not
 // greater than eps.
+template <int kChunk>
 __device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double eps, double* recips, bool& bad,
                                             double (*col)[64]) {
     const int lane = opaque_lane();
@@ -107,11 +108,17 @@ __device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double ep
         col[j][lane] = row[j];
         if (j + 1 < NB) {
             row[j + 1] = row[j + 1] - row[j] * lane_value(row[j], j + 1);
-            double t[NB];  // the column's broadcasts, loaded together before any use
 #pragma unroll
-            for (int c = j + 2; c < NB; ++c) t[c] = col[j][c];
+            for (int c0 = j + 2; c0 < NB; c0 += kChunk) {
+                double t[kChunk];  // the column's broadcasts, kChunk loaded together before any use
 #pragma unroll
-            for (int c = j + 2; c < NB; ++c) row[c] = row[c] - row[j] * t[c];
+                for (int u = 0; u < kChunk; ++u)
+                    if (c0 + u < NB) t[u] = col[j][c0 + u];
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u)
+                    if (c0 + u < NB) row[c0 + u] = row[c0 + u] - row[j] * t[u];
+                if (kChunk < NB) __builtin_amdgcn_sched_barrier(0);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -122,6 +129,7 @@ __device__ __forceinline__ bool factor_rows(double (&row)[NB], int nr, double ep
 // estimate and two Newton steps, L_jj = A_jj / sqrt(A_jj), L_rj = A_rj / L_jj as a
 // product with that reciprocal, updates as fused multiply-adds.  recips[j]
 // receives the reciprocal.
+template <int kChunk>
 __device__ __forceinline__ bool factor_rows_fast(double (&row)[NB], int nr, double eps, double* recips,
                                                  double (*col)[64]) {
     const int lane = opaque_lane();
@@ -138,11 +146,17 @@ __device__ __forceinline__ bool factor_rows_fast(double (&row)[NB], int nr, doub
         col[j][lane] = row[j];
         if (j + 1 < NB) {
             row[j + 1] = fma(-row[j], lane_value(row[j], j + 1), row[j + 1]);
-            double t[NB];
 #pragma unroll
-            for (int c = j + 2; c < NB; ++c) t[c] = col[j][c];
+            for (int c0 = j + 2; c0 < NB; c0 += kChunk) {
+                double t[kChunk];
 #pragma unroll
-            for (int c = j + 2; c < NB; ++c) row[c] = fma(-row[j], t[c], row[c]);
+                for (int u = 0; u < kChunk; ++u)
+                    if (c0 + u < NB) t[u] = col[j][c0 + u];
+#pragma unroll
+                for (int u = 0; u < kChunk; ++u)
+                    if (c0 + u < NB) row[c0 + u] = fma(-row[j], t[u], row[c0 + u]);
+                if (kChunk < NB) __builtin_amdgcn_sched_barrier(0);
+            }
         }
         __builtin_amdgcn_sched_barrier(0);
     }
@@ -190,7 +204,7 @@ __device__ __forceinline__ bool factor_diag_tile(double (*D)[NB + 1], int nr, do
 #pragma unroll
         for (int c = 0; c < NB; ++c) row[c] = D[lr][c];
         bool bad = false;
-        bool ok = factor_rows(row, nr, eps, recips, bad, col);
+        bool ok = factor_rows<NB>(row, nr, eps, recips, bad, col);
         if (!__any(bad)) {
             if (lane < nr) {
 #pragma unroll
@@ -271,7 +285,7 @@ __device__ __noinline__ void solve_row_slow(double* arow, const double (*col)[64
     }
 }
 
-template <int kTiles, bool kFast>
+template <int kTiles, bool kFast, int kChunk = NB>
 __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double eps, double (*PT)[NB][kPT],
                              int* ok_sh, double* recips, double (*col)[64]) {
     const int T = (n + NB - 1) / NB;
@@ -295,7 +309,8 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
                 v[c] = real ? a : ((c == lr) ? 1.0 : 0.0);
             }
             bool bad = false;
-            bool ok = kFast ? factor_rows_fast(v, nk, eps, recips, col) : factor_rows(v, nk, eps, recips, bad, col);
+            bool ok = kFast ? factor_rows_fast<kChunk>(v, nk, eps, recips, col)
+                            : factor_rows<kChunk>(v, nk, eps, recips, bad, col);
             if (!kFast && __any(bad)) {
                 // rare: redo the tile in LDS with divisions, from the unfactored rows
                 double (*D)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&PT[0][0][0]);
@@ -330,17 +345,17 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
             bool bad = false;
 #pragma unroll
             for (int m = 0; m < NB; ++m) {
-                double t[NB];  // column m of L_kk, loaded together before any use
+                v[m] = kFast ? v[m] * recips[m] : div_cr_nofb(v[m], col[m][m], recips[m], bad);
 #pragma unroll
-                for (int c = m; c < NB; ++c) t[c] = col[m][c];
-                if (kFast) {
-                    v[m] = v[m] * recips[m];
+                for (int c0 = m + 1; c0 < NB; c0 += kChunk) {
+                    double t[kChunk];  // column m of L_kk, kChunk loaded together before any use
 #pragma unroll
-                    for (int c = m + 1; c < NB; ++c) v[c] = fma(-v[m], t[c], v[c]);
-                } else {
-                    v[m] = div_cr_nofb(v[m], t[m], recips[m], bad);
+                    for (int u = 0; u < kChunk; ++u)
+                        if (c0 + u < NB) t[u] = col[m][c0 + u];
 #pragma unroll
-                    for (int c = m + 1; c < NB; ++c) v[c] = v[c] - v[m] * t[c];
+                    for (int u = 0; u < kChunk; ++u)
+                        if (c0 + u < NB) v[c0 + u] = kFast ? fma(-v[m], t[u], v[c0 + u]) : v[c0 + u] - v[m] * t[u];
+                    if (kChunk < NB) __builtin_amdgcn_sched_barrier(0);
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }

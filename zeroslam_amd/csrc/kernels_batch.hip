@@ -84,17 +84,18 @@ struct phase_clock {
     }
 };
 
-// The workgroup factorisation and solve as calls of their own: their register
-// demand (a 32-wide row per lane) then does not add to what the megakernel keeps
-// live around them.
-__device__ __noinline__ bool batch_factor(int n, double* S, const int32_t* tile_last, double eps,
+// The workgroup factorisation and solve, inlined (as calls they saved and
+// restored the megakernel's registers through scratch memory, several times
+// slower).  The factorisation loads its broadcasts 8 at a time: the same
+// arithmetic as the per-graph kernel's 32 at a time, in fewer registers.
+__device__ __forceinline__ bool batch_factor(int n, double* S, const int32_t* tile_last, double eps,
                                           double (*PT)[wg::kTile][wg::kPT], int* flag, double* recips,
                                           double (*col)[64]) {
-    return wg::fused_factor<kBatchTiles, true>(n, S, tile_last, eps, PT, flag, recips, col);
+    return wg::fused_factor<kBatchTiles, true, 8>(n, S, tile_last, eps, PT, flag, recips, col);
 }
 
 template <int kThreads>
-__device__ __noinline__ int batch_solve(int n, const double* S, const double* b, double* x, const int32_t* tile_first,
+__device__ __forceinline__ int batch_solve(int n, const double* S, const double* b, double* x, const int32_t* tile_first,
                                         const int32_t* tile_last, double (*buf)[wg::kTile][wg::kTile + 1], double* sh,
                                         int* flag) {
     return wg::triangular_solve_fast<kThreads>(n, S, b, x, tile_first, tile_last, buf, sh, flag);
@@ -167,7 +168,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
         __syncthreads();
         clk.mark(PH_INVERT);
         for (int blk = tid >> 6; blk < p.nblk; blk += nwaves)
-            mixed::schur_block(p, B.Y, B.W, B.Hpp, w.bp_m, w.bl_m, w.S, w.bs, lam, blk, lane);
+            mixed::schur_block<TS, 1>(p, B.Y, B.W, B.Hpp, w.bp_m, w.bl_m, w.S, w.bs, lam, blk, lane);
         __syncthreads();
         clk.mark(PH_SCHUR);
         // factorisation and solve

@@ -237,7 +237,10 @@ __device__ inline void form_Y_item(const dev_problem& p, const TS* Wbuf, const T
 // binary64; the block is assembled in binary64:
 //   S_ik = [i == k] (H_pp,i + lambda I) - sum Y W^T,  b_s,i = b_p,i - sum Y b_l
 // (factor_graph.hpp:768-781, matrix_sparse_block.hpp:158-196).
-template <class TS>
+// kInFlight: contributions per lane loaded before they are added (2 hides the
+// re-reads' latency; 1 needs fewer registers).  Either way each lane adds its
+// contributions c, c + 64, ... in order, so the result is the same.
+template <class TS, int kInFlight = 2>
 __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* Wbuf, const TS* Hpp, const double* bp,
                                    const double* bl, double* S, double* bs, double lam, int blk, int lane) {
     const int row = p.blk_row[blk], col = p.blk_col[blk];
@@ -276,15 +279,17 @@ __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* 
         }
     };
     int64_t c = c0 + lane;
-    for (; c + 64 < c1; c += 128) {
-        TS Ya0[18], Wc0[18], Ya1[18], Wc1[18];
-        double b0[3], b1[3];
-        fetch(c, Ya0, Wc0, b0);
-        fetch(c + 64, Ya1, Wc1, b1);
-        add(Ya0, Wc0, b0);
-        add(Ya1, Wc1, b1);
+    if (kInFlight == 2) {
+        for (; c + 64 < c1; c += 128) {
+            TS Ya0[18], Wc0[18], Ya1[18], Wc1[18];
+            double b0[3], b1[3];
+            fetch(c, Ya0, Wc0, b0);
+            fetch(c + 64, Ya1, Wc1, b1);
+            add(Ya0, Wc0, b0);
+            add(Ya1, Wc1, b1);
+        }
     }
-    if (c < c1) {
+    for (; c < c1; c += 64) {
         TS Ya0[18], Wc0[18];
         double b0[3];
         fetch(c, Ya0, Wc0, b0);
