@@ -134,6 +134,34 @@ __device__ __forceinline__ bool factor_rows_fast(double (&row)[NB], int nr, doub
                                                  double (*col)[64]) {
     const int lane = opaque_lane();
     bool ok = true;
+    if (kChunk == NB) {
+        // lagged updates: column j - 1's broadcasts, loaded during step j - 1, are
+        // applied during step j, so their LDS round trip overlaps step j's pivot
+        // chain; entry j gets column j - 1's update first, by readlane.  Each
+        // element still receives its updates in ascending column order.
+        double tp[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j >= 1) row[j] = fma(-row[j - 1], lane_value(row[j - 1], j), row[j]);
+            const double d = lane_value(row[j], j);
+            ok &= (j >= nr) | (d > eps);
+            double y = __builtin_amdgcn_rsq(d);
+            y = y * fma(-0.5 * d * y, y, 1.5);
+            y = y * fma(-0.5 * d * y, y, 1.5);
+            row[j] = (lane == j) ? d * y : row[j] * y;
+            recips[j] = y;
+            col[j][lane] = row[j];
+            if (j >= 1) {
+#pragma unroll
+                for (int c = j + 1; c < NB; ++c) row[c] = fma(-row[j - 1], tp[c], row[c]);
+            }
+#pragma unroll
+            for (int c = j + 2; c < NB; ++c) tp[c] = col[j][c];
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // the last column's update has no entries left to reach
+        return ok;
+    }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const double d = lane_value(row[j], j);
@@ -343,21 +371,44 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
         if (*ok_sh == 0) return false;
         if (tid >= 64 && rt < nbelow) {
             bool bad = false;
+            if (kChunk == NB) {
+                // column m + 1 of L_kk (and its reciprocal) loaded while column m is applied
+                double t[NB], tn[NB];
+                double y = recips[0], yn = 0.0;
 #pragma unroll
-            for (int m = 0; m < NB; ++m) {
-                v[m] = kFast ? v[m] * recips[m] : div_cr_nofb(v[m], col[m][m], recips[m], bad);
+                for (int c = 0; c < NB; ++c) t[c] = col[0][c];
 #pragma unroll
-                for (int c0 = m + 1; c0 < NB; c0 += kChunk) {
-                    double t[kChunk];  // column m of L_kk, kChunk loaded together before any use
+                for (int m = 0; m < NB; ++m) {
+                    if (m + 1 < NB) {
 #pragma unroll
-                    for (int u = 0; u < kChunk; ++u)
-                        if (c0 + u < NB) t[u] = col[m][c0 + u];
+                        for (int c = m + 1; c < NB; ++c) tn[c] = col[m + 1][c];
+                        yn = recips[m + 1];
+                    }
+                    v[m] = kFast ? v[m] * y : div_cr_nofb(v[m], t[m], y, bad);
 #pragma unroll
-                    for (int u = 0; u < kChunk; ++u)
-                        if (c0 + u < NB) v[c0 + u] = kFast ? fma(-v[m], t[u], v[c0 + u]) : v[c0 + u] - v[m] * t[u];
-                    if (kChunk < NB) __builtin_amdgcn_sched_barrier(0);
+                    for (int c = m + 1; c < NB; ++c) v[c] = kFast ? fma(-v[m], t[c], v[c]) : v[c] - v[m] * t[c];
+#pragma unroll
+                    for (int c = m + 1; c < NB; ++c) t[c] = tn[c];
+                    y = yn;
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
+            } else {
+#pragma unroll
+                for (int m = 0; m < NB; ++m) {
+                    v[m] = kFast ? v[m] * recips[m] : div_cr_nofb(v[m], col[m][m], recips[m], bad);
+#pragma unroll
+                    for (int c0 = m + 1; c0 < NB; c0 += kChunk) {
+                        double t[kChunk];  // column m of L_kk, kChunk loaded together before any use
+#pragma unroll
+                        for (int u = 0; u < kChunk; ++u)
+                            if (c0 + u < NB) t[u] = col[m][c0 + u];
+#pragma unroll
+                        for (int u = 0; u < kChunk; ++u)
+                            if (c0 + u < NB) v[c0 + u] = kFast ? fma(-v[m], t[u], v[c0 + u]) : v[c0 + u] - v[m] * t[u];
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
             }
             double* arow = A + static_cast<int64_t>(k0 + nk + rt) * n + k0;
             if (bad) {
