@@ -148,6 +148,27 @@ def test_exact_repeated_solves_share_hash_state():
         assert np.array_equal(bits(P), bits(Po)) and np.array_equal(bits(X), bits(Xo))
 
 
+@pytest.mark.parametrize("frames", [60, 130])
+def test_fast_blocked_factorisation(frames):
+    """The per-panel launches in blocks of 8 panels, each block's update of the
+    tiles right of it on FP64 matrix cores (FP64_FAST and MIXED), against the
+    oracle and against the one-workgroup factorisation."""
+    pr = synth(1, n_frames=frames, n_points=40 * frames, outlier_rate=0.0, loss_kind=1)
+    print(f"blocked factorisation frames={frames} obs={pr.n_obs}", flush=True)
+    o = CpuGraph(pr, "oracle")
+    acc_o = o.solve(6)
+    Po, Xo = o.state()
+    chi_o = o.current_chi(True)
+    for path in (2, 1):
+        g = Graph.from_problem(pr, factor_path=path)
+        assert g.solve(6, False, FP64_FAST) == acc_o
+        chi = g.current_chi(True)
+        err = state_rel_err(g.poses(), g.points(), Po, Xo)
+        print(f"  path {path}: chi2 rel {(chi - chi_o) / chi_o:+.2e} state {err:.2e}", flush=True)
+        assert abs(chi - chi_o) <= TOL * abs(chi_o)
+        assert err <= TOL
+
+
 @pytest.mark.parametrize("cfg,kw,iters", [
     (0, {}, 10),
     (3, dict(n_frames=12, n_points=800), 10),

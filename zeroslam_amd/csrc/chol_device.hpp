@@ -247,6 +247,32 @@ __device__ __forceinline__ bool factor_diag_tile(double (*D)[NB + 1], int nr, do
     return *ok_sh != 0;
 }
 
+// factor_diag_tile for the reassociating precisions (factor_rows_fast); recips[j]
+// receives 1 / L_jj.
+__device__ __forceinline__ bool factor_diag_tile_fast(double (*D)[NB + 1], int nr, double eps, int* ok_sh,
+                                                      double* recips, double (*col)[64]) {
+    if (threadIdx.x < 64) {
+        const int lane = opaque_lane();
+        const int lr = lane & (NB - 1);
+        if (nr < NB && lane >= nr && lane < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) D[lane][c] = 0.0;
+            D[lane][lane] = 1.0;
+        }
+        double row[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c) row[c] = D[lr][c];
+        const bool ok = factor_rows_fast<NB>(row, nr, eps, recips, col);
+        if (lane < nr) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) D[lane][c] = row[c];
+        }
+        if (lane == 0) *ok_sh = ok ? 1 : 0;
+    }
+    __syncthreads();
+    return *ok_sh != 0;
+}
+
 // Row i of a panel below the diagonal tile: L_ic = (A_ic - sum_{m<c} L_im L_cm) / L_cc
 // in ascending c, subtractions in ascending m (the row-Crout order), divisions by
 // the reciprocals of the diagonal (div_cr, correctly rounded).  The row is solved
@@ -279,6 +305,24 @@ __device__ __forceinline__ void solve_panel_row(double* row, const double (*D)[N
     } else {
         solve_panel_row_slow(row, D);
     }
+}
+
+// solve_panel_row for the reassociating precisions: fused multiply-adds and
+// products with the reciprocals of the diagonal.
+__device__ __forceinline__ void solve_panel_row_fast(double* row, const double (*D)[NB + 1], const double* recips) {
+    double x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = row[c];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        double v = x[c];
+#pragma unroll
+        for (int m = 0; m < c; ++m) v = fma(-x[m], D[c][m], v);
+        x[c] = v * recips[c];
+        asm volatile("" ::: "memory");  // keep later columns' LDS reads from being hoisted
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) row[c] = x[c];
 }
 
 // ---- whole factorisation in one workgroup -------------------------------------

@@ -91,6 +91,30 @@ __device__ __forceinline__ void panel_update(double (*own)[NB + 1], const double
     }
 }
 
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
+// panel_update on FP64 matrix cores (the reassociating precisions; 256 threads):
+// wave w forms quadrant (w / 2, w % 2) of LI LJ^T with v_mfma_f64_16x16x4f64
+// (lane l: operand row l % 16 at depth l / 16; result element v at row
+// l / 16 + 4 v, column l % 16) and subtracts it; lower_only skips the upper
+// quadrant and the entries above the diagonal.
+__device__ __forceinline__ void panel_update_mfma(double (*own)[NB + 1], const double (*LI)[NB + 1],
+                                                  const double (*LJ)[NB + 1], int nr, int nc, bool lower_only) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int qi = wave >> 1, qj = wave & 1;
+    if (lower_only && qj > qi) return;
+    const int ar = qi * 16 + (lane & 15), br = qj * 16 + (lane & 15), kq = lane >> 4;
+    mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < NB / 4; ++kk)
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(LI[ar][kk * 4 + kq], LJ[br][kk * 4 + kq], acc, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = qi * 16 + kq + 4 * v;
+        if (r < nr && br < nc && (!lower_only || br <= r)) own[r][br] -= acc[v];
+    }
+}
+
 __device__ __forceinline__ void load_tile(double (*dst)[NB + 1], const double* A, int n, int r0, int c0, int nr, int nc,
                                           bool lower_only) {
     for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
@@ -106,8 +130,20 @@ __device__ __forceinline__ void store_tile(const double (*src)[NB + 1], double* 
     }
 }
 
+// Blocked form (kb0 > 0 or kb1 < T): launch k only updates the tiles of the
+// columns kb0 .. kb1 - 1; the panels of that block reach the tiles right of it
+// through k_chol_syrk, which runs after the block's last launch, so launch kb0
+// applies no panel (the block before it arrived through k_chol_syrk).
+__host__ __device__ inline int64_t step_trailing_tiles(int64_t m, int64_t w) {
+    return w * (w + 1) / 2 + (m - w) * w;  // tiles k < J < kb1, J <= I < T (m = T - k - 1, w = kb1 - k - 1)
+}
+
+// kFast (the reassociating precisions): every tile's loads issued together,
+// panel updates on matrix cores, factor_diag_tile_fast and solve_panel_row_fast.
+template <bool kFast>
 __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const double* eps_p, int32_t* status, int k,
-                                                   const int32_t* tile_first, const lm_state* lm, int gated) {
+                                                   int kb0, int kb1, const int32_t* tile_first, const lm_state* lm,
+                                                   int gated) {
     if ((gated && lm->done) || status[0] != 0) return;
     __shared__ double own[NB][NB + 1];
     __shared__ double D[NB][NB + 1];
@@ -125,9 +161,47 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
         if (tile_first[I] > k) return;  // tile (I, k) lies outside the envelope
         const int r0 = I * NB;
         const int nr = (n - r0) < NB ? (n - r0) : NB;
+        if (kFast) {
+            const bool upd_k = k > kb0 && tile_first[k] <= k - 1;
+            const bool upd_i = I != k && upd_k && tile_first[I] <= k - 1;
+            constexpr int kPer = NB * NB / 256;
+            double vd[kPer], vj[kPer], vo[kPer], vi[kPer];
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int idx = threadIdx.x + e * 256, r = idx / NB, c = idx % NB;
+                vd[e] = (r < nk && c <= r) ? A[static_cast<int64_t>(k0 + r) * n + k0 + c] : 0.0;
+                vj[e] = (upd_k && r < nk) ? A[static_cast<int64_t>(k0 + r) * n + kp0 + c] : 0.0;
+                vo[e] = (I != k && r < nr) ? A[static_cast<int64_t>(r0 + r) * n + k0 + c] : 0.0;
+                vi[e] = (upd_i && r < nr) ? A[static_cast<int64_t>(r0 + r) * n + kp0 + c] : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) {
+                const int idx = threadIdx.x + e * 256, r = idx / NB, c = idx % NB;
+                D[r][c] = vd[e];
+                LJ[r][c] = vj[e];
+                own[r][c] = vo[e];
+                LI[r][c] = vi[e];
+            }
+            __syncthreads();
+            if (upd_k) panel_update_mfma(D, LJ, LJ, nk, nk, true);
+            if (upd_i) panel_update_mfma(own, LI, LJ, nr, nk, false);
+            __syncthreads();
+            if (!wg::factor_diag_tile_fast(D, nk, eps_p[0], &ok_sh, recips, col)) {
+                if (threadIdx.x == 0) atomicCAS(status, 0, 1);
+                return;
+            }
+            if (I == k) {
+                store_tile(D, A, n, k0, k0, nk, nk, true);
+                return;
+            }
+            if (threadIdx.x < nr) wg::solve_panel_row_fast(own[threadIdx.x], D, recips);
+            __syncthreads();
+            store_tile(own, A, n, r0, k0, nr, nk, false);
+            return;
+        }
         // diagonal tile (k, k) with its panel k-1 update
         load_tile(D, A, n, k0, k0, nk, nk, true);
-        if (k > 0 && tile_first[k] <= k - 1) {
+        if (k > kb0 && tile_first[k] <= k - 1) {
             load_tile(LJ, A, n, k0, kp0, nk, NB, false);
             __syncthreads();
             panel_update(D, LJ, LJ, nk, nk, true);
@@ -143,7 +217,7 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
         }
         // row tile (I, k): panel k-1 update, then solve against L_kk^T
         load_tile(own, A, n, r0, k0, nr, nk, false);
-        if (k > 0 && tile_first[I] <= k - 1 && tile_first[k] <= k - 1) {
+        if (k > kb0 && tile_first[I] <= k - 1 && tile_first[k] <= k - 1) {
             load_tile(LI, A, n, r0, kp0, nr, NB, false);
             __syncthreads();
             panel_update(own, LI, LJ, nr, nk, false);
@@ -156,13 +230,21 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
         store_tile(own, A, n, r0, k0, nr, nk, false);
         return;
     }
-    if (k == 0) return;
-    // trailing tile (I, J), J > k: panel k-1 update
+    if (k == kb0) return;
+    // trailing tile (I, J), k < J < kb1: panel k-1 update.  The first w rows form
+    // a triangle, the rows below it full rows of w tiles.
     const int64_t u = static_cast<int64_t>(blockIdx.x) - np_wg;
-    int64_t Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) / 2.0);
-    while ((Ip + 1) * (Ip + 2) / 2 <= u) ++Ip;
-    while (Ip * (Ip + 1) / 2 > u) --Ip;
-    const int64_t Jp = u - Ip * (Ip + 1) / 2;
+    const int64_t w = kb1 - 1 - k, tri = w * (w + 1) / 2;
+    int64_t Ip, Jp;
+    if (u < tri) {
+        Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) / 2.0);
+        while ((Ip + 1) * (Ip + 2) / 2 <= u) ++Ip;
+        while (Ip * (Ip + 1) / 2 > u) --Ip;
+        Jp = u - Ip * (Ip + 1) / 2;
+    } else {
+        Ip = w + (u - tri) / w;
+        Jp = (u - tri) % w;
+    }
     const int I = k + 1 + static_cast<int>(Ip), J = k + 1 + static_cast<int>(Jp);
     if (tile_first[I] > k - 1 || tile_first[J] > k - 1) return;  // panel k-1 is zero in row tile I or J
     const int r0 = I * NB, c0 = J * NB;
@@ -171,9 +253,82 @@ __global__ void __launch_bounds__(256) k_chol_step(int n, double* A, const doubl
     load_tile(LI, A, n, r0, kp0, nr, NB, false);
     load_tile(LJ, A, n, c0, kp0, nc, NB, false);
     __syncthreads();
-    panel_update(own, LI, LJ, nr, nc, I == J);
+    if (kFast)
+        panel_update_mfma(own, LI, LJ, nr, nc, I == J);
+    else
+        panel_update(own, LI, LJ, nr, nc, I == J);
     __syncthreads();
     store_tile(own, A, n, r0, c0, nr, nc, I == J);
+}
+
+// ---- block update on FP64 matrix cores ----------------------------------------
+// A(I, J) -= L(I, P) L(J, P)^T for every lower tile (I, J) with J >= kb1, where P
+// are the panel columns kb0 .. kb1 - 1 (only those inside both rows' envelopes).
+// Four waves, each one 16 x 16 quadrant of the 32 x 32 tile, accumulate with
+// v_mfma_f64_16x16x4f64 (lane l: A operand row l % 16, depth l / 16, B operand
+// depth l / 16, column l % 16; result element v: row l / 16 + 4 v, column l % 16).  The depth is walked 32 columns
+// at a time through LDS, the next chunk's loads in flight during the products.
+// The sums are reassociated (depth order 4 at a time), so the blocked path
+// serves the reassociating precisions only.
+constexpr int kBlockTiles = 8;  // panels per block of the blocked factorisation
+
+__global__ void __launch_bounds__(256) k_chol_syrk(int n, double* A, int kb0, int kb1, const int32_t* tile_first,
+                                                   const int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || status[0] != 0) return;
+    __shared__ double LI[NB][NB + 1];
+    __shared__ double LJ[NB][NB + 1];
+    const int64_t u = blockIdx.x;
+    int64_t Ip = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(u) + 1.0) - 1.0) / 2.0);
+    while ((Ip + 1) * (Ip + 2) / 2 <= u) ++Ip;
+    while (Ip * (Ip + 1) / 2 > u) --Ip;
+    const int64_t Jp = u - Ip * (Ip + 1) / 2;
+    const int I = kb1 + static_cast<int>(Ip), J = kb1 + static_cast<int>(Jp);
+    int c_lo = tile_first[I] > tile_first[J] ? tile_first[I] : tile_first[J];
+    c_lo = c_lo > kb0 ? c_lo : kb0;
+    if (c_lo >= kb1) return;  // the block's columns are zero in row tile I or J
+    const int r0 = I * NB, c0 = J * NB;
+    const int nr = (n - r0) < NB ? (n - r0) : NB, nc = (n - c0) < NB ? (n - c0) : NB;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int qi = wave >> 1, qj = wave & 1;
+    const bool active = !(I == J && qj > qi);  // the diagonal tile's upper quadrant
+    constexpr int kPer = NB * NB / 256;
+    double pa[kPer], pb[kPer];
+    auto fetch = [&](int c) {
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int idx = tid + e * 256, r = idx / NB, cc = idx % NB;
+            pa[e] = r < nr ? A[static_cast<int64_t>(r0 + r) * n + c * NB + cc] : 0.0;
+            pb[e] = r < nc ? A[static_cast<int64_t>(c0 + r) * n + c * NB + cc] : 0.0;
+        }
+    };
+    fetch(c_lo);
+    mfma_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int ar = qi * 16 + (lane & 15), br = qj * 16 + (lane & 15), kq = lane >> 4;
+    for (int c = c_lo; c < kb1; ++c) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const int idx = tid + e * 256, r = idx / NB, cc = idx % NB;
+            LI[r][cc] = pa[e];
+            LJ[r][cc] = pb[e];
+        }
+        __syncthreads();
+        if (c + 1 < kb1) fetch(c + 1);
+        if (active) {
+#pragma unroll
+            for (int kk = 0; kk < NB / 4; ++kk)
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(LI[ar][kk * 4 + kq], LJ[br][kk * 4 + kq], acc, 0, 0, 0);
+        }
+    }
+    if (!active) return;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = qi * 16 + kq + 4 * v, cc = qj * 16 + (lane & 15);
+        if (r < nr && cc < nc && (I != J || cc <= r)) {
+            double* a = A + static_cast<int64_t>(r0 + r) * n + c0 + cc;
+            *a = *a - acc[v];
+        }
+    }
 }
 
 // ---- whole factorisation in one workgroup -------------------------------------
@@ -352,11 +507,26 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
         else
             k_chol_fused<true><<<1, 512, 0, s>>>(n, w.S, w.tile_last, w.chol_scale, w.chol_status, w.lm, g);
     } else {
-        for (int64_t k = 0; k < tiles; ++k) {
-            const int64_t m = tiles - k - 1;
-            const int64_t grid = (tiles - k) + (k > 0 ? m * (m + 1) / 2 : 0);
-            k_chol_step<<<static_cast<unsigned>(grid), 256, 0, s>>>(n, w.S, w.chol_scale, w.chol_status,
-                                                                   static_cast<int>(k), w.tile_first, w.lm, g);
+        // exact: one block holding every panel (the reference's operation order);
+        // otherwise blocks of kBlockTiles panels, each followed by its MFMA update
+        const int64_t bt = exact ? tiles : kBlockTiles;
+        for (int64_t kb0 = 0; kb0 < tiles; kb0 += bt) {
+            const int64_t kb1 = (kb0 + bt) < tiles ? (kb0 + bt) : tiles;
+            for (int64_t k = kb0; k < kb1; ++k) {
+                const int64_t m = tiles - k - 1, wd = kb1 - 1 - k;
+                const int64_t grid = (tiles - k) + (k > kb0 ? step_trailing_tiles(m, wd) : 0);
+                const int ik = static_cast<int>(k), ib0 = static_cast<int>(kb0), ib1 = static_cast<int>(kb1);
+                if (exact)
+                    k_chol_step<false><<<static_cast<unsigned>(grid), 256, 0, s>>>(
+                        n, w.S, w.chol_scale, w.chol_status, ik, ib0, ib1, w.tile_first, w.lm, g);
+                else
+                    k_chol_step<true><<<static_cast<unsigned>(grid), 256, 0, s>>>(
+                        n, w.S, w.chol_scale, w.chol_status, ik, ib0, ib1, w.tile_first, w.lm, g);
+            }
+            const int64_t m = tiles - kb1;
+            if (m > 0)
+                k_chol_syrk<<<static_cast<unsigned>(m * (m + 1) / 2), 256, 0, s>>>(
+                    n, w.S, static_cast<int>(kb0), static_cast<int>(kb1), w.tile_first, w.chol_status, w.lm, g);
         }
     }
     ZB_TIME_END(w, s);
