@@ -172,8 +172,9 @@ def test_fast_blocked_factorisation(frames):
 @pytest.mark.parametrize("cfg,kw,iters", [
     (0, {}, 10),
     (3, dict(n_frames=12, n_points=800), 10),
-    # loop closure, 2160-row reduced system (the backward substitution walks row
-    # envelopes); no outliers or robust loss, so the LM path does not hang on rounding
+    # loop closure, 2160-row reduced system: the blocked factorisation and the
+    # blocked substitution; no outliers or robust loss, so the LM path does not
+    # hang on rounding
     (4, dict(n_frames=360, n_points=12000, outlier_rate=0.0, loss_kind=1), 4),
 ])
 def test_fast_solve_within_tolerance(cfg, kw, iters):

@@ -632,9 +632,6 @@ __device__ inline int triangular_solve(int n, const double* L, const double* b, 
 // depend on the workgroup size.  Scratch: 6 LDS tiles (two diagonal, two
 // coupling tiles, two tiles of partial sums); blockDim.x == kThreads >= 128.
 constexpr int kSlots = 30;
-// Systems of more tiles run the backward pass right-looking over each tile's
-// row envelope (every sum still in a fixed order, independent of blockDim).
-constexpr int kRightLookingTiles = 64;
 
 // Tile (r0.., c0..) of L, zero outside nr x nc (and above the diagonal when
 // lower), staged into dst by threads first, first + stride, ...: stage_load
@@ -695,7 +692,7 @@ __device__ __forceinline__ double slot_total(const double* ps) {
     return (a[0] + a[1]) + (a[2] + a[3]);
 }
 
-template <int kThreads, int kMaxTiles = 1 << 20>
+template <int kThreads>
 __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, const double* b, double* x,
                                                      const int32_t* tile_first, const int32_t* tile_last,
                                                      double (*buf)[NB][NB + 1], double* sh, int* flag) {
@@ -717,10 +714,6 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
         x[i] = b[i];
     }
     const int T = (n + NB - 1) / NB;
-    // backward direction: above kRightLookingTiles tiles the envelope of a column
-    // (which a loop closure stretches to the last rows) holds far more zeros than
-    // the row envelope, so the backward pass walks rows instead
-    const bool right = kMaxTiles > kRightLookingTiles && T > kRightLookingTiles;
     auto rows_of = [n](int t) { return (n - t * NB) < NB ? (n - t * NB) : NB; };
     double* part = &buf[4][0][0];  // forward [2][NB]; backward [2][kSlots][NB]
     stage_tile(buf[0], L, n, 0, 0, rows_of(0), rows_of(0), true, tid, blockDim.x);
@@ -806,8 +799,8 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
             const double (*D)[NB + 1] = buf[p];
             double xc = 0.0;
             if (lane < nb) {
-                double red = 0.0;  // right-looking: the other tiles' terms are already subtracted
-                if (!right && t < T - 1) red = slot_total(part + p * kSlots * NB + lane);
+                double red = 0.0;
+                if (t < T - 1) red = slot_total(part + p * kSlots * NB + lane);
                 xc = x[i0 + lane] - (red + own);
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -836,46 +829,22 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
                 }
                 own = (a0 + a1) + (a2 + a3);
             }
-        } else {
-            const int first = tid - 64, stride = nwk * 64;
+        } else if (t >= 1) {
             const int u = t - 1, iu = u * NB;
-            const bool has_d = t >= 1;
-            const bool has_o = u >= 1 && tile_last[u - 1] >= u;
-            // the next tile's factor tiles, loaded first (they do not depend on x)
+            const int jbeg = i0 + NB;
+            const int jend = (tile_last[u] + 1) * NB < n ? (tile_last[u] + 1) * NB : n;
+            const int first = tid - 64, stride = nwk * 64;
             double sd[kPer], so[kPer];
-            if (has_d) stage_load(sd, L, n, iu, iu, NB, NB, true, first, stride);
+            const bool has_o = u >= 1 && tile_last[u - 1] >= u;
+            stage_load(sd, L, n, iu, iu, NB, NB, true, first, stride);
             if (has_o) stage_load(so, L, n, iu, iu - NB, NB, NB, false, first, stride);
-            if (!right) {
-                // left-looking: slot s of column c of tile u sums rows jbeg + s,
-                // jbeg + s + kSlots, ... down to the column's envelope end
-                if (has_d) {
-                    const int jbeg = i0 + NB;
-                    const int jend = (tile_last[u] + 1) * NB < n ? (tile_last[u] + 1) * NB : n;
-                    const int c = lane & (NB - 1);
-                    const double* Lc = L + iu + c;
-                    for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
-                        const double acc = slot_sum([Lc, n](int j) { return Lc[static_cast<int64_t>(j) * n]; }, x, jbeg + s, jend);
-                        part[(u & 1) * kSlots * NB + s * NB + c] = acc;
-                    }
-                }
-            } else if (t + 1 < T) {
-                // right-looking over the row envelope: tile t + 1, solved in the
-                // previous step, subtracts its rows' terms from the unknowns of tiles
-                // tile_first[t + 1] .. t - 1 (tile t is wave 0's coupling product);
-                // each unknown receives one sum per tile, tiles in descending order
-                const int r0 = (t + 1) * NB, nr = rows_of(t + 1), j0 = tile_first[t + 1] * NB, j1 = t * NB;
-                const double* Lr = L + static_cast<int64_t>(r0) * n;
-                for (int j = j0 + first; j < j1; j += stride) {
-                    double acc0 = 0.0, acc1 = 0.0;
-                    for (int r = 0; r + 1 < nr; r += 2) {
-                        acc0 = fma(Lr[static_cast<int64_t>(r) * n + j], x[r0 + r], acc0);
-                        acc1 = fma(Lr[static_cast<int64_t>(r + 1) * n + j], x[r0 + r + 1], acc1);
-                    }
-                    if (nr & 1) acc0 = fma(Lr[static_cast<int64_t>(nr - 1) * n + j], x[r0 + nr - 1], acc0);
-                    x[j] -= acc0 + acc1;
-                }
+            const int c = lane & (NB - 1);
+            const double* Lc = L + iu + c;
+            for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
+                const double acc = slot_sum([Lc, n](int j) { return Lc[static_cast<int64_t>(j) * n]; }, x, jbeg + s, jend);
+                part[(u & 1) * kSlots * NB + s * NB + c] = acc;
             }
-            if (has_d) stage_store(buf[u & 1], sd, first, stride);
+            stage_store(buf[u & 1], sd, first, stride);
             if (has_o) stage_store(buf[2 + (u & 1)], so, first, stride);
         }
         __syncthreads();

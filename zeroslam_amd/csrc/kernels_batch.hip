@@ -98,7 +98,7 @@ template <int kThreads>
 __device__ __forceinline__ int batch_solve(int n, const double* S, const double* b, double* x, const int32_t* tile_first,
                                         const int32_t* tile_last, double (*buf)[wg::kTile][wg::kTile + 1], double* sh,
                                         int* flag) {
-    return wg::triangular_solve_fast<kThreads, kBatchTiles>(n, S, b, x, tile_first, tile_last, buf, sh, flag);
+    return wg::triangular_solve_fast<kThreads>(n, S, b, x, tile_first, tile_last, buf, sh, flag);
 }
 
 template <class TS, int kThreads>
