@@ -114,7 +114,9 @@ typedef struct zsba_stats {
     double setup_ms;         /* upload + structure analysis (0 when cached)           */
     double kernel_ms[ZSBA_K_COUNT];        /* summed device time per kernel class     */
     int64_t kernel_launches[ZSBA_K_COUNT]; /* launches per kernel class               */
-    double reserved[8];
+    int32_t linearisations;  /* linearisations (make_hessian) in the last solve          */
+    int32_t pad;
+    double reserved[7];
 } zsba_stats;
 
 typedef struct zsba_graph zsba_graph;
@@ -220,8 +222,13 @@ typedef struct zsba_batch_info {
     double phase_ms[8];       /* mean per-window device time by phase: cost,
                                  linearisation, damping + inverses + Y, Schur,
                                  factorisation, triangular solve, refinement,
-                                 update + controller                                 */
-    double reserved[4];
+                                 update + controller; linearisation is the
+                                 grid-wide phase that every window shares, so its
+                                 entry is lin_ms                                     */
+    double lin_ms;            /* grid-wide linearisation launches of the solve,
+                                 device clock (first workgroup of a round's
+                                 linearisation to first workgroup of its attempts)  */
+    double reserved[3];
 } zsba_batch_info;
 zsba_batch* zsba_batch_create(zsba_graph* const* graphs, int32_t count);
 void        zsba_batch_destroy(zsba_batch* batch);

@@ -34,6 +34,7 @@ def test_batch_matches_individual_solves(precision, tol):
         print(f"window {k}: obs {p.n_obs} batch acc {acc[k]} it {st_b['iterations']} chi2 {st_b['final_chi2']!r} | "
               f"single acc {a} chi2 {st['final_chi2']!r} | state {err:.2e}", flush=True)
         assert acc[k] == a
+        assert st_b["linearisations"] == st["linearisations"] >= 1
         assert abs(st_b["final_chi2"] - st["final_chi2"]) <= tol * st["final_chi2"]
         assert err <= tol
 

@@ -32,10 +32,12 @@ class Stats(C.Structure):
                 ("n_obs", C.c_int64), ("n_poses", C.c_int64), ("n_points", C.c_int64),
                 ("initial_chi2", C.c_double), ("final_chi2", C.c_double), ("lambda_", C.c_double),
                 ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("kernel_ms", C.c_double * 16),
-                ("kernel_launches", C.c_int64 * 16), ("reserved", C.c_double * 8)]
+                ("kernel_launches", C.c_int64 * 16), ("linearisations", C.c_int32), ("pad", C.c_int32),
+                ("reserved", C.c_double * 7)]
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "kernel_ms", "kernel_launches")}
+        d = {k: getattr(self, k) for k, _ in self._fields_
+             if k not in ("reserved", "pad", "kernel_ms", "kernel_launches")}
         d["kernel_ms"] = {name: self.kernel_ms[i] for i, name in enumerate(KERNEL_NAMES)}
         d["kernel_launches"] = {name: int(self.kernel_launches[i]) for i, name in enumerate(KERNEL_NAMES)}
         return d
@@ -55,7 +57,7 @@ class BatchInfo(C.Structure):
     _fields_ = [("windows", C.c_int32), ("individual", C.c_int32), ("threads", C.c_int32),
                 ("graph_launches", C.c_int32), ("tune_ms", C.c_float * 3), ("pad", C.c_float),
                 ("autotune_wall_ms", C.c_double), ("kernel_wall_ms", C.c_double), ("solve_ms", C.c_double),
-                ("phase_ms", C.c_double * 8), ("reserved", C.c_double * 4)]
+                ("phase_ms", C.c_double * 8), ("lin_ms", C.c_double), ("reserved", C.c_double * 3)]
 
     PHASES = ("cost", "linearise", "invert", "schur", "factor", "solve", "refine", "update")
 

@@ -137,8 +137,9 @@ struct lm_state {
     int32_t iterations, use_relative;
     int32_t iter, failures, success, attempts, rejected, solve_failures;
     int32_t done, need_linearize, need_rollback, solve_ok, step_applied, executed_iterations;
-    int32_t lin_count;  // linearisations performed in this solve
-    int32_t pad[3];
+    int32_t lin_count;       // linearisation sequences ended (the per-graph path ends one per attempt)
+    int32_t linearisations;  // linearisations performed in this solve (make_hessian calls)
+    int32_t pad[2];
 };
 
 // Working buffers of the binary64 path.
@@ -256,8 +257,9 @@ void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hip
 void launch_obs_grad(const dev_problem& p, const dev_work64& w, const double* lam, double* dobs_by_e, hipStream_t s);
 
 // ---- batched local BA (kernels_batch.hip) --------------------------------------
-// One workgroup per window solves the window's whole LM loop.  Windows are
-// limited to reduced systems of kBatchMaxN rows (42 poses).
+// One workgroup per window runs the window's LM attempts; the linearisations of
+// every window run as grid-wide launches between them.  Windows are limited to
+// reduced systems of kBatchMaxN rows (42 poses).
 constexpr int kBatchMaxN = 256;
 struct batch_window {
     dev_problem p;
@@ -270,10 +272,24 @@ enum batch_phase : int {
     PH_COST = 0, PH_LINEARISE = 1, PH_INVERT = 2, PH_SCHUR = 3, PH_FACTOR = 4, PH_SOLVE = 5, PH_REFINE = 6, PH_UPDATE = 7
 };
 constexpr int kBatchPhases = 8;
-// threads per window: 256, 512 or 1024; out[i] receives window i's final
-// controller, phase_ticks[i * kBatchPhases + k] its time in phase k (100 MHz ticks).
+// Sizes of the grid-wide linearisation launches: prefix sums (count + 1 entries
+// each, on the device) of the windows' points, pose chunks (np * kPoseChunks)
+// and pose-block entries (np * 27), with their totals.
+struct batch_grid {
+    const int32_t* points = nullptr;
+    const int32_t* chunks = nullptr;
+    const int32_t* entries = nullptr;
+    int32_t n_points = 0, n_chunks = 0, n_entries = 0, pad = 0;
+};
+// Launches of one batched solve: `rounds` rounds of (linearisation of the
+// windows that need one, then every unfinished window's attempts up to its next
+// accepted step).  threads per window: 256, 512 or 1024; out[i] receives window
+// i's controller, phase_ticks[i * kBatchPhases + k] its time in phase k (100 MHz
+// ticks), lin_clock[2 r], lin_clock[2 r + 1] the device clock at the start of
+// round r's linearisation and of its attempts (earliest workgroup).
 void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out,
-                        int64_t* phase_ticks, hipStream_t s);
+                        int64_t* phase_ticks, const batch_grid& grid, uint64_t* lin_clock, int rounds,
+                        hipStream_t s);
 
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from

@@ -759,6 +759,7 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     }
     stats.iterations = lm_pinned->executed_iterations;
     stats.attempts = lm_pinned->attempts;
+    stats.linearisations = lm_pinned->linearisations;
     stats.accepted = lm_pinned->success;
     stats.rejected = lm_pinned->rejected;
     stats.solve_failures = lm_pinned->solve_failures;
@@ -908,13 +909,14 @@ int32_t zsba_graph::adjoint(const double* dL_dpoints, const double* dL_dposes, d
 }
 
 // ---- batched local BA -----------------------------------------------------------
-// One persistent launch solves every window whose reduced system fits one
+// One sequence of launches solves every window whose reduced system fits one
 // workgroup (zb::kBatchMaxN rows); larger windows and the reference-order FP64
-// precision go through the per-graph solve.  The launch is captured once into a
-// hipGraph and replayed by later solves of the same batch; the window table it
-// reads is refreshed before each replay.  The workgroup size is chosen per batch
-// shape by timing the candidates on the batch itself (states restored after each
-// trial) and remembered for the process.
+// precision go through the per-graph solve.  The sequence (iterations + 1 rounds
+// of grid-wide linearisation and per-window attempts) is captured once into a
+// hipGraph and replayed by later solves of the same batch; the window table and
+// the linearisation grid it reads are refreshed before each replay.  The
+// workgroup size is chosen per batch shape by timing the candidates on the batch
+// itself (states restored after each trial) and remembered for the process.
 namespace {
 struct tune_key {
     int32_t count, max_n, obs_bucket, mixed;
@@ -937,12 +939,19 @@ struct zsba_batch {
     lm_state* h_out = nullptr;
     int64_t* d_ticks = nullptr;
     int64_t* h_ticks = nullptr;
-    int32_t capacity = 0;
+    int32_t* d_prefix = nullptr;     // 3 (capacity + 1): points, pose chunks, pose-block entries
+    uint64_t* d_linclk = nullptr;    // 2 rounds
+    uint64_t* h_linclk = nullptr;
+    int32_t capacity = 0, clock_capacity = 0;
     std::vector<zb::batch_window> h_wins;
+    std::vector<int32_t> h_prefix;
+    zb::batch_grid grid{};
+    int32_t rounds = 0;
     std::vector<int32_t> win_graph;  // window -> graph index
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int32_t graph_threads = 0, graph_mixed = -1, graph_count = -1;
+    int32_t graph_threads = 0, graph_mixed = -1, graph_count = -1, graph_rounds = -1;
+    int32_t graph_points = -1, graph_chunks = -1, graph_entries = -1;
 
     ~zsba_batch() { release(); }
     void release() {
@@ -956,14 +965,21 @@ struct zsba_batch {
         if (h_out) (void)hipHostFree(h_out);
         if (d_ticks) (void)hipFree(d_ticks);
         if (h_ticks) (void)hipHostFree(h_ticks);
+        if (d_prefix) (void)hipFree(d_prefix);
+        if (d_linclk) (void)hipFree(d_linclk);
+        if (h_linclk) (void)hipHostFree(h_linclk);
         if (stream) (void)hipStreamDestroy(stream);
         d_wins = nullptr;
         d_out = nullptr;
         h_out = nullptr;
         d_ticks = nullptr;
         h_ticks = nullptr;
+        d_prefix = nullptr;
+        d_linclk = nullptr;
+        h_linclk = nullptr;
         stream = nullptr;
         capacity = 0;
+        clock_capacity = 0;
     }
     int32_t fail(int32_t code, const std::string& m) {
         err = m;
@@ -973,7 +989,7 @@ struct zsba_batch {
         return fail(ZSBA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
     }
 
-    int32_t ensure(int32_t count, int dev) {
+    int32_t ensure(int32_t count, int dev, int32_t n_rounds) {
         hipError_t e;
         if (device != dev) {
             release();
@@ -982,6 +998,22 @@ struct zsba_batch {
         if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
         if (!stream && (e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess)
             return hip_fail(e, "hipStreamCreate");
+        if (n_rounds > clock_capacity) {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+            if (d_linclk) (void)hipFree(d_linclk);
+            if (h_linclk) (void)hipHostFree(h_linclk);
+            d_linclk = nullptr;
+            h_linclk = nullptr;
+            clock_capacity = 0;
+            const size_t bytes = sizeof(uint64_t) * 2 * n_rounds;
+            if ((e = hipMalloc(&d_linclk, bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_linclk), bytes, hipHostMallocDefault)) != hipSuccess)
+                return hip_fail(e, "hipHostMalloc");
+            clock_capacity = n_rounds;
+        }
         if (count > capacity) {
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
@@ -992,11 +1024,15 @@ struct zsba_batch {
             if (h_out) (void)hipHostFree(h_out);
             if (d_ticks) (void)hipFree(d_ticks);
             if (h_ticks) (void)hipHostFree(h_ticks);
+            if (d_prefix) (void)hipFree(d_prefix);
             d_wins = nullptr;
             d_out = nullptr;
             h_out = nullptr;
             d_ticks = nullptr;
             h_ticks = nullptr;
+            d_prefix = nullptr;
+            if ((e = hipMalloc(&d_prefix, sizeof(int32_t) * 3 * (count + 1))) != hipSuccess)
+                return hip_fail(e, "hipMalloc");
             const size_t tick_bytes = sizeof(int64_t) * zb::kBatchPhases * count;
             if ((e = hipMalloc(&d_ticks, tick_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_ticks), tick_bytes, hipHostMallocDefault)) != hipSuccess)
@@ -1015,20 +1051,26 @@ struct zsba_batch {
     int32_t launch(int32_t threads, bool mixed) {
         hipError_t e;
         const int32_t count = static_cast<int32_t>(h_wins.size());
-        if (!exec || graph_threads != threads || graph_mixed != (mixed ? 1 : 0) || graph_count != count) {
+        if (!exec || graph_threads != threads || graph_mixed != (mixed ? 1 : 0) || graph_count != count ||
+            graph_rounds != rounds || graph_points != grid.n_points || graph_chunks != grid.n_chunks ||
+            graph_entries != grid.n_entries) {
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
             exec = nullptr;
             graph = nullptr;
             if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
                 return hip_fail(e, "hipStreamBeginCapture");
-            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, d_ticks, stream);
+            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, d_ticks, grid, d_linclk, rounds, stream);
             if ((e = hipStreamEndCapture(stream, &graph)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
             if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
                 return hip_fail(e, "hipGraphInstantiate");
             graph_threads = threads;
             graph_mixed = mixed ? 1 : 0;
             graph_count = count;
+            graph_rounds = rounds;
+            graph_points = grid.n_points;
+            graph_chunks = grid.n_chunks;
+            graph_entries = grid.n_entries;
         }
         if ((e = hipGraphLaunch(exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         info.graph_launches += 1;
@@ -1061,7 +1103,8 @@ struct zsba_batch {
         int32_t rc = 0;
         for (int32_t threads : {256, 512, 1024}) {
             (void)hipEventRecord(t0, stream);
-            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, d_ticks, stream);
+            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, d_ticks, grid,
+                                   d_linclk, rounds, stream);
             (void)hipEventRecord(t1, stream);
             if ((e = hipEventSynchronize(t1)) != hipSuccess) {
                 rc = hip_fail(e, "autotune");
@@ -1136,12 +1179,40 @@ struct zsba_batch {
             info.solve_ms = now_ms() - t_start;
             return 0;
         }
-        int32_t rc = ensure(nw, dev);
+        // iterations + 1 rounds: a window linearises after each accepted step that
+        // does not end its solve, so it pauses at most iterations - 1 times
+        rounds = std::max(iterations, 0) + 1;
+        int32_t rc = ensure(nw, dev, rounds);
         if (rc) return rc;
         hipError_t e;
         if ((e = hipMemcpyAsync(d_wins, h_wins.data(), sizeof(zb::batch_window) * nw, hipMemcpyHostToDevice, stream)) !=
             hipSuccess)
             return hip_fail(e, "upload windows");
+        {
+            h_prefix.assign(3 * (nw + 1), 0);
+            int64_t pts = 0, chunks = 0, entries = 0;
+            for (int32_t i = 0; i < nw; ++i) {
+                h_prefix[i] = static_cast<int32_t>(pts);
+                h_prefix[(nw + 1) + i] = static_cast<int32_t>(chunks);
+                h_prefix[2 * (nw + 1) + i] = static_cast<int32_t>(entries);
+                pts += h_wins[i].p.nl;
+                chunks += static_cast<int64_t>(h_wins[i].p.np) * zb::kPoseChunks;
+                entries += static_cast<int64_t>(h_wins[i].p.np) * 27;
+            }
+            if (entries > INT32_MAX || 64 * chunks > INT32_MAX) return fail(ZSBA_EINVAL, "batch too large");
+            h_prefix[nw] = static_cast<int32_t>(pts);
+            h_prefix[(nw + 1) + nw] = static_cast<int32_t>(chunks);
+            h_prefix[2 * (nw + 1) + nw] = static_cast<int32_t>(entries);
+            grid.points = d_prefix;
+            grid.chunks = d_prefix + (nw + 1);
+            grid.entries = d_prefix + 2 * (nw + 1);
+            grid.n_points = static_cast<int32_t>(pts);
+            grid.n_chunks = static_cast<int32_t>(chunks);
+            grid.n_entries = static_cast<int32_t>(entries);
+            if ((e = hipMemcpyAsync(d_prefix, h_prefix.data(), sizeof(int32_t) * h_prefix.size(), hipMemcpyHostToDevice,
+                                    stream)) != hipSuccess)
+                return hip_fail(e, "upload linearisation grid");
+        }
         const bool mixed = precision == ZSBA_PRECISION_MIXED;
         int32_t max_n = 0;
         int64_t obs = 0;
@@ -1171,6 +1242,9 @@ struct zsba_batch {
         if ((e = hipMemcpyAsync(h_ticks, d_ticks, sizeof(int64_t) * zb::kBatchPhases * nw, hipMemcpyDeviceToHost,
                                 stream)) != hipSuccess)
             return hip_fail(e, "download phase clocks");
+        if ((e = hipMemcpyAsync(h_linclk, d_linclk, sizeof(uint64_t) * 2 * rounds, hipMemcpyDeviceToHost, stream)) !=
+            hipSuccess)
+            return hip_fail(e, "download linearisation clocks");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batch solve");
         info.kernel_wall_ms = now_ms() - t0;
         for (int32_t i = 0; i < nw; ++i) {
@@ -1179,6 +1253,7 @@ struct zsba_batch {
             *g->lm_pinned = s;
             g->stats.iterations = s.executed_iterations;
             g->stats.attempts = s.attempts;
+            g->stats.linearisations = s.linearisations;
             g->stats.accepted = s.success;
             g->stats.rejected = s.rejected;
             g->stats.solve_failures = s.solve_failures;
@@ -1195,6 +1270,10 @@ struct zsba_batch {
             for (int32_t i = 0; i < nw; ++i) sum += static_cast<double>(h_ticks[i * zb::kBatchPhases + k]);
             info.phase_ms[k] = sum / nw * 1e-5;
         }
+        info.lin_ms = 0.0;
+        for (int32_t r = 0; r < rounds; ++r)
+            if (h_linclk[2 * r + 1] >= h_linclk[2 * r]) info.lin_ms += static_cast<double>(h_linclk[2 * r + 1] - h_linclk[2 * r]) * 1e-5;
+        info.phase_ms[zb::PH_LINEARISE] = info.lin_ms;
         info.solve_ms = now_ms() - t_start;
         return 0;
     }

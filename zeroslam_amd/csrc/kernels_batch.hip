@@ -1,12 +1,17 @@
-// Batched local bundle adjustment: one workgroup per window runs that window's
-// whole Levenberg-Marquardt solve (factor_graph::solve, factor_graph.hpp:489-615),
-// the same phases zsba_solve enqueues as separate launches: cost, linearisation,
-// damping and point inverses, Schur complement, envelope Cholesky, triangular
-// solves, binary64 refinement (mixed), update, accept / reject, rollback.
-// Workgroup barriers separate the phases; windows never wait on each other, so
-// one launch of `count` workgroups solves the batch and a window that finishes
-// early frees its compute unit.  Each window uses the device buffers of its own
-// graph handle; the controller is the shared lm_control.hpp code.
+// Batched local bundle adjustment (factor_graph::solve, factor_graph.hpp:489-615,
+// on many independent windows).  The LM loop of each window runs in one
+// workgroup: damping and point inverses, Schur complement, envelope Cholesky,
+// triangular solves, binary64 refinement (mixed), update, cost, accept / reject
+// and rollback, with workgroup barriers between the phases.  The reference
+// linearises once per outer iteration (after an accepted step), so a window's
+// workgroup pauses there, and the linearisations of all paused windows run as
+// grid-wide launches over every window's points and poses at once, the
+// HBM-bound phase spread over the whole GPU.  A solve is a fixed sequence of
+// rounds (linearisation, then attempts), captured once in a hipGraph; windows
+// never wait on each other inside a launch.  Each window uses the device buffers
+// of its own graph handle; the controller is the shared lm_control.hpp code and
+// the per-item arithmetic the shared mixed_items.hpp code, so every window's
+// result equals its per-graph solve.
 #include "ba_math.hpp"
 #include "chol_device.hpp"
 #include "common.hpp"
@@ -45,18 +50,60 @@ template <> struct blocks<double> {
         : pterms(w.pose_terms), Hpp(w.Hpp), Hll(w.Hll), W(w.W), Hinv(w.Hinv), Y(w.Y) {}
 };
 
+// Window holding global item g: the k with prefix[k] <= g < prefix[k + 1].
+__device__ __forceinline__ int find_window(const int32_t* prefix, int count, int g) {
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool needs_linearisation(const lm_state* lm) { return !lm->done && lm->need_linearize; }
+
+// Grid-wide linearisation, three launches: per point (H_ll, b_l, W and the
+// per-observation pose terms), per pose chunk, per pose-block entry.
 template <class TS>
-__device__ void linearise(const dev_problem& p, const dev_work64& w, const blocks<TS>& B) {
-    for (int l = threadIdx.x; l < p.nl; l += blockDim.x)
-        mixed::lin_point(p, w.pose, w.point, B.pterms, w.pose_grad, B.Hll, w.bl_m, B.W, l);
-    __syncthreads();
-    const int lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
-    for (int ic = threadIdx.x >> 6; ic < p.np * kPoseChunks; ic += nwaves)
-        mixed::reduce_pose_chunk(p, B.pterms, w.pose_grad, w.pose_part, ic / kPoseChunks, ic % kPoseChunks, lane);
-    __syncthreads();
-    for (int ij = threadIdx.x; ij < p.np * 27; ij += blockDim.x)
-        mixed::reduce_pose_finish(w.pose_part, B.Hpp, w.bp_m, ij / 27, ij % 27);
-    __syncthreads();
+__global__ void __launch_bounds__(256) k_batch_lin_points(const batch_window* wins, int count, const int32_t* prefix,
+                                                          int total, uint64_t* lin_clock, int round) {
+    if (threadIdx.x == 0) atomicMin(reinterpret_cast<unsigned long long*>(&lin_clock[2 * round]),
+                                    static_cast<unsigned long long>(wall_clock64()));
+    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= total) return;
+    const int k = find_window(prefix, count, g);
+    const batch_window& bw = wins[k];
+    if (!needs_linearisation(bw.w.lm)) return;
+    const blocks<TS> B(bw.w);
+    mixed::lin_point(bw.p, bw.w.pose, bw.w.point, B.pterms, bw.w.pose_grad, B.Hll, bw.w.bl_m, B.W, g - prefix[k]);
+}
+
+template <class TS>
+__global__ void __launch_bounds__(256) k_batch_lin_chunks(const batch_window* wins, int count, const int32_t* prefix,
+                                                          int total) {
+    const int g = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);  // one wave per chunk
+    if (g >= total) return;
+    const int k = find_window(prefix, count, g);
+    const batch_window& bw = wins[k];
+    if (!needs_linearisation(bw.w.lm)) return;
+    const blocks<TS> B(bw.w);
+    const int ic = g - prefix[k];
+    mixed::reduce_pose_chunk(bw.p, B.pterms, bw.w.pose_grad, bw.w.pose_part, ic / kPoseChunks, ic % kPoseChunks,
+                             threadIdx.x & 63);
+}
+
+template <class TS>
+__global__ void __launch_bounds__(256) k_batch_lin_entries(const batch_window* wins, int count, const int32_t* prefix,
+                                                           int total) {
+    const int g = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (g >= total) return;
+    const int k = find_window(prefix, count, g);
+    const batch_window& bw = wins[k];
+    if (!needs_linearisation(bw.w.lm)) return;
+    const blocks<TS> B(bw.w);
+    const int ij = g - prefix[k];
+    mixed::reduce_pose_finish(bw.w.pose_part, B.Hpp, bw.w.bp_m, ij / 27, ij % 27);
 }
 
 __device__ double cost(const dev_problem& p, const dev_work64& w, double* sh) {
@@ -65,8 +112,8 @@ __device__ double cost(const dev_problem& p, const dev_work64& w, double* sh) {
     return block_sum(acc, sh);
 }
 
-// Phase clocks (thread 0, after the phase's closing barrier), summed per window
-// into phase_ticks[window * kBatchPhases + phase] in 100 MHz ticks.
+// Phase clocks (thread 0, after the phase's closing barrier), added per window
+// to phase_ticks[window * kBatchPhases + phase] in 100 MHz ticks.
 struct phase_clock {
     int64_t* out;
     int64_t last;
@@ -80,7 +127,7 @@ struct phase_clock {
         last = t;
     }
     __device__ void flush() {
-        for (int i = 0; i < kBatchPhases; ++i) out[i] = acc[i];
+        for (int i = 0; i < kBatchPhases; ++i) out[i] += acc[i];
     }
 };
 
@@ -101,13 +148,51 @@ __device__ __forceinline__ int batch_solve(int n, const double* S, const double*
     return wg::triangular_solve_fast<kThreads>(n, S, b, x, tile_first, tile_last, buf, sh, flag);
 }
 
+// Start of a solve: controller reset, phase clocks cleared, the initial cost, and
+// the first linearisation requested.
+__global__ void __launch_bounds__(256) k_batch_begin(const batch_window* wins, int64_t* phase_ticks,
+                                                     uint64_t* lin_clock, int lin_slots) {
+    __shared__ double sh[256];
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0)
+        for (int i = tid; i < lin_slots; i += blockDim.x) lin_clock[i] = ~0ULL;
+    const batch_window& bw = wins[blockIdx.x];
+    const dev_problem& p = bw.p;
+    const dev_work64& w = bw.w;
+    lm_state* lm = w.lm;
+    int64_t* ticks = phase_ticks + static_cast<int64_t>(blockIdx.x) * kBatchPhases;
+    if (tid == 0) {
+        for (int i = 0; i < kBatchPhases; ++i) ticks[i] = 0;
+        *lm = lm_state{};
+        lm->iterations = bw.iterations;
+        lm->use_relative = bw.use_relative;
+        lm->need_linearize = 1;
+    }
+    phase_clock clk(ticks);
+    __syncthreads();
+    const double c = cost(p, w, sh);
+    clk.mark(PH_COST);
+    if (tid == 0) {
+        lm->chi2 = c;
+        clk.flush();
+    }
+}
+
+// A window's attempts up to its next accepted step that asks for a
+// linearisation (the workgroup pauses there; the next round resumes it) or to the
+// end of its solve.  The first round starts the controller from the linearised
+// system (max diagonal, lm_init).
 template <class TS, int kThreads>
-__global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wins, lm_state* out, int64_t* phase_ticks) {
+__global__ void __launch_bounds__(kThreads) k_batch_attempts(const batch_window* wins, lm_state* out,
+                                                             int64_t* phase_ticks, uint64_t* lin_clock, int round) {
+    if (threadIdx.x == 0) atomicMin(reinterpret_cast<unsigned long long*>(&lin_clock[2 * round + 1]),
+                                    static_cast<unsigned long long>(wall_clock64()));
     const batch_window& bw = wins[blockIdx.x];
     const dev_problem& p = bw.p;
     const dev_work64& w = bw.w;
     const blocks<TS> B(w);
     lm_state* lm = w.lm;
+    if (lm->done) return;
     __shared__ double sh[kThreads];
     __shared__ max_scan ms[kThreads];
     __shared__ double PT[kBatchTiles - 1][wg::kTile][wg::kPT];  // factor panels, then the solve's tiles
@@ -122,20 +207,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
     const bool mixed_path = sizeof(TS) == 4;
     phase_clock clk(phase_ticks + static_cast<int64_t>(blockIdx.x) * kBatchPhases);
 
-    if (tid == 0) {
-        *lm = lm_state{};
-        lm->iterations = bw.iterations;
-        lm->use_relative = bw.use_relative;
-    }
-    __syncthreads();
-    {
-        const double c = cost(p, w, sh);
-        if (tid == 0) lm->chi2 = c;
-    }
-    clk.mark(PH_COST);
-    linearise(p, w, B);
-    clk.mark(PH_LINEARISE);
-    {
+    if (lm->lin_count == 0) {
         // max_diagonal over pose then point diagonal entries, in order
         const int64_t total = 6L * p.np + 3L * p.nl;
         const int64_t per = (total + blockDim.x - 1) / blockDim.x;
@@ -154,8 +226,13 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
             for (unsigned t = 0; t < blockDim.x; ++t) acc = scan_combine(acc, ms[t]);
             lm_init(lm, scan_max_diagonal(acc));
         }
-        __syncthreads();
+    } else if (tid == 0) {
+        // the accepted attempt that paused for the linearisation ends here
+        lm_linearise_done(lm);
+        lm_end_attempt(lm);
     }
+    __syncthreads();
+    clk.mark(PH_UPDATE);
     const int max_attempts = 11 * bw.iterations + 1;
     for (int attempt = 0; attempt < max_attempts; ++attempt) {
         if (lm->done) break;
@@ -239,18 +316,14 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
                 for (int t = tid; t < p.np + p.nl; t += blockDim.x)
                     mixed::rollback_item(p, w.pose, w.pose_backup, w.point, w.point_backup, t);
             }
-            if (!lm->done && lm->need_linearize) {
-                __syncthreads();
-                clk.mark(PH_UPDATE);
-                linearise(p, w, B);
-                clk.mark(PH_LINEARISE);
-                if (tid == 0) lm_linearise_done(lm);
-            }
+            if (!lm->done && lm->need_linearize) break;  // pause: the next round linearises
         }
         if (tid == 0) lm_end_attempt(lm);
         __syncthreads();
         clk.mark(PH_UPDATE);
     }
+    __syncthreads();
+    clk.mark(PH_UPDATE);
     if (tid == 0) {
         out[blockIdx.x] = *lm;
         clk.flush();
@@ -261,16 +334,33 @@ __global__ void __launch_bounds__(kThreads) k_batch_solve(const batch_window* wi
 
 
 void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out,
-                        int64_t* phase_ticks, hipStream_t s) {
+                        int64_t* phase_ticks, const batch_grid& grid, uint64_t* lin_clock, int rounds,
+                        hipStream_t s) {
     if (count <= 0) return;
-    if (mixed_path) {
-        if (threads == 256) k_batch_solve<float, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks);
-        else if (threads == 512) k_batch_solve<float, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks);
-        else k_batch_solve<float, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks);
-    } else {
-        if (threads == 256) k_batch_solve<double, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks);
-        else if (threads == 512) k_batch_solve<double, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks);
-        else k_batch_solve<double, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks);
+    auto blocks_for = [](int64_t items) { return static_cast<unsigned>(items > 0 ? (items + 255) / 256 : 1); };
+    k_batch_begin<<<count, 256, 0, s>>>(wins, phase_ticks, lin_clock, 2 * rounds);
+    for (int r = 0; r < rounds; ++r) {
+        if (mixed_path) {
+            k_batch_lin_points<float><<<blocks_for(grid.n_points), 256, 0, s>>>(wins, count, grid.points, grid.n_points,
+                                                                              lin_clock, r);
+            k_batch_lin_chunks<float><<<blocks_for(64L * grid.n_chunks), 256, 0, s>>>(wins, count, grid.chunks,
+                                                                                    grid.n_chunks);
+            k_batch_lin_entries<float><<<blocks_for(grid.n_entries), 256, 0, s>>>(wins, count, grid.entries,
+                                                                                grid.n_entries);
+            if (threads == 256) k_batch_attempts<float, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+            else if (threads == 512) k_batch_attempts<float, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+            else k_batch_attempts<float, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+        } else {
+            k_batch_lin_points<double><<<blocks_for(grid.n_points), 256, 0, s>>>(wins, count, grid.points, grid.n_points,
+                                                                               lin_clock, r);
+            k_batch_lin_chunks<double><<<blocks_for(64L * grid.n_chunks), 256, 0, s>>>(wins, count, grid.chunks,
+                                                                                     grid.n_chunks);
+            k_batch_lin_entries<double><<<blocks_for(grid.n_entries), 256, 0, s>>>(wins, count, grid.entries,
+                                                                                 grid.n_entries);
+            if (threads == 256) k_batch_attempts<double, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+            else if (threads == 512) k_batch_attempts<double, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+            else k_batch_attempts<double, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
+        }
     }
 }
 

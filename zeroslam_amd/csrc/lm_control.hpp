@@ -70,6 +70,7 @@ __device__ inline void lm_init(lm_state* lm, double max_diagonal) {
     lm->step_applied = 0;
     lm->solve_ok = 0;
     lm->lin_count = 1;
+    lm->linearisations = 1;
     lm->done = (lm->iterations <= 0) ? 1 : 0;
 }
 
@@ -147,6 +148,7 @@ __device__ inline void lm_end_attempt(lm_state* lm) {
 }
 
 __device__ inline void lm_linearise_done(lm_state* lm) {
+    if (lm->need_linearize) lm->linearisations += 1;
     lm->need_linearize = 0;
     lm->lin_count += 1;
 }
