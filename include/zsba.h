@@ -200,10 +200,12 @@ int32_t zsba_restore(zsba_graph* graph);
 /* ---- batched local BA (no reference counterpart: many independent graphs) -----
  * A batch solves many independent graphs (local BA windows) with the same
  * iteration count, convergence rule and precision.  Every window whose reduced
- * system has at most 256 rows (42 poses) is solved by one workgroup of a single
- * persistent launch that runs the window's whole LM loop; the launch is captured
- * in a hipGraph and replayed by later zsba_batch_solve calls.  The workgroup size
- * is chosen by timing the candidates on the first solve of each batch shape.
+ * system has at most 256 rows (42 poses) runs its LM attempts in one workgroup;
+ * the windows' linearisations (once per accepted step, factor_graph.hpp:598-611)
+ * run between them as grid-wide launches over every window.  The sequence of
+ * launches is captured in a hipGraph and replayed by later zsba_batch_solve
+ * calls.  The attempts' workgroup size is chosen by timing the candidates on the
+ * first solve of each batch shape.
  * Larger windows, and ZSBA_PRECISION_FP64 (the reference's operation order), are
  * solved one by one with zsba_solve.  accepted[i] receives the accepted-step
  * count of graphs[i]; per-graph zsba_get_stats reflect the batch solve.  The

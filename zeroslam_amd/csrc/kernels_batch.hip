@@ -63,8 +63,9 @@ __device__ __forceinline__ int find_window(const int32_t* prefix, int count, int
 
 __device__ __forceinline__ bool needs_linearisation(const lm_state* lm) { return !lm->done && lm->need_linearize; }
 
-// Grid-wide linearisation, three launches: per point (H_ll, b_l, W and the
-// per-observation pose terms), per pose chunk, per pose-block entry.
+// Grid-wide linearisation, three launches: per point (H_ll, b_l and W), per pose
+// chunk (the pose terms formed again from the observations and summed in a fixed
+// order, so they never go through memory), per pose-block entry.
 template <class TS>
 __global__ void __launch_bounds__(256) k_batch_lin_points(const batch_window* wins, int count, const int32_t* prefix,
                                                           int total, uint64_t* lin_clock, int round) {
@@ -76,21 +77,24 @@ __global__ void __launch_bounds__(256) k_batch_lin_points(const batch_window* wi
     const batch_window& bw = wins[k];
     if (!needs_linearisation(bw.w.lm)) return;
     const blocks<TS> B(bw.w);
-    mixed::lin_point(bw.p, bw.w.pose, bw.w.point, B.pterms, bw.w.pose_grad, B.Hll, bw.w.bl_m, B.W, g - prefix[k]);
+    mixed::lin_point<TS, false>(bw.p, bw.w.pose, bw.w.point, B.pterms, bw.w.pose_grad, B.Hll, bw.w.bl_m, B.W,
+                                g - prefix[k]);
 }
 
 template <class TS>
 __global__ void __launch_bounds__(256) k_batch_lin_chunks(const batch_window* wins, int count, const int32_t* prefix,
                                                           int total) {
+    __shared__ TS ht[4][64][21];
+    __shared__ double gt[4][64][6];
+    const int wave = threadIdx.x >> 6;
     const int g = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);  // one wave per chunk
     if (g >= total) return;
     const int k = find_window(prefix, count, g);
     const batch_window& bw = wins[k];
     if (!needs_linearisation(bw.w.lm)) return;
-    const blocks<TS> B(bw.w);
     const int ic = g - prefix[k];
-    mixed::reduce_pose_chunk(bw.p, B.pterms, bw.w.pose_grad, bw.w.pose_part, ic / kPoseChunks, ic % kPoseChunks,
-                             threadIdx.x & 63);
+    mixed::reduce_pose_chunk_recompute(bw.p, bw.w.pose, bw.w.point, bw.w.pose_part, ic / kPoseChunks,
+                                       ic % kPoseChunks, threadIdx.x & 63, ht[wave], gt[wave]);
 }
 
 template <class TS>

@@ -61,12 +61,46 @@ __device__ inline double cost_item(const dev_problem& p, const double* pose, con
     return robust_chi2(p.loss_kind[li], p.loss_delta[li], r, W);
 }
 
-// make_hessian (factor_graph.hpp:648-732) for the observations of point l, in
-// insertion order: H_ll and the W blocks accumulate in TS, b_l in binary64; the
-// pose-side terms go to pose-major slots.  Residual, robust weight and Jacobians
-// are binary64 in the reference's order; the Hessian products use the Jacobians
-// rounded to TS.
+// Linearisation of observation s between pose parameters P and point X: the
+// binary64 residual, information, robust weight and Jacobians in the reference's
+// order, and the Jacobians and weighted information rounded to TS.  Shared by
+// every kernel that forms Hessian terms, so that recomputed terms equal stored
+// ones bit for bit.
 template <class TS>
+struct obs_lin {
+    double r[2], W[4], Wr[4], drho, Jp[12], Jl[6];
+    TS Jpf[12], Jlf[6], Wrf[4];
+};
+
+template <class TS>
+__device__ __forceinline__ void linearise_obs(const dev_problem& p, const double (&P)[7], const double (&X)[3], int s,
+                                              obs_lin<TS>& o) {
+    const int32_t param = p.obs_param[s];
+    const double* K = p.cams + 4 * (param & 0xFFFF);
+    double Pc[3], q[4];
+    pose_apply(P, X, Pc);
+    pose_quat(P, q);
+    const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+    residual(Pc, uv, K, o.r);
+    obs_weight(p, s, o.W);
+    const int li = param >> 16;
+    robust_info(p.loss_kind[li], p.loss_delta[li], o.W, o.r, o.drho, o.Wr);
+    jacobians(Pc, q, K, o.Jp, o.Jl);
+#pragma unroll
+    for (int i = 0; i < 12; ++i) o.Jpf[i] = static_cast<TS>(o.Jp[i]);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) o.Jlf[i] = static_cast<TS>(o.Jl[i]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.Wrf[i] = static_cast<TS>(o.Wr[i]);
+}
+
+// make_hessian (factor_graph.hpp:648-732) for the observations of point l, in
+// insertion order: H_ll and the W blocks accumulate in TS, b_l in binary64; with
+// kPoseTerms the pose-side terms go to pose-major slots (otherwise
+// reduce_pose_chunk_recompute forms them again).  Residual, robust weight and
+// Jacobians are binary64 in the reference's order; the Hessian products use the
+// Jacobians rounded to TS.
+template <class TS, bool kPoseTerms = true>
 __device__ inline void lin_point(const dev_problem& p, const double* pose, const double* point, TS* pterms, double* pgrad,
                                  TS* Hll, double* bl, TS* Wbuf, int l) {
     const bool lfixed = p.point_fixed[l] != 0;
@@ -78,37 +112,23 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
         const int ip = p.obs_pose[s];
         const bool pfixed = p.pose_fixed[ip] != 0;
         if (pfixed && lfixed) continue;
-        const int32_t param = p.obs_param[s];
-        const double* K = p.cams + 4 * (param & 0xFFFF);
-        double P[7], Pc[3], q[4];
+        double P[7];
 #pragma unroll
         for (int i = 0; i < 7; ++i) P[i] = pose[7L * ip + i];
-        pose_apply(P, X, Pc);
-        pose_quat(P, q);
-        double r[2], W[4], Wr[4], drho, Jp[12], Jl[6];
-        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
-        residual(Pc, uv, K, r);
-        obs_weight(p, s, W);
-        const int li = param >> 16;
-        robust_info(p.loss_kind[li], p.loss_delta[li], W, r, drho, Wr);
-        jacobians(Pc, q, K, Jp, Jl);
-        TS Jpf[12], Jlf[6], Wrf[4];
-#pragma unroll
-        for (int i = 0; i < 12; ++i) Jpf[i] = static_cast<TS>(Jp[i]);
-#pragma unroll
-        for (int i = 0; i < 6; ++i) Jlf[i] = static_cast<TS>(Jl[i]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) Wrf[i] = static_cast<TS>(Wr[i]);
+        obs_lin<TS> o;
+        linearise_obs(p, P, X, s, o);
         if (!pfixed) {
             TS H21[21], Wpl[18];
-            double v[6];
-            pose_hessian(Jpf, Jlf, Wrf, H21, Wpl);
-            pose_grad(Jp, W, r, drho, v);
-            const int64_t slot = p.obs_pose_pos[s];
+            pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
+            if (kPoseTerms) {
+                double v[6];
+                pose_grad(o.Jp, o.W, o.r, o.drho, v);
+                const int64_t slot = p.obs_pose_pos[s];
 #pragma unroll
-            for (int i = 0; i < 21; ++i) pterms[21 * slot + i] = H21[i];
+                for (int i = 0; i < 21; ++i) pterms[21 * slot + i] = H21[i];
 #pragma unroll
-            for (int i = 0; i < 6; ++i) pgrad[6 * slot + i] = v[i];
+                for (int i = 0; i < 6; ++i) pgrad[6 * slot + i] = v[i];
+            }
             if (!lfixed) {
                 const int pr = p.obs_pair[s];
                 TS* Wp = Wbuf + 18L * pr;
@@ -124,8 +144,8 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
         if (!lfixed) {
             TS Hl[9];
             double v[3];
-            point_hessian(Jlf, Wrf, Hl);
-            point_grad(Jl, W, r, drho, v);
+            point_hessian(o.Jlf, o.Wrf, Hl);
+            point_grad(o.Jl, o.W, o.r, o.drho, v);
 #pragma unroll
             for (int i = 0; i < 9; ++i) H[i] = H[i] + Hl[i];
 #pragma unroll
@@ -182,6 +202,67 @@ __device__ inline void reduce_pose_chunk(const dev_problem& p, const TS* pterms,
         for (; t < t1; ++t) acc[0] += src[6L * t];
         out[lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
     }
+}
+
+// reduce_pose_chunk without the stored terms: the wave forms the pose terms of
+// 64 slots at a time (lane = slot, linearise_obs + pose_hessian + pose_grad, the
+// arithmetic lin_point stores), stages them in LDS (ht: 64 x 21 TS, gt: 64 x 6
+// binary64 per wave), and lane j < 27 adds term j over the slots in
+// reduce_pose_chunk's order (accumulator (t - t0) mod 4 over whole blocks of 8
+// slots, the tail into accumulator 0), so part receives the same values.  The
+// caller's wave must be convergent (all 64 lanes call with the same i, c).
+template <class TS>
+__device__ inline void reduce_pose_chunk_recompute(const dev_problem& p, const double* pose, const double* point,
+                                                   double* part, int i, int c, int lane, TS (*ht)[21],
+                                                   double (*gt)[6]) {
+    const bool fixed = p.pose_fixed[i] != 0;
+    int t0, t1;
+    chunk_range(p.pose_start[i], fixed ? p.pose_start[i] : p.pose_start[i + 1], c, t0, t1);
+    const int full = t0 + ((t1 - t0) & ~7);  // slots before this are in whole blocks of 8
+    double P[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) P[k] = pose[7L * i + k];
+    TS hacc[4] = {0, 0, 0, 0};
+    double gacc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int base = t0; base < t1; base += 64) {
+        const int t = base + lane;
+        if (t < t1) {
+            const int s = p.pose_list[t];
+            const int l = p.obs_point[s];
+            const double X[3] = {point[3L * l], point[3L * l + 1], point[3L * l + 2]};
+            obs_lin<TS> o;
+            linearise_obs(p, P, X, s, o);
+            TS H21[21], Wpl[18];
+            double v[6];
+            pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
+            pose_grad(o.Jp, o.W, o.r, o.drho, v);
+#pragma unroll
+            for (int k = 0; k < 21; ++k) ht[lane][k] = H21[k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) gt[lane][k] = v[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int m = (t1 - base) < 64 ? (t1 - base) : 64;
+        if (lane < 21) {
+            for (int k = 0; k < m; ++k) {
+                const int tt = base + k;
+                hacc[tt < full ? ((tt - t0) & 3) : 0] += ht[k][lane];
+            }
+        } else if (lane < 27) {
+            for (int k = 0; k < m; ++k) {
+                const int tt = base + k;
+                gacc[tt < full ? ((tt - t0) & 3) : 0] += gt[k][lane - 21];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    double* out = part + (static_cast<int64_t>(i) * kPoseChunks + c) * 27;
+    if (lane < 21) out[lane] = static_cast<double>((hacc[0] + hacc[1]) + (hacc[2] + hacc[3]));
+    else if (lane < 27) out[lane] = (gacc[0] + gacc[1]) + (gacc[2] + gacc[3]);
 }
 
 // Thread for (pose i, term j < 27): the chunk partials added in order; H_pp term

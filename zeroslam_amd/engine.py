@@ -253,8 +253,9 @@ class Graph:
 class Batch:
     """Many independent graphs (local BA windows) solved together (zsba_batch_*).
 
-    Windows that fit one workgroup run in one persistent, hipGraph-replayed launch;
-    the graphs must outlive the batch."""
+    Windows that fit one workgroup run in one hipGraph launch per solve (rounds of
+    grid-wide linearisation and per-window attempts); the graphs must outlive the
+    batch."""
 
     def __init__(self, graphs: list[Graph]):
         self.lib = _lib.load()
