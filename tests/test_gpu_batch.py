@@ -73,6 +73,37 @@ def test_batch_repeated_pairs_match_individual_solves():
         assert np.array_equal(batch_graphs[k].points(), g.points())
 
 
+def with_closing_edges(p, a=2, b=None, count=20):
+    """The window with pose a also observing `count` of the points pose b sees
+    (same pixels), so the reduced system couples the first and last poses."""
+    b = p.n_poses - 1 if b is None else b
+    idx = np.flatnonzero(p.obs_pose == b)[:count]
+    extra = {"obs_pose": np.full(len(idx), a, np.int32), "obs_point": p.obs_point[idx], "obs_uv": p.obs_uv[idx],
+             "obs_K": p.obs_K[idx], "obs_loss": p.obs_loss[idx], "obs_delta": p.obs_delta[idx]}
+    fields = {k: np.concatenate([getattr(p, k), v]) for k, v in extra.items()}
+    if p.outlier is not None:
+        fields["outlier"] = np.concatenate([p.outlier, np.zeros(len(idx), p.outlier.dtype)])
+    return dataclasses.replace(p, **fields)
+
+
+def test_batch_tall_panels_skip_small_workgroups():
+    """Windows of 40 poses whose first free pose shares points with the last one
+    (8-tile panels, 224 rows below the first diagonal tile) need more than 256
+    threads in the factorisation: the autotuner does not consider 256, and the
+    results equal the per-graph solves."""
+    probs = [with_closing_edges(synth(4, seed=60 + i, n_frames=40, n_points=500)) for i in range(3)]
+    graphs = [Graph.from_problem(p) for p in probs]
+    b = Batch(graphs)
+    acc = b.solve(5, False, MIXED)
+    info = b.info()
+    print(f"tall panels: {info}", flush=True)
+    assert info["windows"] == 3 and info["threads"] >= 512 and info["tune_ms"][0] == 0.0
+    for k, p in enumerate(probs):
+        g = Graph.from_problem(p)
+        assert acc[k] == g.solve(5, False, MIXED)
+        assert np.array_equal(graphs[k].poses(), g.poses())
+
+
 def test_batch_fp64_fast_matches_oracle():
     probs = [clean_window(i) for i in range(4)]
     graphs = [Graph.from_problem(p) for p in probs]

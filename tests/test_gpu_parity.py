@@ -121,6 +121,17 @@ def test_exact_solve_both_factor_paths(cfg, kw, iters, path):
     exact_compare(pr, iters, factor_path=path)
 
 
+@pytest.mark.parametrize("frames", [80, 85])
+def test_exact_one_workgroup_panel_limit(frames):
+    """Loop closures of 15 and 16 tiles with the one-workgroup factorisation
+    requested: 15-tile panels (448 rows below the diagonal tile, one row thread
+    each) stay in the one workgroup, 16-tile panels go to the per-panel launches;
+    both reproduce the reference bit for bit."""
+    pr = synth(4, n_frames=frames, n_points=3000)
+    print(f"one-workgroup limit frames={frames} n={6 * (frames - 2)}", flush=True)
+    exact_compare(pr, 3, factor_path=1)
+
+
 def test_exact_loop_closure_dense_envelope():
     """A loop with revisits: the envelope is dense and the per-panel launches run."""
     pr = synth(4, n_frames=120, n_points=6000)

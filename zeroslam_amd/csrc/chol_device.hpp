@@ -340,6 +340,9 @@ __device__ __forceinline__ void solve_panel_row_fast(double* row, const double (
 //   3. the trailing tiles (I, J), k < J <= I, are updated in memory from PT,
 //      each thread holding a 2 x 4 block: a -= L_Iq L_Jq for q ascending.
 // Workgroup barriers order the phases; nothing waits on another workgroup.
+// Every panel's rows below its diagonal tile need a thread of waves 1..: the
+// caller keeps (tallest panel - 1) x 32 <= blockDim.x - 64 (returns false
+// otherwise).
 // kFast (the reassociating precisions): reciprocal square roots, products with
 // reciprocals and fused multiply-adds instead of the reference's operation
 // sequence (factor_rows_fast).
@@ -368,6 +371,7 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
         const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
         const int nbelow = rows_end - k0 - nk;  // rows below the diagonal tile (full panels only)
         const int rt = tid - 64;                // row-solve thread index (waves 1..)
+        if (nbelow > static_cast<int>(blockDim.x) - 64) return false;  // callers keep panels within the row threads
         double v[NB];
         if (tid < 64) {
             const int lane = opaque_lane();

@@ -944,6 +944,7 @@ struct zsba_batch {
     uint64_t* h_linclk = nullptr;
     int32_t capacity = 0, clock_capacity = 0;
     std::vector<zb::batch_window> h_wins;
+    int32_t min_threads = 256;       // one factorisation row thread per row below a diagonal tile
     std::vector<int32_t> h_prefix;
     zb::batch_grid grid{};
     int32_t rounds = 0;
@@ -1102,6 +1103,7 @@ struct zsba_batch {
         best = 0;
         int32_t rc = 0;
         for (int32_t threads : {256, 512, 1024}) {
+            if (threads < min_threads) continue;
             (void)hipEventRecord(t0, stream);
             zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, d_ticks, grid,
                                    d_linclk, rounds, stream);
@@ -1214,13 +1216,18 @@ struct zsba_batch {
                 return hip_fail(e, "upload linearisation grid");
         }
         const bool mixed = precision == ZSBA_PRECISION_MIXED;
-        int32_t max_n = 0;
+        int32_t max_n = 0, max_panel = 1;
         int64_t obs = 0;
-        for (const zb::batch_window& bw : h_wins) {
-            max_n = std::max(max_n, bw.p.n);
-            obs += bw.p.no;
+        for (int32_t i = 0; i < nw; ++i) {
+            max_n = std::max(max_n, h_wins[i].p.n);
+            max_panel = std::max(max_panel, graphs[win_graph[i]]->w.panel_tiles);
+            obs += h_wins[i].p.no;
         }
-        const tune_key key{nw, max_n, static_cast<int32_t>(std::log2(static_cast<double>(obs) + 1.0)), mixed ? 1 : 0};
+        // the factorisation gives every row below a diagonal tile its own thread
+        // of waves 1..: (tallest panel - 1) x 32 rows need that many threads + 64
+        min_threads = (max_panel - 1) * 32 + 64 <= 256 ? 256 : 512;
+        const tune_key key{nw, max_n * 64 + max_panel, static_cast<int32_t>(std::log2(static_cast<double>(obs) + 1.0)),
+                           mixed ? 1 : 0};
         int32_t threads = 0;
         {
             std::lock_guard<std::mutex> lock(tune_mutex);

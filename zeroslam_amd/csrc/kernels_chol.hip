@@ -333,8 +333,9 @@ __global__ void __launch_bounds__(256) k_chol_syrk(int n, double* A, int kb0, in
 
 // ---- whole factorisation in one workgroup -------------------------------------
 // Used when every column panel of the envelope is at most kFusedTiles tiles tall
-// (wg::fused_factor).
-constexpr int kFusedTiles = 16;
+// (wg::fused_factor: one thread of waves 1.. per row below the diagonal tile).
+constexpr int kFusedTiles = 15;
+static_assert((kFusedTiles - 1) * NB <= 512 - 64, "a row thread for every row below the diagonal tile");
 
 template <bool kFast>
 __global__ void __launch_bounds__(512) k_chol_fused(int n, double* A, const int32_t* tile_last, const double* eps_p,
