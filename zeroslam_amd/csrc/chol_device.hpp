@@ -365,58 +365,137 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
                              int* ok_sh, double* recips, double (*col)[64]) {
     const int T = (n + NB - 1) / NB;
     const int tid = threadIdx.x;
-    for (int k = 0; k < T; ++k) {
-        const int k0 = k * NB;
-        const int nk = (n - k0) < NB ? (n - k0) : NB;
+    const int rt = tid - 64;  // row-solve thread index (waves 1..)
+    auto rows_below = [&](int k) {  // rows below diagonal tile k in its panel (full tiles only)
+        const int k0 = k * NB, nk = (n - k0) < NB ? (n - k0) : NB;
         const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
-        const int nbelow = rows_end - k0 - nk;  // rows below the diagonal tile (full panels only)
-        const int rt = tid - 64;                // row-solve thread index (waves 1..)
-        if (nbelow > static_cast<int>(blockDim.x) - 64) return false;  // callers keep panels within the row threads
-        double v[NB];
-        if (tid < 64) {
-            const int lane = opaque_lane();
-            const int lr = lane & (NB - 1);
-            const bool real = lr < nk;
-            const double* src = A + static_cast<int64_t>(k0 + (real ? lr : 0)) * n + k0;
+        return rows_end - k0 - nk;
+    };
+    // Wave 0: diagonal tile k from memory into registers, factored (col: L_kk
+    // transposed, recips: reciprocals of its diagonal), written back; ok_sh[0] =
+    // the pivots passed.  Exact path: when a division needs div_cr's fallback it
+    // returns true without writing back, for diag_redo.
+    auto diag = [&](int k, double (&v)[NB]) -> bool {
+        const int k0 = k * NB, nk = (n - k0) < NB ? (n - k0) : NB;
+        const int lane = opaque_lane();
+        const int lr = lane & (NB - 1);
+        const bool real = lr < nk;
+        const double* src = A + static_cast<int64_t>(k0 + (real ? lr : 0)) * n + k0;
+#pragma unroll
+        for (int c = 0; c < NB; ++c) {
+            double a = 0.0;
+            if (real && c <= lr) a = src[c];
+            v[c] = real ? a : ((c == lr) ? 1.0 : 0.0);
+        }
+        bool bad = false;
+        const bool ok = kFast ? factor_rows_fast<kChunk>(v, nk, eps, recips, col)
+                              : factor_rows<kChunk>(v, nk, eps, recips, bad, col);
+        if (!kFast && __any(bad)) return true;
+        if (lane < nk) {
+            double* dst = A + static_cast<int64_t>(k0 + lane) * n + k0;
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c <= lane) dst[c] = v[c];
+        }
+        if (lane == 0) ok_sh[0] = ok ? 1 : 0;
+        return false;
+    };
+    // Wave 0: the rare redo of diagonal tile k in LDS (PT[0] as scratch) with
+    // true divisions, from the unfactored rows still in memory.
+    auto diag_redo = [&](int k, double (&v)[NB]) {
+        const int k0 = k * NB, nk = (n - k0) < NB ? (n - k0) : NB;
+        const int lane = opaque_lane();
+        const int lr = lane & (NB - 1);
+        const bool real = lr < nk;
+        const double* src = A + static_cast<int64_t>(k0 + (real ? lr : 0)) * n + k0;
+        double (*D)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&PT[0][0][0]);
+        if (lane < NB) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) D[lane][c] = (real && c <= lr) ? src[c] : ((c == lr) ? 1.0 : 0.0);
+        }
+        const bool ok = factor_tile_lds(D, nk, eps, recips);
+        if (lane < NB) {
 #pragma unroll
             for (int c = 0; c < NB; ++c) {
-                double a = 0.0;
-                if (real && c <= lr) a = src[c];
-                v[c] = real ? a : ((c == lr) ? 1.0 : 0.0);
+                v[c] = D[lane][c];
+                col[c][lane] = D[lane][c];
             }
-            bool bad = false;
-            bool ok = kFast ? factor_rows_fast<kChunk>(v, nk, eps, recips, col)
-                            : factor_rows<kChunk>(v, nk, eps, recips, bad, col);
-            if (!kFast && __any(bad)) {
-                // rare: redo the tile in LDS with divisions, from the unfactored rows
-                double (*D)[NB + 1] = reinterpret_cast<double (*)[NB + 1]>(&PT[0][0][0]);
-                if (lane < NB) {
+        }
+        if (lane < nk) {
+            double* dst = A + static_cast<int64_t>(k0 + lane) * n + k0;
 #pragma unroll
-                    for (int c = 0; c < NB; ++c) D[lane][c] = (real && c <= lr) ? src[c] : ((c == lr) ? 1.0 : 0.0);
-                }
-                ok = factor_tile_lds(D, nk, eps, recips);
-                if (lane < NB) {
+            for (int c = 0; c < NB; ++c)
+                if (c <= lane) dst[c] = v[c];
+        }
+        if (lane == 0) ok_sh[0] = ok ? 1 : 0;
+    };
+    // Block b (2 x 4 entries) of trailing tile (k + 1 + Ip, k + 1 + Jp): a -= L_Iq L_Jq
+    // over panel k's columns q in ascending order, from PT.
+    auto tile_block = [&](int k, int rows_end, int Ip, int Jp, int b) {
+        const int k0 = k * NB;
+        const int r0 = (b >> 3) * 2, c0 = (b & 7) * 4;
+        if (Ip == Jp && c0 > r0 + 1) return;  // above the diagonal
+        const int gr = k0 + NB * (1 + Ip) + r0, gc = k0 + NB * (1 + Jp) + c0;
+        double acc[2][4];
 #pragma unroll
-                    for (int c = 0; c < NB; ++c) {
-                        v[c] = D[lane][c];
-                        col[c][lane] = D[lane][c];
-                    }
-                }
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const bool in = (Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end;
+                acc[a][c] = in ? A[static_cast<int64_t>(gr + a) * n + gc + c] : 0.0;
             }
-            if (lane < nk) {
-                double* dst = A + static_cast<int64_t>(k0 + lane) * n + k0;
+        const double (*LI)[kPT] = PT[Ip];
+        const double (*LJ)[kPT] = PT[Jp];
+#pragma unroll 1
+        for (int q0 = 0; q0 < NB; q0 += 4) {
+            double li[4][2], lj[4][4];  // four steps' operands, loaded before any use
 #pragma unroll
-                for (int c = 0; c < NB; ++c)
-                    if (c <= lane) dst[c] = v[c];
+            for (int u = 0; u < 4; ++u) {
+                li[u][0] = LI[q0 + u][r0];
+                li[u][1] = LI[q0 + u][r0 + 1];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) lj[u][c] = LJ[q0 + u][c0 + c];
             }
-            if (lane == 0) *ok_sh = ok ? 1 : 0;
-        } else if (rt < nbelow) {
-            const double* src = A + static_cast<int64_t>(k0 + nk + rt) * n + k0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        acc[a][c] = kFast ? fma(-li[u][a], lj[u][c], acc[a][c]) : acc[a][c] - li[u][a] * lj[u][c];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if ((Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end)
+                    A[static_cast<int64_t>(gr + a) * n + gc + c] = acc[a][c];
+    };
+
+    double v[NB];  // wave 0: the diagonal tile's rows; waves 1..: a row below it
+    {
+        const int nb0 = rows_below(0);
+        if (nb0 > static_cast<int>(blockDim.x) - 64) return false;  // callers keep panels within the row threads
+        if (tid < 64) {
+            if (diag(0, v)) diag_redo(0, v);
+        } else if (rt < nb0) {
+            const double* src = A + static_cast<int64_t>(NB + rt) * n;
 #pragma unroll
             for (int c = 0; c < NB; ++c) v[c] = src[c];
         }
         __syncthreads();
-        if (*ok_sh == 0) return false;
+        if (ok_sh[0] == 0) return false;
+    }
+    // Per panel k (diagonal tile factored, rows below loaded): the rows are solved
+    // (waves 1..); panel k's update goes first to column k + 1 (all threads); then
+    // wave 0 factors diagonal tile k + 1 while waves 1.. load the rows below it
+    // and update the rest of the trailing tiles.  Every element receives the same
+    // operations in the same order as without the overlap.
+    for (int k = 0; k < T; ++k) {
+        const int k0 = k * NB;
+        const int nk = (n - k0) < NB ? (n - k0) : NB;
+        const int rows_end = (tile_last[k] + 1) * NB < n ? (tile_last[k] + 1) * NB : n;
+        const int nbelow = rows_end - k0 - nk;
         if (tid >= 64 && rt < nbelow) {
             bool bad = false;
             if (kChunk == NB) {
@@ -472,54 +551,36 @@ __device__ bool fused_factor(int n, double* A, const int32_t* tile_last, double 
             for (int c = 0; c < NB; ++c) PT[t][c][rr] = v[c];
         }
         __syncthreads();
-        // trailing tiles (I, J) = (k + 1 + Ip, k + 1 + Jp), 0 <= Jp <= Ip < m: 128
-        // threads per tile, a 2 x 4 block each
+        if (k + 1 >= T) break;
         const int m = (nbelow + NB - 1) / NB;
-        const int blocks = m * (m + 1) / 2 * 128;
-        for (int bidx = tid; bidx < blocks; bidx += blockDim.x) {
-            const int pair = bidx >> 7, b = bidx & 127;
-            int Ip = 0;
-            while ((Ip + 1) * (Ip + 2) / 2 <= pair) ++Ip;
-            const int Jp = pair - Ip * (Ip + 1) / 2;
-            const int r0 = (b >> 3) * 2, c0 = (b & 7) * 4;
-            if (Ip == Jp && c0 > r0 + 1) continue;  // above the diagonal
-            const int gr = k0 + NB * (1 + Ip) + r0, gc = k0 + NB * (1 + Jp) + c0;
-            double acc[2][4];
+        for (int bidx = tid; bidx < m * 128; bidx += blockDim.x) tile_block(k, rows_end, bidx >> 7, 0, bidx & 127);
+        __syncthreads();
+        const int nb1 = rows_below(k + 1);
+        if (nb1 > static_cast<int>(blockDim.x) - 64) return false;
+        if (tid < 64) {
+            const bool redo = diag(k + 1, v);
+            if (!kFast && opaque_lane() == 0) ok_sh[1] = redo ? 1 : 0;
+        } else {
+            if (rt < nb1) {
+                const double* src = A + static_cast<int64_t>((k + 2) * NB + rt) * n + (k + 1) * NB;
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const bool in = (Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end;
-                    acc[a][c] = in ? A[static_cast<int64_t>(gr + a) * n + gc + c] : 0.0;
-                }
-            const double (*LI)[kPT] = PT[Ip];
-            const double (*LJ)[kPT] = PT[Jp];
-#pragma unroll 1
-            for (int q0 = 0; q0 < NB; q0 += 4) {
-                double li[4][2], lj[4][4];  // four steps' operands, loaded before any use
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    li[u][0] = LI[q0 + u][r0];
-                    li[u][1] = LI[q0 + u][r0 + 1];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) lj[u][c] = LJ[q0 + u][c0 + c];
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int a = 0; a < 2; ++a)
-#pragma unroll
-                        for (int c = 0; c < 4; ++c)
-                            acc[a][c] = kFast ? fma(-li[u][a], lj[u][c], acc[a][c]) : acc[a][c] - li[u][a] * lj[u][c];
+                for (int c = 0; c < NB; ++c) v[c] = src[c];
             }
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int c = 0; c < 4; ++c)
-                    if ((Ip != Jp || c0 + c <= r0 + a) && gr + a < rows_end && gc + c < rows_end)
-                        A[static_cast<int64_t>(gr + a) * n + gc + c] = acc[a][c];
+            // trailing tiles with 1 <= Jp <= Ip < m
+            const int pairs = m * (m - 1) / 2;
+            for (int bidx = rt; bidx < pairs * 128; bidx += static_cast<int>(blockDim.x) - 64) {
+                const int pr = bidx >> 7;
+                int Ip = 1;
+                while (Ip * (Ip + 1) / 2 <= pr) ++Ip;
+                tile_block(k, rows_end, Ip, 1 + pr - (Ip - 1) * Ip / 2, bidx & 127);
+            }
         }
         __syncthreads();
+        if (!kFast && ok_sh[1]) {
+            if (tid < 64) diag_redo(k + 1, v);
+            __syncthreads();
+        }
+        if (ok_sh[0] == 0) return false;
     }
     return true;
 }

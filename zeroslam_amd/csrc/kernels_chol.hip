@@ -344,8 +344,8 @@ __global__ void __launch_bounds__(512) k_chol_fused(int n, double* A, const int3
     __shared__ double col[NB][64];
     __shared__ double recips[NB];
     __shared__ double PT[kFusedTiles - 1][NB][wg::kPT];
-    __shared__ int ok_sh;
-    if (!wg::fused_factor<kFusedTiles, kFast>(n, A, tile_last, eps_p[0], PT, &ok_sh, recips, col) && threadIdx.x == 0)
+    __shared__ int ok_sh[2];
+    if (!wg::fused_factor<kFusedTiles, kFast>(n, A, tile_last, eps_p[0], PT, ok_sh, recips, col) && threadIdx.x == 0)
         status[0] = 1;
 }
 
