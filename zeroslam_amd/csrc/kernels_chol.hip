@@ -602,32 +602,38 @@ __global__ void __launch_bounds__(256) k_trsv_block_bwd(int n, const double* L, 
 }
 
 // Unknowns left of block p: x_j -= sum_{i in block} L_ij x_i over the rows whose
-// envelope reaches column j.  Lane = column; wave w sums the rows of tiles
-// t0 + 2w and t0 + 2w + 1; the four partial sums are added in a fixed order.
-__global__ void __launch_bounds__(256) k_trsv_cols_update(int n, const double* L, double* x, int p,
+// envelope reaches column j.  Lane = column; wave w sums the rows of tile t0 + w
+// in four interleaved chains; the eight tile sums are added in a fixed order.
+__global__ void __launch_bounds__(512) k_trsv_cols_update(int n, const double* L, double* x, int p,
                                                           const int32_t* tile_first, const int32_t* status,
                                                           const lm_state* lm, int gated) {
-    static_assert(kSolveBlockTiles == 8, "four waves of two tiles");
+    static_assert(kSolveBlockTiles == 8, "eight waves of one tile");
     if ((gated && lm->done) || *status != 0) return;
-    __shared__ double red[4][64];
+    __shared__ double red[8][64];
     const int T = (n + NB - 1) / NB, t0 = p * kSolveBlockTiles;
     const int t1 = (t0 + kSolveBlockTiles) < T ? (t0 + kSolveBlockTiles) : T;
     const int c0 = t0 * NB;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j = static_cast<int>(blockIdx.x) * 64 + lane;
-    double acc = 0.0;
-    if (j < c0) {
-        for (int tt = 0; tt < 2; ++tt) {
-            const int t = t0 + 2 * wave + tt;
-            if (t >= t1 || tile_first[t] * NB > j) continue;
-            const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-            const double* Lj = L + static_cast<int64_t>(i0) * n + j;
-            for (int r = 0; r < nb; ++r) acc = fma(Lj[static_cast<int64_t>(r) * n], x[i0 + r], acc);
+    const int t = t0 + wave;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (j < c0 && t < t1 && tile_first[t] * NB <= j) {
+        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
+        const double* Lj = L + static_cast<int64_t>(i0) * n + j;
+        int r = 0;
+        for (; r + 3 < nb; r += 4) {
+            a0 = fma(Lj[static_cast<int64_t>(r) * n], x[i0 + r], a0);
+            a1 = fma(Lj[static_cast<int64_t>(r + 1) * n], x[i0 + r + 1], a1);
+            a2 = fma(Lj[static_cast<int64_t>(r + 2) * n], x[i0 + r + 2], a2);
+            a3 = fma(Lj[static_cast<int64_t>(r + 3) * n], x[i0 + r + 3], a3);
         }
+        for (; r < nb; ++r) a0 = fma(Lj[static_cast<int64_t>(r) * n], x[i0 + r], a0);
     }
-    red[wave][lane] = acc;
+    red[wave][lane] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    if (wave == 0 && j < c0) x[j] -= (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+    if (wave == 0 && j < c0)
+        x[j] -= ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])) +
+                ((red[4][lane] + red[5][lane]) + (red[6][lane] + red[7][lane]));
 }
 
 void launch_trsv_blocked(int n, const double* L, const double* b, double* x, const dev_work64& w, hipStream_t s,
@@ -645,7 +651,7 @@ void launch_trsv_blocked(int n, const double* L, const double* b, double* x, con
         k_trsv_block_bwd<<<1, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
         const int c0 = p * kSolveBlockTiles * NB;
         if (c0 > 0)
-            k_trsv_cols_update<<<(c0 + 63) / 64, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
+            k_trsv_cols_update<<<(c0 + 63) / 64, 512, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
     }
 }
 
