@@ -758,36 +758,40 @@ __device__ __forceinline__ double slot_total(const double* ps) {
 }
 
 template <int kThreads>
-__device__ __forceinline__ int triangular_solve_fast(int n, const double* L, const double* b, double* x,
-                                                     const int32_t* tile_first, const int32_t* tile_last,
-                                                     double (*buf)[NB][NB + 1], double* sh, int* flag) {
+__device__ __forceinline__ int triangular_solve_fast_ld(int n, int ld, const double* L, const double* b, double* x,
+                                                        const int32_t* tile_first, const int32_t* tile_last,
+                                                        double (*buf)[NB][NB + 1], double* sh, int* flag, int passes,
+                                                        bool check) {
     static_assert(kThreads >= 128 && kThreads % 64 == 0, "one wave for the recurrences, at least one worker wave");
     constexpr int nwk = kThreads / 64 - 1;                           // worker waves
     constexpr int kPer = (NB * NB + nwk * 64 - 1) / (nwk * 64);      // staged elements per worker thread
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double m = 0;
-    for (int i = tid; i < n; i += blockDim.x) {
-        const double a = fabs(L[static_cast<int64_t>(i) * n + i]);
-        m = (m < a) ? a : m;
+    if (check) {
+        double m = 0;
+        for (int i = tid; i < n; i += blockDim.x) {
+            const double a = fabs(L[static_cast<int64_t>(i) * ld + i]);
+            m = (m < a) ? a : m;
+        }
+        const double mx = block_max(m, sh);
+        if (tid == 0) *flag = 0;
+        __syncthreads();
+        const double eps = mx * 1e-7;
+        for (int i = tid; i < n; i += blockDim.x)
+            if (!(fabs(L[static_cast<int64_t>(i) * ld + i]) > eps)) *flag = 1;
     }
-    const double mx = block_max(m, sh);
-    if (tid == 0) *flag = 0;
-    __syncthreads();
-    const double eps = mx * 1e-7;
-    for (int i = tid; i < n; i += blockDim.x) {
-        if (!(fabs(L[static_cast<int64_t>(i) * n + i]) > eps)) *flag = 1;
-        x[i] = b[i];
-    }
+    if (b != x)
+        for (int i = tid; i < n; i += blockDim.x) x[i] = b[i];
     const int T = (n + NB - 1) / NB;
     auto rows_of = [n](int t) { return (n - t * NB) < NB ? (n - t * NB) : NB; };
     double* part = &buf[4][0][0];  // forward [2][NB]; backward [2][kSlots][NB]
-    stage_tile(buf[0], L, n, 0, 0, rows_of(0), rows_of(0), true, tid, blockDim.x);
-    if (T > 1 && tile_first[1] <= 0) stage_tile(buf[2], L, n, NB, 0, rows_of(1), NB, false, tid, blockDim.x);
+    double own = 0.0;              // wave 0: the coupling tile's product for the next tile
+    if (passes & 1) {
+    stage_tile(buf[0], L, ld, 0, 0, rows_of(0), rows_of(0), true, tid, blockDim.x);
+    if (T > 1 && tile_first[1] <= 0) stage_tile(buf[2], L, ld, NB, 0, rows_of(1), NB, false, tid, blockDim.x);
     __syncthreads();
-    if (*flag) return 2;
+    if (check && *flag) return 2;
 
     // forward: L y = b
-    double own = 0.0;  // wave 0: the coupling tile's product for the next tile
     for (int t = 0; t < T; ++t) {
         const int p = t & 1, i0 = t * NB, nb = rows_of(t);
         if (wave == 0) {
@@ -833,11 +837,11 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
             // the next tile's factor tiles, loaded first (they do not depend on x)
             double sd[kPer], so[kPer];
             const bool has_o = u + 1 < T && tile_first[u + 1] <= u;
-            stage_load(sd, L, n, iu, iu, nbu, nbu, true, first, stride);
-            if (has_o) stage_load(so, L, n, iu + NB, iu, rows_of(u + 1), NB, false, first, stride);
+            stage_load(sd, L, ld, iu, iu, nbu, nbu, true, first, stride);
+            if (has_o) stage_load(so, L, ld, iu + NB, iu, rows_of(u + 1), NB, false, first, stride);
             // slot s, row r: columns c0 + s, c0 + s + kSlots, ... left of this tile
             const int r = lane & (NB - 1);
-            const double* Lr = L + static_cast<int64_t>(iu + (r < nbu ? r : 0)) * n;
+            const double* Lr = L + static_cast<int64_t>(iu + (r < nbu ? r : 0)) * ld;
             for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
                 const double acc = (r < nbu) ? slot_sum([Lr](int j) { return Lr[j]; }, x, c0 + s, i0) : 0.0;
                 part[(u & 1) * kSlots * NB + s * NB + r] = acc;
@@ -848,12 +852,19 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
         __syncthreads();
     }
 
+    }
+    if (!(passes & 2)) return 0;
+    if (!(passes & 1)) {
+        __syncthreads();
+        if (check && *flag) return 2;
+    }
+
     // backward: L^T x = y
     {
         const int t = T - 1;
-        stage_tile(buf[t & 1], L, n, t * NB, t * NB, rows_of(t), rows_of(t), true, tid, blockDim.x);
+        stage_tile(buf[t & 1], L, ld, t * NB, t * NB, rows_of(t), rows_of(t), true, tid, blockDim.x);
         if (t >= 1 && tile_last[t - 1] >= t)
-            stage_tile(buf[2 + (t & 1)], L, n, t * NB, (t - 1) * NB, rows_of(t), NB, false, tid, blockDim.x);
+            stage_tile(buf[2 + (t & 1)], L, ld, t * NB, (t - 1) * NB, rows_of(t), NB, false, tid, blockDim.x);
     }
     __syncthreads();
     own = 0.0;
@@ -901,12 +912,12 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
             const int first = tid - 64, stride = nwk * 64;
             double sd[kPer], so[kPer];
             const bool has_o = u >= 1 && tile_last[u - 1] >= u;
-            stage_load(sd, L, n, iu, iu, NB, NB, true, first, stride);
-            if (has_o) stage_load(so, L, n, iu, iu - NB, NB, NB, false, first, stride);
+            stage_load(sd, L, ld, iu, iu, NB, NB, true, first, stride);
+            if (has_o) stage_load(so, L, ld, iu, iu - NB, NB, NB, false, first, stride);
             const int c = lane & (NB - 1);
             const double* Lc = L + iu + c;
             for (int s = 2 * (wave - 1) + (lane >> 5); s < kSlots; s += 2 * nwk) {
-                const double acc = slot_sum([Lc, n](int j) { return Lc[static_cast<int64_t>(j) * n]; }, x, jbeg + s, jend);
+                const double acc = slot_sum([Lc, ld](int j) { return Lc[static_cast<int64_t>(j) * ld]; }, x, jbeg + s, jend);
                 part[(u & 1) * kSlots * NB + s * NB + c] = acc;
             }
             stage_store(buf[u & 1], sd, first, stride);
@@ -915,6 +926,15 @@ __device__ __forceinline__ int triangular_solve_fast(int n, const double* L, con
         __syncthreads();
     }
     return 0;
+}
+
+
+// The whole system (leading dimension n), both passes, with the failure check.
+template <int kThreads>
+__device__ __forceinline__ int triangular_solve_fast(int n, const double* L, const double* b, double* x,
+                                                     const int32_t* tile_first, const int32_t* tile_last,
+                                                     double (*buf)[NB][NB + 1], double* sh, int* flag) {
+    return triangular_solve_fast_ld<kThreads>(n, n, L, b, x, tile_first, tile_last, buf, sh, flag, 3, true);
 }
 
 }  // namespace wg

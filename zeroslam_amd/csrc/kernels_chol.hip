@@ -442,11 +442,12 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
 
 // ---- blocked substitution for large systems (reassociating precisions) ------
 // Blocks of kSolveBlockTiles row tiles.  Forward: per block, one workgroup solves
-// the diagonal block (its unknowns already hold every earlier block's terms),
-// then a grid-wide launch subtracts the block's terms from every later row of the
-// envelope, one wave per row.  Backward: per block from the last, one workgroup
-// solves the transposed diagonal block, then a grid-wide launch subtracts its
-// terms from every earlier unknown the block's rows reach, one column per lane.
+// the diagonal block with the pipelined substitution (its unknowns already hold
+// every earlier block's terms), then a grid-wide launch subtracts the block's
+// terms from every later row of the envelope, one wave per row.  Backward: per
+// block from the last, one workgroup solves the transposed diagonal block, then a
+// grid-wide launch subtracts its terms from every earlier unknown the block's rows
+// reach, one column per lane.
 // The L tiles a solve reads never leave the envelope; every sum has a fixed
 // order.  Failure rule as in the one-workgroup solve (|L_ii| > 1e-7 max |L_ii|).
 constexpr int kSolveBlockTiles = 8;
@@ -480,56 +481,34 @@ __global__ void __launch_bounds__(1024) k_trsv_begin(int n, const double* L, con
     }
 }
 
-// Diagonal tile t of L into D (identity rows and columns past n).
-__device__ __forceinline__ void stage_diag(double (*D)[NB + 1], const double* L, int n, int i0, int nb) {
-    for (int idx = threadIdx.x; idx < NB * NB; idx += blockDim.x) {
-        const int r = idx / NB, c = idx % NB;
-        D[r][c] = (r < nb && c <= r) ? L[static_cast<int64_t>(i0 + r) * n + i0 + c] : ((r == c) ? 1.0 : 0.0);
-    }
-}
-
-__global__ void __launch_bounds__(256) k_trsv_block_fwd(int n, const double* L, double* x, int p,
-                                                        const int32_t* tile_first, const int32_t* status,
-                                                        const lm_state* lm, int gated) {
-    if ((gated && lm->done) || *status != 0) return;
-    __shared__ double D[NB][NB + 1];
+// Diagonal block p (tiles t0 .. t1 - 1): one pass of the pipelined substitution
+// on the block, its envelope clipped to the block (the earlier or later
+// blocks' terms are already in x).  512 threads.
+__device__ __forceinline__ void solve_diag_block(int n, const double* L, double* x, int p, const int32_t* tile_first,
+                                                 const int32_t* tile_last, int passes) {
+    __shared__ double sh[512];
+    __shared__ double buf[6][NB][NB + 1];
+    __shared__ int32_t tf[kSolveBlockTiles], tl[kSolveBlockTiles];
+    __shared__ int flag;
     const int T = (n + NB - 1) / NB, t0 = p * kSolveBlockTiles;
     const int t1 = (t0 + kSolveBlockTiles) < T ? (t0 + kSolveBlockTiles) : T;
-    const int rend = (t1 * NB) < n ? (t1 * NB) : n;
-    const int tid = threadIdx.x;
-    for (int t = t0; t < t1; ++t) {
-        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-        stage_diag(D, L, n, i0, nb);
-        __syncthreads();
-        if (tid < 64) {  // unit-diagonal recurrence: row i scaled by 1 / L_ii
-            const int lane = wg::opaque_lane(), lr = lane & (NB - 1);
-            double xi = (lane < nb) ? x[i0 + lane] : 0.0;
-            const double yl = 1.0 / D[lr][lr];
-            double rowv[NB];
-#pragma unroll
-            for (int c = 0; c < NB; ++c) rowv[c] = D[lr][c] * ((c < lane) ? yl : 0.0);
-            xi *= yl;
-#pragma unroll
-            for (int j = 0; j < NB; ++j) xi = fma(-rowv[j], lane_value(xi, j), xi);
-            if (lane < nb) x[i0 + lane] = xi;
-        }
-        __syncthreads();
-        // the block's later rows: x_i -= L_i,t x_t
-        for (int i = i0 + NB + tid; i < rend; i += blockDim.x) {
-            if (tile_first[i / NB] > t) continue;
-            const double* Li = L + static_cast<int64_t>(i) * n + i0;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-#pragma unroll
-            for (int c = 0; c < NB; c += 4) {
-                a0 = fma(Li[c], x[i0 + c], a0);
-                a1 = fma(Li[c + 1], x[i0 + c + 1], a1);
-                a2 = fma(Li[c + 2], x[i0 + c + 2], a2);
-                a3 = fma(Li[c + 3], x[i0 + c + 3], a3);
-            }
-            x[i] -= (a0 + a1) + (a2 + a3);
-        }
-        __syncthreads();
+    if (threadIdx.x < static_cast<unsigned>(t1 - t0)) {
+        const int f = tile_first[t0 + threadIdx.x] - t0, l = tile_last[t0 + threadIdx.x];
+        tf[threadIdx.x] = f > 0 ? f : 0;
+        tl[threadIdx.x] = (l < t1 - 1 ? l : t1 - 1) - t0;
     }
+    __syncthreads();
+    const int r0 = t0 * NB, m = ((t1 * NB) < n ? (t1 * NB) : n) - r0;
+    double* xb = x + r0;
+    wg::triangular_solve_fast_ld<512>(m, n, L + static_cast<int64_t>(r0) * n + r0, xb, xb, tf, tl, buf, sh, &flag,
+                                      passes, false);
+}
+
+__global__ void __launch_bounds__(512) k_trsv_block_fwd(int n, const double* L, double* x, int p,
+                                                        const int32_t* tile_first, const int32_t* tile_last,
+                                                        const int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || *status != 0) return;
+    solve_diag_block(n, L, x, p, tile_first, tile_last, 1);
 }
 
 // Rows below block p: x_i -= L_i,block x_block over the row's envelope, one wave
@@ -555,50 +534,12 @@ __global__ void __launch_bounds__(256) k_trsv_rows_update(int n, const double* L
     if (lane == 0) x[i] -= acc;
 }
 
-__global__ void __launch_bounds__(256) k_trsv_block_bwd(int n, const double* L, double* x, int p,
-                                                        const int32_t* tile_first, const int32_t* status,
-                                                        lm_state* lm, int gated) {
+__global__ void __launch_bounds__(512) k_trsv_block_bwd(int n, const double* L, double* x, int p,
+                                                        const int32_t* tile_first, const int32_t* tile_last,
+                                                        const int32_t* status, lm_state* lm, int gated) {
     if ((gated && lm->done) || *status != 0) return;
-    __shared__ double D[NB][NB + 1];
-    const int T = (n + NB - 1) / NB, t0 = p * kSolveBlockTiles;
-    const int t1 = (t0 + kSolveBlockTiles) < T ? (t0 + kSolveBlockTiles) : T;
-    const int c0 = t0 * NB;
-    const int tid = threadIdx.x;
-    for (int t = t1 - 1; t >= t0; --t) {
-        const int i0 = t * NB, nb = (n - i0) < NB ? (n - i0) : NB;
-        stage_diag(D, L, n, i0, nb);
-        __syncthreads();
-        if (tid < 64) {  // unit-diagonal recurrence on L_tt^T
-            const int lane = wg::opaque_lane(), lr = lane & (NB - 1);
-            double xc = (lane < nb) ? x[i0 + lane] : 0.0;
-            const double yl = 1.0 / D[lr][lr];
-            double colv[NB];
-#pragma unroll
-            for (int j = 0; j < NB; ++j) colv[j] = D[j][lr] * ((j > lane) ? yl : 0.0);
-            xc *= yl;
-#pragma unroll
-            for (int j = NB - 1; j >= 0; --j) xc = fma(-colv[j], lane_value(xc, j), xc);
-            if (lane < nb) x[i0 + lane] = xc;
-        }
-        __syncthreads();
-        // the block's earlier unknowns: x_j -= sum_{i in tile t} L_ij x_i
-        const int js = (tile_first[t] * NB) > c0 ? (tile_first[t] * NB) : c0;
-        for (int j = js + tid; j < i0; j += blockDim.x) {
-            const double* Lj = L + static_cast<int64_t>(i0) * n + j;
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int r = 0;
-            for (; r + 3 < nb; r += 4) {
-                a0 = fma(Lj[static_cast<int64_t>(r) * n], x[i0 + r], a0);
-                a1 = fma(Lj[static_cast<int64_t>(r + 1) * n], x[i0 + r + 1], a1);
-                a2 = fma(Lj[static_cast<int64_t>(r + 2) * n], x[i0 + r + 2], a2);
-                a3 = fma(Lj[static_cast<int64_t>(r + 3) * n], x[i0 + r + 3], a3);
-            }
-            for (; r < nb; ++r) a0 = fma(Lj[static_cast<int64_t>(r) * n], x[i0 + r], a0);
-            x[j] -= (a0 + a1) + (a2 + a3);
-        }
-        __syncthreads();
-    }
-    if (p == 0 && tid == 0) lm->solve_ok = 1;  // the solve's last launch
+    solve_diag_block(n, L, x, p, tile_first, tile_last, 2);
+    if (p == 0 && threadIdx.x == 0) lm->solve_ok = 1;  // the solve's last launch
 }
 
 // Unknowns left of block p: x_j -= sum_{i in block} L_ij x_i over the rows whose
@@ -642,13 +583,13 @@ void launch_trsv_blocked(int n, const double* L, const double* b, double* x, con
     const int P = (T + kSolveBlockTiles - 1) / kSolveBlockTiles;
     k_trsv_begin<<<1, 1024, 0, s>>>(n, L, b, x, w.chol_status, w.lm, gated);
     for (int p = 0; p < P; ++p) {
-        k_trsv_block_fwd<<<1, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
+        k_trsv_block_fwd<<<1, 512, 0, s>>>(n, L, x, p, w.tile_first, w.tile_last, w.chol_status, w.lm, gated);
         const int c1 = ((p + 1) * kSolveBlockTiles * NB) < n ? ((p + 1) * kSolveBlockTiles * NB) : n;
         if (c1 < n)
             k_trsv_rows_update<<<(n - c1 + 3) / 4, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
     }
     for (int p = P - 1; p >= 0; --p) {
-        k_trsv_block_bwd<<<1, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
+        k_trsv_block_bwd<<<1, 512, 0, s>>>(n, L, x, p, w.tile_first, w.tile_last, w.chol_status, w.lm, gated);
         const int c0 = p * kSolveBlockTiles * NB;
         if (c0 > 0)
             k_trsv_cols_update<<<(c0 + 63) / 64, 512, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
