@@ -185,6 +185,20 @@ double  zsba_current_chi(zsba_graph* graph, int32_t recompute);
  * region (e2 <= delta^2 for Huber, factor_graph.hpp:873) at the current state,
  * 1 for every observation without a loss.  Removed ids are left untouched. */
 int32_t zsba_get_inlier_flags(zsba_graph* graph, uint8_t* flags /* n_obs_added */);
+/* The reprojection test of map::cull (map.hpp:287-312), the part that needs the
+ * optimised state, for every live point at the current state: the mean over its
+ * observations, in insertion order, of the reprojection error in pixels, each
+ * rounded to binary32 and added in binary32, with the camera-frame point taken as
+ * R X + t for R the matrix of the pose quaternion (the frame rotation map.hpp:243
+ * sets) and `threshold` counted for a point the camera cannot project
+ * (map.hpp:297-299; the reference uses 5.991 for both).  mean_error[id] receives
+ * the mean (+inf for a point without observations), erase[id] 1 when the
+ * reference erases the point on this test: no observation, or a mean of at least
+ * `threshold` (map.hpp:305-312).  Removed ids are left untouched.  The id and
+ * recency rules of map.hpp:271-285 stay with the caller.  Returns the number of
+ * points flagged. */
+int32_t zsba_reprojection_cull(zsba_graph* graph, float threshold, float* mean_error /* n_points_added */,
+                               uint8_t* erase /* n_points_added */);
 /* Per-observation residuals (obs - projection, factor_graph.hpp:990-1009) at the
  * current state, indexed by id.  */
 int32_t zsba_get_residuals(zsba_graph* graph, double* residuals /* n_obs_added*2 */);

@@ -55,6 +55,7 @@ def _bind(lib, prefix: str) -> None:
             "zo_so3_exp": (None, [_D, _D]),
             "zo_so3_left_jacobian": (None, [_D, _D]),
             "zo_loss": (None, [C.c_int32, C.c_double, C.c_double, _D]),
+            "zo_reprojection_cull": (C.c_int32, [C.c_void_p, C.c_float, C.POINTER(C.c_float), _U8]),
         }
         for name, (res, args) in extra.items():
             fn = getattr(lib, name)
@@ -160,6 +161,14 @@ class CpuGraph:
         return out
 
     # oracle-only diagnostics
+    def reprojection_cull(self, threshold: float = 5.991):
+        """map::cull's reprojection test at the current state: per point, the mean
+        error (float32) and the erase flag."""
+        mean = np.zeros(self.n_points, np.float32)
+        erase = np.zeros(self.n_points, np.uint8)
+        self.lib.zo_reprojection_cull(self.h, C.c_float(threshold), _p(mean, C.c_float), _p(erase, C.c_uint8))
+        return mean, erase
+
     def reduced_system(self, lam: float):
         P, M = 6 * self.n_poses, 3 * self.n_points
         S = np.zeros((P, P)); bs = np.zeros(P); xp = np.zeros(P); xl = np.zeros(M)

@@ -872,6 +872,66 @@ int64_t zo_trace(void* h, char* buf, int64_t size) {
 }
 
 // Standalone kernels for known-answer tests.
+// map::cull's reprojection test (map.hpp:287-312) for every point: over its
+// observations in insertion order, mapped = R X + t with R = so3::get_matrix of
+// the pose quaternion (lie.hpp:188-204; the frame rotation set at map.hpp:243),
+// the product in matrix.hpp's order (sum from zero, ascending k) and the
+// translation added after; `threshold` for a point pinhole::project rejects
+// (z < 1e-12, camera.hpp:97-99), otherwise float(sqrt(|uv - projection|^2)) with
+// the squared length summed from zero (matrix.hpp:177-183); the error summed in
+// binary32, the mean in binary32; erase without observations or when the mean
+// reaches the threshold.  Returns the number of points flagged.
+int32_t zo_reprojection_cull(void* h, float threshold, float* mean, uint8_t* erase) {
+    const graph* g = static_cast<const graph*>(h);
+    std::vector<float> err(static_cast<size_t>(g->nl), 0.0f);
+    std::vector<int64_t> count(static_cast<size_t>(g->nl), 0);
+    for (int32_t e = 0; e < g->no; ++e) {
+        const int32_t l = g->obs_point[e];
+        const double* P = &g->pose[7L * g->obs_pose[e]];
+        const double* X = &g->point[3L * l];
+        const double w = P[6], x = P[3], y = P[4], z = P[5];
+        const double tx = 2.0 * x, ty = 2.0 * y, tz = 2.0 * z;
+        const double txx = tx * x, txy = tx * y, txz = tx * z, txw = tx * w;
+        const double tyy = ty * y, tyz = ty * z, tyw = ty * w;
+        const double tzz = tz * z, tzw = tz * w;
+        const double R[9] = {1.0 - (tyy + tzz), txy - tzw, txz + tyw,
+                             txy + tzw, 1.0 - (txx + tzz), tyz - txw,
+                             txz - tyw, tyz + txw, 1.0 - (txx + tyy)};
+        double m[3];
+        for (int i = 0; i < 3; ++i) {
+            double sum = 0;
+            for (int k = 0; k < 3; ++k) sum += R[3 * i + k] * X[k];
+            m[i] = sum + P[i];
+        }
+        if (m[2] < 1.0e-12) {
+            err[l] += threshold;
+        } else {
+            const double* K = &g->obs_K[4L * e];
+            const double iz = 1.0 / m[2];
+            const double u = K[0] * (m[0] * iz) + K[2];
+            const double v = K[1] * (m[1] * iz) + K[3];
+            const double d[2] = {g->obs_uv[2L * e] - u, g->obs_uv[2L * e + 1] - v};
+            double l2 = 0;
+            for (int i = 0; i < 2; ++i) l2 += d[i] * d[i];
+            err[l] += static_cast<float>(std::sqrt(l2));
+        }
+        ++count[l];
+    }
+    int32_t flagged = 0;
+    for (int32_t l = 0; l < g->nl; ++l) {
+        if (count[l] == 0) {
+            mean[l] = INFINITY;
+            erase[l] = 1;
+        } else {
+            const float mm = err[l] / static_cast<float>(count[l]);
+            mean[l] = mm;
+            erase[l] = (mm >= threshold) ? 1 : 0;
+        }
+        flagged += erase[l];
+    }
+    return flagged;
+}
+
 int32_t zo_cholesky(const double* A, int32_t n, double* L) { return decompose_cholesky(A, n, L) ? 1 : 0; }
 int32_t zo_cholesky_solve(const double* L, const double* b, int32_t n, double* x) { return solve_cholesky(L, b, n, x) ? 1 : 0; }
 int32_t zo_invert3(const double* A, double* out) { return invert3(A, out) ? 1 : 0; }

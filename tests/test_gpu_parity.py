@@ -199,3 +199,43 @@ def test_fast_solve_within_tolerance(cfg, kw, iters):
     chi, chi_o = g.current_chi(True), o.current_chi(True)
     assert abs(chi - chi_o) <= TOL * abs(chi_o)
     assert state_rel_err(P, X, Po, Xo) <= TOL
+
+
+@pytest.mark.parametrize("cfg,kw,iters", [(1, dict(n_frames=20, n_points=1500), 0), (1, dict(n_frames=20, n_points=1500), 5),
+                                          (4, dict(n_frames=60, n_points=3000), 3)])
+def test_reprojection_cull_matches_oracle(cfg, kw, iters):
+    """map::cull's reprojection test on the device against the oracle, at the
+    initial state and after exact solves: the float32 means bit for bit, the same
+    erase flags; outliers and points behind cameras give flagged points."""
+    pr = synth(cfg, **kw)
+    print(f"reprojection cull cfg={cfg} {kw} iters={iters}", flush=True)
+    g = Graph.from_problem(pr)
+    o = CpuGraph(pr, "oracle")
+    if iters:
+        assert g.solve(iters, False, FP64) == o.solve(iters)
+    for thr in (5.991, 50.0):
+        mean, erase = g.reprojection_cull(thr)
+        mean_o, erase_o = o.reprojection_cull(thr)
+        print(f"  threshold {thr}: flagged {int(erase.sum())} of {len(erase)}", flush=True)
+        assert np.array_equal(mean.view(np.uint32), mean_o.view(np.uint32))
+        assert np.array_equal(erase, erase_o)
+        assert 0 < int(erase.sum()) < len(erase)
+
+
+def test_reprojection_cull_after_edits():
+    """Ids after removals and additions: a point whose observations were all
+    removed and a point added without any are flagged with an infinite mean, a
+    removed point's entries are left untouched, the rest match the oracle."""
+    pr = synth(1, n_frames=10, n_points=400)
+    g = Graph.from_problem(pr)
+    for e in np.flatnonzero(pr.obs_point == 0):
+        g.remove_observation(int(e))
+    g.remove_point(1)
+    extra = g.add_points(np.array([[0.5, 0.5, 4.0]]))
+    mean, erase = g.reprojection_cull()
+    mean_o, erase_o = CpuGraph(pr, "oracle").reprojection_cull()
+    assert np.isinf(mean[0]) and erase[0] == 1
+    assert np.isnan(mean[1]) and erase[1] == 0
+    assert np.isinf(mean[extra]) and erase[extra] == 1
+    assert np.array_equal(mean[2:pr.n_points].view(np.uint32), mean_o[2:].view(np.uint32))
+    assert np.array_equal(erase[2:pr.n_points], erase_o[2:])

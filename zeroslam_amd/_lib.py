@@ -98,6 +98,7 @@ _SIGNATURES = {
     "zsba_solve": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32]),
     "zsba_current_chi": (C.c_double, [_P, C.c_int32]),
     "zsba_get_inlier_flags": (C.c_int32, [_P, _U8]),
+    "zsba_reprojection_cull": (C.c_int32, [_P, C.c_float, C.POINTER(C.c_float), _U8]),
     "zsba_get_residuals": (C.c_int32, [_P, _D]),
     "zsba_get_stats": (C.c_int32, [_P, C.POINTER(Stats)]),
     "zsba_snapshot": (C.c_int32, [_P]),

@@ -225,6 +225,9 @@ void launch_decide64(const dev_work64& w, hipStream_t s);
 void launch_rollback64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out, hipStream_t s);
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s);
+// map::cull's reprojection test per point slot (mean error, erase flag).
+void launch_reprojection_cull(const dev_problem& p, const dev_work64& w, float threshold, float* mean, uint8_t* erase,
+                              hipStream_t s);
 void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 // Point step dx_l for the solved dx_p and the update of every free vertex (no LM gating).
 void launch_gn_apply64(const dev_problem& p, const dev_work64& w, hipStream_t s);

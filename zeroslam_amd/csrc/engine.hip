@@ -122,6 +122,8 @@ struct zsba_graph {
     lm_state* lm_pinned = nullptr;
     double* lambda_dev = nullptr;
     uint8_t* flags_dev = nullptr;
+    float* cull_mean_dev = nullptr;
+    uint8_t* cull_erase_dev = nullptr;
     double* eval_dev = nullptr;
     size_t eval_capacity = 0;
     double* snap_pose = nullptr;
@@ -518,6 +520,8 @@ int32_t zsba_graph::build_structure() {
     w.lm = mem.get<lm_state>(1);
     lambda_dev = mem.get<double>(1);
     flags_dev = mem.get<uint8_t>(no);
+    cull_mean_dev = mem.get<float>(nl);
+    cull_erase_dev = mem.get<uint8_t>(nl);
     snap_pose = mem.get<double>(7L * np);
     snap_point = mem.get<double>(3L * nl);
     if (mem.status != hipSuccess) {
@@ -1505,6 +1509,38 @@ int32_t zsba_get_inlier_flags(zsba_graph* g, uint8_t* flags) {
     if (e != hipSuccess) return g->hip_fail(e, "inlier flags");
     for (int32_t s = 0; s < g->dp.no; ++s) flags[g->obs_e_id[g->sorted_e[s]]] = f[s];
     return 0;
+}
+
+int32_t zsba_reprojection_cull(zsba_graph* g, float threshold, float* mean_error, uint8_t* erase) {
+    if (!g || !mean_error || !erase) return ZSBA_EINVAL;
+    if (g->live_points() == 0) return 0;
+    if (g->live_obs() == 0) {  // every live point is without observations
+        int32_t n = 0;
+        for (size_t j = 0; j < g->point_alive.size(); ++j)
+            if (g->point_alive[j]) {
+                mean_error[j] = INFINITY;
+                erase[j] = 1;
+                ++n;
+            }
+        return n;
+    }
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    const int32_t nl = g->dp.nl;
+    zb::launch_reprojection_cull(g->dp, g->w, threshold, g->cull_mean_dev, g->cull_erase_dev, g->stream);
+    std::vector<float> m(nl);
+    std::vector<uint8_t> f(nl);
+    hipError_t e = g->copy(m.data(), g->cull_mean_dev, m.size() * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = g->copy(f.data(), g->cull_erase_dev, f.size(), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return g->hip_fail(e, "reprojection cull");
+    int32_t n = 0;
+    for (int32_t l = 0; l < nl; ++l) {
+        const int32_t id = g->point_slot_id[l];
+        mean_error[id] = m[l];
+        erase[id] = f[l];
+        n += f[l];
+    }
+    return n;
 }
 
 int32_t zsba_get_residuals(zsba_graph* g, double* out) {

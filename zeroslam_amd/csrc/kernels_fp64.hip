@@ -1020,6 +1020,59 @@ void launch_eval_edges64(const dev_problem& p, const dev_work64& w, double* out,
     if (p.no > 0) k_eval_edges<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, out);
 }
 
+// map::cull's reprojection test (map.hpp:287-312) for point l: over its
+// observations in insertion order, the camera-frame point R X + t with R the
+// matrix of the pose quaternion (lie::so3::get_matrix, lie.hpp:188-204, the
+// frame rotation map.hpp:243 sets), |uv - project(.)| rounded to binary32 and
+// added in binary32, `threshold` for a point the camera cannot project
+// (camera.hpp:97-99, map.hpp:297-299); the mean, and erase when there is no
+// observation or the mean reaches the threshold (map.hpp:305-312).
+__global__ void __launch_bounds__(kBlock) k_reprojection_cull(dev_problem p, const double* pose, const double* point,
+                                                              float threshold, float* mean, uint8_t* erase) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p.nl) return;
+    double X[3];
+    load3(point + 3L * l, X);
+    float err = 0.0f;
+    int count = 0;
+    for (int s = p.lm_start[l]; s < p.lm_start[l + 1]; ++s) {
+        double P[7], q[4], R[9], m[3];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) P[i] = pose[7L * p.obs_pose[s] + i];
+        pose_quat(P, q);
+        qmatrix(q, R);
+        mul<3, 3, 1>(R, X, m);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = m[i] + P[i];
+        if (m[2] < 1.0e-12) {
+            err = err + threshold;
+        } else {
+            const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+            double r[2];
+            residual(m, uv, p.cams + 4 * (p.obs_param[s] & 0xFFFF), r);
+            double l2 = 0.0;  // matrix::get_length_squared, matrix.hpp:177-183
+            l2 = l2 + r[0] * r[0];
+            l2 = l2 + r[1] * r[1];
+            err = err + static_cast<float>(sqrt(l2));
+        }
+        ++count;
+    }
+    if (count == 0) {
+        mean[l] = INFINITY;
+        erase[l] = 1;
+    } else {
+        const float mm = err / static_cast<float>(count);
+        mean[l] = mm;
+        erase[l] = (mm >= threshold) ? 1 : 0;
+    }
+}
+
+void launch_reprojection_cull(const dev_problem& p, const dev_work64& w, float threshold, float* mean, uint8_t* erase,
+                              hipStream_t s) {
+    if (p.nl > 0)
+        k_reprojection_cull<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, threshold, mean, erase);
+}
+
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s) {
     if (p.no > 0) k_inliers<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, flags_sorted);
 }

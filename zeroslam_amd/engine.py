@@ -182,6 +182,17 @@ class Graph:
         self._check(self.lib.zsba_get_inlier_flags(self.h, ptr(out, C.c_uint8)))
         return out
 
+    def reprojection_cull(self, threshold: float = 5.991) -> tuple[np.ndarray, np.ndarray]:
+        """map::cull's reprojection test at the current state (zsba_reprojection_cull):
+        per point id, the mean reprojection error (float32, +inf without
+        observations) and the erase flag."""
+        n = self.counts()[1]
+        mean = np.full(n, np.nan, np.float32)
+        erase = np.zeros(n, np.uint8)
+        self._check(self.lib.zsba_reprojection_cull(self.h, C.c_float(threshold), ptr(mean, C.c_float),
+                                                    ptr(erase, C.c_uint8)))
+        return mean, erase
+
     def residuals(self) -> np.ndarray:
         out = np.full((self.counts()[2], 2), np.nan)
         self._check(self.lib.zsba_get_residuals(self.h, ptr(out)))
