@@ -9,7 +9,7 @@ import pytest
 from oracle.oracle import CpuGraph
 from tests.helpers import rel_err, state_rel_err
 from zeroslam_amd.engine import FP64, FP64_FAST, MIXED, Batch, Graph, solve_batch
-from zeroslam_amd.problem import synth, synth_window
+from zeroslam_amd.problem import motion_only, synth, synth_window
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +180,29 @@ def test_batch_fallbacks():
         r = Graph.from_problem(p)
         r.solve(4, False, FP64)
         assert np.array_equal(g.points(), r.points())
+
+
+@pytest.mark.parametrize("precision,tol", [(FP64_FAST, 1e-9), (MIXED, 1e-6)])
+def test_batch_motion_only_frames(precision, tol):
+    """Tracking-step BAs of many frames at once: each window one free frame and
+    its fixed landmarks (a 6x6 system), without outliers or robust loss; the
+    batch against the per-graph solves, and FP64_FAST against the oracle."""
+    probs = [motion_only(synth(1, seed=70 + i, n_frames=12, n_points=600, outlier_rate=0.0, loss_kind=1), 1)
+             for i in range(16)]
+    graphs = [Graph.from_problem(p) for p in probs]
+    acc = solve_batch(graphs, 50, False, precision)
+    for k, p in enumerate(probs):
+        g = Graph.from_problem(p)
+        a = g.solve(50, False, precision)
+        chi_b, chi = graphs[k].stats()["final_chi2"], g.stats()["final_chi2"]
+        print(f"frame {k}: obs {p.n_obs} batch acc {acc[k]} chi2 {chi_b!r} | single acc {a} chi2 {chi!r}", flush=True)
+        assert acc[k] == a
+        assert abs(chi_b - chi) <= tol * chi
+        assert rel_err(graphs[k].poses(), g.poses()) <= tol
+        if precision == FP64_FAST:
+            # run to convergence, the last accepted steps change the cost at the
+            # rounding floor: whether one more is taken may differ by one
+            o = CpuGraph(p, "oracle")
+            assert abs(int(acc[k]) - o.solve(50)) <= 1
+            assert abs(chi_b - o.current_chi(True)) <= tol * chi_b
+            assert rel_err(graphs[k].poses(), o.state()[0]) <= tol

@@ -14,7 +14,7 @@ from oracle.oracle import CpuGraph
 from tests.golden.make_golden import CASES, load
 from tests.helpers import rel_err, state_rel_err
 from zeroslam_amd.engine import FP64, FP64_FAST, Graph, math_probe
-from zeroslam_amd.problem import synth
+from zeroslam_amd.problem import motion_only, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -239,3 +239,12 @@ def test_reprojection_cull_after_edits():
     assert np.isinf(mean[extra]) and erase[extra] == 1
     assert np.array_equal(mean[2:pr.n_points].view(np.uint32), mean_o[2:].view(np.uint32))
     assert np.array_equal(erase[2:pr.n_points], erase_o[2:])
+
+
+@pytest.mark.parametrize("n_free", [1, 3])
+def test_exact_motion_only(n_free):
+    """map::optimise(n, true, 50), the tracking step's BA for one free frame:
+    every landmark fixed, edges only to the free frames, 50 iterations."""
+    pr = motion_only(synth(1, seed=30 + n_free, n_frames=20, n_points=1500), n_free)
+    print(f"motion-only n_free={n_free} obs={pr.n_obs}", flush=True)
+    exact_compare(pr, 50)

@@ -6,7 +6,7 @@ import pytest
 
 from oracle.oracle import CpuGraph, have_reference
 from tests.golden.make_golden import mixed_problem
-from zeroslam_amd.problem import synth
+from zeroslam_amd.problem import motion_only, synth
 
 pytestmark = pytest.mark.skipif(not have_reference(), reason="oracle/_ref not built (no reference checkout)")
 
@@ -40,3 +40,11 @@ def test_mixed_losses_information_cameras():
 def test_config1_three_iterations():
     """The 100-keyframe / 20k-point / 160k-observation workload."""
     bitwise_same(synth(1), 3)
+
+
+@pytest.mark.parametrize("n_free", [1, 3])
+def test_motion_only_graphs(n_free):
+    """The graph of map::optimise(n, true, 50): fixed landmarks, edges only to the
+    free frames; one free frame is the tracking step's 6x6 system."""
+    pr = motion_only(synth(1, seed=30 + n_free, n_frames=20, n_points=1500), n_free)
+    bitwise_same(pr, 50)

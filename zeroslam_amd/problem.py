@@ -130,6 +130,31 @@ def synth(config: int, seed: int | None = None, **overrides) -> Problem:
                    true_poses=true_poses, true_points=true_points, outlier=outlier)
 
 
+def motion_only(pr: Problem, n_free: int = 1) -> Problem:
+    """The graph ``map::optimise(n_free, true, ...)`` builds from a map
+    (``map.hpp:79-192``): the last ``n_free`` frames free, every landmark fixed,
+    edges only to the free frames (an edge to a fixed frame is skipped when the
+    landmarks are fixed, ``map.hpp:150-152``), and frames and landmarks left
+    without edges dropped (``map.hpp:181-192``).  With one free frame this is the
+    motion-only BA of the tracking step (``slam.hpp:316,387,469``): a single 6x6
+    system per iteration."""
+    free = np.zeros(pr.n_poses, bool)
+    free[pr.n_poses - n_free:] = True
+    keep = free[pr.obs_pose]
+    pose_ids = np.flatnonzero(free)
+    point_ids = np.unique(pr.obs_point[keep])
+    pose_map = np.full(pr.n_poses, -1, np.int32); pose_map[pose_ids] = np.arange(len(pose_ids))
+    point_map = np.full(pr.n_points, -1, np.int32); point_map[point_ids] = np.arange(len(point_ids))
+    sub = lambda a, m: None if a is None else a[m].copy()
+    return Problem(poses=pr.poses[pose_ids].copy(), pose_fixed=np.zeros(len(pose_ids), np.uint8),
+                   points=pr.points[point_ids].copy(), point_fixed=np.ones(len(point_ids), np.uint8),
+                   obs_pose=pose_map[pr.obs_pose[keep]], obs_point=point_map[pr.obs_point[keep]],
+                   obs_uv=sub(pr.obs_uv, keep), obs_K=sub(pr.obs_K, keep), obs_loss=sub(pr.obs_loss, keep),
+                   obs_delta=sub(pr.obs_delta, keep), obs_info=sub(pr.obs_info, keep),
+                   true_poses=sub(pr.true_poses, pose_ids), true_points=sub(pr.true_points, point_ids),
+                   outlier=sub(pr.outlier, keep))
+
+
 def synth_window(index: int, seed_base: int = 2) -> Problem:
     """One independent local-BA window of the batched workload: keyframes
     U{5..30}, points U{200..3000}, track length U{2..window}, camera and noise
