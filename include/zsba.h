@@ -256,6 +256,51 @@ int32_t     zsba_batch_get_info(const zsba_batch* batch, zsba_batch_info* info);
 int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t iterations,
                          int32_t use_relative_convergence, int32_t precision, int32_t* accepted);
 
+/* ---- data association by projection (slam.hpp:341-384) ------------------------
+ * A new frame's matched keypoints against the map's landmarks: the loop of
+ * slam.hpp:341-384 with feature::distance<256> (feature.hpp:1134-1188).  Inputs,
+ * in the reference's terms:
+ *   frame:              frame_current.rotation (row-major) and .translation, the
+ *                       pinhole intrinsics, image_grey's cols and rows;
+ *   landmark_xyz[o]:    the o-th landmark in the map's iteration order
+ *                       (reconstruction.landmarks), 3 doubles;
+ *   landmark_descriptor[o]: the descriptor of its first observation, 32 bytes
+ *                       (slam.hpp:374);
+ *   landmark_observed[o]: 1 when frame_current already observes it (slam.hpp:352-362);
+ *   keypoint_xy[i]:     match_point_current[i];
+ *   keypoint_descriptor[i]: descriptor of match_index_current[i], 32 bytes;
+ *   keypoint_key[i]:    match_index_previous[i], in [0, n_keys);
+ *   key_taken[k]:       1 when frame_previous_points holds k before the loop.
+ * The matches are written in the order the reference calls add_observation:
+ * (match_landmark[j], match_keypoint[j]) = (o, i); the caller adds the
+ * observation of landmark o with keypoint match_index_current[i] and marks
+ * match_index_previous[i] taken.  At most max_matches are written; the return
+ * value is the number of matches (it may exceed max_matches) or a negative
+ * ZSBA_E* code.  A matcher keeps its device buffers between calls; calls on one
+ * matcher must not overlap. */
+typedef struct zsba_matcher zsba_matcher;
+typedef struct zsba_match_frame {
+    double rotation[9];     /* row-major R: camera = R * world + t */
+    double translation[3];
+    double K[4];            /* fx, fy, cx, cy */
+    int64_t cols, rows;     /* image size; the border is 10 px (slam.hpp:348) */
+} zsba_match_frame;
+zsba_matcher* zsba_matcher_create(int32_t device);
+void          zsba_matcher_destroy(zsba_matcher* matcher);
+const char*   zsba_matcher_last_error(const zsba_matcher* matcher);
+int32_t zsba_match_by_projection(zsba_matcher* matcher, const zsba_match_frame* frame,
+                                 int32_t n_landmarks, const double* landmark_xyz /* n*3 */,
+                                 const uint8_t* landmark_descriptor /* n*32 */,
+                                 const uint8_t* landmark_observed /* n */,
+                                 int32_t n_keypoints, const double* keypoint_xy /* n*2 */,
+                                 const uint8_t* keypoint_descriptor /* n*32 */,
+                                 const int32_t* keypoint_key /* n */,
+                                 int32_t n_keys, const uint8_t* key_taken /* n_keys */,
+                                 int32_t max_matches, int32_t* match_landmark, int32_t* match_keypoint);
+/* Device time of the last call (from its first kernel to its last, including the
+ * read-back of the candidate count) and its number of candidate pairs. */
+int32_t zsba_matcher_stats(const zsba_matcher* matcher, double* device_ms, int32_t* candidates);
+
 /* ---- implicit differentiation (no reference counterpart) ----------------------
  * zsba_gauss_newton applies `steps` undamped Gauss-Newton steps unconditionally
  * (lambda = 0; no cost test), polishing a converged solve to the stationary

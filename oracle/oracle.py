@@ -56,6 +56,9 @@ def _bind(lib, prefix: str) -> None:
             "zo_so3_left_jacobian": (None, [_D, _D]),
             "zo_loss": (None, [C.c_int32, C.c_double, C.c_double, _D]),
             "zo_reprojection_cull": (C.c_int32, [C.c_void_p, C.c_float, C.POINTER(C.c_float), _U8]),
+            "zo_hamming256": (C.c_int32, [_U8, _U8]),
+            "zo_match_by_projection": (C.c_int32, [_D, _D, _D, C.c_int64, C.c_int64, C.c_int32, _D, _U8, _U8,
+                                                   C.c_int32, _D, _U8, _I32, C.c_int32, _U8, C.c_int32, _I32, _I32]),
         }
         for name, (res, args) in extra.items():
             fn = getattr(lib, name)
@@ -195,3 +198,32 @@ class CpuGraph:
         buf = C.create_string_buffer(int(n) + 1)
         self.lib.zo_trace(self.h, buf, n + 1)
         return buf.value.decode()
+
+
+def match_by_projection(scene, max_matches: int | None = None):
+    """The oracle's projection search (zo_match_by_projection) on a
+    zeroslam_amd.problem.MatchScene: the (landmark, keypoint) matches in the
+    order the reference adds them."""
+    lib = oracle_lib()
+    L, N = len(scene.landmark_xyz), len(scene.keypoint_xy)
+    # a keypoint's previous-frame index is taken by its match: at most N matches
+    cap = max(1, N if max_matches is None else max_matches)
+    lm = np.zeros(cap, np.int32)
+    kp = np.zeros(cap, np.int32)
+    a = {k: np.ascontiguousarray(getattr(scene, k)) for k in ("rotation", "translation", "K", "landmark_xyz",
+                                                             "landmark_descriptor", "landmark_observed", "keypoint_xy",
+                                                             "keypoint_descriptor", "keypoint_key", "key_taken")}
+    n = lib.zo_match_by_projection(_p(a["rotation"]), _p(a["translation"]), _p(a["K"]), int(scene.cols), int(scene.rows),
+                                   L, _p(a["landmark_xyz"]), _p(a["landmark_descriptor"], C.c_uint8),
+                                   _p(a["landmark_observed"], C.c_uint8), N, _p(a["keypoint_xy"]),
+                                   _p(a["keypoint_descriptor"], C.c_uint8), _p(a["keypoint_key"], C.c_int32),
+                                   len(scene.key_taken), _p(a["key_taken"], C.c_uint8), cap, _p(lm, C.c_int32),
+                                   _p(kp, C.c_int32))
+    k = min(n, cap)
+    return lm[:k].copy(), kp[:k].copy(), n
+
+
+def hamming256(a, b) -> int:
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    return int(oracle_lib().zo_hamming256(_p(a, C.c_uint8), _p(b, C.c_uint8)))

@@ -932,6 +932,72 @@ int32_t zo_reprojection_cull(void* h, float threshold, float* mean, uint8_t* era
     return flagged;
 }
 
+// feature::distance<256> (feature.hpp:1134-1188): the number of differing bits
+// of two 32-byte descriptors, as four 64-bit popcounts of the XOR.
+int32_t zo_hamming256(const uint8_t* a, const uint8_t* b) {
+    int32_t d = 0;
+    for (int w = 0; w < 4; ++w) {
+        uint64_t x, y;
+        std::memcpy(&x, a + 8 * w, 8);
+        std::memcpy(&y, b + 8 * w, 8);
+        d += __builtin_popcountll(x ^ y);
+    }
+    return d;
+}
+
+// The projection search of slam.hpp:341-384, literally: landmarks in the given
+// order; mapped = R X + t (products summed from zero in ascending k, the
+// translation added after); pinhole::project (camera.hpp:97-112: no projection
+// below z = 1e-12); the 10-pixel border against cols - 11 and rows - 11 in the
+// reference's size_t arithmetic (slam.hpp:348-350); landmarks the frame already
+// observes skipped (:352-362); then every keypoint in index order: skipped when
+// its squared distance to the projection, summed from zero, is above 4
+// (:365-368), when its previous-frame index is taken (:369-373), or when its
+// descriptor is 64 or more bits from the landmark's (:374-376); otherwise a match,
+// and its previous-frame index is taken (:377-379).  Writes at most max_matches
+// (landmark, keypoint) pairs in that order; returns the number of matches.
+int32_t zo_match_by_projection(const double* R, const double* t, const double* K, int64_t cols, int64_t rows,
+                               int32_t n_landmarks, const double* xyz, const uint8_t* lm_desc, const uint8_t* observed,
+                               int32_t n_keypoints, const double* kp_xy, const uint8_t* kp_desc, const int32_t* key,
+                               int32_t n_keys, const uint8_t* key_taken, int32_t max_matches, int32_t* match_lm,
+                               int32_t* match_kp) {
+    std::vector<uint8_t> taken(key_taken, key_taken + n_keys);
+    const double u_max = static_cast<double>(static_cast<uint64_t>(cols) - 11u);
+    const double v_max = static_cast<double>(static_cast<uint64_t>(rows) - 11u);
+    int32_t n = 0;
+    for (int32_t o = 0; o < n_landmarks; ++o) {
+        const double* X = &xyz[3L * o];
+        double m[3];
+        for (int i = 0; i < 3; ++i) {
+            double sum = 0;
+            for (int k = 0; k < 3; ++k) sum += R[3 * i + k] * X[k];
+            m[i] = sum + t[i];
+        }
+        if (m[2] < 1.0e-12) continue;
+        const double iz = 1.0 / m[2];
+        const double ux = m[0] * iz, uy = m[1] * iz;
+        const double u = K[0] * ux + K[2], v = K[1] * uy + K[3];
+        if ((u < 10) || (u > u_max) || (v < 10) || (v > v_max)) continue;
+        if (observed[o]) continue;
+        for (int32_t i = 0; i < n_keypoints; ++i) {
+            const double d[2] = {kp_xy[2L * i] - u, kp_xy[2L * i + 1] - v};
+            double l2 = 0;
+            for (int k = 0; k < 2; ++k) l2 += d[k] * d[k];
+            if (l2 > 4) continue;
+            if (taken[key[i]]) continue;
+            if (zo_hamming256(&lm_desc[32L * o], &kp_desc[32L * i]) < 64) {
+                if (n < max_matches) {
+                    match_lm[n] = o;
+                    match_kp[n] = i;
+                }
+                ++n;
+                taken[key[i]] = 1;
+            }
+        }
+    }
+    return n;
+}
+
 int32_t zo_cholesky(const double* A, int32_t n, double* L) { return decompose_cholesky(A, n, L) ? 1 : 0; }
 int32_t zo_cholesky_solve(const double* L, const double* b, int32_t n, double* x) { return solve_cholesky(L, b, n, x) ? 1 : 0; }
 int32_t zo_invert3(const double* A, double* out) { return invert3(A, out) ? 1 : 0; }

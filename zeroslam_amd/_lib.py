@@ -68,6 +68,11 @@ class BatchInfo(C.Structure):
         return d
 
 
+class MatchFrame(C.Structure):
+    _fields_ = [("rotation", C.c_double * 9), ("translation", C.c_double * 3), ("K", C.c_double * 4),
+                ("cols", C.c_int64), ("rows", C.c_int64)]
+
+
 _P = C.c_void_p
 _D = C.POINTER(C.c_double)
 _I32 = C.POINTER(C.c_int32)
@@ -110,6 +115,12 @@ _SIGNATURES = {
     "zsba_batch_solve": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, _I32]),
     "zsba_batch_get_info": (C.c_int32, [_P, C.POINTER(BatchInfo)]),
     "zsba_adjoint": (C.c_int32, [_P, _D, _D, _D, _D]),
+    "zsba_matcher_create": (_P, [C.c_int32]),
+    "zsba_matcher_destroy": (None, [_P]),
+    "zsba_matcher_last_error": (C.c_char_p, [_P]),
+    "zsba_matcher_stats": (C.c_int32, [_P, _D, _I32]),
+    "zsba_match_by_projection": (C.c_int32, [_P, C.POINTER(MatchFrame), C.c_int32, _D, _U8, _U8, C.c_int32, _D, _U8,
+                                             _I32, C.c_int32, _U8, C.c_int32, _I32, _I32]),
     "zsba_gauss_newton": (C.c_int32, [_P, C.c_int32]),
     "zsba_debug_adjoint_ops": (C.c_int32, [_P, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),

@@ -43,6 +43,7 @@ UNITS = [
     ("kernels_chol.hip", ["-ffp-contract=off"]),
     ("kernels_batch.hip", ["-ffp-contract=off"]),
     ("kernels_adjoint.hip", ["-ffp-contract=off"]),
+    ("kernels_match.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -1,0 +1,396 @@
+// Projection-based data association of a new frame against the map
+// (slam.hpp:341-384) with the 256-bit Hamming distance of feature.hpp:1134-1188.
+//
+// The reference walks the landmarks in the map's iteration order and, for each
+// landmark in front of the frame and inside its 10-pixel border that the frame
+// does not observe yet, every matched keypoint of the frame in index order.  A
+// keypoint within 2 px of the landmark's projection (squared distance not above
+// 4), whose previous-frame index has not been taken, and whose descriptor is
+// within 63 bits of the landmark's first observation's descriptor, becomes an
+// observation; its previous-frame index is then taken.
+//
+// On the device the test is split in two exact halves:
+//   1. every (landmark, keypoint) pair is tested independently against the
+//      indices taken before the walk: one thread per landmark, the keypoint
+//      positions staged through LDS in tiles shared by the workgroup, the
+//      descriptor held in registers, Hamming distances by popcount.  A count
+//      pass, a device-wide exclusive scan and a write pass lay the candidates
+//      out in the walk's order (landmark order, then keypoint order);
+//   2. the taking of indices during the walk: a candidate survives exactly when
+//      it is the first candidate, in the walk's order, that carries its
+//      previous-frame index (an earlier accepted candidate with that index would
+//      have taken it, and the first one is always accepted).  One 64-bit atomic
+//      minimum of (landmark, keypoint) per index finds it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "zsba.h"
+
+namespace {
+
+constexpr int kMatchBlock = 256;
+constexpr int kScanBlock = 1024;
+constexpr int kDescWords = 8;  // 256 bits
+
+struct match_frame {
+    double R[9];
+    double t[3];
+    double K[4];
+    double u_max, v_max;  // cols - 11 and rows - 11, converted as the reference's size_t arithmetic
+};
+
+// mapped = R X + t (matrix.hpp products: sum from zero in ascending k, the
+// translation added after), pinhole::project (camera.hpp:91-112), and the
+// border test of slam.hpp:348-350.
+__device__ __forceinline__ bool project_landmark(const match_frame& f, const double* X, double& u, double& v) {
+    double m[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) s = s + f.R[3 * i + k] * X[k];
+        m[i] = s + f.t[i];
+    }
+    if (m[2] < 1.0e-12) return false;
+    const double iz = 1.0 / m[2];
+    const double x = m[0] * iz, y = m[1] * iz;
+    u = f.K[0] * x + f.K[2];
+    v = f.K[1] * y + f.K[3];
+    return !((u < 10.0) || (u > f.u_max) || (v < 10.0) || (v > f.v_max));
+}
+
+// One thread per landmark.  kWrite = false counts each landmark's candidates,
+// kWrite = true writes them from the landmark's exclusive offset.  Every thread
+// of the workgroup takes part in staging every keypoint tile.
+template <bool kWrite>
+__global__ void __launch_bounds__(kMatchBlock)
+k_match_scan_keypoints(match_frame f, int32_t nl, const double* __restrict__ lm_xyz, const uint32_t* __restrict__ lm_desc,
+                       const uint8_t* __restrict__ lm_observed, int32_t nk, const double* __restrict__ kp_xy,
+                       const uint32_t* __restrict__ kp_desc, const int32_t* __restrict__ kp_key,
+                       const uint8_t* __restrict__ key_taken, int32_t* __restrict__ count,
+                       const int32_t* __restrict__ offset, int32_t* __restrict__ cand_lm, int32_t* __restrict__ cand_kp) {
+    __shared__ double2 tile[kMatchBlock];
+    const int32_t o = blockIdx.x * kMatchBlock + threadIdx.x;
+    double u = 0.0, v = 0.0;
+    bool live = false;
+    uint32_t d[kDescWords];
+    if (o < nl && !lm_observed[o]) {
+        const double X[3] = {lm_xyz[3L * o], lm_xyz[3L * o + 1], lm_xyz[3L * o + 2]};
+        live = project_landmark(f, X, u, v);
+    }
+#pragma unroll
+    for (int w = 0; w < kDescWords; ++w) d[w] = live ? lm_desc[kDescWords * static_cast<int64_t>(o) + w] : 0u;
+    int32_t n = 0;
+    const int32_t base = (kWrite && live) ? offset[o] : 0;
+    for (int32_t j0 = 0; j0 < nk; j0 += kMatchBlock) {
+        __syncthreads();
+        const int32_t j = j0 + threadIdx.x;
+        if (j < nk) tile[threadIdx.x] = make_double2(kp_xy[2L * j], kp_xy[2L * j + 1]);
+        __syncthreads();
+        if (!live) continue;
+        const int32_t jn = min(kMatchBlock, nk - j0);
+        for (int32_t jj = 0; jj < jn; ++jj) {
+            const double2 p = tile[jj];
+            const double d0 = p.x - u, d1 = p.y - v;
+            double l2 = 0.0;  // matrix::get_length_squared
+            l2 = l2 + d0 * d0;
+            l2 = l2 + d1 * d1;
+            if (l2 > 4.0) continue;
+            const int32_t k = j0 + jj;
+            if (key_taken[kp_key[k]]) continue;
+            uint32_t h = 0;
+#pragma unroll
+            for (int w = 0; w < kDescWords; ++w) h += __popc(d[w] ^ kp_desc[kDescWords * static_cast<int64_t>(k) + w]);
+            if (h >= 64u) continue;
+            if (kWrite) {
+                cand_lm[base + n] = o;
+                cand_kp[base + n] = k;
+            }
+            ++n;
+        }
+    }
+    if (!kWrite && o < nl) count[o] = n;
+}
+
+// Exclusive scan of per-landmark counts: block totals, one workgroup over the
+// totals, then each block's scan from its base.
+__device__ __forceinline__ int32_t block_exclusive_scan(int32_t x, int32_t& total) {
+    __shared__ int32_t wsum[kScanBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int32_t inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int32_t before = 0;
+    total = 0;
+    for (int w = 0; w < kScanBlock / 64; ++w) {
+        if (w < wv) before += wsum[w];
+        total += wsum[w];
+    }
+    return before + inc - x;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_match_block_totals(int32_t nl, const int32_t* count, int32_t* totals) {
+    const int32_t o = blockIdx.x * kScanBlock + threadIdx.x;
+    int32_t total;
+    (void)block_exclusive_scan(o < nl ? count[o] : 0, total);
+    if (threadIdx.x == 0) totals[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_match_scan_totals(int32_t nb, int32_t* totals, int32_t nl, int32_t* offset) {
+    int32_t carry = 0;
+    for (int32_t b0 = 0; b0 < nb; b0 += kScanBlock) {
+        const int32_t b = b0 + threadIdx.x;
+        const int32_t x = b < nb ? totals[b] : 0;
+        int32_t total;
+        const int32_t ex = block_exclusive_scan(x, total);
+        __syncthreads();
+        if (b < nb) totals[b] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) offset[nl] = carry;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_match_scan_apply(int32_t nl, const int32_t* count, const int32_t* base,
+                                                                 int32_t* offset) {
+    const int32_t o = blockIdx.x * kScanBlock + threadIdx.x;
+    int32_t total;
+    const int32_t ex = block_exclusive_scan(o < nl ? count[o] : 0, total);
+    if (o < nl) offset[o] = base[blockIdx.x] + ex;
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_best_fill(int32_t n_keys, unsigned long long* best) {
+    for (int32_t k = blockIdx.x * kMatchBlock + threadIdx.x; k < n_keys; k += gridDim.x * kMatchBlock) best[k] = ~0ull;
+}
+
+__device__ __forceinline__ unsigned long long walk_position(int32_t o, int32_t k) {
+    return (static_cast<unsigned long long>(static_cast<uint32_t>(o)) << 32) | static_cast<uint32_t>(k);
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_first_per_key(int32_t nc, const int32_t* cand_lm, const int32_t* cand_kp,
+                                                                   const int32_t* kp_key, unsigned long long* best) {
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t k = cand_kp[c];
+    atomicMin(&best[kp_key[k]], walk_position(cand_lm[c], k));
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_accept(int32_t nc, const int32_t* cand_lm, const int32_t* cand_kp,
+                                                            const int32_t* kp_key, const unsigned long long* best,
+                                                            uint8_t* accepted) {
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= nc) return;
+    const int32_t k = cand_kp[c];
+    accepted[c] = best[kp_key[k]] == walk_position(cand_lm[c], k) ? 1 : 0;
+}
+
+int32_t blocks_for(int64_t n, int block) { return static_cast<int32_t>((n + block - 1) / block); }
+
+}  // namespace
+
+struct zsba_matcher {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::string err;
+    // device buffers, grown on demand
+    struct buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    buf lm_xyz, lm_desc, lm_obs, kp_xy, kp_desc, kp_key, key_taken, count, offset, totals, best, cand_lm, cand_kp, accepted;
+    std::vector<int32_t> h_lm, h_kp;
+    std::vector<uint8_t> h_acc;
+    double last_ms = 0.0;
+    int32_t last_candidates = 0;
+
+    ~zsba_matcher() {
+        if (stream) {
+            (void)hipSetDevice(device);
+            (void)hipStreamSynchronize(stream);
+        }
+        for (buf* b : {&lm_xyz, &lm_desc, &lm_obs, &kp_xy, &kp_desc, &kp_key, &key_taken, &count, &offset, &totals, &best,
+                       &cand_lm, &cand_kp, &accepted})
+            if (b->p) (void)hipFree(b->p);
+        for (hipEvent_t& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    int32_t fail(int32_t code, const char* fmt, ...) {
+        char text[512];
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(text, sizeof text, fmt, ap);
+        va_end(ap);
+        err = text;
+        return code;
+    }
+    int32_t hip_fail(hipError_t e, const char* what) { return fail(ZSBA_EDEVICE, "%s: %s", what, hipGetErrorString(e)); }
+    hipError_t reserve(buf& b, size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (b.bytes >= bytes) return hipSuccess;
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        const size_t grown = bytes + bytes / 2;
+        const hipError_t e = hipMalloc(&b.p, grown);
+        if (e == hipSuccess) b.bytes = grown;
+        return e;
+    }
+    hipError_t upload(buf& b, const void* src, size_t bytes) {
+        hipError_t e = reserve(b, bytes);
+        if (e == hipSuccess && bytes) e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, stream);
+        return e;
+    }
+    template <class T>
+    T* as(buf& b) { return static_cast<T*>(b.p); }
+};
+
+extern "C" {
+
+zsba_matcher* zsba_matcher_create(int32_t device) {
+    zsba_matcher* m = new (std::nothrow) zsba_matcher();
+    if (!m) return nullptr;
+    m->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&m->ev[0]) != hipSuccess || hipEventCreate(&m->ev[1]) != hipSuccess) {
+        delete m;
+        return nullptr;
+    }
+    return m;
+}
+
+void zsba_matcher_destroy(zsba_matcher* m) { delete m; }
+
+const char* zsba_matcher_last_error(const zsba_matcher* m) { return m ? m->err.c_str() : "null handle"; }
+
+int32_t zsba_matcher_stats(const zsba_matcher* m, double* device_ms, int32_t* candidates) {
+    if (!m) return ZSBA_EINVAL;
+    if (device_ms) *device_ms = m->last_ms;
+    if (candidates) *candidates = m->last_candidates;
+    return 0;
+}
+
+int32_t zsba_match_by_projection(zsba_matcher* m, const zsba_match_frame* frame, int32_t n_landmarks,
+                                 const double* landmark_xyz, const uint8_t* landmark_descriptor,
+                                 const uint8_t* landmark_observed, int32_t n_keypoints, const double* keypoint_xy,
+                                 const uint8_t* keypoint_descriptor, const int32_t* keypoint_key, int32_t n_keys,
+                                 const uint8_t* key_taken, int32_t max_matches, int32_t* match_landmark,
+                                 int32_t* match_keypoint) {
+    if (!m) return ZSBA_EINVAL;
+    if (!frame || n_landmarks < 0 || n_keypoints < 0 || n_keys < 0 || max_matches < 0)
+        return m->fail(ZSBA_EINVAL, "bad arguments to match_by_projection");
+    if ((n_landmarks > 0 && (!landmark_xyz || !landmark_descriptor || !landmark_observed)) ||
+        (n_keypoints > 0 && (!keypoint_xy || !keypoint_descriptor || !keypoint_key || !key_taken)) ||
+        (max_matches > 0 && (!match_landmark || !match_keypoint)))
+        return m->fail(ZSBA_EINVAL, "match_by_projection: null array");
+    for (int32_t i = 0; i < n_keypoints; ++i)
+        if (keypoint_key[i] < 0 || keypoint_key[i] >= n_keys)
+            return m->fail(ZSBA_EINVAL, "match_by_projection: keypoint %d has previous-frame index %d outside [0, %d)", i,
+                           keypoint_key[i], n_keys);
+    m->last_ms = 0.0;
+    m->last_candidates = 0;
+    if (n_landmarks == 0 || n_keypoints == 0) return 0;
+
+    match_frame f;
+    for (int i = 0; i < 9; ++i) f.R[i] = frame->rotation[i];
+    for (int i = 0; i < 3; ++i) f.t[i] = frame->translation[i];
+    for (int i = 0; i < 4; ++i) f.K[i] = frame->K[i];
+    f.u_max = static_cast<double>(static_cast<uint64_t>(frame->cols) - 11u);
+    f.v_max = static_cast<double>(static_cast<uint64_t>(frame->rows) - 11u);
+
+    hipError_t e = hipSetDevice(m->device);
+    const int64_t nl = n_landmarks, nk = n_keypoints;
+    const int32_t nb = blocks_for(nl, kScanBlock);
+    if (e == hipSuccess) e = m->upload(m->lm_xyz, landmark_xyz, 24 * nl);
+    if (e == hipSuccess) e = m->upload(m->lm_desc, landmark_descriptor, 32 * nl);
+    if (e == hipSuccess) e = m->upload(m->lm_obs, landmark_observed, nl);
+    if (e == hipSuccess) e = m->upload(m->kp_xy, keypoint_xy, 16 * nk);
+    if (e == hipSuccess) e = m->upload(m->kp_desc, keypoint_descriptor, 32 * nk);
+    if (e == hipSuccess) e = m->upload(m->kp_key, keypoint_key, 4 * nk);
+    if (e == hipSuccess) e = m->upload(m->key_taken, key_taken, static_cast<size_t>(n_keys));
+    if (e == hipSuccess) e = m->reserve(m->count, 4 * nl);
+    if (e == hipSuccess) e = m->reserve(m->offset, 4 * (nl + 1));
+    if (e == hipSuccess) e = m->reserve(m->totals, 4 * static_cast<size_t>(nb));
+    if (e == hipSuccess) e = m->reserve(m->best, 8 * static_cast<size_t>(n_keys));
+    if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: upload");
+
+    const hipStream_t s = m->stream;
+    (void)hipEventRecord(m->ev[0], s);
+    const int32_t grid = blocks_for(nl, kMatchBlock);
+    k_match_scan_keypoints<false><<<grid, kMatchBlock, 0, s>>>(
+        f, n_landmarks, m->as<double>(m->lm_xyz), m->as<uint32_t>(m->lm_desc), m->as<uint8_t>(m->lm_obs), n_keypoints,
+        m->as<double>(m->kp_xy), m->as<uint32_t>(m->kp_desc), m->as<int32_t>(m->kp_key), m->as<uint8_t>(m->key_taken),
+        m->as<int32_t>(m->count), nullptr, nullptr, nullptr);
+    k_match_block_totals<<<nb, kScanBlock, 0, s>>>(n_landmarks, m->as<int32_t>(m->count), m->as<int32_t>(m->totals));
+    k_match_scan_totals<<<1, kScanBlock, 0, s>>>(nb, m->as<int32_t>(m->totals), n_landmarks, m->as<int32_t>(m->offset));
+    k_match_scan_apply<<<nb, kScanBlock, 0, s>>>(n_landmarks, m->as<int32_t>(m->count), m->as<int32_t>(m->totals),
+                                                  m->as<int32_t>(m->offset));
+    int32_t nc = 0;
+    e = hipMemcpyAsync(&nc, m->as<int32_t>(m->offset) + nl, sizeof nc, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: candidate count");
+    m->last_candidates = nc;
+    if (nc == 0) {
+        (void)hipEventRecord(m->ev[1], s);
+        (void)hipEventSynchronize(m->ev[1]);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, m->ev[0], m->ev[1]);
+        m->last_ms = ms;
+        return 0;
+    }
+    e = m->reserve(m->cand_lm, 4 * static_cast<size_t>(nc));
+    if (e == hipSuccess) e = m->reserve(m->cand_kp, 4 * static_cast<size_t>(nc));
+    if (e == hipSuccess) e = m->reserve(m->accepted, static_cast<size_t>(nc));
+    if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: candidate buffers");
+    k_match_scan_keypoints<true><<<grid, kMatchBlock, 0, s>>>(
+        f, n_landmarks, m->as<double>(m->lm_xyz), m->as<uint32_t>(m->lm_desc), m->as<uint8_t>(m->lm_obs), n_keypoints,
+        m->as<double>(m->kp_xy), m->as<uint32_t>(m->kp_desc), m->as<int32_t>(m->kp_key), m->as<uint8_t>(m->key_taken),
+        nullptr, m->as<int32_t>(m->offset), m->as<int32_t>(m->cand_lm), m->as<int32_t>(m->cand_kp));
+    k_match_best_fill<<<std::min(blocks_for(n_keys, kMatchBlock), 1024), kMatchBlock, 0, s>>>(
+        n_keys, m->as<unsigned long long>(m->best));
+    const int32_t cgrid = blocks_for(nc, kMatchBlock);
+    k_match_first_per_key<<<cgrid, kMatchBlock, 0, s>>>(nc, m->as<int32_t>(m->cand_lm), m->as<int32_t>(m->cand_kp),
+                                                        m->as<int32_t>(m->kp_key), m->as<unsigned long long>(m->best));
+    k_match_accept<<<cgrid, kMatchBlock, 0, s>>>(nc, m->as<int32_t>(m->cand_lm), m->as<int32_t>(m->cand_kp),
+                                                 m->as<int32_t>(m->kp_key), m->as<unsigned long long>(m->best),
+                                                 m->as<uint8_t>(m->accepted));
+    (void)hipEventRecord(m->ev[1], s);
+    m->h_lm.resize(nc);
+    m->h_kp.resize(nc);
+    m->h_acc.resize(nc);
+    e = hipMemcpyAsync(m->h_lm.data(), m->cand_lm.p, 4 * static_cast<size_t>(nc), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->h_kp.data(), m->cand_kp.p, 4 * static_cast<size_t>(nc), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->h_acc.data(), m->accepted.p, static_cast<size_t>(nc), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return m->hip_fail(e, "match_by_projection");
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, m->ev[0], m->ev[1]);
+    m->last_ms = ms;
+    // The candidates are in the walk's order; the accepted ones are its matches.
+    int32_t n = 0;
+    for (int32_t c = 0; c < nc; ++c) {
+        if (!m->h_acc[c]) continue;
+        if (n < max_matches) {
+            match_landmark[n] = m->h_lm[c];
+            match_keypoint[n] = m->h_kp[c];
+        }
+        ++n;
+    }
+    return n;
+}
+
+}  // extern "C"

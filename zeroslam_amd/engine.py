@@ -11,7 +11,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import ptr
-from .problem import Problem
+from .problem import MatchScene, Problem
 
 FP64 = _lib.ZSBA_PRECISION_FP64
 MIXED = _lib.ZSBA_PRECISION_MIXED
@@ -311,3 +311,56 @@ def solve_batch(graphs: list[Graph], iterations: int, use_relative_convergence: 
         msgs = [lib.zsba_last_error(g.h).decode() for g in graphs if lib.zsba_last_error(g.h)]
         raise ZsbaError(rc, "; ".join(m for m in msgs if m))
     return acc
+
+
+class Matcher:
+    """Projection search of a new frame against the map on the device
+    (zsba_match_by_projection, the loop of slam.hpp:341-384).  Keeps its device
+    buffers between calls."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        self.h = self.lib.zsba_matcher_create(int(device))
+        if not self.h:
+            raise ZsbaError(-3, "zsba_matcher_create failed")
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.zsba_matcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def match(self, scene: MatchScene, max_matches: int | None = None) -> tuple[np.ndarray, np.ndarray, int]:
+        """(landmark, keypoint) index pairs in the order the reference adds the
+        observations, and the number of matches."""
+        f = _lib.MatchFrame()
+        f.rotation[:] = [float(x) for x in np.asarray(scene.rotation, np.float64).reshape(9)]
+        f.translation[:] = [float(x) for x in np.asarray(scene.translation, np.float64)]
+        f.K[:] = [float(x) for x in np.asarray(scene.K, np.float64)]
+        f.cols, f.rows = int(scene.cols), int(scene.rows)
+        xyz = np.ascontiguousarray(scene.landmark_xyz, np.float64)
+        lm_desc = np.ascontiguousarray(scene.landmark_descriptor, np.uint8)
+        observed = np.ascontiguousarray(scene.landmark_observed, np.uint8)
+        kp_xy = np.ascontiguousarray(scene.keypoint_xy, np.float64)
+        kp_desc = np.ascontiguousarray(scene.keypoint_descriptor, np.uint8)
+        key = np.ascontiguousarray(scene.keypoint_key, np.int32)
+        taken = np.ascontiguousarray(scene.key_taken, np.uint8)
+        cap = max(1, len(kp_xy) if max_matches is None else int(max_matches))
+        lm = np.zeros(cap, np.int32)
+        kp = np.zeros(cap, np.int32)
+        n = self.lib.zsba_match_by_projection(
+            self.h, C.byref(f), len(xyz), ptr(xyz), ptr(lm_desc, C.c_uint8), ptr(observed, C.c_uint8), len(kp_xy),
+            ptr(kp_xy), ptr(kp_desc, C.c_uint8), ptr(key, C.c_int32), len(taken), ptr(taken, C.c_uint8), cap,
+            ptr(lm, C.c_int32), ptr(kp, C.c_int32))
+        if n < 0:
+            raise ZsbaError(n, self.lib.zsba_matcher_last_error(self.h).decode())
+        k = min(n, cap)
+        return lm[:k].copy(), kp[:k].copy(), n
+
+    def stats(self) -> dict:
+        ms = C.c_double()
+        nc = C.c_int32()
+        self.lib.zsba_matcher_stats(self.h, C.byref(ms), C.byref(nc))
+        return {"device_ms": ms.value, "candidates": nc.value}

@@ -164,3 +164,87 @@ def synth_window(index: int, seed_base: int = 2) -> Problem:
     n_points = int(rng.integers(200, 3001))
     return synth(2, seed=1000003 * seed_base + index, n_frames=n_frames, n_points=n_points,
                  track_min=2, track_max=n_frames)
+
+
+@dataclasses.dataclass
+class MatchScene:
+    """The inputs of the projection search of a new frame against the map
+    (``slam.hpp:341-384``), as arrays: the frame's rotation (row-major, camera =
+    R world + t), translation, intrinsics and image size; the landmarks in the
+    map's iteration order with the descriptor of each one's first observation and
+    whether the frame already observes it; the frame's matched keypoints with
+    their descriptors and previous-frame indices; the previous-frame indices
+    taken before the search."""
+    rotation: np.ndarray             # (3, 3) float64
+    translation: np.ndarray          # (3,) float64
+    K: np.ndarray                    # (4,) float64
+    cols: int
+    rows: int
+    landmark_xyz: np.ndarray         # (L, 3) float64
+    landmark_descriptor: np.ndarray  # (L, 32) uint8
+    landmark_observed: np.ndarray    # (L,) uint8
+    keypoint_xy: np.ndarray          # (N, 2) float64
+    keypoint_descriptor: np.ndarray  # (N, 32) uint8
+    keypoint_key: np.ndarray         # (N,) int32
+    key_taken: np.ndarray            # (n_keys,) uint8
+
+
+def match_scene(n_landmarks: int, n_keypoints: int, seed: int = 0, cols: int = 640, rows: int = 480) -> MatchScene:
+    """A synthetic frame and map for the projection search.  Most landmarks are
+    in view; some are behind the camera or outside the image; some come in pairs
+    that project within a pixel of each other.  About half the keypoints sit
+    within 0-2.5 px of a landmark's projection with that landmark's descriptor
+    with 40-90 bits flipped (either side of the 64-bit bound); the rest are
+    scattered with random descriptors.  Some previous-frame indices are shared by
+    two keypoints, some are taken before the search, and some landmarks are
+    already observed."""
+    rng = np.random.default_rng(seed)
+    K = np.array([525.0, 525.0, 319.5, 239.5])
+    axis = rng.standard_normal(3)
+    axis /= np.linalg.norm(axis)
+    a = 0.1
+    Kx = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    R = np.eye(3) + np.sin(a) * Kx + (1 - np.cos(a)) * Kx @ Kx
+    t = rng.normal(0, 0.2, 3)
+    # landmarks in the camera frame, then to the world
+    z = rng.uniform(1.5, 25.0, n_landmarks)
+    u = rng.uniform(-40, cols + 40, n_landmarks)
+    v = rng.uniform(-40, rows + 40, n_landmarks)
+    twin = rng.random(n_landmarks) < 0.1
+    src = rng.integers(0, n_landmarks, n_landmarks)
+    u[twin] = u[src[twin]] + rng.uniform(-0.7, 0.7, twin.sum())
+    v[twin] = v[src[twin]] + rng.uniform(-0.7, 0.7, twin.sum())
+    z[twin] = z[src[twin]] * rng.uniform(0.9, 1.1, twin.sum())
+    cam = np.stack([(u - K[2]) / K[0] * z, (v - K[3]) / K[1] * z, z], axis=1)
+    behind = rng.random(n_landmarks) < 0.05
+    cam[behind, 2] *= -1.0
+    world = (cam - t) @ R  # R^T (c - t) per row
+    lm_desc = rng.integers(0, 256, (n_landmarks, 32), dtype=np.uint8)
+    lm_desc[twin] = lm_desc[src[twin]]
+    observed = (rng.random(n_landmarks) < 0.05).astype(np.uint8)
+    # keypoints: half near landmark projections, half scattered
+    n_near = n_keypoints // 2
+    owner = rng.integers(0, n_landmarks, n_near)
+    ang = rng.uniform(0, 2 * np.pi, n_near)
+    rad = rng.uniform(0.0, 2.5, n_near)
+    kp = np.empty((n_keypoints, 2))
+    kp[:n_near, 0] = u[owner] + rad * np.cos(ang)
+    kp[:n_near, 1] = v[owner] + rad * np.sin(ang)
+    kp[n_near:, 0] = rng.uniform(0, cols, n_keypoints - n_near)
+    kp[n_near:, 1] = rng.uniform(0, rows, n_keypoints - n_near)
+    kp_desc = rng.integers(0, 256, (n_keypoints, 32), dtype=np.uint8)
+    bits = np.unpackbits(lm_desc[owner], axis=1)
+    for r in range(n_near):
+        flip = rng.choice(256, size=int(rng.integers(40, 91)), replace=False)
+        bits[r, flip] ^= 1
+    kp_desc[:n_near] = np.packbits(bits, axis=1)
+    order = rng.permutation(n_keypoints)
+    kp, kp_desc = kp[order], kp_desc[order]
+    n_keys = int(n_keypoints * 1.25) + 1
+    key = rng.choice(n_keys, size=n_keypoints, replace=False).astype(np.int32)
+    shared = rng.random(n_keypoints) < 0.1
+    key[shared] = key[rng.integers(0, n_keypoints, shared.sum())]
+    taken = (rng.random(n_keys) < 0.1).astype(np.uint8)
+    return MatchScene(rotation=R, translation=t, K=K, cols=cols, rows=rows, landmark_xyz=np.ascontiguousarray(world),
+                      landmark_descriptor=lm_desc, landmark_observed=observed, keypoint_xy=np.ascontiguousarray(kp),
+                      keypoint_descriptor=np.ascontiguousarray(kp_desc), keypoint_key=key, key_taken=taken)
