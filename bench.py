@@ -19,6 +19,7 @@ throughput on a sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -360,6 +361,65 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_match(args) -> int:
+    """Projection search of a new frame against the map (zsba_match_by_projection,
+    the loop of slam.hpp:341-384): 200k landmarks (the loop-closure map's size)
+    against 2000 matched keypoints, one step = one search.  The metric counts the
+    landmark-keypoint tests the reference's loop makes; the roofline prices the 6
+    binary64 operations of each of those tests' distance (2 differences, 2 squares,
+    2 sums) against the FP64 vector peak over the device span of a call.  The
+    banded search executes only the tests of keypoints in the landmark's and the
+    neighbouring 8-px image bands (about 5% of them at this size), so the fraction
+    is that of a brute-force search reaching the same time."""
+    from zeroslam_amd.engine import Matcher
+    from zeroslam_amd.problem import match_scene
+
+    L, N = 200_000, 2000
+    scene = match_scene(L, N, seed=11)
+    m = Matcher()
+    print(f"bench: projection search {L} landmarks x {N} keypoints", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        m.match(scene)
+    dev_ms, wall = [], []
+    n = 0
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        _, _, n = m.match(scene)
+        wall.append(time.perf_counter() - t0)
+        dev_ms.append(m.stats()["device_ms"])
+    st = m.stats()
+    pairs = float(L) * N
+    dev_s = float(np.mean(dev_ms)) * 1e-3
+    achieved = 6.0 * pairs / dev_s / 1e12
+    out = {"metric": "projection-search landmark-keypoint tests per second", "value": pairs / dev_s,
+           "unit": "tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dev_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64+u32", "data": "synthetic",
+           "config": {"workload": f"projection search: {L} landmarks x {N} keypoints, 640x480 frame",
+                      "matches": int(n), "candidates": int(st["candidates"])},
+           "end_to_end_tests_per_s": pairs / float(np.mean(wall)),
+           "roofline": {"bound": "fp64-valu", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                        "note": "brute-force equivalent: 6 flops x every landmark-keypoint test of the reference; "
+                                "the banded search executes about 5% of the tests"}}
+    if not args.no_cpu:
+        from oracle.oracle import match_by_projection
+        sub = dataclasses.replace(scene, landmark_xyz=scene.landmark_xyz[:20000],
+                                  landmark_descriptor=scene.landmark_descriptor[:20000],
+                                  landmark_observed=scene.landmark_observed[:20000])
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            match_by_projection(sub)
+            reps += 1
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": reps * 20000.0 * N / dt, "unit": "tests/s", "cores": 1, "kind": "port",
+                               "sample": f"the first 20000 landmarks x {N} keypoints, {reps} searches in {dt:.1f} s, "
+                                         f"single-threaded on {cpu_model()}"}
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -372,6 +432,8 @@ def main() -> int:
                     help="time only --precision at the workload's iteration count (no FP64 comparison; profiling)")
     ap.add_argument("--cpu-iters", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--match", action="store_true",
+                    help="bench the projection search (data association) instead of the solver")
     ap.add_argument("--seeds", type=int, default=4,
                     help="repeat the equal-cost comparison on this many other instances of the workload")
     args = ap.parse_args()
@@ -382,6 +444,8 @@ def main() -> int:
 
     if args.config == 2:
         return bench_batch(args)
+    if args.match:
+        return bench_match(args)
 
     from zeroslam_amd.engine import FP64, FP64_FAST, MIXED, Graph
     from zeroslam_amd.problem import ITERATIONS, synth

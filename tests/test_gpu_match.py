@@ -75,3 +75,56 @@ def test_truncation_reuse_and_errors(matcher):
     bad.keypoint_key[3] = len(bad.key_taken)
     with pytest.raises(ZsbaError):
         matcher.match(bad)
+
+
+def test_banded_search_edge_inputs(matcher):
+    """Keypoints off the image (clamped to the first or last band), with infinite
+    or NaN coordinates (tested against every landmark), and landmarks at the
+    border rows."""
+    sc = match_scene(20000, 2000, seed=9)
+    rng = np.random.default_rng(9)
+    xy = sc.keypoint_xy.copy()
+    idx = rng.choice(len(xy), 60, replace=False)
+    xy[idx[:10], 1] = -rng.uniform(0, 50, 10)
+    xy[idx[10:20], 1] = sc.rows + rng.uniform(0, 50, 10)
+    xy[idx[20:25], 0] = np.inf
+    xy[idx[25:30], 1] = -np.inf
+    xy[idx[30:35], 0] = np.nan
+    xy[idx[35:40], 1] = np.nan
+    # keypoints 1 px outside the border rows next to landmarks on them
+    lm = sc.landmark_xyz.copy()
+    for r, (v_lm, v_kp) in enumerate([(10.0, 9.0), (sc.rows - 11.0, sc.rows - 10.0)]):
+        lm[r] = kat.at_pixel(300.0, v_lm, 5.0)
+        xy[idx[40 + r]] = [300.0, v_kp]
+        sc.keypoint_descriptor[idx[40 + r]] = sc.landmark_descriptor[r]
+    sc.keypoint_xy, sc.landmark_xyz = xy, lm
+    sc.landmark_observed = sc.landmark_observed.copy()
+    sc.landmark_observed[:2] = 0
+    sc.key_taken = sc.key_taken.copy()
+    sc.key_taken[sc.keypoint_key[idx[40:42]]] = 0
+    sc.rotation, sc.translation, sc.K = np.eye(3), np.zeros(3), np.array([kat.FX, kat.FX, kat.CX, kat.CY])
+    n = same(matcher, sc)
+    print(f"edge inputs: {n} matches", flush=True)
+    assert n > 0
+
+
+def test_two_pass_fallbacks(matcher):
+    """Inputs the banded search leaves to the ordered two-pass search: a landmark
+    whose projection is NaN, more candidates than one workgroup sorts, a frame
+    taller than the band table."""
+    sc = match_scene(3000, 1500, seed=10)
+    sc.landmark_xyz = sc.landmark_xyz.copy()
+    sc.landmark_xyz[5] = [np.nan, 0.0, 1.0]
+    sc.landmark_observed = sc.landmark_observed.copy()
+    sc.landmark_observed[5] = 0
+    assert same(matcher, sc) > 0
+    # 5000 landmarks on a 5-px grid, each with its own keypoint and descriptor
+    rng = np.random.default_rng(11)
+    gu, gv = np.meshgrid(np.arange(20.0, 620.0, 5.0), np.arange(20.0, 460.0, 5.0))
+    uv = np.stack([gu.ravel(), gv.ravel()], 1)[:5000]
+    desc = rng.integers(0, 256, (5000, 32), dtype=np.uint8)
+    dense = kat.scene([kat.at_pixel(u, v) for u, v in uv], uv, desc, desc, list(range(5000)))
+    assert same(matcher, dense) == 5000
+    assert matcher.stats()["candidates"] == 5000
+    tall = match_scene(3000, 1500, seed=12, rows=40000)
+    same(matcher, tall)

@@ -126,3 +126,17 @@ def test_synthetic_scene_invariants():
     # truncated output: the count stays, the first pairs are the same
     lm2, kp2, n2 = match_by_projection(sc, max_matches=10)
     assert n2 == n and np.array_equal(lm2, lm[:10]) and np.array_equal(kp2, kp[:10])
+
+
+def test_not_a_number_passes_the_distance_test(rng):
+    """The reference skips a keypoint when the squared distance is above 4; a NaN
+    distance is not, so such a keypoint is near every landmark, and a landmark
+    whose projection is NaN (it passes the border test too) is near every
+    keypoint; an infinite distance is above 4."""
+    d = rng.integers(0, 256, 32, dtype=np.uint8)
+    nan = float("nan")
+    sc = scene([kat_pt for kat_pt in (at_pixel(100, 100), at_pixel(400, 300))],
+               [[nan, 300], [float("inf"), 100], [400, 300]], [d, d], [d, d, d], [0, 1, 2])
+    assert matches(sc) == [(0, 0), (1, 2)]
+    sc = scene([[nan, 0, 1], at_pixel(400, 300)], [[50, 50], [400, 300]], [d, d], [d, d], [0, 1])
+    assert matches(sc) == [(0, 0), (0, 1)]

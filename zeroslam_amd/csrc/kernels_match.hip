@@ -9,7 +9,9 @@
 // within 63 bits of the landmark's first observation's descriptor, becomes an
 // observation; its previous-frame index is then taken.
 //
-// On the device the test is split in two exact halves:
+// On the device the test is split in two exact halves (section "banded search"
+// below gives the fast form of the first; the form described here is the one
+// that handles every input):
 //   1. every (landmark, keypoint) pair is tested independently against the
 //      indices taken before the walk: one thread per landmark, the keypoint
 //      positions staged through LDS in tiles shared by the workgroup, the
@@ -198,6 +200,230 @@ __global__ void __launch_bounds__(kMatchBlock) k_match_accept(int32_t nc, const 
 
 int32_t blocks_for(int64_t n, int block) { return static_cast<int32_t>((n + block - 1) / block); }
 
+// ---- banded search ------------------------------------------------------------
+// Keypoints and landmarks are bucketed by image row in bands of kBandPx pixels.
+// A keypoint passes the 2-pixel test only if its row differs from the
+// landmark's by at most 2 px (the rounded sum of squares is never below its
+// second term), so a landmark in band b needs the keypoints of bands b-1..b+1,
+// plus those with a coordinate that is not finite: a NaN distance passes the
+// test, as it does in the reference.  Rows outside the image are clamped to the
+// first or last band, where they are still tested wherever they could match.
+// Landmarks whose projection is not finite but passed the border test (NaN)
+// and candidate lists longer than one workgroup sorts are left to the ordered
+// two-pass search above, which handles every input.
+constexpr int kBandPx = 8;
+constexpr int kMaxBands = 4096;
+constexpr int kSortMax = 4096;
+
+__device__ __forceinline__ int row_band(double y, int nbands) {  // y finite
+    const double b = floor(y / kBandPx);
+    return b < 0.0 ? 0 : (b >= static_cast<double>(nbands - 1) ? nbands - 1 : static_cast<int>(b));
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_zero(int32_t n, int32_t* a) {
+    for (int32_t i = blockIdx.x * kMatchBlock + threadIdx.x; i < n; i += gridDim.x * kMatchBlock) a[i] = 0;
+}
+
+// One workgroup: keypoints sorted by band (a counting sort; order inside a band
+// does not matter), the last bin holding those with a coordinate that is not
+// finite.  bin_start has nbands + 2 entries.
+__global__ void __launch_bounds__(kScanBlock) k_match_bin_keypoints(int32_t nk, const double* __restrict__ kp_xy, int32_t nbands,
+                                                                   int32_t* __restrict__ bin_start, double2* __restrict__ kp_sorted,
+                                                                   int32_t* __restrict__ kp_sorted_id) {
+    __shared__ int32_t hist[kMaxBands + 1];
+    const int nbins = nbands + 1;
+    for (int b = threadIdx.x; b < nbins; b += kScanBlock) hist[b] = 0;
+    __syncthreads();
+    auto bin_of = [&](double x, double y) { return (isfinite(x) && isfinite(y)) ? row_band(y, nbands) : nbands; };
+    for (int32_t j = threadIdx.x; j < nk; j += kScanBlock) atomicAdd(&hist[bin_of(kp_xy[2L * j], kp_xy[2L * j + 1])], 1);
+    __syncthreads();
+    int32_t carry = 0;
+    for (int b0 = 0; b0 < nbins; b0 += kScanBlock) {
+        const int b = b0 + threadIdx.x;
+        const int32_t x = b < nbins ? hist[b] : 0;
+        int32_t total;
+        const int32_t ex = block_exclusive_scan(x, total);
+        __syncthreads();
+        if (b < nbins) {
+            hist[b] = carry + ex;
+            bin_start[b] = carry + ex;
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bin_start[nbins] = carry;
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < nk; j += kScanBlock) {
+        const double x = kp_xy[2L * j], y = kp_xy[2L * j + 1];
+        const int32_t pos = atomicAdd(&hist[bin_of(x, y)], 1);
+        kp_sorted[pos] = make_double2(x, y);
+        kp_sorted_id[pos] = j;
+    }
+}
+
+// Per landmark: the projection and border tests, then its band, or -1.  Each
+// workgroup counts its landmarks per band in LDS and writes the counts to
+// column blockIdx.x of the band-major table block_hist[band][workgroup], whose
+// exclusive scan gives every (band, workgroup) its range of the band order.
+// counters[0] counts landmarks that pass with a projection that is not finite.
+__global__ void __launch_bounds__(kMatchBlock) k_match_project(match_frame f, int32_t nl, const double* __restrict__ lm_xyz,
+                                                             const uint8_t* __restrict__ lm_observed, int32_t nbands,
+                                                             double2* __restrict__ lm_uv, int32_t* __restrict__ lm_band,
+                                                             int32_t* __restrict__ block_hist, int32_t* __restrict__ counters) {
+    __shared__ int32_t hist[kMaxBands];
+    for (int b = threadIdx.x; b < nbands; b += kMatchBlock) hist[b] = 0;
+    __syncthreads();
+    const int32_t o = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (o < nl) {
+        double u = 0.0, v = 0.0;
+        bool live = false;
+        if (!lm_observed[o]) {
+            const double X[3] = {lm_xyz[3L * o], lm_xyz[3L * o + 1], lm_xyz[3L * o + 2]};
+            live = project_landmark(f, X, u, v);
+        }
+        int32_t b = -1;
+        if (live) {
+            if (isfinite(u) && isfinite(v)) {
+                b = row_band(v, nbands);
+                atomicAdd(&hist[b], 1);
+            } else {
+                atomicAdd(&counters[0], 1);
+            }
+        }
+        lm_uv[o] = make_double2(u, v);
+        lm_band[o] = b;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbands; b += kMatchBlock)
+        block_hist[static_cast<int64_t>(b) * gridDim.x + blockIdx.x] = hist[b];
+}
+
+// Same grid as k_match_project: each landmark takes the next place in its
+// (band, workgroup) range.
+__global__ void __launch_bounds__(kMatchBlock) k_match_lm_scatter(int32_t nl, int32_t nbands, const int32_t* __restrict__ lm_band,
+                                                                const int32_t* __restrict__ range_start,
+                                                                int32_t* __restrict__ lm_sorted) {
+    __shared__ int32_t cursor[kMaxBands];
+    for (int b = threadIdx.x; b < nbands; b += kMatchBlock)
+        cursor[b] = range_start[static_cast<int64_t>(b) * gridDim.x + blockIdx.x];
+    __syncthreads();
+    const int32_t o = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (o >= nl) return;
+    const int32_t b = lm_band[o];
+    if (b >= 0) lm_sorted[atomicAdd(&cursor[b], 1)] = o;
+}
+
+// One workgroup per 256 landmarks in band order: the keypoints of the bands
+// from the first landmark's band - 1 to the last one's band + 1, then those not
+// finite, staged through LDS.  Candidates are appended as (landmark, keypoint)
+// walk positions; counters[1] counts them (only the first cap are stored).
+__global__ void __launch_bounds__(kMatchBlock)
+k_match_band_search(int32_t nbands, const int32_t* __restrict__ live_count, const int32_t* __restrict__ lm_sorted,
+                    const double2* __restrict__ lm_uv, const int32_t* __restrict__ lm_band, const uint32_t* __restrict__ lm_desc,
+                    const int32_t* __restrict__ bin_start, const double2* __restrict__ kp_sorted,
+                    const int32_t* __restrict__ kp_sorted_id, const uint32_t* __restrict__ kp_desc,
+                    const int32_t* __restrict__ kp_key, const uint8_t* __restrict__ key_taken, int32_t cap,
+                    unsigned long long* __restrict__ cand, int32_t* __restrict__ counters) {
+    __shared__ double2 tile[kMatchBlock];
+    __shared__ int32_t tile_id[kMatchBlock];
+    const int32_t n_live = *live_count;
+    const int32_t first = blockIdx.x * kMatchBlock;
+    if (first >= n_live) return;  // the whole workgroup
+    const int32_t idx = first + threadIdx.x;
+    const bool valid = idx < n_live;
+    const int32_t last = min(n_live, first + kMatchBlock) - 1;
+    const int b_lo = max(lm_band[lm_sorted[first]] - 1, 0);
+    const int b_hi = min(lm_band[lm_sorted[last]] + 1, nbands - 1);
+    int32_t o = 0;
+    double u = 0.0, v = 0.0;
+    uint32_t d[kDescWords];
+    if (valid) {
+        o = lm_sorted[idx];
+        const double2 p = lm_uv[o];
+        u = p.x;
+        v = p.y;
+    }
+#pragma unroll
+    for (int w = 0; w < kDescWords; ++w) d[w] = valid ? lm_desc[kDescWords * static_cast<int64_t>(o) + w] : 0u;
+    for (int r = 0; r < 2; ++r) {
+        const int32_t k0 = r == 0 ? bin_start[b_lo] : bin_start[nbands];
+        const int32_t k1 = r == 0 ? bin_start[b_hi + 1] : bin_start[nbands + 1];
+        for (int32_t j0 = k0; j0 < k1; j0 += kMatchBlock) {
+            __syncthreads();
+            const int32_t j = j0 + threadIdx.x;
+            if (j < k1) {
+                tile[threadIdx.x] = kp_sorted[j];
+                tile_id[threadIdx.x] = kp_sorted_id[j];
+            }
+            __syncthreads();
+            if (!valid) continue;
+            const int32_t jn = min(kMatchBlock, k1 - j0);
+            for (int32_t jj = 0; jj < jn; ++jj) {
+                const double2 p = tile[jj];
+                const double d0 = p.x - u, d1 = p.y - v;
+                double l2 = 0.0;
+                l2 = l2 + d0 * d0;
+                l2 = l2 + d1 * d1;
+                if (l2 > 4.0) continue;
+                const int32_t k = tile_id[jj];
+                if (key_taken[kp_key[k]]) continue;
+                uint32_t h = 0;
+#pragma unroll
+                for (int w = 0; w < kDescWords; ++w) h += __popc(d[w] ^ kp_desc[kDescWords * static_cast<int64_t>(k) + w]);
+                if (h >= 64u) continue;
+                const int32_t slot = atomicAdd(&counters[1], 1);
+                if (slot < cap) cand[slot] = walk_position(o, k);
+            }
+        }
+    }
+}
+
+// One workgroup: bitonic sort of the n = counters[1] <= kSortMax walk positions.
+__global__ void __launch_bounds__(kScanBlock) k_match_sort_small(unsigned long long* __restrict__ keys,
+                                                                const int32_t* __restrict__ counters) {
+    __shared__ unsigned long long sh[kSortMax];
+    const int32_t n = min(counters[1], kSortMax);
+    int32_t p = 1;
+    while (p < n) p <<= 1;
+    for (int32_t i = threadIdx.x; i < p; i += kScanBlock) sh[i] = i < n ? keys[i] : ~0ull;
+    __syncthreads();
+    for (int32_t k = 2; k <= p; k <<= 1) {
+        for (int32_t j = k >> 1; j > 0; j >>= 1) {
+            for (int32_t i = threadIdx.x; i < p; i += kScanBlock) {
+                const int32_t l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & k) == 0;
+                    const unsigned long long a = sh[i], b = sh[l];
+                    if ((a > b) == up) {
+                        sh[i] = b;
+                        sh[l] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int32_t i = threadIdx.x; i < n; i += kScanBlock) keys[i] = sh[i];
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_first_per_key_packed(int32_t nc, const unsigned long long* cand,
+                                                                          const int32_t* kp_key, unsigned long long* best) {
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= nc) return;
+    const unsigned long long w = cand[c];
+    atomicMin(&best[kp_key[static_cast<uint32_t>(w)]], w);
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_match_accept_packed(int32_t nc, const unsigned long long* cand,
+                                                                   const int32_t* kp_key, const unsigned long long* best,
+                                                                   uint8_t* accepted) {
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= nc) return;
+    const unsigned long long w = cand[c];
+    accepted[c] = best[kp_key[static_cast<uint32_t>(w)]] == w ? 1 : 0;
+}
+
+
 }  // namespace
 
 struct zsba_matcher {
@@ -211,6 +437,8 @@ struct zsba_matcher {
         size_t bytes = 0;
     };
     buf lm_xyz, lm_desc, lm_obs, kp_xy, kp_desc, kp_key, key_taken, count, offset, totals, best, cand_lm, cand_kp, accepted;
+    buf lm_uv, lm_band, lm_sorted, band_count, band_start, cursor, bin_start, kp_sorted, kp_sorted_id, counters, candw;
+    std::vector<unsigned long long> h_w;
     std::vector<int32_t> h_lm, h_kp;
     std::vector<uint8_t> h_acc;
     double last_ms = 0.0;
@@ -222,7 +450,8 @@ struct zsba_matcher {
             (void)hipStreamSynchronize(stream);
         }
         for (buf* b : {&lm_xyz, &lm_desc, &lm_obs, &kp_xy, &kp_desc, &kp_key, &key_taken, &count, &offset, &totals, &best,
-                       &cand_lm, &cand_kp, &accepted})
+                       &cand_lm, &cand_kp, &accepted, &lm_uv, &lm_band, &lm_sorted, &band_count, &band_start, &cursor,
+                       &bin_start, &kp_sorted, &kp_sorted_id, &counters, &candw})
             if (b->p) (void)hipFree(b->p);
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -330,6 +559,91 @@ int32_t zsba_match_by_projection(zsba_matcher* m, const zsba_match_frame* frame,
     const hipStream_t s = m->stream;
     (void)hipEventRecord(m->ev[0], s);
     const int32_t grid = blocks_for(nl, kMatchBlock);
+
+    // Banded search; the ordered two-pass search below takes the inputs it leaves.
+    const int64_t nbands64 = (frame->rows + kBandPx - 1) / kBandPx;
+    if (nbands64 >= 1 && nbands64 <= kMaxBands && nbands64 * grid < (int64_t(1) << 30)) {
+        const int32_t nbands = static_cast<int32_t>(nbands64);
+        const int64_t ncell = nbands64 * grid;  // (band, workgroup) table
+        const int32_t ncb = blocks_for(ncell, kScanBlock);
+        e = m->reserve(m->lm_uv, 16 * nl);
+        if (e == hipSuccess) e = m->reserve(m->lm_band, 4 * nl);
+        if (e == hipSuccess) e = m->reserve(m->lm_sorted, 4 * nl);
+        if (e == hipSuccess) e = m->reserve(m->band_count, 4 * static_cast<size_t>(ncell));
+        if (e == hipSuccess) e = m->reserve(m->band_start, 4 * static_cast<size_t>(ncell + 1));
+        if (e == hipSuccess) e = m->reserve(m->cursor, 4 * static_cast<size_t>(ncb));
+        if (e == hipSuccess) e = m->reserve(m->bin_start, 4 * static_cast<size_t>(nbands + 2));
+        if (e == hipSuccess) e = m->reserve(m->kp_sorted, 16 * nk);
+        if (e == hipSuccess) e = m->reserve(m->kp_sorted_id, 4 * nk);
+        if (e == hipSuccess) e = m->reserve(m->counters, 8);
+        if (e == hipSuccess) e = m->reserve(m->candw, 8 * static_cast<size_t>(kSortMax));
+        if (e == hipSuccess) e = m->reserve(m->accepted, kSortMax);
+        if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: band buffers");
+        int32_t* const cell_count = m->as<int32_t>(m->band_count);
+        int32_t* const cell_start = m->as<int32_t>(m->band_start);
+        k_match_zero<<<1, kMatchBlock, 0, s>>>(2, m->as<int32_t>(m->counters));
+        k_match_bin_keypoints<<<1, kScanBlock, 0, s>>>(n_keypoints, m->as<double>(m->kp_xy), nbands, m->as<int32_t>(m->bin_start),
+                                                       m->as<double2>(m->kp_sorted), m->as<int32_t>(m->kp_sorted_id));
+        k_match_project<<<grid, kMatchBlock, 0, s>>>(f, n_landmarks, m->as<double>(m->lm_xyz), m->as<uint8_t>(m->lm_obs), nbands,
+                                                     m->as<double2>(m->lm_uv), m->as<int32_t>(m->lm_band), cell_count,
+                                                     m->as<int32_t>(m->counters));
+        const int32_t ncell32 = static_cast<int32_t>(ncell);
+        k_match_block_totals<<<ncb, kScanBlock, 0, s>>>(ncell32, cell_count, m->as<int32_t>(m->cursor));
+        k_match_scan_totals<<<1, kScanBlock, 0, s>>>(ncb, m->as<int32_t>(m->cursor), ncell32, cell_start);
+        k_match_scan_apply<<<ncb, kScanBlock, 0, s>>>(ncell32, cell_count, m->as<int32_t>(m->cursor), cell_start);
+        k_match_lm_scatter<<<grid, kMatchBlock, 0, s>>>(n_landmarks, nbands, m->as<int32_t>(m->lm_band), cell_start,
+                                                        m->as<int32_t>(m->lm_sorted));
+        k_match_band_search<<<grid, kMatchBlock, 0, s>>>(
+            nbands, cell_start + ncell, m->as<int32_t>(m->lm_sorted), m->as<double2>(m->lm_uv),
+            m->as<int32_t>(m->lm_band), m->as<uint32_t>(m->lm_desc), m->as<int32_t>(m->bin_start), m->as<double2>(m->kp_sorted),
+            m->as<int32_t>(m->kp_sorted_id), m->as<uint32_t>(m->kp_desc), m->as<int32_t>(m->kp_key), m->as<uint8_t>(m->key_taken),
+            kSortMax, m->as<unsigned long long>(m->candw), m->as<int32_t>(m->counters));
+        int32_t cnt[2] = {0, 0};
+        e = hipMemcpyAsync(cnt, m->counters.p, sizeof cnt, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: banded search");
+        if (cnt[0] == 0 && cnt[1] <= kSortMax) {
+            const int32_t nc = cnt[1];
+            m->last_candidates = nc;
+            if (nc > 0) {
+                k_match_sort_small<<<1, kScanBlock, 0, s>>>(m->as<unsigned long long>(m->candw), m->as<int32_t>(m->counters));
+                k_match_best_fill<<<std::min(blocks_for(n_keys, kMatchBlock), 1024), kMatchBlock, 0, s>>>(
+                    n_keys, m->as<unsigned long long>(m->best));
+                const int32_t cgrid = blocks_for(nc, kMatchBlock);
+                k_match_first_per_key_packed<<<cgrid, kMatchBlock, 0, s>>>(nc, m->as<unsigned long long>(m->candw),
+                                                                          m->as<int32_t>(m->kp_key),
+                                                                          m->as<unsigned long long>(m->best));
+                k_match_accept_packed<<<cgrid, kMatchBlock, 0, s>>>(nc, m->as<unsigned long long>(m->candw),
+                                                                   m->as<int32_t>(m->kp_key), m->as<unsigned long long>(m->best),
+                                                                   m->as<uint8_t>(m->accepted));
+            }
+            (void)hipEventRecord(m->ev[1], s);
+            m->h_w.resize(nc);
+            m->h_acc.resize(nc);
+            e = hipSuccess;
+            if (nc > 0) {
+                e = hipMemcpyAsync(m->h_w.data(), m->candw.p, 8 * static_cast<size_t>(nc), hipMemcpyDeviceToHost, s);
+                if (e == hipSuccess) e = hipMemcpyAsync(m->h_acc.data(), m->accepted.p, nc, hipMemcpyDeviceToHost, s);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess) e = hipGetLastError();
+            if (e != hipSuccess) return m->hip_fail(e, "match_by_projection");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, m->ev[0], m->ev[1]);
+            m->last_ms = ms;
+            // sorted walk positions: the accepted ones are the matches, in order
+            int32_t n = 0;
+            for (int32_t c = 0; c < nc; ++c) {
+                if (!m->h_acc[c]) continue;
+                if (n < max_matches) {
+                    match_landmark[n] = static_cast<int32_t>(m->h_w[c] >> 32);
+                    match_keypoint[n] = static_cast<int32_t>(m->h_w[c] & 0xFFFFFFFFull);
+                }
+                ++n;
+            }
+            return n;
+        }
+    }
     k_match_scan_keypoints<false><<<grid, kMatchBlock, 0, s>>>(
         f, n_landmarks, m->as<double>(m->lm_xyz), m->as<uint32_t>(m->lm_desc), m->as<uint8_t>(m->lm_obs), n_keypoints,
         m->as<double>(m->kp_xy), m->as<uint32_t>(m->kp_desc), m->as<int32_t>(m->kp_key), m->as<uint8_t>(m->key_taken),
