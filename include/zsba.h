@@ -276,7 +276,8 @@ int32_t zsba_solve_batch(zsba_graph* const* graphs, int32_t count, int32_t itera
  * observation of landmark o with keypoint match_index_current[i] and marks
  * match_index_previous[i] taken.  At most max_matches are written; the return
  * value is the number of matches (it may exceed max_matches) or a negative
- * ZSBA_E* code.  A matcher keeps its device buffers between calls; calls on one
+ * ZSBA_E* code (ZSBA_EUNSUP for more than 2^31 - 1 landmark-keypoint pairs
+ * when a projection is NaN or there are more than 4096 candidates).  A matcher keeps its device buffers between calls; calls on one
  * matcher must not overlap. */
 typedef struct zsba_matcher zsba_matcher;
 typedef struct zsba_match_frame {

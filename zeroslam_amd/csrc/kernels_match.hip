@@ -602,7 +602,7 @@ int32_t zsba_match_by_projection(zsba_matcher* m, const zsba_match_frame* frame,
         e = hipMemcpyAsync(cnt, m->counters.p, sizeof cnt, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return m->hip_fail(e, "match_by_projection: banded search");
-        if (cnt[0] == 0 && cnt[1] <= kSortMax) {
+        if (cnt[0] == 0 && cnt[1] >= 0 && cnt[1] <= kSortMax) {
             const int32_t nc = cnt[1];
             m->last_candidates = nc;
             if (nc > 0) {
@@ -644,6 +644,10 @@ int32_t zsba_match_by_projection(zsba_matcher* m, const zsba_match_frame* frame,
             return n;
         }
     }
+    // Ordered two-pass search: the candidate offsets are 32-bit.
+    if (nl * nk > static_cast<int64_t>(INT32_MAX))
+        return m->fail(ZSBA_EUNSUP, "match_by_projection: %d landmarks x %d keypoints need the ordered search, "
+                       "whose candidate offsets are 32-bit", n_landmarks, n_keypoints);
     k_match_scan_keypoints<false><<<grid, kMatchBlock, 0, s>>>(
         f, n_landmarks, m->as<double>(m->lm_xyz), m->as<uint32_t>(m->lm_desc), m->as<uint8_t>(m->lm_obs), n_keypoints,
         m->as<double>(m->kp_xy), m->as<uint32_t>(m->kp_desc), m->as<int32_t>(m->kp_key), m->as<uint8_t>(m->key_taken),
