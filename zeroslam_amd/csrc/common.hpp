@@ -14,6 +14,21 @@
 
 namespace zb {
 
+// The workgroups of a launch are dealt to the 8 XCDs in turn (workgroup i to
+// XCD i mod 8), and each XCD has its own L2.  This renumbering gives each XCD
+// chunks of kChunk consecutive logical workgroups, the chunks dealt in turn, so
+// neighbouring work items, which share operands, share an L2 while the chunks
+// keep the XCDs evenly loaded.  It is a bijection of [0, n) (the tail that
+// does not fill a round of chunks keeps its numbers): the same work is done,
+// by other workgroups.
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+    constexpr int kXcds = 8, kChunk = 16;
+    const int full = n / (kXcds * kChunk) * (kXcds * kChunk);
+    if (bid >= full) return bid;
+    const int x = bid % kXcds, i = bid / kXcds;
+    return ((i / kChunk) * kXcds + x) * kChunk + i % kChunk;
+}
+
 // Per-kernel-class device time from HIP events recorded around launches on the
 // launching stream (enabled by zsba_options.timing).
 struct kernel_timer {

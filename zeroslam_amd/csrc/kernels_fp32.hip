@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_schur_f(dev_problem p, const float* Y, 
                                                  const double* bp, const double* bl, double* S, double* bs,
                                                  const lm_state* lm, const double* lambda_override) {
     if (!lambda_override && lm->done) return;
-    const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int blk = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     if (blk < p.nblk)
         mixed::schur_block(p, Y, Wbuf, Hpp, bp, bl, S, bs, lambda_override ? *lambda_override : lm->lambda, blk,
                            threadIdx.x & 63);

@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, c
                                                const double* bp, const double* bl, double* S, double* bs,
                                                const lm_state* lm, const double* lambda_override) {
     if (!lambda_override && lm->done) return;
-    const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int blk = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (blk >= p.nblk) return;
     const int row = p.blk_row[blk], col = p.blk_col[blk];
@@ -497,7 +497,7 @@ __device__ __forceinline__ int order_set(const lm_state* lm) { return lm->lin_co
 __global__ void __launch_bounds__(256) k_schur_exact(dev_problem p, const double* Y, const double* Wbuf,
                                                      const double* Hpp, double* S, const lm_state* lm) {
     if (lm->done) return;
-    const int blk = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int blk = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (blk >= p.nblk || lane >= 36) return;
     const int row = p.blk_row[blk], col = p.blk_col[blk];
