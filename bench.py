@@ -210,7 +210,7 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_mixed_hbm_traffic.md")
 GROUP_KERNELS = {
     "cholesky": ["k_chol_begin", "k_chol_fused<true>"],
     "linearisation": ["k_lin_points_f", "k_reduce_pose_part_f", "k_reduce_pose_f"],
-    "schur": ["k_damp_invert_f", "k_form_Y_f", "k_fill_zero_d", "k_schur_f"],
+    "schur": ["k_damp_invert_f", "k_form_Y_f", "k_fill_zero_d", "k_schur_f<2>"],
     "cost": ["k_cost"],
     "backsub": ["k_backsub_update<float>"],
 }

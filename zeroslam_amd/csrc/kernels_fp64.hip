@@ -404,6 +404,11 @@ __device__ __forceinline__ double wave_sum(double v) {
 //   S_ik = [i == k] (H_pp,i + lambda I) - sum_l Y_il W_kl^T
 //   b_s,i = b_p,i - sum_l Y_il b_l
 // (factor_graph.hpp:768-781 with matrix_sparse_block.hpp:158-196).
+// Each lane adds its contributions c, c + 64, ... in order, kInFlight of them
+// loaded before they are added; each product entry is formed from zero in
+// ascending k and added to the lane's sum, so kInFlight changes the registers
+// in use, not the result.
+template <int kInFlight>
 __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, const double* Wbuf, const double* Hpp,
                                                const double* bp, const double* bl, double* S, double* bs,
                                                const lm_state* lm, const double* lambda_override) {
@@ -418,7 +423,6 @@ __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, c
     for (int i = 0; i < 36; ++i) acc[i] = 0.0;
     double accb[6] = {0, 0, 0, 0, 0, 0};
     const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
-    // contributions c, c + 64 of this lane: two in flight (loads first), added in order
     auto fetch = [&](int64_t c, double (&Ya)[18], double (&Wc)[18], double (&b3)[3]) {
         const int2 ab = p.contrib[c];
 #pragma unroll
@@ -434,27 +438,38 @@ __global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, c
         }
     };
     auto add = [&](const double (&Ya)[18], const double (&Wc)[18], const double (&b3)[3]) {
-        double lp[18], prod[36];
-        transpose<6, 3>(Wc, lp);
-        mul<6, 3, 6>(Ya, lp, prod);
 #pragma unroll
-        for (int i = 0; i < 36; ++i) acc[i] = acc[i] + prod[i];
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 6; ++cc) {
+                double t = 0.0;
+                t = t + Ya[r * 3] * Wc[cc * 3];
+                t = t + Ya[r * 3 + 1] * Wc[cc * 3 + 1];
+                t = t + Ya[r * 3 + 2] * Wc[cc * 3 + 2];
+                acc[r * 6 + cc] = acc[r * 6 + cc] + t;
+            }
         if (diag) {
-            double yb[6];
-            mul<6, 3, 1>(Ya, b3, yb);
 #pragma unroll
-            for (int i = 0; i < 6; ++i) accb[i] = accb[i] + yb[i];
+            for (int r = 0; r < 6; ++r) {
+                double t = 0.0;
+                t = t + Ya[r * 3] * b3[0];
+                t = t + Ya[r * 3 + 1] * b3[1];
+                t = t + Ya[r * 3 + 2] * b3[2];
+                accb[r] = accb[r] + t;
+            }
         }
     };
     int64_t c = c0 + lane;
-    for (; c + 64 < c1; c += 128) {
-        double Ya0[18], Wc0[18], Ya1[18], Wc1[18], b0[3], b1[3];
-        fetch(c, Ya0, Wc0, b0);
-        fetch(c + 64, Ya1, Wc1, b1);
-        add(Ya0, Wc0, b0);
-        add(Ya1, Wc1, b1);
+    if (kInFlight == 2) {
+        for (; c + 64 < c1; c += 128) {
+            double Ya0[18], Wc0[18], Ya1[18], Wc1[18], b0[3], b1[3];
+            fetch(c, Ya0, Wc0, b0);
+            fetch(c + 64, Ya1, Wc1, b1);
+            add(Ya0, Wc0, b0);
+            add(Ya1, Wc1, b1);
+        }
     }
-    if (c < c1) {
+    for (; c < c1; c += 64) {
         double Ya0[18], Wc0[18], b0[3];
         fetch(c, Ya0, Wc0, b0);
         add(Ya0, Wc0, b0);
@@ -819,6 +834,10 @@ __global__ void k_math_probe(int fn, int n, const double* in, double* out) {
 }
 
 inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
+
+// Below this many workgroups k_schur keeps two contributions in flight per
+// lane (more latency hidden); above it, one (higher occupancy).
+constexpr int kSchurSmallGrid = 2048;
 inline int cost_grid(const dev_problem& p) {
     const int g = grid_for(p.no, kBlock);
     return g < kPartials ? (g > 0 ? g : 1) : kPartials;
@@ -936,7 +955,11 @@ void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStrea
             k_schur_exact<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.S, w.lm);
             k_bs_exact<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.Y, w.bp, w.bl, w.bs, w.lm);
         } else {
-            k_schur<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+            if (grid_for(64L * p.nblk, 256) < kSchurSmallGrid)
+                k_schur<2><<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+                                                                lambda_override);
+            else
+                k_schur<1><<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
                                                                 lambda_override);
         }
         ZB_TIME_END(w, s);
