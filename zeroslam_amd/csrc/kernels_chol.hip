@@ -441,13 +441,13 @@ __global__ void __launch_bounds__(64) k_backward_exact(int n, const double* L, c
 }
 
 // ---- blocked substitution for large systems (reassociating precisions) ------
-// Blocks of kSolveBlockTiles row tiles.  Forward: per block, one workgroup solves
-// the diagonal block with the pipelined substitution (its unknowns already hold
-// every earlier block's terms), then a grid-wide launch subtracts the block's
-// terms from every later row of the envelope, one wave per row.  Backward: per
-// block from the last, one workgroup solves the transposed diagonal block, then a
-// grid-wide launch subtracts its terms from every earlier unknown the block's rows
-// reach, one column per lane.
+// Blocks of kSolveBlockTiles row tiles.  Forward: per block, one workgroup
+// multiplies the block's unknowns (which already hold every earlier block's
+// terms) by the inverse of the diagonal block, formed once per factorisation;
+// then a grid-wide launch subtracts the block's terms from every later row of
+// the envelope, one wave per row.  Backward: per block from the last, the
+// product with the transposed inverse, then a grid-wide launch subtracts its
+// terms from every earlier unknown the block's rows reach, one column per lane.
 // The L tiles a solve reads never leave the envelope; every sum has a fixed
 // order.  Failure rule as in the one-workgroup solve (|L_ii| > 1e-7 max |L_ii|).
 constexpr int kSolveBlockTiles = 8;
@@ -481,36 +481,6 @@ __global__ void __launch_bounds__(1024) k_trsv_begin(int n, const double* L, con
     }
 }
 
-// Diagonal block p (tiles t0 .. t1 - 1): one pass of the pipelined substitution
-// on the block, its envelope clipped to the block (the earlier or later
-// blocks' terms are already in x).  512 threads.
-__device__ __forceinline__ void solve_diag_block(int n, const double* L, double* x, int p, const int32_t* tile_first,
-                                                 const int32_t* tile_last, int passes) {
-    __shared__ double sh[512];
-    __shared__ double buf[6][NB][NB + 1];
-    __shared__ int32_t tf[kSolveBlockTiles], tl[kSolveBlockTiles];
-    __shared__ int flag;
-    const int T = (n + NB - 1) / NB, t0 = p * kSolveBlockTiles;
-    const int t1 = (t0 + kSolveBlockTiles) < T ? (t0 + kSolveBlockTiles) : T;
-    if (threadIdx.x < static_cast<unsigned>(t1 - t0)) {
-        const int f = tile_first[t0 + threadIdx.x] - t0, l = tile_last[t0 + threadIdx.x];
-        tf[threadIdx.x] = f > 0 ? f : 0;
-        tl[threadIdx.x] = (l < t1 - 1 ? l : t1 - 1) - t0;
-    }
-    __syncthreads();
-    const int r0 = t0 * NB, m = ((t1 * NB) < n ? (t1 * NB) : n) - r0;
-    double* xb = x + r0;
-    wg::triangular_solve_fast_ld<512>(m, n, L + static_cast<int64_t>(r0) * n + r0, xb, xb, tf, tl, buf, sh, &flag,
-                                      passes, false);
-}
-
-__global__ void __launch_bounds__(512) k_trsv_block_fwd(int n, const double* L, double* x, int p,
-                                                        const int32_t* tile_first, const int32_t* tile_last,
-                                                        const int32_t* status, const lm_state* lm, int gated) {
-    if ((gated && lm->done) || *status != 0) return;
-    solve_diag_block(n, L, x, p, tile_first, tile_last, 1);
-}
-
 // Rows below block p: x_i -= L_i,block x_block over the row's envelope, one wave
 // per row (lane-strided products, then a butterfly sum).
 __global__ void __launch_bounds__(256) k_trsv_rows_update(int n, const double* L, double* x, int p,
@@ -532,14 +502,6 @@ __global__ void __launch_bounds__(256) k_trsv_rows_update(int n, const double* L
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off);
     if (lane == 0) x[i] -= acc;
-}
-
-__global__ void __launch_bounds__(512) k_trsv_block_bwd(int n, const double* L, double* x, int p,
-                                                        const int32_t* tile_first, const int32_t* tile_last,
-                                                        const int32_t* status, lm_state* lm, int gated) {
-    if ((gated && lm->done) || *status != 0) return;
-    solve_diag_block(n, L, x, p, tile_first, tile_last, 2);
-    if (p == 0 && threadIdx.x == 0) lm->solve_ok = 1;  // the solve's last launch
 }
 
 // Unknowns left of block p: x_j -= sum_{i in block} L_ij x_i over the rows whose
@@ -577,19 +539,132 @@ __global__ void __launch_bounds__(512) k_trsv_cols_update(int n, const double* L
                 ((red[4][lane] + red[5][lane]) + (red[6][lane] + red[7][lane]));
 }
 
+// Inverses of the diagonal blocks, formed once per factorisation so that the
+// diagonal steps of every solve with that factor (the step and the mixed
+// path's refinement corrections) are matrix-vector products instead of
+// dependent substitutions.  Block p (rows r0 .. r0 + m - 1, m <= 256) gets a
+// 256 x 256 slot holding inv(L_bb) column-major (slot p) and row-major (slot
+// P + p), zeros above the diagonal, so that both products read coalesced
+// columns.  One wave per 32-column chunk: column j of inv(L_bb) by forward
+// substitution of e_j, the chunk's columns kept in LDS, the two halves of the
+// wave summing alternate terms (eight loads in flight), added in a fixed order.
+constexpr int kInvRows = kSolveBlockTiles * NB;
+constexpr int kInvCols = 32;
+constexpr int64_t kInvSlot = static_cast<int64_t>(kInvRows) * kInvRows;
+
+__global__ void __launch_bounds__(64) k_trsv_block_inverse(int n, const double* L, double* Linv, int P,
+                                                           const int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || *status != 0) return;
+    __shared__ double ys[kInvRows * kInvCols];
+    const int p = blockIdx.x / (kInvRows / kInvCols), chunk = blockIdx.x % (kInvRows / kInvCols);
+    const int r0 = p * kInvRows;
+    const int m = (n - r0) < kInvRows ? (n - r0) : kInvRows;
+    const int c0 = chunk * kInvCols;
+    const int col = threadIdx.x & (kInvCols - 1), half = threadIdx.x >> 5;
+    const int j = c0 + col;
+    double* Bc = Linv + p * kInvSlot;        // column-major: Bc[j * 256 + i] = inv[i][j]
+    double* Br = Linv + (P + p) * kInvSlot;  // row-major:    Br[i * 256 + j] = inv[i][j]
+    auto put = [&](int i, double y) {
+        Bc[j * kInvRows + i] = y;
+        Br[i * kInvRows + j] = y;
+    };
+    for (int i = half; i < c0; i += 2) put(i, 0.0);
+    if (c0 >= m) {
+        for (int i = c0 + half; i < kInvRows; i += 2) put(i, 0.0);
+        return;
+    }
+    const double* Lb = L + static_cast<int64_t>(r0) * n + r0;
+    for (int i = c0; i < m; ++i) {
+        const double* Li = Lb + static_cast<int64_t>(i) * n;
+        double acc = 0.0;
+        int k = c0 + half;
+        for (; k + 14 < i; k += 16) {
+            double l[8], yv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                l[u] = Li[k + 2 * u];
+                yv[u] = ys[(k + 2 * u - c0) * kInvCols + col];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = fma(l[u], yv[u], acc);
+        }
+        for (; k < i; k += 2) acc = fma(Li[k], ys[(k - c0) * kInvCols + col], acc);
+        const double other = __shfl_xor(acc, 32);
+        const double sum = half == 0 ? acc + other : other + acc;
+        double y;
+        if (j > i || j >= m) y = 0.0;
+        else if (j == i) y = 1.0 / Li[i];
+        else y = -sum / Li[i];
+        // both halves hold the same y (the same two partial sums, added in the
+        // same order) and each reads back only its own store
+        ys[(i - c0) * kInvCols + col] = y;
+        if (half == 0) put(i, y);
+    }
+    for (int i = m + half; i < kInvRows; i += 2) put(i, 0.0);
+}
+
+// out_r = sum_q M[q][r] v_q for r < m: thread pair per r
+// (alternate q, eight loads in flight), M column-major in a 256 x 256 slot so
+// that the threads of a wave read consecutive addresses; the two partial sums
+// added in a fixed order.  Forward (lower): M = inv(L_bb) column-major, q <= r;
+// backward (upper): M = inv(L_bb) row-major, i.e. inv(L_bb)^T column-major, all q.
+template <bool kForward>
+__device__ __forceinline__ void block_inverse_product(int m, const double* M, double* x, double* xs, double* part) {
+    for (int i = threadIdx.x; i < m; i += 512) xs[i] = x[i];
+    __syncthreads();
+    const int r = threadIdx.x & (kInvRows - 1), half = threadIdx.x / kInvRows;
+    double acc = 0.0;
+    if (r < m) {
+        // backward: from q = 0 too (inv[q][r] = 0 for q < r), so that the
+        // threads of a wave read the same row of the slot at each step
+        const int qb = half, qe = kForward ? r + 1 : m;
+        int q = qb;
+        for (; q + 14 < qe; q += 16) {
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = M[(q + 2 * u) * kInvRows + r];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = fma(a[u], xs[q + 2 * u], acc);
+        }
+        for (; q < qe; q += 2) acc = fma(M[q * kInvRows + r], xs[q], acc);
+    }
+    if (half == 1) part[r] = acc;
+    __syncthreads();
+    if (half == 0 && r < m) x[r] = acc + part[r];
+}
+
+__global__ void __launch_bounds__(512) k_trsv_block_fwd_inv(int n, const double* Linv, double* x, int p,
+                                                            const int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || *status != 0) return;
+    __shared__ double xs[kInvRows], part[kInvRows];
+    const int r0 = p * kInvRows;
+    const int m = (n - r0) < kInvRows ? (n - r0) : kInvRows;
+    block_inverse_product<true>(m, Linv + p * kInvSlot, x + r0, xs, part);
+}
+
+__global__ void __launch_bounds__(512) k_trsv_block_bwd_inv(int n, const double* Linv, double* x, int p, int P,
+                                                            const int32_t* status, lm_state* lm, int gated) {
+    if ((gated && lm->done) || *status != 0) return;
+    __shared__ double xs[kInvRows], part[kInvRows];
+    const int r0 = p * kInvRows;
+    const int m = (n - r0) < kInvRows ? (n - r0) : kInvRows;
+    block_inverse_product<false>(m, Linv + (P + p) * kInvSlot, x + r0, xs, part);
+    if (p == 0 && threadIdx.x == 0) lm->solve_ok = 1;  // the solve's last launch
+}
+
 void launch_trsv_blocked(int n, const double* L, const double* b, double* x, const dev_work64& w, hipStream_t s,
                          int gated) {
     const int T = (n + NB - 1) / NB;
     const int P = (T + kSolveBlockTiles - 1) / kSolveBlockTiles;
     k_trsv_begin<<<1, 1024, 0, s>>>(n, L, b, x, w.chol_status, w.lm, gated);
     for (int p = 0; p < P; ++p) {
-        k_trsv_block_fwd<<<1, 512, 0, s>>>(n, L, x, p, w.tile_first, w.tile_last, w.chol_status, w.lm, gated);
+        k_trsv_block_fwd_inv<<<1, 512, 0, s>>>(n, w.Lt, x, p, w.chol_status, w.lm, gated);
         const int c1 = ((p + 1) * kSolveBlockTiles * NB) < n ? ((p + 1) * kSolveBlockTiles * NB) : n;
         if (c1 < n)
             k_trsv_rows_update<<<(n - c1 + 3) / 4, 256, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
     }
     for (int p = P - 1; p >= 0; --p) {
-        k_trsv_block_bwd<<<1, 512, 0, s>>>(n, L, x, p, w.tile_first, w.tile_last, w.chol_status, w.lm, gated);
+        k_trsv_block_bwd_inv<<<1, 512, 0, s>>>(n, w.Lt, x, p, P, w.chol_status, w.lm, gated);
         const int c0 = p * kSolveBlockTiles * NB;
         if (c0 > 0)
             k_trsv_cols_update<<<(c0 + 63) / 64, 512, 0, s>>>(n, L, x, p, w.tile_first, w.chol_status, w.lm, gated);
@@ -689,6 +764,11 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
                 k_chol_syrk<<<static_cast<unsigned>(m * (m + 1) / 2), 256, 0, s>>>(
                     n, w.S, static_cast<int>(kb0), static_cast<int>(kb1), w.tile_first, w.chol_status, w.lm, g);
         }
+    }
+    if (!exact && tiles > kBlockedSolveTiles) {
+        const int64_t P = (tiles + kSolveBlockTiles - 1) / kSolveBlockTiles;
+        k_trsv_block_inverse<<<static_cast<unsigned>(P * (kInvRows / kInvCols)), 64, 0, s>>>(
+            n, w.S, w.Lt, static_cast<int>(P), w.chol_status, w.lm, g);
     }
     ZB_TIME_END(w, s);
     ZB_TIME_BEGIN(w, K_TRSV, s);
