@@ -545,11 +545,13 @@ __global__ void __launch_bounds__(512) k_trsv_cols_update(int n, const double* L
 // dependent substitutions.  Block p (rows r0 .. r0 + m - 1, m <= 256) gets a
 // 256 x 256 slot holding inv(L_bb) column-major (slot p) and row-major (slot
 // P + p), zeros above the diagonal, so that both products read coalesced
-// columns.  One wave per 32-column chunk: column j of inv(L_bb) by forward
-// substitution of e_j, the chunk's columns kept in LDS, the two halves of the
-// wave summing alternate terms (eight loads in flight), added in a fixed order.
+// columns.  One wave per 8-column chunk: column j of inv(L_bb) by forward
+// substitution of e_j, eight lanes per column each summing every eighth term
+// (four loads in flight), the eight partial sums combined by a butterfly (the
+// same total on every lane of the column); the chunk's columns kept in LDS.
 constexpr int kInvRows = kSolveBlockTiles * NB;
-constexpr int kInvCols = 32;
+constexpr int kInvCols = 8;
+constexpr int kInvLanes = 64 / kInvCols;  // lanes per column
 constexpr int64_t kInvSlot = static_cast<int64_t>(kInvRows) * kInvRows;
 
 __global__ void __launch_bounds__(64) k_trsv_block_inverse(int n, const double* L, double* Linv, int P,
@@ -560,7 +562,7 @@ __global__ void __launch_bounds__(64) k_trsv_block_inverse(int n, const double* 
     const int r0 = p * kInvRows;
     const int m = (n - r0) < kInvRows ? (n - r0) : kInvRows;
     const int c0 = chunk * kInvCols;
-    const int col = threadIdx.x & (kInvCols - 1), half = threadIdx.x >> 5;
+    const int col = threadIdx.x % kInvCols, sub = threadIdx.x / kInvCols;
     const int j = c0 + col;
     double* Bc = Linv + p * kInvSlot;        // column-major: Bc[j * 256 + i] = inv[i][j]
     double* Br = Linv + (P + p) * kInvSlot;  // row-major:    Br[i * 256 + j] = inv[i][j]
@@ -568,39 +570,39 @@ __global__ void __launch_bounds__(64) k_trsv_block_inverse(int n, const double* 
         Bc[j * kInvRows + i] = y;
         Br[i * kInvRows + j] = y;
     };
-    for (int i = half; i < c0; i += 2) put(i, 0.0);
+    for (int i = sub; i < c0; i += kInvLanes) put(i, 0.0);
     if (c0 >= m) {
-        for (int i = c0 + half; i < kInvRows; i += 2) put(i, 0.0);
+        for (int i = c0 + sub; i < kInvRows; i += kInvLanes) put(i, 0.0);
         return;
     }
     const double* Lb = L + static_cast<int64_t>(r0) * n + r0;
     for (int i = c0; i < m; ++i) {
         const double* Li = Lb + static_cast<int64_t>(i) * n;
         double acc = 0.0;
-        int k = c0 + half;
-        for (; k + 14 < i; k += 16) {
-            double l[8], yv[8];
+        int k = c0 + sub;
+        constexpr int S = kInvLanes;
+        for (; k + 3 * S < i; k += 4 * S) {
+            double l[4], yv[4];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                l[u] = Li[k + 2 * u];
-                yv[u] = ys[(k + 2 * u - c0) * kInvCols + col];
+            for (int u = 0; u < 4; ++u) {
+                l[u] = Li[k + u * S];
+                yv[u] = ys[(k + u * S - c0) * kInvCols + col];
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc = fma(l[u], yv[u], acc);
+            for (int u = 0; u < 4; ++u) acc = fma(l[u], yv[u], acc);
         }
-        for (; k < i; k += 2) acc = fma(Li[k], ys[(k - c0) * kInvCols + col], acc);
-        const double other = __shfl_xor(acc, 32);
-        const double sum = half == 0 ? acc + other : other + acc;
+        for (; k < i; k += S) acc = fma(Li[k], ys[(k - c0) * kInvCols + col], acc);
+#pragma unroll
+        for (int off = kInvCols; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
         double y;
         if (j > i || j >= m) y = 0.0;
         else if (j == i) y = 1.0 / Li[i];
-        else y = -sum / Li[i];
-        // both halves hold the same y (the same two partial sums, added in the
-        // same order) and each reads back only its own store
+        else y = -acc / Li[i];
+        // every lane of the column holds the same y and reads back only its own store
         ys[(i - c0) * kInvCols + col] = y;
-        if (half == 0) put(i, y);
+        if (sub == 0) put(i, y);
     }
-    for (int i = m + half; i < kInvRows; i += 2) put(i, 0.0);
+    for (int i = m + sub; i < kInvRows; i += kInvLanes) put(i, 0.0);
 }
 
 // out_r = sum_q M[q][r] v_q for r < m: thread pair per r
