@@ -543,66 +543,124 @@ __global__ void __launch_bounds__(512) k_trsv_cols_update(int n, const double* L
 // diagonal steps of every solve with that factor (the step and the mixed
 // path's refinement corrections) are matrix-vector products instead of
 // dependent substitutions.  Block p (rows r0 .. r0 + m - 1, m <= 256) gets a
-// 256 x 256 slot holding inv(L_bb) column-major (slot p) and row-major (slot
-// P + p), zeros above the diagonal, so that both products read coalesced
-// columns.  One wave per 8-column chunk: column j of inv(L_bb) by forward
-// substitution of e_j, eight lanes per column each summing every eighth term
-// (four loads in flight), the eight partial sums combined by a butterfly (the
-// same total on every lane of the column); the chunk's columns kept in LDS.
+// 256 x 256 slot holding X = inv(L_bb) column-major (slot p) and row-major
+// (slot P + p), zeros above the diagonal, so that both products read
+// coalesced columns.  X is formed by 32-row tiles: the tiles' own inverses
+// X_ii = inv(L_ii), then down each tile column j
+//   X_ij = -X_ii sum_{k = j}^{i-1} L_ik X_kj   (i > j),
+// a chain of at most 8 tile steps.  Entries whose row or column is m or more
+// are never read by the products.
 constexpr int kInvRows = kSolveBlockTiles * NB;
-constexpr int kInvCols = 8;
-constexpr int kInvLanes = 64 / kInvCols;  // lanes per column
 constexpr int64_t kInvSlot = static_cast<int64_t>(kInvRows) * kInvRows;
 
-__global__ void __launch_bounds__(64) k_trsv_block_inverse(int n, const double* L, double* Linv, int P,
-                                                           const int32_t* status, const lm_state* lm, int gated) {
+// One wave per diagonal tile: column c of inv(L_ii) by forward substitution
+// of e_c, two lanes per column summing alternate terms, the tile's columns in
+// LDS (each lane reads back only its own stores: both lanes of a column hold
+// the same value).
+__global__ void __launch_bounds__(64) k_inv_diag_tiles(int n, const double* L, double* Linv, int P, const int32_t* status,
+                                                       const lm_state* lm, int gated) {
     if ((gated && lm->done) || *status != 0) return;
-    __shared__ double ys[kInvRows * kInvCols];
-    const int p = blockIdx.x / (kInvRows / kInvCols), chunk = blockIdx.x % (kInvRows / kInvCols);
+    __shared__ double ys[NB * NB];
+    const int p = blockIdx.x / kSolveBlockTiles, t = blockIdx.x % kSolveBlockTiles;
+    const int R = p * kInvRows + t * NB;  // first row of the tile in the system
+    if (R >= n) return;
+    const int mt = (n - R) < NB ? (n - R) : NB;
+    const int col = threadIdx.x & (NB - 1), half = threadIdx.x >> 5;
+    double* Bc = Linv + p * kInvSlot;
+    double* Br = Linv + (P + p) * kInvSlot;
+    const double* Lt = L + static_cast<int64_t>(R) * n + R;
+    for (int i = 0; i < NB; ++i) {
+        double acc = 0.0;
+        if (i < mt) {
+            const double* Li = Lt + static_cast<int64_t>(i) * n;
+            for (int k = half; k < i; k += 2) acc = fma(Li[k], ys[k * NB + col], acc);
+        }
+        const double other = __shfl_xor(acc, 32);
+        const double sum = half == 0 ? acc + other : other + acc;
+        double y = 0.0;
+        if (i < mt && col < mt && col <= i) {
+            const double d = Lt[static_cast<int64_t>(i) * n + i];
+            y = col == i ? 1.0 / d : -sum / d;
+        }
+        ys[i * NB + col] = y;
+        if (half == 0) {
+            const int gi = t * NB + i, gj = t * NB + col;
+            Bc[gj * kInvRows + gi] = y;
+            Br[gi * kInvRows + gj] = y;
+        }
+    }
+}
+
+// One workgroup (256 threads) per tile column j of a block: the tiles above the
+// diagonal set to zero, then X_ij for i = j + 1 .. in order.  Thread (r, c4)
+// owns row r, columns c4 .. c4 + 3 of the tile being formed.  X_jj is read from
+// the row-major slot (formed by the previous launch), the tiles formed here
+// from LDS.
+constexpr int kInvKeep = kSolveBlockTiles - 2;  // tiles X_ij kept in LDS for later steps
+
+__global__ void __launch_bounds__(256) k_inv_offdiag_tiles(int n, const double* L, double* Linv, int P,
+                                                          const int32_t* status, const lm_state* lm, int gated) {
+    if ((gated && lm->done) || *status != 0) return;
+    __shared__ double xs[kInvKeep][NB][NB];
+    __shared__ double ts[NB][NB + 1];
+    const int p = blockIdx.x / kSolveBlockTiles, j = blockIdx.x % kSolveBlockTiles;
     const int r0 = p * kInvRows;
     const int m = (n - r0) < kInvRows ? (n - r0) : kInvRows;
-    const int c0 = chunk * kInvCols;
-    const int col = threadIdx.x % kInvCols, sub = threadIdx.x / kInvCols;
-    const int j = c0 + col;
-    double* Bc = Linv + p * kInvSlot;        // column-major: Bc[j * 256 + i] = inv[i][j]
-    double* Br = Linv + (P + p) * kInvSlot;  // row-major:    Br[i * 256 + j] = inv[i][j]
-    auto put = [&](int i, double y) {
-        Bc[j * kInvRows + i] = y;
-        Br[i * kInvRows + j] = y;
-    };
-    for (int i = sub; i < c0; i += kInvLanes) put(i, 0.0);
-    if (c0 >= m) {
-        for (int i = c0 + sub; i < kInvRows; i += kInvLanes) put(i, 0.0);
-        return;
-    }
-    const double* Lb = L + static_cast<int64_t>(r0) * n + r0;
-    for (int i = c0; i < m; ++i) {
-        const double* Li = Lb + static_cast<int64_t>(i) * n;
-        double acc = 0.0;
-        int k = c0 + sub;
-        constexpr int S = kInvLanes;
-        for (; k + 3 * S < i; k += 4 * S) {
-            double l[4], yv[4];
+    const int nt = (m + NB - 1) / NB;
+    if (j >= nt) return;
+    const int r = threadIdx.x >> 3, c4 = (threadIdx.x & 7) * 4;
+    double* Bc = Linv + p * kInvSlot;
+    double* Br = Linv + (P + p) * kInvSlot;
+    auto put = [&](int ti, const double (&v)[4]) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                l[u] = Li[k + u * S];
-                yv[u] = ys[(k + u * S - c0) * kInvCols + col];
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc = fma(l[u], yv[u], acc);
+        for (int u = 0; u < 4; ++u) {
+            const int gi = ti * NB + r, gj = j * NB + c4 + u;
+            Bc[gj * kInvRows + gi] = v[u];
+            Br[gi * kInvRows + gj] = v[u];
         }
-        for (; k < i; k += S) acc = fma(Li[k], ys[(k - c0) * kInvCols + col], acc);
+    };
+    const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int ti = 0; ti < j; ++ti) put(ti, zero);
+    for (int i = j + 1; i < nt; ++i) {
+        // T = sum_k L_ik X_kj, row r of it in acc
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const int gr = r0 + i * NB + r;  // row of L
+        for (int k = j; k < i; ++k) {
+            const int gc = r0 + k * NB;  // first column of tile k
+#pragma unroll 8
+            for (int kk = 0; kk < NB; ++kk) {
+                const double a = (gr < n && gc + kk < n) ? L[static_cast<int64_t>(gr) * n + gc + kk] : 0.0;
+                double x[4];
+                if (k == j) {
 #pragma unroll
-        for (int off = kInvCols; off < 64; off <<= 1) acc += __shfl_xor(acc, off);
-        double y;
-        if (j > i || j >= m) y = 0.0;
-        else if (j == i) y = 1.0 / Li[i];
-        else y = -acc / Li[i];
-        // every lane of the column holds the same y and reads back only its own store
-        ys[(i - c0) * kInvCols + col] = y;
-        if (sub == 0) put(i, y);
+                    for (int u = 0; u < 4; ++u) x[u] = Br[(j * NB + kk) * kInvRows + j * NB + c4 + u];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[u] = xs[k - j - 1][kk][c4 + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[u] = fma(a, x[u], acc[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) ts[r][c4 + u] = acc[u];
+        __syncthreads();
+        // X_ij = -X_ii T
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = 0; q < NB; ++q) {
+            const double e = Br[(i * NB + r) * kInvRows + i * NB + q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = fma(e, ts[q][c4 + u], v[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = -v[u];
+        put(i, v);
+        if (i - j - 1 < kInvKeep) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xs[i - j - 1][r][c4 + u] = v[u];
+        }
+        __syncthreads();
     }
-    for (int i = m + sub; i < kInvRows; i += kInvLanes) put(i, 0.0);
 }
 
 // out_r = sum_q M[q][r] v_q for r < m: thread pair per r
@@ -769,8 +827,9 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
     }
     if (!exact && tiles > kBlockedSolveTiles) {
         const int64_t P = (tiles + kSolveBlockTiles - 1) / kSolveBlockTiles;
-        k_trsv_block_inverse<<<static_cast<unsigned>(P * (kInvRows / kInvCols)), 64, 0, s>>>(
-            n, w.S, w.Lt, static_cast<int>(P), w.chol_status, w.lm, g);
+        const unsigned tiles_p = static_cast<unsigned>(P * kSolveBlockTiles);
+        k_inv_diag_tiles<<<tiles_p, 64, 0, s>>>(n, w.S, w.Lt, static_cast<int>(P), w.chol_status, w.lm, g);
+        k_inv_offdiag_tiles<<<tiles_p, 256, 0, s>>>(n, w.S, w.Lt, static_cast<int>(P), w.chol_status, w.lm, g);
     }
     ZB_TIME_END(w, s);
     ZB_TIME_BEGIN(w, K_TRSV, s);
