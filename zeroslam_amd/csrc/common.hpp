@@ -95,6 +95,17 @@ enum kernel_class : int {
     K_CHOLESKY = 7, K_TRSV = 8, K_BACKSUB = 9, K_CONTROL = 10, K_REFINE = 11
 };
 
+// The fields of one observation that the linearisation reads together,
+// packed so that a thread walking a track (or gathering a pose's observations)
+// touches one 32-byte record per observation instead of one line per array.
+struct obs_record {
+    int32_t pose;   // pose slot
+    int32_t point;  // point slot
+    int32_t param;  // camera index | (loss index << 16)
+    int32_t pairf;  // 2 * W block + 1 if this is the block's first observation; -1 without a block
+    double u, v;    // observed pixel
+};
+
 // One W block (6x3 pose-point coupling) exists per distinct (pose, point) pair
 // whose pose and point are both free (factor_graph.hpp:669-709).
 struct dev_problem {
@@ -112,6 +123,7 @@ struct dev_problem {
     const int32_t* obs_orig = nullptr;   // [no] insertion-order index
     const double* obs_uv = nullptr;      // [no*2]
     const double* obs_info = nullptr;    // [no*4] (unused when identity_info)
+    const obs_record* obs_rec = nullptr; // [no] the fields above, packed (sorted order)
     // segments
     const int32_t* lm_start = nullptr;   // [nl+1] observations of a point
     const int32_t* pose_start = nullptr; // [np+1] into pose_list

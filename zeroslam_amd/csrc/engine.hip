@@ -467,6 +467,7 @@ int32_t zsba_graph::build_structure() {
     int32_t *d_obs_pose = mem.get<int32_t>(no), *d_obs_point = mem.get<int32_t>(no), *d_obs_pair = mem.get<int32_t>(no),
             *d_obs_param = mem.get<int32_t>(no), *d_obs_orig = mem.get<int32_t>(no);
     double* d_uv = mem.get<double>(2L * no);
+    zb::obs_record* d_rec = mem.get<zb::obs_record>(no);
     double* d_info = identity ? nullptr : mem.get<double>(4L * no);
     int32_t *d_lm_start = mem.get<int32_t>(nl + 1), *d_pose_start = mem.get<int32_t>(np + 1), *d_pose_list = mem.get<int32_t>(no),
             *d_lm_pair_start = mem.get<int32_t>(nl + 1), *d_pair_pose = mem.get<int32_t>(npair),
@@ -533,6 +534,13 @@ int32_t zsba_graph::build_structure() {
     p.obs_pose = up(d_obs_pose, obs_pose_s); p.obs_point = up(d_obs_point, obs_point_s);
     p.obs_pair = up(d_obs_pair, obs_pair); p.obs_param = up(d_obs_param, obs_param); p.obs_orig = up(d_obs_orig, orig);
     p.obs_uv = up(d_uv, uv); p.obs_info = identity ? nullptr : up(d_info, info);
+    std::vector<zb::obs_record> rec(no);
+    for (int32_t s = 0; s < no; ++s) {
+        const int32_t a = obs_pair[s];
+        rec[s] = {obs_pose_s[s], obs_point_s[s], obs_param[s], a < 0 ? -1 : 2 * a + (pair_first[a] == s ? 1 : 0),
+                  uv[2L * s], uv[2L * s + 1]};
+    }
+    p.obs_rec = up(d_rec, rec);
     p.lm_start = up(d_lm_start, lm_start); p.pose_start = up(d_pose_start, pose_start); p.pose_list = up(d_pose_list, pose_list);
     p.lm_pair_start = up(d_lm_pair_start, lm_pair_start); p.pair_pose = up(d_pair_pose, pair_pose);
     p.pair_point = up(d_pair_point, pair_point); p.pair_first = up(d_pair_first, pair_first);

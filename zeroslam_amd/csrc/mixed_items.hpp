@@ -74,13 +74,13 @@ struct obs_lin {
 
 template <class TS>
 __device__ __forceinline__ void linearise_obs(const dev_problem& p, const double (&P)[7], const double (&X)[3], int s,
-                                              obs_lin<TS>& o) {
-    const int32_t param = p.obs_param[s];
+                                              const obs_record& rec, obs_lin<TS>& o) {
+    const int32_t param = rec.param;
     const double* K = p.cams + 4 * (param & 0xFFFF);
     double Pc[3], q[4];
     pose_apply(P, X, Pc);
     pose_quat(P, q);
-    const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+    const double uv[2] = {rec.u, rec.v};
     residual(Pc, uv, K, o.r);
     obs_weight(p, s, o.W);
     const int li = param >> 16;
@@ -109,14 +109,15 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
     double b[3] = {0, 0, 0};
     const int s0 = p.lm_start[l], s1 = p.lm_start[l + 1];
     for (int s = s0; s < s1; ++s) {
-        const int ip = p.obs_pose[s];
+        const obs_record rec = p.obs_rec[s];
+        const int ip = rec.pose;
         const bool pfixed = p.pose_fixed[ip] != 0;
         if (pfixed && lfixed) continue;
         double P[7];
 #pragma unroll
         for (int i = 0; i < 7; ++i) P[i] = pose[7L * ip + i];
         obs_lin<TS> o;
-        linearise_obs(p, P, X, s, o);
+        linearise_obs(p, P, X, s, rec, o);
         if (!pfixed) {
             TS H21[21], Wpl[18];
             pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
@@ -130,9 +131,9 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
                 for (int i = 0; i < 6; ++i) pgrad[6 * slot + i] = v[i];
             }
             if (!lfixed) {
-                const int pr = p.obs_pair[s];
+                const int pr = rec.pairf >> 1;
                 TS* Wp = Wbuf + 18L * pr;
-                if (p.pair_first[pr] == s) {
+                if (rec.pairf & 1) {
 #pragma unroll
                     for (int i = 0; i < 18; ++i) Wp[i] = Wpl[i];
                 } else {
@@ -228,10 +229,11 @@ __device__ inline void reduce_pose_chunk_recompute(const dev_problem& p, const d
         const int t = base + lane;
         if (t < t1) {
             const int s = p.pose_list[t];
-            const int l = p.obs_point[s];
+            const obs_record rec = p.obs_rec[s];
+            const int l = rec.point;
             const double X[3] = {point[3L * l], point[3L * l + 1], point[3L * l + 2]};
             obs_lin<TS> o;
-            linearise_obs(p, P, X, s, o);
+            linearise_obs(p, P, X, s, rec, o);
             TS H21[21], Wpl[18];
             double v[6];
             pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
