@@ -55,6 +55,10 @@ def algorithmic_bytes(n_obs, n_poses, n_points, n_pairs, s):
         "cost": n_obs * 24 + n_points * 24 + n_poses * 56,
         "schur": n_pairs * 18 * s + n_points * (9 * s + 9 * s) + 8 * n * n // 2 + 8 * n,
         "backsub": n_pairs * 18 * s + n_points * (9 * s + 24) + 8 * n,
+        # mixed refinement round: the matrix-free binary64 residual (observation
+        # record, pose and point in, W blocks, point rows out) and the reduced residual
+        "refinement": n_obs * 32 + n_pairs * 18 * s + n_points * (24 + 9 * s + 3 * 8 * 3) + n_poses * (56 + 6 * 8 * 3)
+                      + 8 * n * 3,
     }
 
 
@@ -148,13 +152,17 @@ def state_error(P, X, Pr, Xr):
                      np.abs(X - Xr).max() / max(np.abs(Xr).max(), 1e-300)))
 
 
-def equal_cost(g, problem, iters, steps, warmup, chi64, ms64, P64, X64):
+def equal_cost(g, gfz, problem, iters, steps, warmup, chi64, ms64, P64, X64):
     """Mixed against the FP64 (reference order) and FP64-fast paths at equal cost.
-    The level is the FP64 cost after `iters` iterations; when the mixed run settles
-    in a local minimum above it (the robust problem has many nearby minima and
-    rounding decides which one a run ends in), the level is the mixed run's
-    converged cost, which the FP64 runs reach first.  Each path is charged the
-    wall time of its smallest iteration budget that reaches the level."""
+
+    The level is the FP64 path's cost after `iters` iterations.  Each path is
+    charged the wall time of its smallest iteration budget that ends at or below
+    (1 + 1e-6) x that cost.  The mixed path runs with the behind-camera freeze
+    (its default, zsba_options.freeze_behind); FP64-fast is measured with the same
+    freeze (`gfz`, a handle with freeze_behind = 1), so that the two differ only in
+    precision.  A mixed run without the freeze is reported beside it.  Should the
+    mixed run never reach the level within 4 x `iters` iterations, the comparison
+    falls back to the mixed run's converged cost and says so."""
     from zeroslam_amd.engine import FP64, FP64_FAST, MIXED
     level = chi64 * (1.0 + 1e-6)
     level_name = f"FP64 cost after {iters} iterations"
@@ -165,44 +173,80 @@ def equal_cost(g, problem, iters, steps, warmup, chi64, ms64, P64, X64):
         g.solve(4 * iters, False, MIXED)
         cmix = g.stats()["final_chi2"]
         level = cmix * (1.0 + 1e-6)
-        level_name = "converged mixed cost (a local minimum above the FP64 cost)"
+        level_name = "converged mixed cost (above the FP64 cost)"
         kmix = iterations_to_reach(g, MIXED, level, 1, 4 * iters)
         k64 = iterations_to_reach(g, FP64, level, 1, iters)
         ms64_level = time_solves(g, FP64, k64, steps, warmup)[1]
         print(f"bench: mixed settles at chi2 {cmix!r} (FP64 {chi64!r}); comparing at that level",
               file=sys.stderr, flush=True)
-    kfast = iterations_to_reach(g, FP64_FAST, level, 1, 4 * iters)
-    msf_level = time_solves(g, FP64_FAST, kfast, steps, warmup)[1] if kfast else None
+    kfast = iterations_to_reach(gfz, FP64_FAST, level, 1, 4 * iters)
+    msf_level = time_solves(gfz, FP64_FAST, kfast, steps, warmup)[1] if kfast else None
     vraw, ms, stats, kms, kl = time_solves(g, MIXED, kmix, steps, warmup)
+    chi_mix = stats["final_chi2"]
+    state_err = state_error(g.poses(), g.points(), P64, X64)
+    # the same budget as the FP64 run, and the mixed run without the freeze
+    g.restore()
+    g.solve(iters, False, MIXED)
+    chi_mix_full = g.stats()["final_chi2"]
     cmp = {"equal_cost_level": level_name, "level_chi2": level,
            "iterations_mixed": kmix, "iterations_fp64": k64, "iterations_fp64_fast": kfast,
+           "iterations_by_precision": {"mixed": kmix},
            "ms_mixed": ms, "ms_fp64": ms64_level, "ms_fp64_fast": msf_level,
-           "mixed_obs_iters_per_s": vraw, "final_chi2": stats["final_chi2"],
-           "chi2_rel_vs_fp64": (stats["final_chi2"] - chi64) / chi64,
-           "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64),
+           "mixed_obs_iters_per_s": vraw, "final_chi2": chi_mix,
+           "chi2_rel_vs_fp64": (chi_mix - chi64) / chi64,
+           "chi2_rel_vs_fp64_at_equal_iterations": (chi_mix_full - chi64) / chi64,
+           "state_rel_err_vs_fp64": state_err,
+           "frozen_points": stats["frozen_points"],
            "speedup_vs_fp64": ms64_level / ms,
-           "speedup_vs_fp64_fast": (msf_level / ms) if msf_level else None}
+           "speedup_vs_fp64_fast": (msf_level / ms) if msf_level else None,
+           "solver_rule": "behind-camera freeze (mixed and FP64-fast); FP64 is the reference's LM unchanged"}
+    gnf = getattr(g, "_no_freeze", None)
+    if gnf is not None:
+        gnf.restore()
+        gnf.solve(iters, False, MIXED)
+        c_nf = gnf.stats()["final_chi2"]
+        k_nf = iterations_to_reach(gnf, MIXED, level, 1, 4 * iters)
+        cmp["mixed_without_freeze"] = {"chi2_rel_vs_fp64_at_equal_iterations": (c_nf - chi64) / chi64,
+                                       "iterations_to_level": k_nf}
     print(f"bench: mixed reaches the level in {kmix} iterations, {ms:.2f} ms (FP64 {k64} iterations, "
           f"{ms64_level:.2f} ms; FP64-fast {kfast} iterations, {msf_level if msf_level else float('nan'):.2f} ms)",
           file=sys.stderr, flush=True)
     return cmp, (vraw, ms, stats, kms, kl), k64
 
 
+def make_graphs(problem, **kw):
+    """The headline handle (mixed with its default freeze), the FP64-fast handle
+    with the freeze, and a mixed handle without it; all at the initial state."""
+    from zeroslam_amd.engine import Graph
+    t0 = time.perf_counter()
+    g = Graph.from_problem(problem, **kw)
+    g.solve(0)
+    g.setup = {"graph_build_and_first_solve_ms": (time.perf_counter() - t0) * 1e3,
+               "structure_and_upload_ms": g.stats()["setup_ms"]}
+    gfz = Graph.from_problem(problem, freeze_behind=1)
+    gnf = Graph.from_problem(problem, freeze_behind=-1)
+    for h in (gfz, gnf):
+        h.solve(0)
+    for h in (g, gfz, gnf):
+        h.snapshot()
+    g._no_freeze = gnf
+    return g, gfz
+
+
 def seed_compare(seed, config, iters):
     """The equal-cost comparison on another instance of the same workload
     (generator seed `seed`), two timed solves each."""
-    from zeroslam_amd.engine import FP64, Graph
+    from zeroslam_amd.engine import FP64
     from zeroslam_amd.problem import synth
     pr = synth(config, seed=seed)
-    g = Graph.from_problem(pr)
-    g.solve(0)
-    g.snapshot()
+    g, gfz = make_graphs(pr)
     _, ms64, st64, _, _ = time_solves(g, FP64, iters, 2, 1)
-    cmp, _, _ = equal_cost(g, pr, iters, 2, 1, st64["final_chi2"], ms64, g.poses(), g.points())
+    cmp, _, _ = equal_cost(g, gfz, pr, iters, 2, 1, st64["final_chi2"], ms64, g.poses(), g.points())
     keep = ("equal_cost_level", "iterations_mixed", "iterations_fp64", "iterations_fp64_fast", "ms_mixed", "ms_fp64",
-            "ms_fp64_fast", "chi2_rel_vs_fp64", "speedup_vs_fp64", "speedup_vs_fp64_fast")
+            "ms_fp64_fast", "chi2_rel_vs_fp64", "chi2_rel_vs_fp64_at_equal_iterations", "frozen_points",
+            "speedup_vs_fp64", "speedup_vs_fp64_fast", "mixed_without_freeze")
     out = {"seed": seed, "n_obs": pr.n_obs}
-    out.update({k: cmp[k] for k in keep})
+    out.update({k: cmp[k] for k in keep if k in cmp})
     return out
 
 
@@ -235,35 +279,45 @@ def measured_traffic(group: str):
     return sum(table[k] for k in names)
 
 
-def roofline_of(problem, precision_name, kernel_ms, kernel_launches, config=1):
+def roofline_of(problem, precision_name, kernel_ms, kernel_launches, stats, steps, config=1):
+    """Roofline of the kernel group with the largest device time over `steps`
+    solves timed class by class with HIP events (each solve's counts in `stats`).
+    achieved = the group's algorithmic bytes (or flops) per unit of work x units
+    / the group's device time; a unit is one linearisation, cost evaluation,
+    attempt (Schur, back-substitution, factorisation) or refinement round."""
     s = 4 if precision_name == "mixed" else 8
     n_pairs = int(problem.n_obs)  # one W block per (pose, point) pair; the workload has no repeats
     B = algorithmic_bytes(problem.n_obs, problem.n_poses, problem.n_points, n_pairs, s)
     avg = {k: kernel_ms[k] / kernel_launches[k] for k in kernel_ms if kernel_launches[k]}
+    attempts = stats["attempts"] * steps
+    refine_rounds = attempts * 2 if precision_name == "mixed" else 0
     groups = {
-        "linearisation": ["linearise", "pose_reduce"],
-        "cost": ["cost"],
-        "schur": ["invert", "form_Y", "fill", "schur"],
-        "backsub": ["backsub"],
-        "cholesky": ["cholesky", "trsv"],
+        "linearisation": (["linearise", "pose_reduce"], stats["linearisations"] * steps),
+        "cost": (["cost"], attempts + steps),
+        "schur": (["invert", "form_Y", "fill", "schur"], attempts),
+        "backsub": (["backsub"], attempts),
+        "refinement": (["refine"], refine_rounds),
+        "cholesky": (["cholesky", "trsv"], attempts),
     }
-    total = {gname: sum(kernel_ms.get(k, 0.0) for k in ks) for gname, ks in groups.items()}
+    total = {gname: sum(kernel_ms.get(k, 0.0) for k in ks) for gname, (ks, _) in groups.items()}
     dominant = max(total, key=total.get)
     traffic = measured_traffic(dominant) if (precision_name == "mixed" and config == 1) else None
+    units = groups[dominant][1]
+    dur = total[dominant] / max(units, 1)  # ms per unit of work
+    share = {gname: round(t / max(sum(total.values()), 1e-30), 4) for gname, t in total.items()}
     if dominant == "cholesky":
         n = 6 * problem.n_poses
-        dur = avg.get("cholesky", float("nan"))
+        fdur = avg.get("cholesky", float("nan"))
         flops = cholesky_flops(problem)
-        achieved = flops / (dur * 1e-3) / 1e12
+        achieved = flops / (fdur * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic, "kernel": "cholesky",
-                    "algorithmic_flops": flops, "avg_ms": dur, "n": n}
+                    "algorithmic_flops": flops, "avg_ms": fdur, "n": n, "group_share": share}
     else:
-        dur = sum(avg.get(k, 0.0) for k in groups[dominant])
         achieved = B[dominant] / (dur * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": dominant,
-                    "algorithmic_bytes": B[dominant], "avg_ms": dur}
+                    "algorithmic_bytes": B[dominant], "avg_ms": dur, "group_share": share}
     return roofline, {k: round(v, 5) for k, v in avg.items()}
 
 
@@ -453,18 +507,27 @@ def main() -> int:
     problem = synth(args.config)
     iters = ITERATIONS[args.config]
     precision = {"fp64": FP64, "fp64_fast": FP64_FAST, "mixed": MIXED}[args.precision]
-    # HIP events time the factorisation and solve classes inside the timed region
-    # (the dominant group; every class with --single).  Events around every class
-    # would add about a fifth to the solve time.
-    g = Graph.from_problem(problem, timing=True, timing_classes=None if args.single else ["cholesky", "trsv"])
-    g.solve(0)          # structure analysis + upload (untimed)
-    g.snapshot()
+    # Set-up as a drop-in call pays it (untimed in `value`): handle creation, the
+    # host-side graph, structure analysis and upload, the first (0-iteration) solve.
+    g, gfz = make_graphs(problem)
+    setup = g.setup
     print(f"bench: c{args.config} obs={problem.n_obs} poses={problem.n_poses} points={problem.n_points}",
           file=sys.stderr, flush=True)
 
+    def class_times(prec, k):
+        """Per-kernel-class device time of `k`-iteration solves from HIP events
+        around every class, on a separate handle (outside the timed region)."""
+        gt = Graph.from_problem(problem, timing=True)
+        gt.solve(0)
+        gt.snapshot()
+        out = time_solves(gt, prec, k, 2, 1)
+        gt.close()
+        return out[3], out[4], out[2]
+
     if args.single:
-        value, ms, stats, kms, kl = time_solves(g, precision, iters, args.steps, args.warmup)
-        roofline, avg = roofline_of(problem, args.precision, kms, kl, args.config)
+        value, ms, stats, _, _ = time_solves(g, precision, iters, args.steps, args.warmup)
+        kms, kl, cst = class_times(precision, iters)
+        roofline, avg = roofline_of(problem, args.precision, kms, kl, cst, 2, args.config)
         print(json.dumps({"metric": "BA observation*iters/s (single precision run)", "value": value,
                           "unit": "obs*iter/s", "ms_per_step": ms, "precision": args.precision,
                           "roofline": roofline, "kernel_avg_ms": avg}))
@@ -477,30 +540,29 @@ def main() -> int:
     print(f"bench: fp64 {v64:.4g} obs*iter/s, chi2 {chi64!r}", file=sys.stderr, flush=True)
     extra = {"fp64": {"value": v64, "ms_per_step": ms64, "lm_iterations": st64["iterations"],
                       "attempts": st64["attempts"], "final_chi2": chi64}}
+    run_iters = iters
     if precision == FP64:
-        value, ms, stats, kms, kl = v64, ms64, st64, k64, l64
-        run_iters = iters
+        value, ms, stats = v64, ms64, st64
     else:
-        vf, msf, stf, kf, lf = time_solves(g, FP64_FAST, iters, args.steps, args.warmup)
+        vf, msf, stf, _, _ = time_solves(gfz, FP64_FAST, iters, args.steps, args.warmup)
         extra["fp64_fast"] = {"value": vf, "ms_per_step": msf, "lm_iterations": stf["iterations"],
-                              "final_chi2": stf["final_chi2"],
+                              "final_chi2": stf["final_chi2"], "freeze_behind": True,
                               "chi2_rel_vs_fp64": (stf["final_chi2"] - chi64) / chi64,
-                              "state_rel_err_vs_fp64": state_error(g.poses(), g.points(), P64, X64)}
+                              "state_rel_err_vs_fp64": state_error(gfz.poses(), gfz.points(), P64, X64)}
         print(f"bench: fp64_fast {vf:.4g} obs*iter/s", file=sys.stderr, flush=True)
         if precision == FP64_FAST:
-            value, ms, stats, kms, kl = vf, msf, stf, kf, lf
-            run_iters = iters
+            value, ms, stats = vf, msf, stf
         else:
-            cmp, (vraw, ms, stats, kms, kl), k64 = equal_cost(g, problem, iters, args.steps, args.warmup, chi64, ms64,
-                                                              P64, X64)
+            cmp, (vraw, ms, stats, _, _), k64 = equal_cost(g, gfz, problem, iters, args.steps, args.warmup, chi64,
+                                                           ms64, P64, X64)
             run_iters = cmp["iterations_mixed"]
             # observation*iterations of the FP64 run to the same cost, per second of the mixed run
             value = problem.n_obs * k64 / (ms * 1e-3)
             extra["mixed"] = cmp
             if args.seeds > 0:
                 extra["other_seeds"] = [seed_compare(seed, args.config, iters) for seed in range(2, 2 + args.seeds)]
-
-    roofline, avg = roofline_of(problem, args.precision, kms, kl, args.config)
+    kms, kl, cst = class_times(precision, run_iters)
+    roofline, avg = roofline_of(problem, args.precision, kms, kl, cst, 2, args.config)
     out = {
         "metric": "BA observation*iters/s, 100KF/20k pts: mixed vs FP64 GPU at equal final cost",
         "value": value, "unit": "obs*iter/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -513,6 +575,7 @@ def main() -> int:
         "roofline": roofline,
         "compare": extra,
         "kernel_avg_ms": avg,
+        "setup": setup,
     }
     if not args.no_cpu:
         print("bench: reference CPU optimiser sample", file=sys.stderr, flush=True)

@@ -67,7 +67,10 @@ extern "C" {
 
 typedef struct zsba_options {
     int32_t device;       /* HIP device ordinal (default 0)                          */
-    int32_t verbose;      /* 1: print the reference's [INIT]/[GOOD]/[ BAD] lines     */
+    int32_t verbose;      /* 1: print one summary line per solve to stderr (iterations,
+                             attempts, accepted, rejected, failed solves, chi2 before
+                             and after, final damping); the reference's per-attempt
+                             [INIT]/[GOOD]/[ BAD] trace is not reproduced            */
     int32_t refine_steps; /* MIXED: refinement steps of the reduced solve: 0 = default
                              (2), negative = none                                      */
     int32_t timing;       /* 1: time kernel launches with HIP events (stats)          */
@@ -80,7 +83,16 @@ typedef struct zsba_options {
                              2 = one launch per 32-column panel                     */
     int32_t timing_classes; /* with timing: bit mask of the kernel classes timed
                              (bit ZSBA_K_*); 0 = every class                        */
-    int32_t reserved[1];
+    int32_t freeze_behind;  /* behind-camera freeze, not a rule of the reference: at
+                             each linearisation, a point with an observation whose
+                             camera cannot project it (camera.hpp:98-100, the
+                             edge's behind-camera residual -0.3 z) is held fixed
+                             until the next linearisation (zero inverse block, no
+                             step).  Such points otherwise sit at the z = 0 edge,
+                             where the residual jumps, and keep the shared damping
+                             from falling (the solve stalls).  0 = default: on for
+                             MIXED, off for FP64_FAST; 1 = on for MIXED and
+                             FP64_FAST; -1 = off.  Never applied to FP64.        */
 } zsba_options;
 
 /* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */
@@ -115,7 +127,8 @@ typedef struct zsba_stats {
     double kernel_ms[ZSBA_K_COUNT];        /* summed device time per kernel class     */
     int64_t kernel_launches[ZSBA_K_COUNT]; /* launches per kernel class               */
     int32_t linearisations;  /* linearisations (make_hessian) in the last solve          */
-    int32_t pad;
+    int32_t frozen_points;   /* points held by the behind-camera freeze at the last
+                                linearisation (0 when the freeze is off)               */
     double reserved[7];
 } zsba_stats;
 

@@ -64,3 +64,12 @@ def test_empty_graph_solve_returns_zero():
     assert g.solve(10) == 0
     g.add_poses([[0, 0, 0, 0, 0, 0, 1.0]])
     assert g.solve(10) == 0
+
+
+def test_package_star_import():
+    # every name in zeroslam_amd.__all__ is importable
+    ns = {}
+    exec("from zeroslam_amd import *", ns)
+    import zeroslam_amd
+    for name in zeroslam_amd.__all__:
+        assert name in ns, name

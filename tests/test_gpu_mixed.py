@@ -102,3 +102,34 @@ def test_mixed_then_fp64_continue():
     Po, Xo = o.state()
     assert abs(g.current_chi(True) - o.current_chi(True)) <= 1e-9 * o.current_chi(True)
     assert state_rel_err(g.poses(), g.points(), Po, Xo) < 1e-9
+
+
+C1_FIXTURE = "tests/golden/c1_seeds.npz"
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5])
+def test_mixed_full_c1_reaches_reference_cost(seed):
+    """The 100-keyframe / 20k-point / ~160k-observation workload (Huber 2 px, 5%
+    outliers), 20 iterations: the mixed solve (with its behind-camera freeze)
+    ends at or below (1 + 1e-6) x the reference optimiser's cost after solve(20)
+    on a fresh graph (tests/golden/make_large.py)."""
+    from pathlib import Path
+
+    from tests.golden.make_large import problem_digest
+    root = Path(__file__).resolve().parents[1]
+    with np.load(root / C1_FIXTURE, allow_pickle=False) as z:
+        i = int(np.nonzero(z["seeds"] == seed)[0][0])
+        digest, chi_ref, P_ref = str(z["digest"][i]), float(z["chi2"][i]), z["poses"][i]
+    pr = synth(1, seed=seed)
+    assert problem_digest(pr) == digest, "the workload generator made a different problem"
+    g = Graph.from_problem(pr)
+    g.solve(20, False, MIXED)
+    st = g.stats()
+    chi = g.current_chi(True)
+    P = g.poses()
+    pose_err = float(np.abs(P - P_ref).max() / np.abs(P_ref).max())
+    print(f"mixed c1 seed {seed}: it {st['iterations']} att {st['attempts']} frozen {st['frozen_points']} "
+          f"chi2 {chi!r} reference {chi_ref!r} rel {(chi - chi_ref) / chi_ref:+.3e} pose rel err {pose_err:.3e}",
+          flush=True)
+    assert chi <= chi_ref * (1 + 1e-6)
+    assert abs(st["final_chi2"] - chi) <= 1e-12 * chi

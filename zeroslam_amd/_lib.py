@@ -23,7 +23,7 @@ ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EDEVICE", -4: "ENOMEM", -5: "EUNSUP"}
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("refine_steps", C.c_int32),
                 ("timing", C.c_int32), ("floor_exp", C.c_int32), ("factor_path", C.c_int32),
-                ("timing_classes", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("timing_classes", C.c_int32), ("freeze_behind", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -32,7 +32,7 @@ class Stats(C.Structure):
                 ("n_obs", C.c_int64), ("n_poses", C.c_int64), ("n_points", C.c_int64),
                 ("initial_chi2", C.c_double), ("final_chi2", C.c_double), ("lambda_", C.c_double),
                 ("solve_ms", C.c_double), ("setup_ms", C.c_double), ("kernel_ms", C.c_double * 16),
-                ("kernel_launches", C.c_int64 * 16), ("linearisations", C.c_int32), ("pad", C.c_int32),
+                ("kernel_launches", C.c_int64 * 16), ("linearisations", C.c_int32), ("frozen_points", C.c_int32),
                 ("reserved", C.c_double * 7)]
 
     def as_dict(self) -> dict:

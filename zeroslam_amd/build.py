@@ -16,6 +16,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -64,7 +65,7 @@ def build(verbose: bool = False) -> Path:
     if not Path(HIPCC).exists() and shutil.which("hipcc") is None:
         raise RuntimeError("hipcc not found; the engine needs ROCm to build")
     BUILD_DIR.mkdir(exist_ok=True)
-    objects = []
+    jobs = []
     for src, flags in UNITS:
         path = CSRC / src
         if not path.exists():
@@ -77,8 +78,13 @@ def build(verbose: bool = False) -> Path:
                    "-o", str(obj)]
         if verbose:
             print(" ".join(cmd))
-        _run(cmd)
-        objects.append(str(obj))
+        jobs.append((cmd, str(obj)))
+    # the translation units are independent: compile them side by side
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "8"))))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        for f in [pool.submit(_run, cmd) for cmd, _ in jobs]:
+            f.result()
+    objects = [obj for _, obj in jobs]
     tmp = OUT.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objects, "-o", str(tmp)]
     if verbose:

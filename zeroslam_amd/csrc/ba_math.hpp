@@ -94,11 +94,18 @@ ZB_HD void pose_apply(const T p[7], const T X[3], T Pc[3]) {
     Pc[2] = r[2] + p[2];
 }
 
+// pinhole::project fails for a point at a depth below 1e-12 (camera.hpp:98-100);
+// the edge then takes its behind-camera residual and Jacobian.
+template <class T>
+ZB_HD bool behind_camera(const T Pc[3]) {
+    return Pc[2] < T(1.0e-12);
+}
+
 // edge_reprojection::compute_residual with pinhole::project,
 // factor_graph.hpp:999-1008, camera.hpp:97-111.  K = (fx, fy, cx, cy).
 template <class T>
 ZB_HD void residual(const T Pc[3], const T uv[2], const T K[4], T r[2]) {
-    if (Pc[2] < T(1.0e-12)) {
+    if (behind_camera(Pc)) {
         const T penalty = T(-0.3) * Pc[2];
         r[0] = penalty;
         r[1] = penalty;
@@ -115,7 +122,7 @@ ZB_HD void residual(const T Pc[3], const T uv[2], const T K[4], T r[2]) {
 template <class T>
 ZB_HD void jacobians(const T Pc[3], const T q[4], const T K[4], T Jp[12], T Jl[6]) {
     T Jc[6];
-    if (Pc[2] < T(1.0e-12)) {
+    if (behind_camera(Pc)) {
         Jc[0] = T(0.0); Jc[1] = T(0.0); Jc[2] = T(0.3);
         Jc[3] = T(0.0); Jc[4] = T(0.0); Jc[5] = T(0.3);
     } else {

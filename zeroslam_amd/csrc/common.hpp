@@ -211,6 +211,12 @@ struct dev_work64 {
     double* adj_partial = nullptr;  // [513]
     double* adj_obs = nullptr;      // [2 no] dL/duv by compacted observation
     uint8_t* adj_frozen = nullptr;  // [nl] point held constant by the adjoint
+    // Behind-camera freeze (not the reference's rule; off on the reference-order
+    // path): null, or [nl] flags set by each linearisation for the points with an
+    // observation the camera cannot project (camera.hpp:98-100).  Until the next
+    // linearisation such a point is held as a fixed one: zero H_ll^-1, so no Schur
+    // terms and no step.
+    uint8_t* frozen = nullptr;
     // envelope of the reduced system in 32 x 32 tiles: row tile I is zero left of
     // tile column tile_first[I]; column tile k is zero below row tile tile_last[k]
     const int32_t* tile_first = nullptr;  // [T]

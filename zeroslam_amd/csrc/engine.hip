@@ -128,6 +128,7 @@ struct zsba_graph {
     size_t eval_capacity = 0;
     double* snap_pose = nullptr;
     double* snap_point = nullptr;
+    uint8_t* frozen_dev = nullptr;  // [nl] behind-camera freeze flags (w.frozen while a solve uses them)
     bool have_snapshot = false;
 
     zsba_stats stats{};
@@ -248,6 +249,18 @@ struct zsba_graph {
         return e < 0 ? 0.0 : std::ldexp(1.0, -e);
     }
     static constexpr int kDefaultFloorExp = 24;
+    // Whether a solve in this precision applies the behind-camera freeze.
+    bool freeze_for(int32_t precision) const {
+        if (precision == ZSBA_PRECISION_FP64 || opt.freeze_behind < 0) return false;
+        return opt.freeze_behind > 0 || precision == ZSBA_PRECISION_MIXED;
+    }
+    // Points flagged by the last linearisation of a solve with the freeze on.
+    int32_t count_frozen() {
+        if (!w.frozen || dp.nl == 0) return 0;
+        std::vector<uint8_t> f(static_cast<size_t>(dp.nl));
+        if (copy(f.data(), w.frozen, f.size(), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        return static_cast<int32_t>(std::count(f.begin(), f.end(), 1));
+    }
 };
 
 // ---- structure analysis -----------------------------------------------------
@@ -506,6 +519,7 @@ int32_t zsba_graph::build_structure() {
     w.adj = mem.get<double>(6 * (6L * np + 3L * nl));
     w.adj_obs = mem.get<double>(2L * no);
     w.adj_frozen = mem.get<uint8_t>(nl);
+    frozen_dev = mem.get<uint8_t>(nl);
     w.adj_partial = mem.get<double>(513);
     int32_t* d_tile_first = mem.get<int32_t>(n_tiles);
     int32_t* d_tile_last = mem.get<int32_t>(n_tiles);
@@ -709,6 +723,7 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
         if (rc) return rc;
     }
     timer.reset();
+    w.frozen = freeze_for(precision) ? frozen_dev : nullptr;
     const double t0 = now_ms();
     std::memset(lm_pinned, 0, sizeof(lm_state));
     lm_pinned->iterations = iterations;
@@ -778,6 +793,8 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     stats.initial_chi2 = lm_pinned->initial_chi2;
     stats.final_chi2 = lm_pinned->chi2;
     stats.lambda = lm_pinned->lambda;
+    stats.frozen_points = count_frozen();
+    w.frozen = nullptr;  // the other entry points follow the reference's rule
     if (opt.verbose) print_trace();
     device_newer = true;
     return lm_pinned->success;
@@ -1180,6 +1197,7 @@ struct zsba_batch {
             bw.p = g->dp;
             bw.w = g->w;
             bw.w.timer = nullptr;
+            bw.w.frozen = g->freeze_for(precision) ? g->frozen_dev : nullptr;
             bw.iterations = iterations;
             bw.use_relative = use_rel ? 1 : 0;
             bw.refine_steps = g->opt.refine_steps;

@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_batch_lin_points(const batch_window* wi
     if (!needs_linearisation(bw.w.lm)) return;
     const blocks<TS> B(bw.w);
     mixed::lin_point<TS, false>(bw.p, bw.w.pose, bw.w.point, B.pterms, bw.w.pose_grad, B.Hll, bw.w.bl_m, B.W,
-                                g - prefix[k]);
+                                bw.w.frozen, g - prefix[k]);
 }
 
 template <class TS>
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kThreads) k_batch_attempts(const batch_window*
         if (lm->done) break;
         const double lam = lm->lambda;
         // reduced camera system
-        for (int l = tid; l < p.nl; l += blockDim.x) mixed::damp_invert_item(p, B.Hll, B.Hinv, lam, l);
+        for (int l = tid; l < p.nl; l += blockDim.x) mixed::damp_invert_item(p, B.Hll, B.Hinv, w.frozen, lam, l);
         for (int64_t i = tid; i < static_cast<int64_t>(n) * n; i += blockDim.x) w.S[i] = 0.0;
         __syncthreads();
         for (int a = tid; a < p.npair; a += blockDim.x) mixed::form_Y_item(p, B.W, B.Hinv, B.Y, a);
@@ -316,12 +316,17 @@ __global__ void __launch_bounds__(kThreads) k_batch_attempts(const batch_window*
                 lm_decide(lm);
             }
             __syncthreads();
-            if (lm->need_rollback) {
+            // Every thread takes the decision into registers; lm_end_attempt below
+            // clears need_rollback only after the barrier that follows these reads.
+            const bool rollback = lm->need_rollback != 0;
+            const bool pause = !lm->done && lm->need_linearize;
+            if (rollback) {
                 for (int t = tid; t < p.np + p.nl; t += blockDim.x)
                     mixed::rollback_item(p, w.pose, w.pose_backup, w.point, w.point_backup, t);
             }
-            if (!lm->done && lm->need_linearize) break;  // pause: the next round linearises
+            if (pause) break;  // the next round linearises
         }
+        __syncthreads();
         if (tid == 0) lm_end_attempt(lm);
         __syncthreads();
         clk.mark(PH_UPDATE);

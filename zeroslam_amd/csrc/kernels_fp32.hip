@@ -38,10 +38,10 @@ constexpr int kBlock = 256;
 
 __global__ void __launch_bounds__(kBlock) k_lin_points_f(dev_problem p, const double* pose, const double* point,
                                                          float* pterms, double* pgrad, float* Hll, double* bl,
-                                                         float* Wbuf, const lm_state* lm, int gated) {
+                                                         float* Wbuf, const lm_state* lm, int gated, uint8_t* frozen) {
     if (gated && (lm->done || !lm->need_linearize)) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < p.nl) mixed::lin_point(p, pose, point, pterms, pgrad, Hll, bl, Wbuf, l);
+    if (l < p.nl) mixed::lin_point(p, pose, point, pterms, pgrad, Hll, bl, Wbuf, frozen, l);
 }
 
 // One wave per (pose, chunk), then one thread per (pose, term).
@@ -63,10 +63,10 @@ __global__ void __launch_bounds__(256) k_reduce_pose_f(dev_problem p, const doub
 __global__ void k_linearise_done_f(lm_state* lm) { lm_linearise_done(lm); }
 
 __global__ void __launch_bounds__(kBlock) k_damp_invert_f(dev_problem p, const float* Hll, float* Hinv, const lm_state* lm,
-                                                          const double* lambda_override) {
+                                                          const double* lambda_override, const uint8_t* frozen) {
     if (!lambda_override && lm->done) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < p.nl) mixed::damp_invert_item(p, Hll, Hinv, lambda_override ? *lambda_override : lm->lambda, l);
+    if (l < p.nl) mixed::damp_invert_item(p, Hll, Hinv, frozen, lambda_override ? *lambda_override : lm->lambda, l);
 }
 
 __global__ void __launch_bounds__(kBlock) k_form_Y_f(dev_problem p, const float* Wbuf, const float* Hinv, float* Y,
@@ -105,7 +105,7 @@ void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_lin_points_f<<<grid_of(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms_f, w.pose_grad, w.Hll_f,
-                                                                 w.bl_m, w.W_f, w.lm, gated ? 1 : 0);
+                                                                 w.bl_m, w.W_f, w.lm, gated ? 1 : 0, w.frozen);
         ZB_TIME_END(w, s);
     }
     if (p.np > 0) {
@@ -121,7 +121,8 @@ void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream
 void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override) {
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_INVERT, s);
-        k_damp_invert_f<<<grid_of(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll_f, w.Hinv_f, w.lm, lambda_override);
+        k_damp_invert_f<<<grid_of(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll_f, w.Hinv_f, w.lm, lambda_override,
+                                                                 w.frozen);
         ZB_TIME_END(w, s);
     }
     if (p.npair > 0) {

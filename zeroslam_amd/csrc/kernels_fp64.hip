@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, in
 // the pose-side terms of each observation go to pose_terms for k_reduce_pose.
 __global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, const double* pose, const double* point,
                                                              double* pterms, double* Hll, double* bl, double* Wbuf,
-                                                             const lm_state* lm, int gated) {
+                                                             const lm_state* lm, int gated, uint8_t* frozen) {
     if (gated && (lm->done || !lm->need_linearize)) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= p.nl) return;
@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, cons
     load3(point + 3L * l, X);
     double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double b[3] = {0, 0, 0};
+    bool behind = false;
     const int s0 = p.lm_start[l], s1 = p.lm_start[l + 1];
     for (int s = s0; s < s1; ++s) {
         const int ip = p.obs_pose[s];
@@ -196,6 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, cons
         double P[7], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
         load7(pose + 7L * ip, P);
         pose_apply(P, X, Pc);
+        behind = behind || behind_camera(Pc);
         const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
         residual(Pc, uv, K, r);
         pose_quat(P, q);
@@ -238,6 +240,7 @@ __global__ void __launch_bounds__(kBlock) k_linearise_points(dev_problem p, cons
     for (int i = 0; i < 9; ++i) Hll[9L * l + i] = H[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) bl[3L * l + i] = b[i];
+    if (frozen) frozen[l] = behind ? 1 : 0;
 }
 
 // One wave per pose; lane j < 27 sums term j over the pose's observations in
@@ -353,7 +356,7 @@ __global__ void __launch_bounds__(1024) k_init_lm(dev_problem p, const TS* Hpp, 
 
 // ---- damping + point-block inverses, factor_graph.hpp:742-765 ------------------
 __global__ void __launch_bounds__(kBlock) k_damp_invert(dev_problem p, const double* Hll, double* Hinv, const lm_state* lm,
-                                                        const double* lambda_override) {
+                                                        const double* lambda_override, const uint8_t* frozen) {
     if (!lambda_override && lm->done) return;
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= p.nl) return;
@@ -362,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_damp_invert(dev_problem p, const dou
 #pragma unroll
     for (int i = 0; i < 9; ++i) H[i] = Hll[9L * l + i];
     H[0] += lam; H[4] += lam; H[8] += lam;
-    if (p.point_fixed[l]) {
+    if (p.point_fixed[l] || (frozen && frozen[l])) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) inv[i] = 0.0;
     } else {
@@ -874,7 +877,7 @@ void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_linearise_points<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms, w.Hll, w.bl, w.W,
-                                                                      w.lm, gated ? 1 : 0);
+                                                                      w.lm, gated ? 1 : 0, w.frozen);
         ZB_TIME_END(w, s);
     }
     if (p.np > 0) {
@@ -935,7 +938,7 @@ void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStrea
                              bool exact) {
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_INVERT, s);
-        k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override);
+        k_damp_invert<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.Hll, w.Hinv, w.lm, lambda_override, w.frozen);
         ZB_TIME_END(w, s);
     }
     if (p.npair > 0) {

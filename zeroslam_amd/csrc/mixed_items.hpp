@@ -70,6 +70,7 @@ template <class TS>
 struct obs_lin {
     double r[2], W[4], Wr[4], drho, Jp[12], Jl[6];
     TS Jpf[12], Jlf[6], Wrf[4];
+    bool behind;  // the camera cannot project the point (behind-camera residual)
 };
 
 template <class TS>
@@ -80,6 +81,7 @@ __device__ __forceinline__ void linearise_obs(const dev_problem& p, const double
     double Pc[3], q[4];
     pose_apply(P, X, Pc);
     pose_quat(P, q);
+    o.behind = behind_camera(Pc);
     const double uv[2] = {rec.u, rec.v};
     residual(Pc, uv, K, o.r);
     obs_weight(p, s, o.W);
@@ -99,10 +101,12 @@ __device__ __forceinline__ void linearise_obs(const dev_problem& p, const double
 // kPoseTerms the pose-side terms go to pose-major slots (otherwise
 // reduce_pose_chunk_recompute forms them again).  Residual, robust weight and
 // Jacobians are binary64 in the reference's order; the Hessian products use the
-// Jacobians rounded to TS.
+// Jacobians rounded to TS.  frozen (nullable) receives whether an observation of
+// the point is behind its camera.
 template <class TS, bool kPoseTerms = true>
 __device__ inline void lin_point(const dev_problem& p, const double* pose, const double* point, TS* pterms, double* pgrad,
-                                 TS* Hll, double* bl, TS* Wbuf, int l) {
+                                 TS* Hll, double* bl, TS* Wbuf, uint8_t* frozen, int l) {
+    bool behind = false;
     const bool lfixed = p.point_fixed[l] != 0;
     const double X[3] = {point[3L * l], point[3L * l + 1], point[3L * l + 2]};
     TS H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -118,6 +122,7 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
         for (int i = 0; i < 7; ++i) P[i] = pose[7L * ip + i];
         obs_lin<TS> o;
         linearise_obs(p, P, X, s, rec, o);
+        behind = behind || o.behind;
         if (!pfixed) {
             TS H21[21], Wpl[18];
             pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
@@ -157,6 +162,7 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
     for (int i = 0; i < 9; ++i) Hll[9L * l + i] = H[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) bl[3L * l + i] = b[i];
+    if (frozen) frozen[l] = behind ? 1 : 0;
 }
 
 // Chunk c of [t0, t1) split into kPoseChunks contiguous pieces.
@@ -285,13 +291,14 @@ __device__ inline void reduce_pose_finish(const double* part, TS* Hpp, double* b
 
 // Damped point block and its inverse (factor_graph.hpp:742-765).
 template <class TS>
-__device__ inline void damp_invert_item(const dev_problem& p, const TS* Hll, TS* Hinv, double lambda, int l) {
+__device__ inline void damp_invert_item(const dev_problem& p, const TS* Hll, TS* Hinv, const uint8_t* frozen,
+                                        double lambda, int l) {
     const TS lam = static_cast<TS>(lambda);
     TS H[9], inv[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) H[i] = Hll[9L * l + i];
     H[0] += lam; H[4] += lam; H[8] += lam;
-    if (p.point_fixed[l]) {
+    if (p.point_fixed[l] || (frozen && frozen[l])) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) inv[i] = TS(0);
     } else {
