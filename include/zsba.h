@@ -334,6 +334,18 @@ int32_t zsba_matcher_stats(const zsba_matcher* matcher, double* device_ms, int32
  * intrinsics applied to every observation's camera.  Outputs are indexed by id
  * (NULL outputs are skipped).  Returns the conjugate-gradient iterations. */
 int32_t zsba_gauss_newton(zsba_graph* graph, int32_t steps);
+/* zsba_hold_indefinite_points: at the current state, set fixed every free point
+ * that has no local optimum in its own coordinates: its undamped Gauss-Newton
+ * block is singular under the reference's inverse rule (matrix.hpp:749-774), or
+ * its block of the exact Hessian (forward-mode derivatives of the gradient) is
+ * not positive definite, as for landmarks seen without parallax: a pivot of the
+ * block's symmetric part is at most rel_tol times its trace (rel_tol in [0, 1);
+ * 0 = 1e-12).  Held points
+ * keep their parameters through later solves, Gauss-Newton steps and the
+ * adjoint (which then treats them as constants).  held[id] is set to 1 for each
+ * (may be NULL; other entries are left untouched).  Returns the number held.
+ * No reference counterpart. */
+int32_t zsba_hold_indefinite_points(zsba_graph* graph, double rel_tol, uint8_t* held /* n_points_added */);
 int32_t zsba_adjoint(zsba_graph* graph, const double* dL_dpoints /* n_points_added*3 */,
                      const double* dL_dposes /* n_poses_added*6 or NULL */,
                      double* dL_dobs /* n_obs_added*2 or NULL */, double dL_dK[4] /* or NULL */);
@@ -383,7 +395,10 @@ typedef struct zsba_synth_params {
     int32_t n_fixed;      /* gauge: the first n_fixed poses are fixed                   */
     double  K[4];
     int32_t width, height;
-    int32_t reserved[4];
+    double  min_parallax; /* layouts 1-2: a landmark is accepted only when the first
+                             and last camera centres of its track subtend at least
+                             this angle at it (rad; 0 = no test)                    */
+    int32_t reserved[2];
 } zsba_synth_params;
 
 typedef struct zsba_synth zsba_synth;

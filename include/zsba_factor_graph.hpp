@@ -87,6 +87,10 @@ private:
 #ifndef ZSBA_SHIM_MATRIX
 #define ZSBA_SHIM_MATRIX ::zsba::column
 #endif
+// Engine precision of a new factor_graph (set_precision changes it per graph).
+#ifndef ZSBA_SHIM_PRECISION
+#define ZSBA_SHIM_PRECISION ZSBA_PRECISION_FP64
+#endif
 
 namespace zsba {
 namespace factor_graph {
@@ -383,7 +387,7 @@ private:
     }
 
     bool verbose_;
-    int32_t precision_ = ZSBA_PRECISION_FP64;
+    int32_t precision_ = ZSBA_SHIM_PRECISION;
     int32_t device_ = 0;
     std::string error_;
     zsba_graph* handle_ = nullptr;

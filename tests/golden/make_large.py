@@ -9,9 +9,9 @@ oracle (bitwise equal to it, tests/test_oracle_reference.py):
     digest of the problem arrays so the test can tell that the generator made
     the same problem.
   * ``c4_full.npz``: the 1000-keyframe loop-closure workload (200k points, ~2M
-    observations, 6000-row reduced system) after solve(1) and solve(2) with the
-    oracle: accepted steps, chi2, every pose, the SHA-256 of the point array and
-    a sample of 2000 points.
+    observations, 6000-row reduced system) after solve(1), solve(2) and
+    solve(15) on fresh graphs with the oracle: accepted steps, chi2, every pose,
+    the SHA-256 of the point array and a sample of 2000 points.
 
 Run from the repository root: ``python tests/golden/make_large.py [c1|c4]``.
 The problems are regenerated by zeroslam_amd.problem.synth; nothing here
@@ -73,9 +73,9 @@ def _c4(k: int):
     return k, problem_digest(pr), acc, g.current_chi(True), P, X, g.stats()
 
 
-def make_c4() -> None:
-    with ProcessPoolExecutor(max_workers=2) as pool:
-        rows = list(pool.map(_c4, (1, 2)))
+def make_c4(ks=(1, 2, 15)) -> None:
+    with ProcessPoolExecutor(max_workers=len(ks)) as pool:
+        rows = list(pool.map(_c4, ks))
     n_points = rows[0][5].shape[0]
     sample = np.linspace(0, n_points - 1, C4_SAMPLE).astype(np.int64)
     out = {"digest": np.array([rows[0][1]]), "sample": sample}

@@ -50,7 +50,7 @@ class SynthParams(C.Structure):
                 ("outlier_rate", C.c_double), ("loss_kind", C.c_int32), ("loss_delta", C.c_double),
                 ("rot_perturb", C.c_double), ("trans_perturb", C.c_double), ("point_perturb", C.c_double),
                 ("n_fixed", C.c_int32), ("K", C.c_double * 4), ("width", C.c_int32), ("height", C.c_int32),
-                ("reserved", C.c_int32 * 4)]
+                ("min_parallax", C.c_double), ("reserved", C.c_int32 * 2)]
 
 
 class BatchInfo(C.Structure):
@@ -122,6 +122,7 @@ _SIGNATURES = {
     "zsba_match_by_projection": (C.c_int32, [_P, C.POINTER(MatchFrame), C.c_int32, _D, _U8, _U8, C.c_int32, _D, _U8,
                                              _I32, C.c_int32, _U8, C.c_int32, _I32, _I32]),
     "zsba_gauss_newton": (C.c_int32, [_P, C.c_int32]),
+    "zsba_hold_indefinite_points": (C.c_int32, [_P, C.c_double, C.POINTER(C.c_uint8)]),
     "zsba_debug_adjoint_ops": (C.c_int32, [_P, _D, _D, _D]),
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),
     "zsba_debug_reduced_system": (C.c_int32, [_P, C.c_double, C.c_int32, _D, _D, _D, _D]),

@@ -17,11 +17,23 @@ from .problem import Problem
 
 
 def solve_problem(problem: Problem, uv: np.ndarray, K: np.ndarray, precision: int = FP64_FAST) -> Graph:
+    """Solve to the stationary point the adjoint differentiates: LM to
+    convergence, then points without a local optimum (landmarks seen without
+    parallax: their exact Hessian block is not positive definite) are held fixed
+    and undamped Gauss-Newton steps polish the rest.  g.held flags the held
+    points by id; the adjoint treats them as constants."""
     pr = problem.copy()
     pr.obs_uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
     pr.obs_K = np.tile(np.asarray(K, dtype=np.float64), (pr.n_obs, 1))
     g = Graph.from_problem(pr)
-    g.solve_to_convergence(precision)
+    g.solve_to_convergence(precision, polish=0)
+    g.held = np.zeros(pr.n_points, np.uint8)
+    for _ in range(3):
+        held = g.hold_indefinite_points()
+        g.held |= held
+        g.gauss_newton(4)
+        if not held.any():
+            break
     return g
 
 

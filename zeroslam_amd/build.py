@@ -96,10 +96,11 @@ def build(verbose: bool = False) -> Path:
 
 def build_oracle(verbose: bool = False) -> None:
     """Compile the CPU oracle (test infrastructure) and, where the reference
-    checkout exists, the reference driver into ``oracle/_ref``."""
+    checkout exists, the reference driver and the map harnesses (the reference's
+    map.hpp with its own factor graph and with the drop-in) into ``oracle/_ref``."""
     targets = ["oracle"]
     if Path("/root/reference/source/factor_graph.hpp").exists():
-        targets.append("ref")
+        targets += ["ref", "map"]
     cmd = ["make", "-s", "-C", str(ROOT / "oracle"), *targets]
     if verbose:
         print(" ".join(cmd))

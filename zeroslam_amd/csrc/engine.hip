@@ -242,6 +242,7 @@ struct zsba_graph {
     int32_t run(int32_t iterations, int32_t use_rel, int32_t precision);
     int32_t adjoint(const double* dL_dpoints, const double* dL_dposes, double* dL_dobs, double* dL_dK);
     int32_t gauss_newton(int32_t steps);
+    int32_t hold_indefinite_points(double rel_tol, uint8_t* held);
     double adjoint_residual = 0.0;  // relative CG residual of the last adjoint
     void print_trace();
     double mixed_floor() const {
@@ -822,6 +823,69 @@ int32_t zsba_graph::gauss_newton(int32_t steps) {
     if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "Gauss-Newton");
     device_newer = true;
     return 0;
+}
+
+// ---- points without a finite optimum ------------------------------------------
+// The exact Hessian's 3x3 block of every free point (three products H e_k with
+// e_k the k-th coordinate of every point, forward-mode duals through the
+// gradient arithmetic) and the undamped Gauss-Newton block.  A point whose
+// Gauss-Newton block is singular under the reference's inverse rule, or whose
+// exact block is not positive definite (pivots of its symmetric part at most
+// rel_tol of its trace), has no local optimum in its own coordinates at this
+// state; it is set fixed.  held[id] = 1 for those points.  Returns their count.
+int32_t zsba_graph::hold_indefinite_points(double rel_tol, uint8_t* held) {
+    hipError_t e;
+    const int64_t np = dp.np, nl = dp.nl, N = 6 * np + 3 * nl;
+    if (nl == 0) return 0;
+    const double zero = 0.0;
+    if ((e = copy(lambda_dev, &zero, sizeof zero, hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "lambda");
+    zb::launch_linearise64(dp, w, stream, false);
+    zb::launch_reduced_system64(dp, w, stream, lambda_dev, false);
+    zb::launch_adjoint_frozen(dp, w, stream);
+    double* dv = w.adj;
+    double* out = dv + N;
+    std::vector<double> v(N, 0.0), blocks(9 * nl, 0.0), col(N);
+    for (int k = 0; k < 3; ++k) {
+        for (int64_t j = 0; j < nl; ++j) {
+            v[6 * np + 3 * j + 0] = 0.0;
+            v[6 * np + 3 * j + 1] = 0.0;
+            v[6 * np + 3 * j + 2] = 0.0;
+            v[6 * np + 3 * j + k] = 1.0;
+        }
+        if ((e = copy(dv, v.data(), N * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return hip_fail(e, "e_k");
+        zb::launch_grad_tangent(dp, w, dv, nullptr, out, stream);
+        if ((e = copy(col.data(), out, N * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+            return hip_fail(e, "H e_k");
+        for (int64_t j = 0; j < nl; ++j)
+            for (int i = 0; i < 3; ++i) blocks[9 * j + 3 * i + k] = col[6 * np + 3 * j + i];
+    }
+    std::vector<uint8_t> frozen(nl);
+    if ((e = copy(frozen.data(), w.adj_frozen, nl, hipMemcpyDeviceToHost)) != hipSuccess) return hip_fail(e, "frozen");
+    int32_t count = 0;
+    for (int64_t j = 0; j < nl; ++j) {
+        const int32_t id = point_slot_id[j];
+        if (point_fixed[id]) continue;
+        bool hold = frozen[j] != 0;
+        if (!hold) {
+            double a[9];
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) a[3 * r + c] = 0.5 * (blocks[9 * j + 3 * r + c] + blocks[9 * j + 3 * c + r]);
+            const double tol = rel_tol * (std::fabs(a[0]) + std::fabs(a[4]) + std::fabs(a[8]));
+            // pivots of the symmetric 3x3 block (Gaussian elimination)
+            const double p1 = a[0];
+            const double p2 = p1 > tol ? a[4] - a[1] * a[3] / p1 : -1.0;
+            const double m12 = a[5] - a[3] * a[2] / p1;
+            const double p3 = p2 > tol ? (a[8] - a[2] * a[6] / p1) - m12 * m12 / p2 : -1.0;
+            hold = !(p1 > tol && p2 > tol && p3 > tol);
+        }
+        if (hold) {
+            point_fixed[id] = 1;
+            structure_dirty = true;
+            ++count;
+            if (held) held[id] = 1;
+        }
+    }
+    return count;
 }
 
 // ---- adjoint of the converged solve --------------------------------------------
@@ -1790,6 +1854,14 @@ int32_t zsba_gauss_newton(zsba_graph* g, int32_t steps) {
     int32_t rc = g->prepare();
     if (rc) return rc;
     return g->gauss_newton(steps);
+}
+
+int32_t zsba_hold_indefinite_points(zsba_graph* g, double rel_tol, uint8_t* held) {
+    if (!g || !(rel_tol >= 0.0) || !(rel_tol < 1.0)) return ZSBA_EINVAL;
+    if (g->live_obs() == 0) return 0;
+    int32_t rc = g->prepare();
+    if (rc) return rc;
+    return g->hold_indefinite_points(rel_tol > 0.0 ? rel_tol : 1e-12, held);
 }
 
 int32_t zsba_adjoint(zsba_graph* g, const double* dL_dpoints, const double* dL_dposes, double* dL_dobs, double* dL_dK) {

@@ -167,6 +167,24 @@ bool project(const zsba_synth_params& p, const pose& T, vec3 X, double& u, doubl
     return u >= 0.0 && u < p.width && v >= 0.0 && v < p.height;
 }
 
+// Camera centre of a world-to-camera transform: -R^T t.
+vec3 centre(const pose& T) {
+    double r[9];
+    qmatrix(T.q, r);
+    return {-(r[0] * T.t.x + r[3] * T.t.y + r[6] * T.t.z), -(r[1] * T.t.x + r[4] * T.t.y + r[7] * T.t.z),
+            -(r[2] * T.t.x + r[5] * T.t.y + r[8] * T.t.z)};
+}
+
+// The angle at X between the rays from the first and the last camera centre of
+// the track is at least min_parallax (always true when the test is off).
+bool parallax_ok(const zsba_synth& s, vec3 X, const std::vector<int32_t>& frames) {
+    if (!(s.params.min_parallax > 0.0) || frames.size() < 2) return true;
+    const vec3 a = normalise(sub(X, centre(s.truth_poses[static_cast<size_t>(frames.front())])));
+    const vec3 b = normalise(sub(X, centre(s.truth_poses[static_cast<size_t>(frames.back())])));
+    const double c = dot(a, b);
+    return std::acos(c < 1.0 ? c : 1.0) >= s.params.min_parallax;
+}
+
 bool visible_in_run(const zsba_synth& s, vec3 X, const std::vector<int32_t>& frames) {
     double u, v;
     for (int32_t f : frames)
@@ -282,7 +300,7 @@ void generate(zsba_synth& s, uint64_t seed) {
                 bool found = false;
                 for (int tries = 0; tries < 200; ++tries) {
                     X = draw_point(s, run, rng);
-                    if (visible_in_run(s, X, run)) { found = true; break; }
+                    if (visible_in_run(s, X, run) && parallax_ok(s, X, run)) { found = true; break; }
                 }
                 if (found || attempt > 1000) break;
             }
@@ -362,9 +380,12 @@ int32_t zsba_synth_default_params(int32_t config, zsba_synth_params* p) {
         p->layout = 1; p->n_frames = 15; p->n_points = 1500; p->track_min = 2; p->track_max = 15;
         p->noise_px = 1.0; p->outlier_rate = 0.05; p->loss_kind = ZSBA_LOSS_HUBER; p->loss_delta = 2.0;
         break;
-    case 3:  // 50 keyframes, 5k points, ~40k observations, no outliers, no loss
+    case 3:  // 50 keyframes, 5k points, ~40k observations, no outliers, no loss; landmarks
+             // seen with at least 2 degrees of parallax over their track, so that each
+             // has a finite optimum to differentiate (the focus of expansion has none)
         p->layout = 1; p->n_frames = 50; p->n_points = 5000; p->track_min = 4; p->track_max = 12;
         p->noise_px = 1.0; p->loss_kind = ZSBA_LOSS_NONE; p->loss_delta = 0.0;
+        p->min_parallax = 2.0 * 3.14159265358979323846 / 180.0;
         break;
     case 4:  // 1000 keyframes on a 2-lap loop, 200k points, geometric tracks 3..40
         p->layout = 2; p->n_frames = 1000; p->n_points = 200000; p->track_min = 3; p->track_max = 40;

@@ -248,6 +248,13 @@ class Graph:
         """Undamped Gauss-Newton steps, every one applied (zsba_gauss_newton)."""
         self._check(self.lib.zsba_gauss_newton(self.h, int(steps)))
 
+    def hold_indefinite_points(self, rel_tol: float = 0.0) -> np.ndarray:
+        """Set fixed the points without a local optimum at the current state
+        (zsba_hold_indefinite_points); returns their flags by point id."""
+        held = np.zeros(self.counts()[1], np.uint8)
+        self._check(self.lib.zsba_hold_indefinite_points(self.h, float(rel_tol), ptr(held, C.c_uint8)))
+        return held
+
     def solve_to_convergence(self, precision: int = FP64_FAST, max_rounds: int = 20, iterations: int = 50,
                              polish: int = 4) -> int:
         """Repeat solve(iterations) until a solve accepts no step (the LM restarts
