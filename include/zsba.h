@@ -314,6 +314,35 @@ int32_t zsba_match_by_projection(zsba_matcher* matcher, const zsba_match_frame* 
 /* Device time of the last call (from its first kernel to its last, including the
  * read-back of the candidate count) and its number of candidate pairs. */
 int32_t zsba_matcher_stats(const zsba_matcher* matcher, double* device_ms, int32_t* candidates);
+/* zsba_match_by_projection_noalloc: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc): the search of
+ * zsba_match_by_projection with every input and output array in device memory
+ * of the current device, enqueued on `stream` (a hipStream_t of that device;
+ * NULL = the null stream) with no allocation, no memset, no host
+ * synchronisation and no read-back.  Its intermediate buffers are one static
+ * device array per GPU of at most 10 MiB, held until the process ends: calls
+ * that overlap on one GPU race (the array keeps the last call's contents; each
+ * call clears what must start at zero).  Limits, else ZSBA_EUNSUP at once:
+ * n_landmarks <= 262144, n_keypoints <= 16384, n_keys <= 65536, frame rows <=
+ * 2048.  *match_count (device) receives the number of matches (the first
+ * max_matches are written, in the order of zsba_match_by_projection) or, when
+ * the banded search cannot finish the call, a negative code: ZSBA_EUNSUP for a
+ * landmark whose projection is not finite or more than 4096 candidate pairs
+ * (zsba_match_by_projection handles both), ZSBA_EINVAL for a keypoint_key
+ * outside [0, n_keys).  Returns 0 once the work is enqueued, else a negative
+ * ZSBA_E* code (nothing is enqueued then). */
+int32_t zsba_match_by_projection_noalloc(const zsba_match_frame* frame,
+                                         int32_t n_landmarks, const double* landmark_xyz /* device n*3 */,
+                                         const uint8_t* landmark_descriptor /* device n*32 */,
+                                         const uint8_t* landmark_observed /* device n */,
+                                         int32_t n_keypoints, const double* keypoint_xy /* device n*2 */,
+                                         const uint8_t* keypoint_descriptor /* device n*32 */,
+                                         const int32_t* keypoint_key /* device n */,
+                                         int32_t n_keys, const uint8_t* key_taken /* device n_keys */,
+                                         int32_t max_matches, int32_t* match_landmark /* device */,
+                                         int32_t* match_keypoint /* device */, int32_t* match_count /* device, 1 */,
+                                         void* stream /* hipStream_t */);
 
 /* ---- implicit differentiation (no reference counterpart) ----------------------
  * zsba_gauss_newton applies `steps` undamped Gauss-Newton steps unconditionally

@@ -121,6 +121,9 @@ _SIGNATURES = {
     "zsba_matcher_stats": (C.c_int32, [_P, _D, _I32]),
     "zsba_match_by_projection": (C.c_int32, [_P, C.POINTER(MatchFrame), C.c_int32, _D, _U8, _U8, C.c_int32, _D, _U8,
                                              _I32, C.c_int32, _U8, C.c_int32, _I32, _I32]),
+    # device pointers (as integers) for every array, then the stream
+    "zsba_match_by_projection_noalloc": (C.c_int32, [C.POINTER(MatchFrame), C.c_int32, _P, _P, _P, C.c_int32, _P, _P,
+                                                     _P, C.c_int32, _P, C.c_int32, _P, _P, _P, _P]),
     "zsba_gauss_newton": (C.c_int32, [_P, C.c_int32]),
     "zsba_hold_indefinite_points": (C.c_int32, [_P, C.c_double, C.POINTER(C.c_uint8)]),
     "zsba_debug_adjoint_ops": (C.c_int32, [_P, _D, _D, _D]),

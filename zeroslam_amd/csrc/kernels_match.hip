@@ -29,6 +29,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -423,6 +424,138 @@ __global__ void __launch_bounds__(kMatchBlock) k_match_accept_packed(int32_t nc,
     accepted[c] = best[kp_key[static_cast<uint32_t>(w)]] == w ? 1 : 0;
 }
 
+// ---- the search that allocates nothing ------------------------------------------
+// zsba_match_by_projection_noalloc runs the banded search above on device
+// inputs and outputs with every intermediate buffer carved from one static
+// device array per GPU (`scratch`, below), and decides on the device what the
+// host decides in zsba_match_by_projection: a call that the banded search
+// cannot finish (a landmark with a projection that is not finite, more than
+// kSortMax candidates, or a keypoint index outside [0, n_keys)) reports an
+// error code instead of falling back to the ordered search.
+constexpr int64_t kNaMaxLandmarks = int64_t(1) << 18;
+constexpr int64_t kNaMaxKeypoints = int64_t(1) << 14;
+constexpr int64_t kNaMaxKeys = int64_t(1) << 16;
+constexpr int64_t kNaMaxBands = 256;  // images up to 2048 rows
+constexpr int64_t kNaMaxGrid = (kNaMaxLandmarks + kMatchBlock - 1) / kMatchBlock;
+constexpr int64_t kNaMaxCells = kNaMaxBands * kNaMaxGrid;
+constexpr int64_t kNaMaxCellBlocks = (kNaMaxCells + kScanBlock - 1) / kScanBlock;
+
+constexpr int64_t na_align(int64_t x) { return (x + 255) / 256 * 256; }
+// byte offsets of the buffers in the scratch array
+constexpr int64_t kNaLmUv = 0;
+constexpr int64_t kNaLmBand = kNaLmUv + na_align(16 * kNaMaxLandmarks);
+constexpr int64_t kNaLmSorted = kNaLmBand + na_align(4 * kNaMaxLandmarks);
+constexpr int64_t kNaCellCount = kNaLmSorted + na_align(4 * kNaMaxLandmarks);
+constexpr int64_t kNaCellStart = kNaCellCount + na_align(4 * kNaMaxCells);
+constexpr int64_t kNaCursor = kNaCellStart + na_align(4 * (kNaMaxCells + 1));
+constexpr int64_t kNaBinStart = kNaCursor + na_align(4 * kNaMaxCellBlocks);
+constexpr int64_t kNaKpSorted = kNaBinStart + na_align(4 * (kNaMaxBands + 2));
+constexpr int64_t kNaKpSortedId = kNaKpSorted + na_align(16 * kNaMaxKeypoints);
+constexpr int64_t kNaKpKey = kNaKpSortedId + na_align(4 * kNaMaxKeypoints);
+constexpr int64_t kNaCounters = kNaKpKey + na_align(4 * kNaMaxKeypoints);
+constexpr int64_t kNaCand = kNaCounters + na_align(4 * 4);
+constexpr int64_t kNaAccepted = kNaCand + na_align(8 * kSortMax);
+constexpr int64_t kNaBest = kNaAccepted + na_align(kSortMax);
+constexpr int64_t kNaScratchBytes = kNaBest + na_align(8 * kNaMaxKeys);
+static_assert(kNaScratchBytes <= (int64_t(10) << 20), "the no-alloc scratch stays within 10 MiB");
+
+// counters[0]: landmarks with a projection that is not finite; [1]: candidates;
+// [2]: a keypoint index outside [0, n_keys).  Keys are copied with bad ones
+// replaced by 0 so that no kernel reads outside key_taken or best.
+__global__ void __launch_bounds__(kMatchBlock) k_na_keys(int32_t nk, const int32_t* __restrict__ kp_key, int32_t n_keys,
+                                                       int32_t* __restrict__ key_safe, int32_t* __restrict__ counters) {
+    const int32_t i = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (i >= nk) return;
+    const int32_t k = kp_key[i];
+    const bool ok = k >= 0 && k < n_keys;
+    key_safe[i] = ok ? k : 0;
+    if (!ok) atomicOr(&counters[2], 1);
+}
+
+__device__ __forceinline__ int32_t na_status(const int32_t* counters) {
+    if (counters[2]) return ZSBA_EINVAL;
+    if (counters[0] != 0 || counters[1] > kSortMax) return ZSBA_EUNSUP;
+    return 0;
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_na_first_per_key(const unsigned long long* cand, const int32_t* kp_key,
+                                                                unsigned long long* best, const int32_t* counters) {
+    if (na_status(counters)) return;
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= counters[1]) return;
+    const unsigned long long w = cand[c];
+    atomicMin(&best[kp_key[static_cast<uint32_t>(w)]], w);
+}
+
+__global__ void __launch_bounds__(kMatchBlock) k_na_accept(const unsigned long long* cand, const int32_t* kp_key,
+                                                         const unsigned long long* best, uint8_t* accepted,
+                                                         const int32_t* counters) {
+    if (na_status(counters)) return;
+    const int32_t c = blockIdx.x * kMatchBlock + threadIdx.x;
+    if (c >= counters[1]) return;
+    const unsigned long long w = cand[c];
+    accepted[c] = best[kp_key[static_cast<uint32_t>(w)]] == w ? 1 : 0;
+}
+
+// One workgroup: the accepted candidates, in walk order, become the matches
+// (the first max_matches written); count receives their number or the error.
+__global__ void __launch_bounds__(kScanBlock) k_na_compact(const unsigned long long* __restrict__ cand,
+                                                          const uint8_t* __restrict__ accepted,
+                                                          const int32_t* __restrict__ counters, int32_t max_matches,
+                                                          int32_t* __restrict__ match_landmark,
+                                                          int32_t* __restrict__ match_keypoint, int32_t* __restrict__ count) {
+    constexpr int kPer = kSortMax / kScanBlock;
+    const int32_t status = na_status(counters);
+    const int32_t nc = status ? 0 : counters[1];
+    const int32_t c0 = threadIdx.x * kPer;
+    int32_t mine = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) mine += (c0 + i < nc && accepted[c0 + i]) ? 1 : 0;
+    int32_t total = 0;
+    int32_t n = block_exclusive_scan(mine, total);
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int32_t c = c0 + i;
+        if (c < nc && accepted[c]) {
+            if (n < max_matches) {
+                match_landmark[n] = static_cast<int32_t>(cand[c] >> 32);
+                match_keypoint[n] = static_cast<int32_t>(cand[c] & 0xFFFFFFFFull);
+            }
+            ++n;
+        }
+    }
+    if (threadIdx.x == 0) *count = status ? status : total;
+}
+
+}  // namespace
+
+// Scratch of zsba_match_by_projection_noalloc: one copy per GPU, held until the
+// process ends.  Host code uses only the address hipGetSymbolAddress gives for
+// the current device (na_scratch), never this name.
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace {
+
+constexpr int kNaMaxDevices = 64;
+std::mutex na_mutex;
+unsigned char* na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+hipError_t na_scratch(unsigned char** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNaMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        na_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = na_ptr[dev];
+    return hipSuccess;
+}
 
 }  // namespace
 
@@ -709,6 +842,82 @@ int32_t zsba_match_by_projection(zsba_matcher* m, const zsba_match_frame* frame,
         ++n;
     }
     return n;
+}
+
+int32_t zsba_match_by_projection_noalloc(const zsba_match_frame* frame, int32_t n_landmarks, const double* landmark_xyz,
+                                         const uint8_t* landmark_descriptor, const uint8_t* landmark_observed,
+                                         int32_t n_keypoints, const double* keypoint_xy,
+                                         const uint8_t* keypoint_descriptor, const int32_t* keypoint_key, int32_t n_keys,
+                                         const uint8_t* key_taken, int32_t max_matches, int32_t* match_landmark,
+                                         int32_t* match_keypoint, int32_t* match_count, void* stream) {
+    if (!frame || !match_count || n_landmarks < 0 || n_keypoints < 0 || n_keys < 0 || max_matches < 0)
+        return ZSBA_EINVAL;
+    if ((n_landmarks > 0 && (!landmark_xyz || !landmark_descriptor || !landmark_observed)) ||
+        (n_keypoints > 0 && (!keypoint_xy || !keypoint_descriptor || !keypoint_key || !key_taken)) ||
+        (max_matches > 0 && (!match_landmark || !match_keypoint)))
+        return ZSBA_EINVAL;
+    if (n_keypoints > 0 && n_keys == 0) return ZSBA_EINVAL;  // every previous-frame index is out of range
+    if (n_landmarks > kNaMaxLandmarks || n_keypoints > kNaMaxKeypoints || n_keys > kNaMaxKeys) return ZSBA_EUNSUP;
+    const int64_t nbands64 = (frame->rows + kBandPx - 1) / kBandPx;
+    if (frame->rows < 1 || frame->cols < 1 || nbands64 > kNaMaxBands) return ZSBA_EUNSUP;
+    unsigned char* base = nullptr;
+    if (na_scratch(&base) != hipSuccess) return ZSBA_EDEVICE;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    int32_t* const counters = reinterpret_cast<int32_t*>(base + kNaCounters);
+    if (n_landmarks == 0 || n_keypoints == 0) {
+        // no candidates: count = 0, written by a kernel (no memset, so the call
+        // can be captured into a graph)
+        k_match_zero<<<1, kMatchBlock, 0, s>>>(1, match_count);
+        return hipGetLastError() == hipSuccess ? 0 : ZSBA_EDEVICE;
+    }
+    match_frame f;
+    for (int i = 0; i < 9; ++i) f.R[i] = frame->rotation[i];
+    for (int i = 0; i < 3; ++i) f.t[i] = frame->translation[i];
+    for (int i = 0; i < 4; ++i) f.K[i] = frame->K[i];
+    f.u_max = static_cast<double>(static_cast<uint64_t>(frame->cols) - 11u);
+    f.v_max = static_cast<double>(static_cast<uint64_t>(frame->rows) - 11u);
+    const int32_t nbands = static_cast<int32_t>(nbands64);
+    const int32_t grid = blocks_for(n_landmarks, kMatchBlock);
+    const int64_t ncell = nbands64 * grid;
+    const int32_t ncb = blocks_for(ncell, kScanBlock);
+    // the carve-up holds these sizes (checked against the limits above)
+    if (grid > kNaMaxGrid || ncell > kNaMaxCells || ncb > kNaMaxCellBlocks) return ZSBA_EUNSUP;
+    double2* const lm_uv = reinterpret_cast<double2*>(base + kNaLmUv);
+    int32_t* const lm_band = reinterpret_cast<int32_t*>(base + kNaLmBand);
+    int32_t* const lm_sorted = reinterpret_cast<int32_t*>(base + kNaLmSorted);
+    int32_t* const cell_count = reinterpret_cast<int32_t*>(base + kNaCellCount);
+    int32_t* const cell_start = reinterpret_cast<int32_t*>(base + kNaCellStart);
+    int32_t* const cursor = reinterpret_cast<int32_t*>(base + kNaCursor);
+    int32_t* const bin_start = reinterpret_cast<int32_t*>(base + kNaBinStart);
+    double2* const kp_sorted = reinterpret_cast<double2*>(base + kNaKpSorted);
+    int32_t* const kp_sorted_id = reinterpret_cast<int32_t*>(base + kNaKpSortedId);
+    int32_t* const kp_key = reinterpret_cast<int32_t*>(base + kNaKpKey);
+    unsigned long long* const cand = reinterpret_cast<unsigned long long*>(base + kNaCand);
+    uint8_t* const accepted = base + kNaAccepted;
+    unsigned long long* const best = reinterpret_cast<unsigned long long*>(base + kNaBest);
+    const int32_t ncell32 = static_cast<int32_t>(ncell);
+    k_match_zero<<<1, kMatchBlock, 0, s>>>(4, counters);
+    k_na_keys<<<blocks_for(n_keypoints, kMatchBlock), kMatchBlock, 0, s>>>(n_keypoints, keypoint_key, n_keys, kp_key,
+                                                                          counters);
+    k_match_bin_keypoints<<<1, kScanBlock, 0, s>>>(n_keypoints, keypoint_xy, nbands, bin_start, kp_sorted, kp_sorted_id);
+    k_match_project<<<grid, kMatchBlock, 0, s>>>(f, n_landmarks, landmark_xyz, landmark_observed, nbands, lm_uv, lm_band,
+                                                 cell_count, counters);
+    k_match_block_totals<<<ncb, kScanBlock, 0, s>>>(ncell32, cell_count, cursor);
+    k_match_scan_totals<<<1, kScanBlock, 0, s>>>(ncb, cursor, ncell32, cell_start);
+    k_match_scan_apply<<<ncb, kScanBlock, 0, s>>>(ncell32, cell_count, cursor, cell_start);
+    k_match_lm_scatter<<<grid, kMatchBlock, 0, s>>>(n_landmarks, nbands, lm_band, cell_start, lm_sorted);
+    k_match_band_search<<<grid, kMatchBlock, 0, s>>>(
+        nbands, cell_start + ncell, lm_sorted, lm_uv, lm_band, reinterpret_cast<const uint32_t*>(landmark_descriptor),
+        bin_start, kp_sorted, kp_sorted_id, reinterpret_cast<const uint32_t*>(keypoint_descriptor), kp_key, key_taken,
+        kSortMax, cand, counters);
+    k_match_sort_small<<<1, kScanBlock, 0, s>>>(cand, counters);
+    k_match_best_fill<<<std::min(blocks_for(n_keys, kMatchBlock), 1024), kMatchBlock, 0, s>>>(n_keys, best);
+    const int32_t cgrid = blocks_for(kSortMax, kMatchBlock);
+    k_na_first_per_key<<<cgrid, kMatchBlock, 0, s>>>(cand, kp_key, best, counters);
+    k_na_accept<<<cgrid, kMatchBlock, 0, s>>>(cand, kp_key, best, accepted, counters);
+    k_na_compact<<<1, kScanBlock, 0, s>>>(cand, accepted, counters, max_matches, match_landmark, match_keypoint,
+                                         match_count);
+    return hipGetLastError() == hipSuccess ? 0 : ZSBA_EDEVICE;
 }
 
 }  // extern "C"
