@@ -404,6 +404,40 @@ int32_t zsba_debug_reduced_system(zsba_graph* graph, double lambda, int32_t prec
  * by a device kernel (on_device = 1). */
 int32_t zsba_debug_math(int32_t fn, int32_t n, const double* in, double* out, int32_t on_device);
 
+/* ---- output formats (reference source/main.cpp:132-289, tools/trajectory_alignment)
+ * zsba_pose_to_frame: the frame pose map::optimise writes back from an
+ * optimised pose vertex (map.hpp:239-244): rotation (row-major 3x3, camera =
+ * R world + t) = lie::so3(qw, qx, qy, qz).get_matrix(), translation = t.
+ * zsba_write_trajectory_tum: save_trajectory_as_txt (main.cpp:132-156), one
+ * line "id x y z qx qy qz qw" per frame in ascending id, with the camera centre
+ * -R^T t and the quaternion of R as lie::so3(R) forms it; byte-identical to the
+ * reference's file for the same frames.
+ * zsba_write_map_ply: save_trajectory_and_map_as_ply (main.cpp:158-289): the
+ * landmarks (landmark_colour in [0,1], NULL = black) and five vertices and
+ * eight edges of each camera's frustum, in the order given (pass the map's
+ * iteration order of its landmarks and frames to reproduce its file byte for
+ * byte); K = (fx, fy, cx, cy).
+ * zsba_trajectory_ate: the alignment and error statistics of
+ * tools/trajectory_alignment/main.cpp:195-276: a similarity transform of the
+ * estimated camera centres onto the ground truth (absolute_orientation.hpp;
+ * first_pose_overlap = its --first option), then the error and path-length
+ * statistics of the aligned trajectory over n_poses >= 3 pose pairs. */
+typedef struct zsba_ate {
+    double scale, rotation[9], translation[3];   /* ground truth = scale R estimate + t */
+    double max, min, mean, median, rmse, sse, std; /* position errors (m)         */
+    double distance_ground_truth, distance_estimated; /* path lengths (m)          */
+    int32_t poses, pad;
+} zsba_ate;
+int32_t zsba_pose_to_frame(const double pose[7], double rotation[9], double translation[3]);
+int32_t zsba_write_trajectory_tum(const char* path, int32_t n_frames, const int32_t* frame_ids,
+                                  const double* rotation /* n*9 */, const double* translation /* n*3 */);
+int32_t zsba_write_map_ply(const char* path, int32_t image_width, int32_t image_height, int32_t n_landmarks,
+                           const double* landmark_xyz /* n*3 */, const double* landmark_colour /* n*3 or NULL */,
+                           int32_t n_frames, const int32_t* frame_ids, const double* rotation /* n*9 */,
+                           const double* translation /* n*3 */, const double K[4]);
+int32_t zsba_trajectory_ate(int32_t n_poses, const double* ground_truth_xyz /* n*3 */,
+                            const double* estimated_xyz /* n*3 */, int32_t first_pose_overlap, zsba_ate* out);
+
 /* ---- synthetic problems (benchmark and test workloads) ------------------------ */
 typedef struct zsba_synth_params {
     int32_t layout;       /* 0 circle look-at, 1 forward walk, 2 double loop facing out */

@@ -43,6 +43,20 @@ class Stats(C.Structure):
         return d
 
 
+class Ate(C.Structure):
+    _fields_ = [("scale", C.c_double), ("rotation", C.c_double * 9), ("translation", C.c_double * 3),
+                ("max", C.c_double), ("min", C.c_double), ("mean", C.c_double), ("median", C.c_double),
+                ("rmse", C.c_double), ("sse", C.c_double), ("std", C.c_double),
+                ("distance_ground_truth", C.c_double), ("distance_estimated", C.c_double),
+                ("poses", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pad", "rotation", "translation")}
+        d["rotation"] = list(self.rotation)
+        d["translation"] = list(self.translation)
+        return d
+
+
 class SynthParams(C.Structure):
     _fields_ = [("layout", C.c_int32), ("n_frames", C.c_int32), ("n_points", C.c_int32),
                 ("track_min", C.c_int32), ("track_max", C.c_int32), ("track_geometric", C.c_int32),
@@ -130,6 +144,11 @@ _SIGNATURES = {
     "zsba_debug_eval_edges": (C.c_int32, [_P, _D]),
     "zsba_debug_reduced_system": (C.c_int32, [_P, C.c_double, C.c_int32, _D, _D, _D, _D]),
     "zsba_debug_math": (C.c_int32, [C.c_int32, C.c_int32, _D, _D, C.c_int32]),
+    "zsba_pose_to_frame": (C.c_int32, [_D, _D, _D]),
+    "zsba_write_trajectory_tum": (C.c_int32, [C.c_char_p, C.c_int32, _I32, _D, _D]),
+    "zsba_write_map_ply": (C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, C.c_int32, _I32, _D, _D,
+                                       _D]),
+    "zsba_trajectory_ate": (C.c_int32, [C.c_int32, _D, _D, C.c_int32, C.POINTER(Ate)]),
     "zsba_synth_default_params": (C.c_int32, [C.c_int32, C.POINTER(SynthParams)]),
     "zsba_synth_create": (_P, [C.POINTER(SynthParams), C.c_uint64]),
     "zsba_synth_destroy": (None, [_P]),

@@ -38,6 +38,7 @@ COMMON = [
 # (source, extra flags).  FP64 parity code: no contraction anywhere.
 UNITS = [
     ("synth.cpp", ["-ffp-contract=off", "-x", "c++"]),
+    ("output.cpp", ["-ffp-contract=off", "-x", "c++"]),
     ("engine.hip", ["-ffp-contract=off"]),
     ("kernels_fp64.hip", ["-ffp-contract=off"]),
     ("kernels_fp32.hip", ["-ffp-contract=off"]),
@@ -100,7 +101,7 @@ def build_oracle(verbose: bool = False) -> None:
     map.hpp with its own factor graph and with the drop-in) into ``oracle/_ref``."""
     targets = ["oracle"]
     if Path("/root/reference/source/factor_graph.hpp").exists():
-        targets += ["ref", "map"]
+        targets += ["ref", "map", "output"]
     cmd = ["make", "-s", "-C", str(ROOT / "oracle"), *targets]
     if verbose:
         print(" ".join(cmd))
