@@ -290,7 +290,7 @@ def roofline_of(problem, precision_name, kernel_ms, kernel_launches, stats, step
     B = algorithmic_bytes(problem.n_obs, problem.n_poses, problem.n_points, n_pairs, s)
     avg = {k: kernel_ms[k] / kernel_launches[k] for k in kernel_ms if kernel_launches[k]}
     attempts = stats["attempts"] * steps
-    refine_rounds = attempts * 2 if precision_name == "mixed" else 0
+    refine_rounds = attempts if precision_name == "mixed" else 0  # one refinement round per attempt
     groups = {
         "linearisation": (["linearise", "pose_reduce"], stats["linearisations"] * steps),
         "cost": (["cost"], attempts + steps),

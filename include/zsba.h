@@ -72,7 +72,7 @@ typedef struct zsba_options {
                              and after, final damping); the reference's per-attempt
                              [INIT]/[GOOD]/[ BAD] trace is not reproduced            */
     int32_t refine_steps; /* MIXED: refinement steps of the reduced solve: 0 = default
-                             (2), negative = none                                      */
+                             (1), negative = none                                      */
     int32_t timing;       /* 1: time kernel launches with HIP events (stats)          */
     int32_t floor_exp;    /* MIXED: the binary32-block reduced system is factorised
                              with its diagonal raised by 2^-floor_exp max|S_ii| (its

@@ -22,11 +22,13 @@
 // Output, after every call: int32 n_frames, then per frame in ascending id
 // { int32 id; double R[9]; double t[3] }; int32 n_landmarks, then per landmark
 // in ascending id { int32 id; double xyz[3] }.  map.hpp's own printf lines go
-// to stdout.  Exit status 0, or 2 for an unreadable scenario.
+// to stdout; the wall time of each call goes to stderr ("call k op ms").  Exit
+// status 0, or 2 for an unreadable scenario.
 // =============================================================================
 #include "map.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <vector>
@@ -130,9 +132,12 @@ int main(int argc, char** argv) {
     for (int32_t c = 0; c < head[3]; ++c) {
         int32_t call[5];
         if (!get(in, call, 5)) return 2;
+        const auto t0 = std::chrono::steady_clock::now();
         if (call[0] == 0) m.optimise(call[1], call[2] != 0, call[3], call[4] != 0);
         else m.cull();
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         std::fflush(stdout);
+        std::fprintf(stderr, "call %d %d %.4f\n", static_cast<int>(c), static_cast<int>(call[0]), ms);
         dump(out, m);
     }
     std::fclose(in);

@@ -24,7 +24,7 @@ S_TOL = 1e-4
 @pytest.mark.parametrize("cfg,kw", [(0, {}), (3, dict(n_frames=12, n_points=800)), (1, dict(n_frames=20, n_points=1500))])
 def test_mixed_reduced_system(cfg, kw):
     pr = synth(cfg, **kw)
-    g = Graph.from_problem(pr)
+    g = Graph.from_problem(pr, refine_steps=2)
     o = CpuGraph(pr, "oracle")
     lam = 1e-3
     ok_o, S_o, bs_o, _, _ = o.reduced_system(lam)

@@ -336,6 +336,13 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
 // Forward + backward substitution with the factor in w.S: x = S^-1 b.
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
 
+// ---- graphs with every point fixed (the tracking step's motion-only BA) ---------
+// The whole FP64 (reference-order) solve in one launch of one workgroup, for at
+// most kMotionMaxPoses poses (kernels_fp64.hip).  w.lm must hold iterations and
+// use_relative with every other field zero.
+constexpr int kMotionMaxPoses = 64;
+void launch_motion_solve(const dev_problem& p, const dev_work64& w, hipStream_t s);
+
 // ---- diagnostics ---------------------------------------------------------------
 void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t s);
 

@@ -43,30 +43,132 @@ struct loss_entry {
     double delta;
 };
 
+// Device memory chunks kept by the process after a handle releases them, per
+// device, so that building a graph (a bundle adjustment per map::optimise call)
+// does not call hipMalloc / hipFree once the pool holds chunks of its size.  At
+// most kPoolCap bytes are kept per device; beyond that chunks are freed.
+struct chunk {
+    void* p;
+    size_t bytes;
+};
+constexpr size_t kPoolCap = size_t(2) << 30;
+constexpr int kPoolDevices = 64;
+std::mutex pool_mutex;
+std::vector<chunk> pool_free[kPoolDevices];
+size_t pool_bytes[kPoolDevices] = {};
+
+hipError_t pool_acquire(int dev, size_t bytes, chunk& out) {
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex);
+        if (dev >= 0 && dev < kPoolDevices) {
+            std::vector<chunk>& v = pool_free[dev];
+            size_t best = v.size();
+            for (size_t i = 0; i < v.size(); ++i)
+                if (v[i].bytes >= bytes && (best == v.size() || v[i].bytes < v[best].bytes)) best = i;
+            if (best != v.size() && v[best].bytes <= 4 * bytes + (size_t(64) << 20)) {
+                out = v[best];
+                v.erase(v.begin() + static_cast<std::ptrdiff_t>(best));
+                pool_bytes[dev] -= out.bytes;
+                return hipSuccess;
+            }
+        }
+    }
+    out = {nullptr, bytes};
+    return hipMalloc(&out.p, bytes);
+}
+
+void pool_release(int dev, const chunk& c) {
+    std::lock_guard<std::mutex> lock(pool_mutex);
+    if (dev < 0 || dev >= kPoolDevices || c.bytes > kPoolCap) {
+        (void)hipFree(c.p);
+        return;
+    }
+    std::vector<chunk>& v = pool_free[dev];
+    while (pool_bytes[dev] + c.bytes > kPoolCap && !v.empty()) {  // drop the largest first
+        size_t big = 0;
+        for (size_t i = 1; i < v.size(); ++i)
+            if (v[i].bytes > v[big].bytes) big = i;
+        (void)hipFree(v[big].p);
+        pool_bytes[dev] -= v[big].bytes;
+        v.erase(v.begin() + static_cast<std::ptrdiff_t>(big));
+    }
+    v.push_back(c);
+    pool_bytes[dev] += c.bytes;
+}
+
+// A handle's device buffers: 256-byte aligned pieces of pooled chunks (at least
+// 8 MiB each), released back to the pool together.
 struct dev_alloc {
-    std::vector<void*> ptrs;
-    size_t bytes = 0;
+    int device = 0;
+    std::vector<chunk> chunks;
+    size_t used = 0;   // bytes used in the last chunk
+    size_t bytes = 0;  // bytes handed out
     hipError_t status = hipSuccess;
     template <class T>
     T* get(size_t n) {
         if (n == 0) n = 1;
-        void* p = nullptr;
-        const hipError_t e = hipMalloc(&p, n * sizeof(T));
-        if (e != hipSuccess) {
-            status = e;
-            return nullptr;
+        const size_t want = (n * sizeof(T) + 255) / 256 * 256;
+        if (chunks.empty() || used + want > chunks.back().bytes) {
+            chunk c;
+            const size_t size = std::max(want, size_t(8) << 20);
+            const hipError_t e = pool_acquire(device, size, c);
+            if (e != hipSuccess) {
+                status = e;
+                return nullptr;
+            }
+            chunks.push_back(c);
+            used = 0;
         }
-        ptrs.push_back(p);
-        bytes += n * sizeof(T);
+        void* p = static_cast<char*>(chunks.back().p) + used;
+        used += want;
+        bytes += want;
         return static_cast<T*>(p);
     }
     void release() {
-        for (void* p : ptrs) (void)hipFree(p);
-        ptrs.clear();
+        for (const chunk& c : chunks) pool_release(device, c);
+        chunks.clear();
+        used = 0;
         bytes = 0;
         status = hipSuccess;
     }
 };
+
+// Streams and pinned controller records kept by the process between handles.
+struct stream_slot {
+    hipStream_t stream;
+    void* pinned;
+};
+std::vector<stream_slot> stream_pool[kPoolDevices];
+
+hipError_t stream_acquire(int dev, hipStream_t& s, void*& pinned, size_t pinned_bytes) {
+    {
+        std::lock_guard<std::mutex> lock(pool_mutex);
+        if (dev >= 0 && dev < kPoolDevices && !stream_pool[dev].empty()) {
+            s = stream_pool[dev].back().stream;
+            pinned = stream_pool[dev].back().pinned;
+            stream_pool[dev].pop_back();
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipHostMalloc(&pinned, pinned_bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        s = nullptr;
+    }
+    return e;
+}
+
+void stream_release(int dev, hipStream_t s, void* pinned) {
+    std::lock_guard<std::mutex> lock(pool_mutex);
+    if (dev >= 0 && dev < kPoolDevices && stream_pool[dev].size() < 64) {
+        stream_pool[dev].push_back({s, pinned});
+        return;
+    }
+    (void)hipHostFree(pinned);
+    (void)hipStreamDestroy(s);
+}
 
 // Key of a block-sparse matrix block with the hash of the reference's
 // sparse_block::block_key (matrix_sparse_block.hpp:47-59).  Used only to learn
@@ -129,6 +231,7 @@ struct zsba_graph {
     double* snap_pose = nullptr;
     double* snap_point = nullptr;
     uint8_t* frozen_dev = nullptr;  // [nl] behind-camera freeze flags (w.frozen while a solve uses them)
+    bool all_points_fixed = false;  // every live point of the current structure is fixed
     bool have_snapshot = false;
 
     zsba_stats stats{};
@@ -154,8 +257,7 @@ struct zsba_graph {
             if (stream) (void)hipStreamSynchronize(stream);
             mem.release();
             if (eval_dev) (void)hipFree(eval_dev);
-            if (lm_pinned) (void)hipHostFree(lm_pinned);
-            if (stream) (void)hipStreamDestroy(stream);
+            if (stream) stream_release(opt.device, stream, lm_pinned);
         }
     }
 
@@ -191,9 +293,11 @@ struct zsba_graph {
         if (e != hipSuccess || count == 0) return fail(ZSBA_EDEVICE, "no HIP device available");
         if (opt.device < 0 || opt.device >= count) return fail(ZSBA_EINVAL, "device %d out of range", opt.device);
         if ((e = hipSetDevice(opt.device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
-        if ((e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
-        if ((e = hipHostMalloc(reinterpret_cast<void**>(&lm_pinned), sizeof(lm_state), hipHostMallocDefault)) != hipSuccess)
-            return hip_fail(e, "hipHostMalloc");
+        void* pinned = nullptr;
+        if ((e = stream_acquire(opt.device, stream, pinned, sizeof(lm_state))) != hipSuccess)
+            return hip_fail(e, "stream and pinned controller");
+        lm_pinned = static_cast<lm_state*>(pinned);
+        mem.device = opt.device;
         std::memset(lm_pinned, 0, sizeof(lm_state));
         hip_ready = true;
         return 0;
@@ -315,6 +419,7 @@ int32_t zsba_graph::build_structure() {
     std::vector<uint8_t> pfix(np), lfix(nl);
     for (int32_t i = 0; i < np; ++i) pfix[i] = pose_fixed[pose_slot_id[i]];
     for (int32_t l = 0; l < nl; ++l) lfix[l] = point_fixed[point_slot_id[l]];
+    all_points_fixed = std::all_of(lfix.begin(), lfix.end(), [](uint8_t f) { return f != 0; });
     std::vector<int32_t> stamp(np, -1), stamp_pair(np, -1);
     for (int32_t l = 0; l < nl; ++l) {
         lm_pair_start[l] = static_cast<int32_t>(pair_pose.size());
@@ -473,9 +578,10 @@ int32_t zsba_graph::build_structure() {
     dev_problem p;
     p.np = np; p.nl = nl; p.no = no; p.npair = npair; p.nblk = nblk; p.ncontrib = ncontrib;
     p.n = static_cast<int32_t>(n); p.identity_info = identity ? 1 : 0;
+    // uploads are queued; the host arrays live until the synchronisation below
     auto up = [&](auto* dst, const auto& src) {
         if (!src.empty() && dst)
-            (void)copy(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice);
+            (void)hipMemcpyAsync(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream);
         return dst;
     };
     int32_t *d_obs_pose = mem.get<int32_t>(no), *d_obs_point = mem.get<int32_t>(no), *d_obs_pair = mem.get<int32_t>(no),
@@ -731,15 +837,22 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     lm_pinned->use_relative = use_rel ? 1 : 0;
     if ((e = hipMemcpyAsync(w.lm, lm_pinned, sizeof(lm_state), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload controller");
-    zb::launch_cost64(dp, w, stream, 0, false, exact);
-    if (mixed) {
-        zb::launch_linearise_mixed(dp, w, stream, false);
-        zb::launch_init_lm_mixed(dp, w, stream);
+    // every point fixed (map::optimise(n, true, ...)): the reference-order solve
+    // in one launch
+    const bool motion = exact && all_points_fixed && dp.np > 0 && dp.np <= zb::kMotionMaxPoses;
+    if (motion) {
+        zb::launch_motion_solve(dp, w, stream);
     } else {
-        zb::launch_linearise64(dp, w, stream, false, exact);
-        zb::launch_init_lm64(dp, w, stream);
+        zb::launch_cost64(dp, w, stream, 0, false, exact);
+        if (mixed) {
+            zb::launch_linearise_mixed(dp, w, stream, false);
+            zb::launch_init_lm_mixed(dp, w, stream);
+        } else {
+            zb::launch_linearise64(dp, w, stream, false, exact);
+            zb::launch_init_lm64(dp, w, stream);
+        }
     }
-    const int32_t max_attempts = iterations * 10;
+    const int32_t max_attempts = motion ? 0 : iterations * 10;
     hipEvent_t ev[2];
     (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
     (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
@@ -1390,7 +1503,7 @@ zsba_graph* zsba_create(const zsba_options* options) {
     if (!g) return nullptr;
     if (options) g->opt = *options;
     if (g->opt.factor_path < 0 || g->opt.factor_path > 2) g->opt.factor_path = 0;
-    if (g->opt.refine_steps == 0) g->opt.refine_steps = 2;
+    if (g->opt.refine_steps == 0) g->opt.refine_steps = 1;
     else if (g->opt.refine_steps < 0) g->opt.refine_steps = 0;
     return g;
 }
@@ -1516,16 +1629,20 @@ int32_t zsba_set_point(zsba_graph* g, int32_t id, const double xyz[3]) {
 int32_t zsba_set_pose_fixed(zsba_graph* g, int32_t id, int32_t fixed) {
     if (!g) return ZSBA_EINVAL;
     if (id < 0 || id >= static_cast<int32_t>(g->pose_alive.size())) return g->fail(ZSBA_EINVAL, "unknown pose id %d", id);
+    const uint8_t f = fixed ? 1 : 0;
+    if (g->pose_fixed[id] == f) return 0;  // unchanged: the structure stands
     if (g->sync_to_host()) return ZSBA_EDEVICE;
-    g->pose_fixed[id] = fixed ? 1 : 0;
+    g->pose_fixed[id] = f;
     g->structure_dirty = true;
     return 0;
 }
 int32_t zsba_set_point_fixed(zsba_graph* g, int32_t id, int32_t fixed) {
     if (!g) return ZSBA_EINVAL;
     if (id < 0 || id >= static_cast<int32_t>(g->point_alive.size())) return g->fail(ZSBA_EINVAL, "unknown point id %d", id);
+    const uint8_t f = fixed ? 1 : 0;
+    if (g->point_fixed[id] == f) return 0;  // unchanged: the structure stands
     if (g->sync_to_host()) return ZSBA_EDEVICE;
-    g->point_fixed[id] = fixed ? 1 : 0;
+    g->point_fixed[id] = f;
     g->structure_dirty = true;
     return 0;
 }

@@ -23,20 +23,6 @@ namespace {
 
 constexpr int kBatchTiles = kBatchMaxN / wg::kTile;
 
-// Deterministic block-wide sum (fixed tree).
-__device__ double block_sum(double v, double* sh) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] = sh[t] + sh[t + w];
-        __syncthreads();
-    }
-    const double r = sh[0];
-    __syncthreads();
-    return r;
-}
-
 // Hessian-block buffers of a window for storage type TS.
 template <class TS> struct blocks;
 template <> struct blocks<float> {
@@ -110,10 +96,49 @@ __global__ void __launch_bounds__(256) k_batch_lin_entries(const batch_window* w
     mixed::reduce_pose_finish(bw.w.pose_part, B.Hpp, bw.w.bp_m, ij / 27, ij % 27);
 }
 
-__device__ double cost(const dev_problem& p, const dev_work64& w, double* sh) {
-    double acc = 0.0;
-    for (int s = threadIdx.x; s < p.no; s += blockDim.x) acc = acc + mixed::cost_item(p, w.pose, w.point, w.residual, s);
-    return block_sum(acc, sh);
+// The sum the per-graph launches form over n items (k_cost and k_backsub_update,
+// each followed by k_sum_partials): a grid of G = min(ceil(n / 256), 1024)
+// blocks of 256 threads, thread (b, t) adding items b 256 + t, + 256 G, ... in
+// turn from 0, each block's 256 values added by a halving tree, and the G block
+// sums, padded with zeros to 1024, by a halving tree.  One workgroup (a
+// multiple of 256 threads) forms it here, blockDim / 256 blocks at a time, so
+// that the batched solver's costs and LM scales, and with them its accept
+// decisions, equal the per-graph solver's bit for bit.  item(i) is called once
+// per item.  sh holds blockDim values, part 1024.
+template <class F>
+__device__ double grid_sum(int64_t n, F item, double* sh, double* part) {
+    const int tid = threadIdx.x, t = tid & 255, per = blockDim.x >> 8;
+    int64_t G = (n + 255) / 256;
+    G = G < 1 ? 1 : (G > 1024 ? 1024 : G);
+    for (int64_t b0 = 0; b0 < G; b0 += per) {
+        const int64_t b = b0 + (tid >> 8);
+        double acc = 0.0;
+        if (b < G)
+            for (int64_t i = b * 256 + t; i < n; i += G * 256) acc = acc + item(i);
+        sh[tid] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) sh[tid] = sh[tid] + sh[tid + w];
+            __syncthreads();
+        }
+        if (t == 0 && b < G) part[b] = sh[tid];
+        __syncthreads();
+    }
+    for (int i = tid; i < 1024; i += blockDim.x)
+        if (i >= G) part[i] = 0.0;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        for (int i = tid; i < w; i += blockDim.x) part[i] = part[i] + part[i + w];
+        __syncthreads();
+    }
+    const double r = part[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double cost(const dev_problem& p, const dev_work64& w, double* sh, double* part) {
+    return grid_sum(p.no, [&](int64_t s) { return mixed::cost_item(p, w.pose, w.point, w.residual, static_cast<int>(s)); },
+                    sh, part);
 }
 
 // Phase clocks (thread 0, after the phase's closing barrier), added per window
@@ -157,6 +182,7 @@ __device__ __forceinline__ int batch_solve(int n, const double* S, const double*
 __global__ void __launch_bounds__(256) k_batch_begin(const batch_window* wins, int64_t* phase_ticks,
                                                      uint64_t* lin_clock, int lin_slots) {
     __shared__ double sh[256];
+    __shared__ double part[1024];
     const int tid = threadIdx.x;
     if (blockIdx.x == 0)
         for (int i = tid; i < lin_slots; i += blockDim.x) lin_clock[i] = ~0ULL;
@@ -174,7 +200,7 @@ __global__ void __launch_bounds__(256) k_batch_begin(const batch_window* wins, i
     }
     phase_clock clk(ticks);
     __syncthreads();
-    const double c = cost(p, w, sh);
+    const double c = cost(p, w, sh, part);
     clk.mark(PH_COST);
     if (tid == 0) {
         lm->chi2 = c;
@@ -302,13 +328,17 @@ __global__ void __launch_bounds__(kThreads) k_batch_attempts(const batch_window*
         }
         __syncthreads();
         if (lm->step_applied) {
-            double acc = 0.0;
-            for (int t = tid; t < p.np + p.nl; t += blockDim.x)
-                acc = acc + mixed::apply_step_item(p, w.pose, w.pose_backup, w.point, w.point_backup, w.dxp, w.dxl, w.bp_m,
-                                                   w.bl_m, lam, t);
-            const double scale = block_sum(acc, sh);
+            // the factorisation's column buffer is free here: the 1024 block sums
+            double* part = &col[0][0];
+            const double scale = grid_sum(
+                static_cast<int64_t>(p.np) + p.nl,
+                [&](int64_t t) {
+                    return mixed::apply_step_item(p, w.pose, w.pose_backup, w.point, w.point_backup, w.dxp, w.dxl, w.bp_m,
+                                                  w.bl_m, lam, static_cast<int>(t));
+                },
+                sh, part);
             clk.mark(PH_UPDATE);
-            const double c = cost(p, w, sh);
+            const double c = cost(p, w, sh, part);
             clk.mark(PH_COST);
             if (tid == 0) {
                 lm->scale = scale + 1e-3;  // factor_graph.hpp:556-560

@@ -1099,6 +1099,261 @@ void launch_reprojection_cull(const dev_problem& p, const dev_work64& w, float t
         k_reprojection_cull<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, threshold, mean, erase);
 }
 
+// ---- the whole solve of a graph with every point fixed, in one launch ---------
+// map::optimise(n, true, ...) (map.hpp:79-192) builds such graphs for tracking:
+// landmarks fixed, edges only to free frames.  The reduced system is then the
+// damped H_pp alone, block-diagonal in 6x6 blocks, and the reference's dense
+// row-Crout factorisation and substitutions of it (matrix_decomposition_
+// cholesky.hpp:26-95) act block by block: every product with an entry outside
+// a block is an exact zero, and the tolerances are the reference's global ones.
+// One workgroup runs the LM loop of factor_graph::solve (factor_graph.hpp:
+// 489-615) with no host round trip: residuals, costs and per-pose sums in the
+// reference's order (sequential from zero, insertion order), so the result is the
+// reference's bit for bit, as the per-graph launch sequence gives it.
+constexpr int kMotionThreads = 256;
+constexpr int kSeqStage = 4096;  // doubles staged in LDS for the sequential sums
+
+// Sum over observations in insertion order of their robust cost at the current
+// state (residuals stored), broadcast to every thread (factor_graph.hpp:618-629).
+__device__ double motion_cost(const dev_problem& p, const dev_work64& w, double* stage, double* out) {
+    for (int s = threadIdx.x; s < p.no; s += blockDim.x) {
+        const int32_t param = p.obs_param[s];
+        double P[7], X[3], Pc[3], r[2], W[4];
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        load7(w.pose + 7L * p.obs_pose[s], P);
+        load3(w.point + 3L * p.obs_point[s], X);
+        pose_apply(P, X, Pc);
+        residual(Pc, uv, cam_of(p, param), r);
+        w.residual[2L * s] = r[0];
+        w.residual[2L * s + 1] = r[1];
+        info_of(p, s, W);
+        int32_t kind;
+        double delta;
+        loss_of(p, param, kind, delta);
+        w.cbuf[p.obs_orig[s]] = robust_chi2(kind, delta, r, W);
+    }
+    __syncthreads();
+    double total = 0.0;
+    for (int c0 = 0; c0 < p.no; c0 += kSeqStage) {
+        const int m = min(kSeqStage, p.no - c0);
+        for (int i = threadIdx.x; i < m; i += blockDim.x) stage[i] = w.cbuf[c0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int i = 0; i < m; ++i) total = total + stage[i];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = total;
+    __syncthreads();
+    const double r = *out;
+    __syncthreads();
+    return r;
+}
+
+// make_hessian's pose blocks (factor_graph.hpp:648-732) with every point fixed:
+// per observation of a free pose its 21 H_pp and 6 gradient terms, then per
+// (pose, term) the sum in insertion order from zero (H_pp += H, b_p -= v), the
+// pose's terms (contiguous, pose-major) staged through LDS.
+__device__ void motion_linearise(const dev_problem& p, const dev_work64& w, double* stage) {
+    for (int s = threadIdx.x; s < p.no; s += blockDim.x) {
+        const int ip = p.obs_pose[s];
+        if (p.pose_fixed[ip]) continue;
+        const int32_t param = p.obs_param[s];
+        const double* K = cam_of(p, param);
+        double P[7], X[3], Pc[3], q[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+        load7(w.pose + 7L * ip, P);
+        load3(w.point + 3L * p.obs_point[s], X);
+        pose_apply(P, X, Pc);
+        const double uv[2] = {p.obs_uv[2L * s], p.obs_uv[2L * s + 1]};
+        residual(Pc, uv, K, r);
+        pose_quat(P, q);
+        jacobians(Pc, q, K, Jp, Jl);
+        info_of(p, s, W);
+        int32_t kind;
+        double delta;
+        loss_of(p, param, kind, delta);
+        robust_info(kind, delta, W, r, drho, Wr);
+        double H21[21], Wpl[18], v[6];
+        pose_terms(Jp, Jl, Wr, W, r, drho, H21, Wpl, v);
+        double* out = w.pose_terms + 27L * p.obs_pose_pos[s];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) out[i] = H21[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) out[21 + i] = v[i];
+    }
+    __syncthreads();
+    constexpr int kChunk = kSeqStage / 27;  // observations per staged chunk
+    for (int i = 0; i < p.np; ++i) {
+        const int j = threadIdx.x;  // term (threads 0..26)
+        double acc = 0.0;
+        if (!p.pose_fixed[i]) {
+            const int t0 = p.pose_start[i], t1 = p.pose_start[i + 1];
+            for (int c0 = t0; c0 < t1; c0 += kChunk) {
+                const int m = min(kChunk, t1 - c0);
+                const double* src = w.pose_terms + 27L * c0;
+                for (int k = threadIdx.x; k < 27 * m; k += blockDim.x) stage[k] = src[k];
+                __syncthreads();
+                if (j < 27)
+                    for (int t = 0; t < m; ++t) acc = j < 21 ? acc + stage[27 * t + j] : acc - stage[27 * t + j];
+                __syncthreads();
+            }
+        }
+        if (j < 21) w.Hpp[21L * i + j] = acc;
+        else if (j < 27) w.bp[6L * i + (j - 21)] = acc;
+    }
+    __syncthreads();
+}
+
+// std::max(scale, |a|) of the reference's decompose / solve (math.hpp:235-237).
+__device__ __forceinline__ double ref_max_abs(double scale, double a) {
+    const double x = fabs(a);
+    return (scale < x) ? x : scale;
+}
+
+// solve_linear_system (factor_graph.hpp:736-806) for the block-diagonal system:
+// damping, decompose_cholesky and solve_cholesky per 6x6 block with the global
+// tolerances.  Returns whether both succeeded; dx in w.dxp.
+__device__ bool motion_linear_solve(const dev_problem& p, const dev_work64& w, double lam, double (*L)[36],
+                                    double* sh, int* flag) {
+    const int np = p.np, n = 6 * np, tid = threadIdx.x;
+    // damped blocks (lower triangle; dense entry = 0 + (H + lambda on the diagonal))
+    for (int i = tid; i < np; i += blockDim.x) {
+        const double* H = w.Hpp + 21L * i;
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b <= a; ++b) {
+                double v = H[a * (a + 1) / 2 + b];
+                if (a == b) v += lam;
+                L[i][6 * a + b] = 0 + v;
+            }
+    }
+    if (tid == 0) *flag = 0;
+    __syncthreads();
+    if (tid == 0) {
+        double scale = 0;
+        for (int r = 0; r < n; ++r) scale = ref_max_abs(scale, L[r / 6][7 * (r % 6)]);
+        sh[0] = scale * 1e-14;
+    }
+    __syncthreads();
+    const double eps = sh[0];
+    for (int i = tid; i < np; i += blockDim.x) {
+        double* A = L[i];  // factorised in place, row by row (A's lower entries are read before written)
+        bool ok = true;
+        for (int r = 0; r < 6 && ok; ++r) {
+            for (int k = 0; k < r; ++k) {
+                double v = A[6 * r + k];
+                for (int j = 0; j < k; ++j) v -= A[6 * r + j] * A[6 * k + j];
+                A[6 * r + k] = v / A[7 * k];
+            }
+            double v = A[7 * r];
+            for (int j = 0; j < r; ++j) v -= A[6 * r + j] * A[6 * r + j];
+            if (!(v > eps)) ok = false;
+            else A[7 * r] = sqrt(v);
+        }
+        if (!ok) atomicOr(flag, 1);
+    }
+    __syncthreads();
+    if (*flag) return false;
+    if (tid == 0) {
+        double scale = 0;
+        for (int r = 0; r < n; ++r) scale = ref_max_abs(scale, L[r / 6][7 * (r % 6)]);
+        sh[0] = scale * 1e-7;
+    }
+    __syncthreads();
+    const double eps2 = sh[0];
+    for (int i = tid; i < np; i += blockDim.x) {
+        const double* A = L[i];
+        double x[6];
+        bool ok = true;
+        for (int r = 0; r < 6; ++r) {
+            x[r] = w.bp[6L * i + r];  // b_s = b_p (no Schur terms)
+            for (int j = 0; j < r; ++j) x[r] -= A[6 * r + j] * x[j];
+            if (!(fabs(A[7 * r]) > eps2)) ok = false;
+            x[r] /= A[7 * r];
+        }
+        for (int r = 6; r-- > 0;) {
+            for (int j = r + 1; j < 6; ++j) x[r] -= A[6 * j + r] * x[j];
+            if (!(fabs(A[7 * r]) > eps2)) ok = false;
+            x[r] /= A[7 * r];
+        }
+        for (int r = 0; r < 6; ++r) w.dxp[6L * i + r] = x[r];
+        if (!ok) atomicOr(flag, 1);
+    }
+    __syncthreads();
+    return *flag == 0;
+}
+
+__global__ void __launch_bounds__(kMotionThreads) k_motion_solve(dev_problem p, dev_work64 w) {
+    __shared__ double L[kMotionMaxPoses][36];
+    __shared__ double stage[kSeqStage];
+    __shared__ double sh[2];
+    __shared__ int flag;
+    lm_state* lm = w.lm;
+    const int tid = threadIdx.x, np = p.np, n = 6 * np;
+    // initial cost, linearisation and damping (factor_graph.hpp:500-522)
+    const double c0 = motion_cost(p, w, stage, sh + 1);
+    motion_linearise(p, w, stage);
+    if (tid == 0) {
+        lm->chi2 = c0;
+        max_scan m = {0.0, 0, 0, 1};
+        for (int r = 0; r < n; ++r) m = scan_push(m, w.Hpp[21L * (r / 6) + (r % 6) * (r % 6 + 1) / 2 + r % 6]);
+        if (p.nl > 0) m = scan_push(m, 0.0);  // the fixed points' empty blocks
+        lm_init(lm, scan_max_diagonal(m));
+    }
+    __syncthreads();
+    const int max_attempts = 11 * lm->iterations + 1;
+    for (int attempt = 0; attempt < max_attempts; ++attempt) {
+        if (lm->done) break;
+        const double lam = lm->lambda;
+        const bool ok = motion_linear_solve(p, w, lam, L, sh, &flag);
+        if (tid == 0) {
+            lm->solve_ok = ok ? 1 : 0;
+            lm_after_solve(lm);
+        }
+        __syncthreads();
+        if (lm->step_applied) {
+            // update_states and the LM scale over every slot (points: zero steps)
+            for (int i = tid; i < np; i += blockDim.x) {
+                if (p.pose_fixed[i]) continue;
+                double P[7], d[6];
+                load7(w.pose + 7L * i, P);
+                for (int k = 0; k < 7; ++k) w.pose_backup[7L * i + k] = P[k];
+                for (int k = 0; k < 6; ++k) d[k] = w.dxp[6L * i + k];
+                pose_plus(P, d);
+                for (int k = 0; k < 7; ++k) w.pose[7L * i + k] = P[k];
+            }
+            if (tid == 0) {
+                double scale = 0;
+                for (int r = 0; r < n; ++r) scale += w.dxp[r] * (lam * w.dxp[r] + w.bp[r]);
+                lm->scale = scale + 1e-3;
+            }
+            __syncthreads();
+            const double c = motion_cost(p, w, stage, sh + 1);
+            if (tid == 0) {
+                lm->chi2_new = c;
+                lm_decide(lm);
+            }
+            __syncthreads();
+            const bool rollback = lm->need_rollback != 0;
+            const bool relinearise = !lm->done && lm->need_linearize;
+            if (rollback)
+                for (int i = tid; i < np; i += blockDim.x)
+                    if (!p.pose_fixed[i])
+                        for (int k = 0; k < 7; ++k) w.pose[7L * i + k] = w.pose_backup[7L * i + k];
+            __syncthreads();
+            if (relinearise) {
+                motion_linearise(p, w, stage);
+                if (tid == 0) lm_linearise_done(lm);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) lm_end_attempt(lm);
+        __syncthreads();
+    }
+}
+
+void launch_motion_solve(const dev_problem& p, const dev_work64& w, hipStream_t s) {
+    if (p.np > 0 && p.np <= kMotionMaxPoses) k_motion_solve<<<1, kMotionThreads, 0, s>>>(p, w);
+}
+
 void launch_inliers64(const dev_problem& p, const dev_work64& w, uint8_t* flags_sorted, hipStream_t s) {
     if (p.no > 0) k_inliers<<<grid_for(p.no, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, flags_sorted);
 }
