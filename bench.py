@@ -54,7 +54,11 @@ def algorithmic_bytes(n_obs, n_poses, n_points, n_pairs, s):
         "linearisation": n_obs * (16 + 8) + n_pairs * 18 * s + n_points * (24 + 9 * s) + n_poses * (56 + 27 * s),
         "cost": n_obs * 24 + n_points * 24 + n_poses * 56,
         "schur": n_pairs * 18 * s + n_points * (9 * s + 9 * s) + 8 * n * n // 2 + 8 * n,
-        "backsub": n_pairs * 18 * s + n_points * (9 * s + 24) + 8 * n,
+        # back-substitution and update: dx_l = H_ll^-1 (b_l - W^T dx_p) per point (the
+        # mixed path receives dx_l from its refinement round and only reads it), then
+        # point and pose read, backed up and written
+        "backsub": ((n_points * 24 if s == 4 else n_pairs * 18 * s + n_points * (9 * s + 24)) + 8 * n
+                    + n_points * 24 * 3 + n_poses * 56 * 3),
         # mixed refinement round: the matrix-free binary64 residual (observation
         # record, pose and point in, W blocks, point rows out) and the reduced residual
         "refinement": n_obs * 32 + n_pairs * 18 * s + n_points * (24 + 9 * s + 3 * 8 * 3) + n_poses * (56 + 6 * 8 * 3)
