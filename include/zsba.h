@@ -80,7 +80,9 @@ typedef struct zsba_options {
                              damping.  0 = default, negative = no floor              */
     int32_t factor_path;  /* reduced-system factorisation: 0 = automatic, 1 = one
                              workgroup (envelope panels up to 16 tiles of 32 rows),
-                             2 = one launch per 32-column panel                     */
+                             2 = one launch per 32-column panel, 3 = banded (FP64
+                             fast and MIXED when every row of the reduced system
+                             spans at most 120 columns; otherwise automatic)      */
     int32_t timing_classes; /* with timing: bit mask of the kernel classes timed
                              (bit ZSBA_K_*); 0 = every class                        */
     int32_t freeze_behind;  /* behind-camera freeze, not a rule of the reference: at

@@ -43,6 +43,7 @@ UNITS = [
     ("kernels_fp64.hip", ["-ffp-contract=off"]),
     ("kernels_fp32.hip", ["-ffp-contract=off"]),
     ("kernels_chol.hip", ["-ffp-contract=off"]),
+    ("kernels_band.hip", ["-ffp-contract=off"]),
     ("kernels_batch.hip", ["-ffp-contract=off"]),
     ("kernels_adjoint.hip", ["-ffp-contract=off"]),
     ("kernels_match.hip", ["-ffp-contract=off"]),

@@ -191,7 +191,8 @@ struct dev_work64 {
     double* partial = nullptr;      // reduction partials
     double* cbuf = nullptr;         // [no] per-observation robust chi2, insertion order (exact sums)
     double* sbuf = nullptr;         // [6np + 3nl] per-slot LM scale terms (exact sums)
-    double* Lt = nullptr;           // [n*n] transposed factor (exact backward substitution)
+    double* Lt = nullptr;           // [n*n + n] transposed factor (exact backward substitution), or the
+                                    // banded factor (n rows of band entries and 1 / L_ii)
     double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
     int32_t* chol_status = nullptr; // [0] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
@@ -222,7 +223,9 @@ struct dev_work64 {
     const int32_t* tile_first = nullptr;  // [T]
     const int32_t* tile_last = nullptr;   // [T]
     int32_t panel_tiles = 0;              // max over k of tile_last[k] - k + 1
-    int32_t factor_path = 0;              // 0 automatic, 1 one workgroup, 2 panel launches
+    int32_t factor_path = 0;              // 0 automatic, 1 one workgroup, 2 panel launches, 3 band
+    // band of the reduced system: row r is zero left of column r - band + 1
+    int32_t band = 0;
     // mixed-path refinement (pose_grad doubles as the per-observation slots)
     double* zl = nullptr;           // [nl*3] point rows r_l of the binary64 residual
     double* gl = nullptr;           // [nl*3] H_ll^-1 r_l
@@ -335,6 +338,17 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
                              double floor_rel = 0.0);
 // Forward + backward substitution with the factor in w.S: x = S^-1 b.
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
+
+// ---- banded factorisation and substitutions (kernels_band.hip) -----------------
+// For the reassociating precisions with factor path 3 when the reduced system's
+// band (w.band) is at most kBandMaxW columns: one workgroup factorises w.S into
+// the compact band w.Lt and solves for w.dxp from w.bs (same status and solve_ok
+// rules).
+constexpr int kBandPanel = 8;
+constexpr int kBandMaxW = 120;
+bool band_path(int32_t n, const dev_work64& w, bool exact);
+void launch_chol_band(int32_t n, const dev_work64& w, hipStream_t s, bool gated, double floor_rel);
+void launch_band_solve(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
 
 // ---- graphs with every point fixed (the tracking step's motion-only BA) ---------
 // The whole FP64 (reference-order) solve in one launch of one workgroup, for at

@@ -512,7 +512,7 @@ int32_t zsba_graph::build_structure() {
     // stays inside it, so tiles left of it stay exactly zero
     const int32_t n_tiles = static_cast<int32_t>((6L * np + 31) / 32);
     std::vector<int32_t> tile_first(n_tiles), tile_last(n_tiles);
-    int32_t panel_tiles = 0;
+    int32_t panel_tiles = 0, band = 0;
     {
         std::vector<int32_t> first_pose(np);
         for (int32_t i = 0; i < np; ++i) first_pose[i] = i;
@@ -527,6 +527,7 @@ int32_t zsba_graph::build_structure() {
         for (int32_t t = 0; t < n_tiles; ++t)
             for (int32_t k = tile_first[t]; k < t; ++k) tile_last[k] = std::max(tile_last[k], t);
         for (int32_t k = 0; k < n_tiles; ++k) panel_tiles = std::max(panel_tiles, tile_last[k] - k + 1);
+        for (int64_t r = 0; r < 6L * np; ++r) band = std::max(band, static_cast<int32_t>(r - 6L * first_pose[r / 6] + 1));
     }
     std::vector<int2> contrib(static_cast<size_t>(ncontrib));
     {
@@ -614,7 +615,7 @@ int32_t zsba_graph::build_structure() {
     w.partial = mem.get<double>(3 * zb::kPartials + 8);
     w.cbuf = mem.get<double>(no);
     w.sbuf = mem.get<double>(6L * np + 3L * nl);
-    w.Lt = mem.get<double>(static_cast<size_t>(n * n));
+    w.Lt = mem.get<double>(static_cast<size_t>(n * n + n));
     w.pose_terms_f = mem.get<float>(21L * no); w.pose_grad = mem.get<double>(6L * no);
     w.Hpp_f = mem.get<float>(21L * np); w.bp_m = mem.get<double>(6L * np);
     w.Hll_f = mem.get<float>(9L * nl); w.bl_m = mem.get<double>(3L * nl);
@@ -673,6 +674,7 @@ int32_t zsba_graph::build_structure() {
     w.tile_first = up(d_tile_first, tile_first);
     w.tile_last = up(d_tile_last, tile_last);
     w.panel_tiles = panel_tiles;
+    w.band = band;
     w.factor_path = opt.factor_path;
     p.obs_pose_pos = up(d_obs_pose_pos, obs_pose_pos);
     for (int set = 0; set < 2; ++set) {
@@ -1502,7 +1504,7 @@ zsba_graph* zsba_create(const zsba_options* options) {
     zsba_graph* g = new (std::nothrow) zsba_graph();
     if (!g) return nullptr;
     if (options) g->opt = *options;
-    if (g->opt.factor_path < 0 || g->opt.factor_path > 2) g->opt.factor_path = 0;
+    if (g->opt.factor_path < 0 || g->opt.factor_path > 3) g->opt.factor_path = 0;
     if (g->opt.refine_steps == 0) g->opt.refine_steps = 1;
     else if (g->opt.refine_steps < 0) g->opt.refine_steps = 0;
     return g;

@@ -793,6 +793,12 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
         k_empty_ok<<<1, 1, 0, s>>>(w.lm, g);
         return;
     }
+    if (band_path(n, w, exact)) {
+        ZB_TIME_BEGIN(w, K_CHOLESKY, s);
+        launch_chol_band(n, w, s, gated, floor_rel);
+        ZB_TIME_END(w, s);
+        return;
+    }
     ZB_TIME_BEGIN(w, K_CHOLESKY, s);
     k_chol_begin<<<1, 1024, 0, s>>>(n, w.S, w.chol_scale, w.chol_status, w.lm, g, floor_rel);
     const int64_t tiles = (n + NB - 1) / NB;
@@ -847,6 +853,10 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
 
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated) {
     if (n <= 0) return;
+    if (band_path(n, w, false)) {
+        launch_band_solve(n, w, b, x, s, gated);
+        return;
+    }
     launch_trsv(n, w.S, b, x, w, s, gated ? 1 : 0, 0);
 }
 
