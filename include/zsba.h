@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define ZSBA_ABI_VERSION 1
+#define ZSBA_ABI_VERSION 2
 
 /* Error codes (negative returns). */
 #define ZSBA_EINVAL   (-1) /* bad argument (null pointer, id out of range, NaN size ...) */
@@ -95,6 +95,11 @@ typedef struct zsba_options {
                              from falling (the solve stalls).  0 = default: on for
                              MIXED, off for FP64_FAST; 1 = on for MIXED and
                              FP64_FAST; -1 = off.  Never applied to FP64.        */
+    int32_t attempt_graph;  /* 0 = default: each LM attempt's launch sequence is
+                             captured once per structure and precision as a HIP
+                             graph and replayed (one graph launch per attempt);
+                             -1 = launch the kernels one by one.  Timing (above)
+                             always launches one by one.                        */
 } zsba_options;
 
 /* Kernel classes timed when zsba_options.timing is set (zsba_stats.kernel_*). */

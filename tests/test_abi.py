@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_handles():
     lib = _lib.load()
-    assert lib.zsba_abi_version() == 1
+    assert lib.zsba_abi_version() == 2
     g = Graph()
     assert g.add_poses(np.tile([0, 0, 0, 0, 0, 0, 1.0], (3, 1)), [1, 0, 0]) == 0
     assert g.add_points(np.zeros((2, 3))) == 0

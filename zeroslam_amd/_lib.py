@@ -23,7 +23,7 @@ ERRORS = {-1: "EINVAL", -2: "ESTATE", -3: "EDEVICE", -4: "ENOMEM", -5: "EUNSUP"}
 class Options(C.Structure):
     _fields_ = [("device", C.c_int32), ("verbose", C.c_int32), ("refine_steps", C.c_int32),
                 ("timing", C.c_int32), ("floor_exp", C.c_int32), ("factor_path", C.c_int32),
-                ("timing_classes", C.c_int32), ("freeze_behind", C.c_int32)]
+                ("timing_classes", C.c_int32), ("freeze_behind", C.c_int32), ("attempt_graph", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -173,7 +173,7 @@ def load() -> C.CDLL:
         fn = getattr(lib, name)  # AttributeError if the symbol is not exported
         fn.restype = restype
         fn.argtypes = argtypes
-    if lib.zsba_abi_version() != 1:
+    if lib.zsba_abi_version() != 2:
         raise RuntimeError("libzsba.so ABI version mismatch")
     _lib = lib
     return lib

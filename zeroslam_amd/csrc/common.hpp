@@ -151,6 +151,22 @@ struct dev_problem {
     // vertices
     const uint8_t* pose_fixed = nullptr;
     const uint8_t* point_fixed = nullptr;
+    // Linearisation blocks (mixed path): runs of whole points, at most kLinBlock
+    // observations each (n_lblk = 0 when some point has more: the per-point
+    // kernels then linearise).  Entry e is one free pose a block touches: its
+    // observations' offsets in the block lo_list[lo_start[e] .. lo_start[e+1]) in
+    // order; block b's entries are [lp_start[b], lp_start[b+1]).  Entry e's partial
+    // sums go to slot e_slot[e]; pose i's slots are [pe_start[i], pe_start[i+1]),
+    // in block order.
+    int32_t n_lblk = 0, n_lent = 0;
+    const int32_t* lb_start = nullptr;   // [n_lblk+1] observation (sorted index) ranges
+    const int32_t* lb_point = nullptr;   // [n_lblk+1] point slot ranges
+    const int32_t* lp_start = nullptr;   // [n_lblk+1]
+    const int32_t* lo_start = nullptr;   // [n_lent+1]
+    const uint8_t* lo_list = nullptr;    // [observations of free poses]
+    const int32_t* e_slot = nullptr;     // [n_lent]
+    const int32_t* pe_start = nullptr;   // [np+1]
+    const uint8_t* pair_multi = nullptr; // [npair] the W block has more than one observation
     // parameter tables
     const double* cams = nullptr;        // [ncam*4]
     const int32_t* loss_kind = nullptr;  // [nloss]
@@ -232,6 +248,10 @@ struct dev_work64 {
     double* rvec = nullptr;         // [n] reduced residual r_p - W H_ll^-1 r_l
     double* corr = nullptr;         // [n] pose correction
     double* pose_part = nullptr;    // [np*kPoseChunks*27] per-chunk partial sums of the per-pose reductions
+    // per-entry partial sums of the linearisation blocks: 21 H_pp terms (storage
+    // type of the blocks) and 6 gradient terms
+    void* lin_part_h = nullptr;     // [n_lent*21]
+    double* lin_part_g = nullptr;   // [n_lent*6]
 };
 
 #define ZB_TIME_BEGIN(w, id, s) do { if ((w).timer) (w).timer->begin((id), (s)); } while (0)
@@ -242,6 +262,8 @@ constexpr int kPartials = 1024;  // reduction partial slots
 // into kPoseChunks contiguous chunks, one wave each, whose partial sums are then
 // added in chunk order: a fixed association, independent of the launch shape.
 constexpr int kPoseChunks = 16;
+// Observations per linearisation block (one thread each).
+constexpr int kLinBlock = 128;
 
 // ---- binary64 kernels (kernels_fp64.hip) ----------------------------------
 // `exact`: reproduce the reference's summation orders (sequential sums over all
@@ -312,13 +334,16 @@ enum batch_phase : int {
 };
 constexpr int kBatchPhases = 8;
 // Sizes of the grid-wide linearisation launches: prefix sums (count + 1 entries
-// each, on the device) of the windows' points, pose chunks (np * kPoseChunks)
-// and pose-block entries (np * 27), with their totals.
+// each, on the device) of the windows' points, pose chunks (np * kPoseChunks),
+// pose-block entries (np * 27) and linearisation blocks, with their totals.
 struct batch_grid {
     const int32_t* points = nullptr;
     const int32_t* chunks = nullptr;
     const int32_t* entries = nullptr;
-    int32_t n_points = 0, n_chunks = 0, n_entries = 0, pad = 0;
+    const int32_t* lblocks = nullptr;  // linearisation blocks (windows with p.n_lblk > 0)
+    int32_t n_points = 0, n_chunks = 0, n_entries = 0, n_lblocks = 0;
+    int32_t n_per_point = 0;  // windows linearised per point (p.n_lblk == 0)
+    int32_t pad = 0;
 };
 // Launches of one batched solve: `rounds` rounds of (linearisation of the
 // windows that need one, then every unfinished window's attempts up to its next

@@ -237,6 +237,20 @@ struct zsba_graph {
     zsba_stats stats{};
     zb::kernel_timer timer;
 
+    // Captured launch sequence of one LM attempt, per precision and freeze state
+    // (the kernels' arguments), for the current structure.
+    hipGraph_t att_graph[6] = {};
+    hipGraphExec_t att_exec[6] = {};
+    bool att_capture_failed = false;
+    void drop_attempt_graphs() {
+        for (int k = 0; k < 6; ++k) {
+            if (att_exec[k]) (void)hipGraphExecDestroy(att_exec[k]);
+            if (att_graph[k]) (void)hipGraphDestroy(att_graph[k]);
+            att_exec[k] = nullptr;
+            att_graph[k] = nullptr;
+        }
+    }
+
     // Exact-order state: the keys of the reference's factor_graph::h_pl container,
     // cleared and refilled by every make_hessian and kept across solves exactly as
     // that member is (factor_graph.hpp:366, 654, 704-709).
@@ -255,6 +269,7 @@ struct zsba_graph {
     ~zsba_graph() {
         if (hip_ready) {
             if (stream) (void)hipStreamSynchronize(stream);
+            drop_attempt_graphs();
             mem.release();
             if (eval_dev) (void)hipFree(eval_dev);
             if (stream) stream_release(opt.device, stream, lm_pinned);
@@ -370,6 +385,8 @@ struct zsba_graph {
 
 // ---- structure analysis -----------------------------------------------------
 int32_t zsba_graph::build_structure() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_attempt_graphs();
     const int32_t np_all = static_cast<int32_t>(pose_alive.size());
     const int32_t nl_all = static_cast<int32_t>(point_alive.size());
     pose_slot_id.clear(); point_slot_id.clear(); obs_e_id.clear();
@@ -440,6 +457,66 @@ int32_t zsba_graph::build_structure() {
     }
     lm_pair_start[nl] = static_cast<int32_t>(pair_pose.size());
     const int32_t npair = static_cast<int32_t>(pair_pose.size());
+
+    // linearisation blocks: whole points, at most kLinBlock observations
+    std::vector<int32_t> lb_start, lb_point, lp_start, le_pose, lo_start, pe_start(np + 1, 0), pe_list;
+    std::vector<uint8_t> lo_list, pair_multi(npair, 0);
+    {
+        std::vector<int32_t> pair_count(npair, 0);
+        for (int32_t s = 0; s < no; ++s)
+            if (obs_pair[s] >= 0) pair_count[obs_pair[s]]++;
+        for (int32_t a = 0; a < npair; ++a) pair_multi[a] = pair_count[a] > 1 ? 1 : 0;
+        bool fits = true;
+        for (int32_t l = 0; l < nl && fits; ++l) fits = lm_start[l + 1] - lm_start[l] <= zb::kLinBlock;
+        if (fits && no > 0) {
+            std::vector<int32_t> slot_of(np, -1), members;
+            std::vector<std::vector<uint8_t>> offs;
+            auto close_block = [&](int32_t b0, int32_t b1) {
+                // entries of the block: free poses in order of first appearance
+                members.clear();
+                offs.clear();
+                for (int32_t s = b0; s < b1; ++s) {
+                    const int32_t k = e_pose[sorted_e[s]];
+                    if (pfix[k]) continue;
+                    if (slot_of[k] < 0) {
+                        slot_of[k] = static_cast<int32_t>(members.size());
+                        members.push_back(k);
+                        offs.emplace_back();
+                    }
+                    offs[slot_of[k]].push_back(static_cast<uint8_t>(s - b0));
+                }
+                for (size_t m = 0; m < members.size(); ++m) {
+                    le_pose.push_back(members[m]);
+                    lo_start.push_back(static_cast<int32_t>(lo_list.size()));
+                    lo_list.insert(lo_list.end(), offs[m].begin(), offs[m].end());
+                    slot_of[members[m]] = -1;
+                }
+                lb_start.push_back(b1);
+                lp_start.push_back(static_cast<int32_t>(le_pose.size()));
+            };
+            lb_start.push_back(0);
+            lb_point.push_back(0);
+            lp_start.push_back(0);
+            int32_t b0 = 0;
+            for (int32_t l = 0; l < nl; ++l) {
+                if (lm_start[l + 1] - b0 > zb::kLinBlock) {
+                    close_block(b0, lm_start[l]);
+                    lb_point.push_back(l);
+                    b0 = lm_start[l];
+                }
+            }
+            close_block(b0, no);
+            lb_point.push_back(nl);
+            lo_start.push_back(static_cast<int32_t>(lo_list.size()));
+            for (size_t e = 0; e < le_pose.size(); ++e) pe_start[le_pose[e] + 1]++;
+            for (int32_t i = 0; i < np; ++i) pe_start[i + 1] += pe_start[i];
+            pe_list.assign(le_pose.size(), 0);  // entry -> pose-major slot
+            std::vector<int32_t> f(pe_start.begin(), pe_start.end() - 1);
+            for (size_t e = 0; e < le_pose.size(); ++e) pe_list[e] = f[le_pose[e]]++;
+        }
+    }
+    const int32_t n_lblk = lb_start.empty() ? 0 : static_cast<int32_t>(lb_start.size()) - 1;
+    const int32_t n_lent = static_cast<int32_t>(le_pose.size());
 
     // key sequence of h_pl (first occurrence in insertion order, factor_graph.hpp:664-709)
     {
@@ -623,6 +700,16 @@ int32_t zsba_graph::build_structure() {
     w.zl = mem.get<double>(3L * nl); w.gl = mem.get<double>(3L * nl);
     w.rvec = mem.get<double>(n); w.corr = mem.get<double>(n);
     w.pose_part = mem.get<double>(static_cast<size_t>(np) * zb::kPoseChunks * 27);
+    w.lin_part_h = mem.get<double>(21L * n_lent);  // room for binary64 storage
+    w.lin_part_g = mem.get<double>(6L * n_lent);
+    int32_t* d_lb_start = mem.get<int32_t>(lb_start.size());
+    int32_t* d_lb_point = mem.get<int32_t>(lb_point.size());
+    int32_t* d_lp_start = mem.get<int32_t>(lp_start.size());
+    int32_t* d_lo_start = mem.get<int32_t>(lo_start.size());
+    uint8_t* d_lo_list = mem.get<uint8_t>(lo_list.size());
+    int32_t* d_pe_start = mem.get<int32_t>(pe_start.size());
+    int32_t* d_pe_list = mem.get<int32_t>(pe_list.size());
+    uint8_t* d_pair_multi = mem.get<uint8_t>(pair_multi.size());
     int32_t* d_pose_pairs = mem.get<int32_t>(npair);
     w.adj = mem.get<double>(6 * (6L * np + 3L * nl));
     w.adj_obs = mem.get<double>(2L * no);
@@ -671,6 +758,14 @@ int32_t zsba_graph::build_structure() {
     p.pose_fixed = up(d_pfix, pfix); p.point_fixed = up(d_lfix, lfix);
     p.pose_pair_start = up(d_pose_pair_start, pose_pair_start);
     p.pose_pairs = up(d_pose_pairs, pose_pairs);
+    p.n_lblk = n_lblk;
+    p.n_lent = n_lent;
+    if (n_lblk > 0) {
+        p.lb_start = up(d_lb_start, lb_start); p.lb_point = up(d_lb_point, lb_point); p.lp_start = up(d_lp_start, lp_start);
+        p.lo_start = up(d_lo_start, lo_start); p.lo_list = up(d_lo_list, lo_list);
+        p.pe_start = up(d_pe_start, pe_start); p.e_slot = up(d_pe_list, pe_list);
+    }
+    p.pair_multi = up(d_pair_multi, pair_multi);
     w.tile_first = up(d_tile_first, tile_first);
     w.tile_last = up(d_tile_last, tile_last);
     w.panel_tiles = panel_tiles;
@@ -693,6 +788,12 @@ int32_t zsba_graph::build_structure() {
     mirror_filled = false;
     p.cams = up(d_cams, cam_table); p.loss_kind = up(d_lk, lk); p.loss_delta = up(d_ld, ld);
     (void)hipMemsetAsync(w.chol_status, 0, 4 * sizeof(int32_t), stream);
+    // points without observations are in no linearisation block: their H_ll, b_l
+    // and freeze flag stay zero
+    (void)hipMemsetAsync(w.Hll_f, 0, sizeof(float) * 9 * static_cast<size_t>(nl), stream);
+    (void)hipMemsetAsync(w.Hll, 0, sizeof(double) * 9 * static_cast<size_t>(nl), stream);
+    (void)hipMemsetAsync(w.bl_m, 0, sizeof(double) * 3 * static_cast<size_t>(nl), stream);
+    (void)hipMemsetAsync(frozen_dev, 0, static_cast<size_t>(nl), stream);
     (void)hipStreamSynchronize(stream);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "structure upload");
@@ -855,11 +956,9 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
         }
     }
     const int32_t max_attempts = motion ? 0 : iterations * 10;
-    hipEvent_t ev[2];
-    (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
-    (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
-    int32_t* done_pinned = &lm_pinned->done;
-    for (int32_t a = 0; a < max_attempts; ++a) {
+    // one attempt: every kernel gated on the controller in device memory, so the
+    // same sequence serves every attempt of the solve
+    auto enqueue_attempt = [&]() {
         if (mixed) {
             zb::launch_reduced_system_mixed(dp, w, stream, nullptr);
             zb::launch_cholesky_solve64(dp.n, w, stream, true, false, mixed_floor());
@@ -878,6 +977,46 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
         if (mixed) zb::launch_linearise_mixed(dp, w, stream, true);
         else zb::launch_linearise64(dp, w, stream, true, exact);
         zb::launch_end_attempt64(w, stream);
+    };
+    hipGraphExec_t att = nullptr;
+    if (max_attempts > 0 && !w.timer && opt.attempt_graph >= 0 && !att_capture_failed) {
+        const int key = 2 * (precision < 0 ? 0 : (precision > 2 ? 2 : precision)) + (w.frozen ? 1 : 0);
+        if (!att_exec[key]) {
+            hipGraph_t gr = nullptr;
+            hipGraphExec_t ex = nullptr;
+            bool ok = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                enqueue_attempt();
+                ok = hipStreamEndCapture(stream, &gr) == hipSuccess && gr;
+            }
+            ok = ok && hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) == hipSuccess;
+            if (ok) {
+                att_graph[key] = gr;
+                att_exec[key] = ex;
+            } else {
+                // launch one by one from now on
+                if (gr) (void)hipGraphDestroy(gr);
+                (void)hipGetLastError();
+                att_capture_failed = true;
+            }
+        }
+        att = att_exec[key];
+    }
+    hipEvent_t ev[2];
+    (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    int32_t* done_pinned = &lm_pinned->done;
+    for (int32_t a = 0; a < max_attempts; ++a) {
+        if (att) {
+            if ((e = hipGraphLaunch(att, stream)) != hipSuccess) {
+                (void)hipStreamSynchronize(stream);
+                (void)hipEventDestroy(ev[0]);
+                (void)hipEventDestroy(ev[1]);
+                return hip_fail(e, "hipGraphLaunch");
+            }
+        } else {
+            enqueue_attempt();
+        }
         (void)hipMemcpyAsync(done_pinned, &w.lm->done, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
         (void)hipEventRecord(ev[a & 1], stream);
         // Keep one attempt queued behind the one whose flag is read.
@@ -1147,7 +1286,7 @@ struct zsba_batch {
     lm_state* h_out = nullptr;
     int64_t* d_ticks = nullptr;
     int64_t* h_ticks = nullptr;
-    int32_t* d_prefix = nullptr;     // 3 (capacity + 1): points, pose chunks, pose-block entries
+    int32_t* d_prefix = nullptr;     // 4 (capacity + 1): points, pose chunks, pose-block entries, blocks
     uint64_t* d_linclk = nullptr;    // 2 rounds
     uint64_t* h_linclk = nullptr;
     int32_t capacity = 0, clock_capacity = 0;
@@ -1160,7 +1299,7 @@ struct zsba_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t graph_threads = 0, graph_mixed = -1, graph_count = -1, graph_rounds = -1;
-    int32_t graph_points = -1, graph_chunks = -1, graph_entries = -1;
+    int32_t graph_points = -1, graph_chunks = -1, graph_entries = -1, graph_lblocks = -1, graph_per_point = -1;
 
     ~zsba_batch() { release(); }
     void release() {
@@ -1240,7 +1379,7 @@ struct zsba_batch {
             d_ticks = nullptr;
             h_ticks = nullptr;
             d_prefix = nullptr;
-            if ((e = hipMalloc(&d_prefix, sizeof(int32_t) * 3 * (count + 1))) != hipSuccess)
+            if ((e = hipMalloc(&d_prefix, sizeof(int32_t) * 4 * (count + 1))) != hipSuccess)
                 return hip_fail(e, "hipMalloc");
             const size_t tick_bytes = sizeof(int64_t) * zb::kBatchPhases * count;
             if ((e = hipMalloc(&d_ticks, tick_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
@@ -1262,7 +1401,7 @@ struct zsba_batch {
         const int32_t count = static_cast<int32_t>(h_wins.size());
         if (!exec || graph_threads != threads || graph_mixed != (mixed ? 1 : 0) || graph_count != count ||
             graph_rounds != rounds || graph_points != grid.n_points || graph_chunks != grid.n_chunks ||
-            graph_entries != grid.n_entries) {
+            graph_entries != grid.n_entries || graph_lblocks != grid.n_lblocks || graph_per_point != grid.n_per_point) {
             if (exec) (void)hipGraphExecDestroy(exec);
             if (graph) (void)hipGraphDestroy(graph);
             exec = nullptr;
@@ -1280,6 +1419,8 @@ struct zsba_batch {
             graph_points = grid.n_points;
             graph_chunks = grid.n_chunks;
             graph_entries = grid.n_entries;
+            graph_lblocks = grid.n_lblocks;
+            graph_per_point = grid.n_per_point;
         }
         if ((e = hipGraphLaunch(exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
         info.graph_launches += 1;
@@ -1400,26 +1541,34 @@ struct zsba_batch {
             hipSuccess)
             return hip_fail(e, "upload windows");
         {
-            h_prefix.assign(3 * (nw + 1), 0);
-            int64_t pts = 0, chunks = 0, entries = 0;
+            h_prefix.assign(4 * (nw + 1), 0);
+            int64_t pts = 0, chunks = 0, entries = 0, lblocks = 0;
+            int32_t per_point = 0;
             for (int32_t i = 0; i < nw; ++i) {
                 h_prefix[i] = static_cast<int32_t>(pts);
                 h_prefix[(nw + 1) + i] = static_cast<int32_t>(chunks);
                 h_prefix[2 * (nw + 1) + i] = static_cast<int32_t>(entries);
+                h_prefix[3 * (nw + 1) + i] = static_cast<int32_t>(lblocks);
                 pts += h_wins[i].p.nl;
                 chunks += static_cast<int64_t>(h_wins[i].p.np) * zb::kPoseChunks;
                 entries += static_cast<int64_t>(h_wins[i].p.np) * 27;
+                lblocks += h_wins[i].p.n_lblk;
+                if (h_wins[i].p.n_lblk == 0) ++per_point;
             }
             if (entries > INT32_MAX || 64 * chunks > INT32_MAX) return fail(ZSBA_EINVAL, "batch too large");
             h_prefix[nw] = static_cast<int32_t>(pts);
             h_prefix[(nw + 1) + nw] = static_cast<int32_t>(chunks);
             h_prefix[2 * (nw + 1) + nw] = static_cast<int32_t>(entries);
+            h_prefix[3 * (nw + 1) + nw] = static_cast<int32_t>(lblocks);
             grid.points = d_prefix;
             grid.chunks = d_prefix + (nw + 1);
             grid.entries = d_prefix + 2 * (nw + 1);
+            grid.lblocks = d_prefix + 3 * (nw + 1);
             grid.n_points = static_cast<int32_t>(pts);
             grid.n_chunks = static_cast<int32_t>(chunks);
             grid.n_entries = static_cast<int32_t>(entries);
+            grid.n_lblocks = static_cast<int32_t>(lblocks);
+            grid.n_per_point = per_point;
             if ((e = hipMemcpyAsync(d_prefix, h_prefix.data(), sizeof(int32_t) * h_prefix.size(), hipMemcpyHostToDevice,
                                     stream)) != hipSuccess)
                 return hip_fail(e, "upload linearisation grid");

@@ -49,6 +49,12 @@ __device__ __forceinline__ int find_window(const int32_t* prefix, int count, int
 
 __device__ __forceinline__ bool needs_linearisation(const lm_state* lm) { return !lm->done && lm->need_linearize; }
 
+// Mixed windows linearise by blocks of observations (as the per-graph mixed path
+// does); binary64 windows per point and pose chunk (as the per-graph binary64
+// path sums).
+template <class TS> constexpr bool kBlocks = false;
+template <> constexpr bool kBlocks<float> = true;
+
 // Grid-wide linearisation, three launches: per point (H_ll, b_l and W), per pose
 // chunk (the pose terms formed again from the observations and summed in a fixed
 // order, so they never go through memory), per pose-block entry.
@@ -61,7 +67,7 @@ __global__ void __launch_bounds__(256) k_batch_lin_points(const batch_window* wi
     if (g >= total) return;
     const int k = find_window(prefix, count, g);
     const batch_window& bw = wins[k];
-    if (!needs_linearisation(bw.w.lm)) return;
+    if ((kBlocks<TS> && bw.p.n_lblk > 0) || !needs_linearisation(bw.w.lm)) return;
     const blocks<TS> B(bw.w);
     mixed::lin_point<TS, false>(bw.p, bw.w.pose, bw.w.point, B.pterms, bw.w.pose_grad, B.Hll, bw.w.bl_m, B.W,
                                 bw.w.frozen, g - prefix[k]);
@@ -77,7 +83,7 @@ __global__ void __launch_bounds__(256) k_batch_lin_chunks(const batch_window* wi
     if (g >= total) return;
     const int k = find_window(prefix, count, g);
     const batch_window& bw = wins[k];
-    if (!needs_linearisation(bw.w.lm)) return;
+    if ((kBlocks<TS> && bw.p.n_lblk > 0) || !needs_linearisation(bw.w.lm)) return;
     const int ic = g - prefix[k];
     mixed::reduce_pose_chunk_recompute(bw.p, bw.w.pose, bw.w.point, bw.w.pose_part, ic / kPoseChunks,
                                        ic % kPoseChunks, threadIdx.x & 63, ht[wave], gt[wave]);
@@ -93,7 +99,50 @@ __global__ void __launch_bounds__(256) k_batch_lin_entries(const batch_window* w
     if (!needs_linearisation(bw.w.lm)) return;
     const blocks<TS> B(bw.w);
     const int ij = g - prefix[k];
+    if (kBlocks<TS> && bw.p.n_lblk > 0) return;  // k_batch_lin_finish
     mixed::reduce_pose_finish(bw.w.pose_part, B.Hpp, bw.w.bp_m, ij / 27, ij % 27);
+}
+
+// The windows linearised by blocks: one workgroup per pose (global pose g, window
+// k with entries[k] = 27 x poses before it).
+template <class TS>
+__global__ void __launch_bounds__(mixed::kFinishThreads) k_batch_lin_finish(const batch_window* wins, int count,
+                                                                           const int32_t* entries, int total) {
+    const int g = static_cast<int>(blockIdx.x);
+    if (g >= total) return;
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (entries[mid] <= 27 * g) lo = mid;
+        else hi = mid;
+    }
+    const batch_window& bw = wins[lo];
+    if (!(kBlocks<TS> && bw.p.n_lblk > 0) || !needs_linearisation(bw.w.lm)) return;
+    __shared__ double red[4][27];
+    const blocks<TS> B(bw.w);
+    mixed::lin_pose_finish(bw.p, static_cast<const TS*>(bw.w.lin_part_h), bw.w.lin_part_g, B.Hpp, bw.w.bp_m,
+                           g - entries[lo] / 27, red);
+}
+
+// The windows linearised by blocks of observations: one workgroup per block.
+template <class TS>
+__global__ void __launch_bounds__(kLinBlock) k_batch_lin_blocks(const batch_window* wins, int count,
+                                                                const int32_t* prefix, int total, uint64_t* lin_clock,
+                                                                int round) {
+    // the first workgroup dispatched stamps the round's start (one atomic, not one
+    // per workgroup of a grid of tens of thousands)
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicMin(reinterpret_cast<unsigned long long*>(&lin_clock[2 * round]),
+                  static_cast<unsigned long long>(wall_clock64()));
+    const int g = static_cast<int>(blockIdx.x);
+    if (g >= total) return;
+    const int k = find_window(prefix, count, g);
+    const batch_window& bw = wins[k];
+    if (!needs_linearisation(bw.w.lm)) return;
+    __shared__ mixed::lin_stage<TS> sh;
+    const blocks<TS> B(bw.w);
+    mixed::lin_block(bw.p, bw.w.pose, bw.w.point, B.Hll, bw.w.bl_m, B.W, bw.w.frozen, static_cast<TS*>(bw.w.lin_part_h),
+                     bw.w.lin_part_g, B.pterms, g - prefix[k], sh);
 }
 
 // The sum the per-graph launches form over n items (k_cost and k_backsub_update,
@@ -380,12 +429,20 @@ void launch_batch_solve(const batch_window* wins, int count, int threads, bool m
     k_batch_begin<<<count, 256, 0, s>>>(wins, phase_ticks, lin_clock, 2 * rounds);
     for (int r = 0; r < rounds; ++r) {
         if (mixed_path) {
-            k_batch_lin_points<float><<<blocks_for(grid.n_points), 256, 0, s>>>(wins, count, grid.points, grid.n_points,
-                                                                              lin_clock, r);
-            k_batch_lin_chunks<float><<<blocks_for(64L * grid.n_chunks), 256, 0, s>>>(wins, count, grid.chunks,
-                                                                                    grid.n_chunks);
-            k_batch_lin_entries<float><<<blocks_for(grid.n_entries), 256, 0, s>>>(wins, count, grid.entries,
-                                                                                grid.n_entries);
+            if (grid.n_lblocks > 0)
+                k_batch_lin_blocks<float><<<grid.n_lblocks, kLinBlock, 0, s>>>(wins, count, grid.lblocks,
+                                                                             grid.n_lblocks, lin_clock, r);
+            if (grid.n_per_point > 0) {
+                k_batch_lin_points<float><<<blocks_for(grid.n_points), 256, 0, s>>>(wins, count, grid.points,
+                                                                                  grid.n_points, lin_clock, r);
+                k_batch_lin_chunks<float><<<blocks_for(64L * grid.n_chunks), 256, 0, s>>>(wins, count, grid.chunks,
+                                                                                        grid.n_chunks);
+                k_batch_lin_entries<float><<<blocks_for(grid.n_entries), 256, 0, s>>>(wins, count, grid.entries,
+                                                                                    grid.n_entries);
+            }
+            if (grid.n_lblocks > 0 && grid.n_entries > 0)
+                k_batch_lin_finish<float><<<grid.n_entries / 27, mixed::kFinishThreads, 0, s>>>(
+                    wins, count, grid.entries, grid.n_entries / 27);
             if (threads == 256) k_batch_attempts<float, 256><<<count, 256, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
             else if (threads == 512) k_batch_attempts<float, 512><<<count, 512, 0, s>>>(wins, out, phase_ticks, lin_clock, r);
             else k_batch_attempts<float, 1024><<<count, 1024, 0, s>>>(wins, out, phase_ticks, lin_clock, r);

@@ -44,6 +44,24 @@ __global__ void __launch_bounds__(kBlock) k_lin_points_f(dev_problem p, const do
     if (l < p.nl) mixed::lin_point(p, pose, point, pterms, pgrad, Hll, bl, Wbuf, frozen, l);
 }
 
+// Linearisation by blocks of observations, then one thread per (pose, term).
+__global__ void __launch_bounds__(kLinBlock) k_lin_blocks_f(dev_problem p, const double* pose, const double* point,
+                                                            float* Hll, double* bl, float* Wbuf, float* part_h,
+                                                            double* part_g, float* wscratch, const lm_state* lm,
+                                                            int gated, uint8_t* frozen) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    __shared__ mixed::lin_stage<float> sh;
+    mixed::lin_block(p, pose, point, Hll, bl, Wbuf, frozen, part_h, part_g, wscratch, static_cast<int>(blockIdx.x), sh);
+}
+
+__global__ void __launch_bounds__(mixed::kFinishThreads) k_lin_finish_f(dev_problem p, const float* part_h,
+                                                                       const double* part_g, float* Hpp, double* bp,
+                                                                       const lm_state* lm, int gated) {
+    if (gated && (lm->done || !lm->need_linearize)) return;
+    __shared__ double red[4][27];
+    mixed::lin_pose_finish(p, part_h, part_g, Hpp, bp, static_cast<int>(blockIdx.x), red);
+}
+
 // One wave per (pose, chunk), then one thread per (pose, term).
 __global__ void __launch_bounds__(256) k_reduce_pose_part_f(dev_problem p, const float* pterms, const double* pgrad,
                                                             double* part, const lm_state* lm, int gated) {
@@ -102,6 +120,21 @@ constexpr int kSchurSmallGrid = 2048;
 }  // namespace
 
 void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
+    if (p.n_lblk > 0) {
+        float* part_h = static_cast<float*>(w.lin_part_h);
+        ZB_TIME_BEGIN(w, K_LINEARISE, s);
+        k_lin_blocks_f<<<p.n_lblk, kLinBlock, 0, s>>>(p, w.pose, w.point, w.Hll_f, w.bl_m, w.W_f, part_h, w.lin_part_g,
+                                                      w.pose_terms_f, w.lm, gated ? 1 : 0, w.frozen);
+        ZB_TIME_END(w, s);
+        if (p.np > 0) {
+            ZB_TIME_BEGIN(w, K_POSE_REDUCE, s);
+            k_lin_finish_f<<<p.np, mixed::kFinishThreads, 0, s>>>(p, part_h, w.lin_part_g, w.Hpp_f, w.bp_m, w.lm,
+                                                                 gated ? 1 : 0);
+            ZB_TIME_END(w, s);
+        }
+        if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);
+        return;
+    }
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_lin_points_f<<<grid_of(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms_f, w.pose_grad, w.Hll_f,

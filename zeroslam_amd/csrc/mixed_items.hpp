@@ -165,6 +165,181 @@ __device__ inline void lin_point(const dev_problem& p, const double* pose, const
     if (frozen) frozen[l] = behind ? 1 : 0;
 }
 
+// ---- linearisation by blocks of observations (one thread each) ---------------
+// Block b holds whole points (p.lb_start).  Each thread linearises one
+// observation once (linearise_obs, the arithmetic lin_point uses) and stages its
+// terms in LDS; then the thread of each point's first observation adds the
+// point's terms in insertion order exactly as lin_point does (H_ll, b_l, the W
+// blocks of pairs with several observations, the behind flag), and threads
+// (entry, term) add the pose terms of each free pose the block touches, in block
+// order, into per-entry partials (lin_part_h in TS, lin_part_g in binary64).
+// lin_pose_finish then adds a pose's partials over its blocks.  The W terms of
+// observations whose pair has several observations pass through wscratch (18
+// per observation slot).
+template <class TS>
+struct lin_stage {
+    TS h[kLinBlock][21];     // pose H_pp terms
+    double g[kLinBlock][6];  // pose gradient terms
+    TS ph[kLinBlock][9];     // point H_ll terms
+    double pg[kLinBlock][3]; // point gradient terms
+    int32_t pairf[kLinBlock];  // obs_record.pairf, or -2 for a fixed pose
+    uint8_t flags[kLinBlock];  // bit 0 behind camera, bit 1 several observations in the pair
+    int32_t lo_s[kLinBlock + 1];  // the block's entries: offset ranges
+    uint8_t lo_l[kLinBlock];      // and offsets
+};
+
+template <class TS>
+__device__ inline void lin_block(const dev_problem& p, const double* pose, const double* point, TS* Hll, double* bl,
+                                 TS* Wbuf, uint8_t* frozen, TS* part_h, double* part_g, TS* wscratch, int b,
+                                 lin_stage<TS>& sh) {
+    const int t = threadIdx.x;
+    const int s0 = p.lb_start[b], m = p.lb_start[b + 1] - s0;
+    const int e0 = p.lp_start[b], ne = p.lp_start[b + 1] - e0;
+    const int l0 = p.lo_start[e0];
+    // the block's entry table into LDS
+    for (int i = t; i <= ne; i += blockDim.x) sh.lo_s[i] = p.lo_start[e0 + i] - l0;
+    for (int i = t; i < p.lo_start[e0 + ne] - l0; i += blockDim.x) sh.lo_l[i] = p.lo_list[l0 + i];
+    if (t < m) {
+        const int s = s0 + t;
+        const obs_record rec = p.obs_rec[s];
+        const bool pfixed = p.pose_fixed[rec.pose] != 0, lfixed = p.point_fixed[rec.point] != 0;
+        bool behind = false, multi = false;
+        if (!(pfixed && lfixed)) {
+            double P[7];
+#pragma unroll
+            for (int i = 0; i < 7; ++i) P[i] = pose[7L * rec.pose + i];
+            const double X[3] = {point[3L * rec.point], point[3L * rec.point + 1], point[3L * rec.point + 2]};
+            obs_lin<TS> o;
+            linearise_obs(p, P, X, s, rec, o);
+            behind = o.behind;
+            if (!pfixed) {
+                TS H21[21], Wpl[18];
+                double v[6];
+                pose_hessian(o.Jpf, o.Jlf, o.Wrf, H21, Wpl);
+                pose_grad(o.Jp, o.W, o.r, o.drho, v);
+#pragma unroll
+                for (int i = 0; i < 21; ++i) sh.h[t][i] = H21[i];
+#pragma unroll
+                for (int i = 0; i < 6; ++i) sh.g[t][i] = v[i];
+                if (!lfixed) {
+                    const int pr = rec.pairf >> 1;
+                    multi = p.pair_multi[pr] != 0;
+                    if (multi) {
+#pragma unroll
+                        for (int i = 0; i < 18; ++i) wscratch[18L * s + i] = Wpl[i];
+                    } else {
+                        TS* Wp = Wbuf + 18L * pr;
+#pragma unroll
+                        for (int i = 0; i < 18; ++i) Wp[i] = Wpl[i];
+                    }
+                }
+            }
+            if (!lfixed) {
+                TS Hl[9];
+                double v[3];
+                point_hessian(o.Jlf, o.Wrf, Hl);
+                point_grad(o.Jl, o.W, o.r, o.drho, v);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) sh.ph[t][i] = Hl[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) sh.pg[t][i] = v[i];
+            }
+        }
+        sh.pairf[t] = pfixed ? -2 : rec.pairf;
+        sh.flags[t] = static_cast<uint8_t>((behind ? 1 : 0) | (multi ? 2 : 0));
+    }
+    __syncthreads();
+    // point sums: thread (point, term) adds the point's terms in insertion order
+    // (terms 0-8 H_ll, 9-11 b_l; term 12 the behind flag and the W blocks of
+    // pairs with several observations)
+    const int pt0 = p.lb_point[b], npt = p.lb_point[b + 1] - pt0;
+    for (int q = t; q < npt * 13; q += blockDim.x) {
+        const int l = pt0 + q / 13, term = q - (q / 13) * 13;
+        const bool lfixed = p.point_fixed[l] != 0;
+        const int k0 = p.lm_start[l] - s0, k1 = p.lm_start[l + 1] - s0;
+        if (term < 9) {
+            TS acc = 0;
+            if (!lfixed)
+                for (int k = k0; k < k1; ++k) acc = acc + sh.ph[k][term];
+            Hll[9L * l + term] = acc;
+        } else if (term < 12) {
+            double acc = 0.0;
+            if (!lfixed)
+                for (int k = k0; k < k1; ++k) acc = acc - sh.pg[k][term - 9];
+            bl[3L * l + (term - 9)] = acc;
+        } else {
+            bool behind = false;
+            for (int k = k0; k < k1; ++k) {
+                const int pf = sh.pairf[k];
+                const bool pfixed = pf == -2;
+                if (pfixed && lfixed) continue;
+                behind = behind || (sh.flags[k] & 1) != 0;
+                if (!pfixed && !lfixed && (sh.flags[k] & 2)) {
+                    TS* Wp = Wbuf + 18L * (pf >> 1);
+                    const TS* Wo = wscratch + 18L * (s0 + k);
+                    if (pf & 1) {
+#pragma unroll
+                        for (int i = 0; i < 18; ++i) Wp[i] = Wo[i];
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < 18; ++i) Wp[i] = Wp[i] + Wo[i];
+                    }
+                }
+            }
+            if (frozen) frozen[l] = behind ? 1 : 0;
+        }
+    }
+    // pose partials, written at the entry's pose-major slot
+    for (int q = t; q < ne * 27; q += blockDim.x) {
+        const int ei = q / 27, j = q - ei * 27;
+        const int o0 = sh.lo_s[ei], o1 = sh.lo_s[ei + 1];
+        const int64_t slot = p.e_slot[e0 + ei];
+        if (j < 21) {
+            TS acc = 0;
+            for (int o = o0; o < o1; ++o) acc = acc + sh.h[sh.lo_l[o]][j];
+            part_h[21 * slot + j] = acc;
+        } else {
+            double acc = 0.0;
+            for (int o = o0; o < o1; ++o) acc = acc + sh.g[sh.lo_l[o]][j - 21];
+            part_g[6 * slot + (j - 21)] = acc;
+        }
+    }
+}
+
+// Pose i by one workgroup of kFinishThreads: thread t adds the partials of slots
+// pe_start[i] + t, + kFinishThreads, ... (binary64, 27 terms), each wave adds its
+// lanes by a fixed butterfly, and the waves' sums are added in wave order; H_pp
+// terms are rounded to TS, b_p = -(sum of gradient terms).  red: 4 x 27 doubles.
+constexpr int kFinishThreads = 256;
+template <class TS>
+__device__ inline void lin_pose_finish(const dev_problem& p, const TS* part_h, const double* part_g, TS* Hpp,
+                                       double* bp, int i, double (*red)[27]) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int k0 = p.pe_start[i], k1 = p.pe_start[i + 1];
+    double acc[27];
+#pragma unroll
+    for (int j = 0; j < 27; ++j) acc[j] = 0.0;
+    for (int k = k0 + t; k < k1; k += kFinishThreads) {
+#pragma unroll
+        for (int j = 0; j < 21; ++j) acc[j] += static_cast<double>(part_h[21L * k + j]);
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc[21 + j] += part_g[6L * k + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 27; ++j) {
+        double v = acc[j];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (t < 27) {
+        const double sum = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+        if (t < 21) Hpp[21L * i + t] = static_cast<TS>(sum);
+        else bp[6L * i + (t - 21)] = -sum;
+    }
+}
+
 // Chunk c of [t0, t1) split into kPoseChunks contiguous pieces.
 __device__ __forceinline__ void chunk_range(int t0, int t1, int c, int& c0, int& c1) {
     const int64_t len = t1 - t0;

@@ -37,11 +37,14 @@ class ZsbaError(RuntimeError):
 
 class Graph:
     def __init__(self, device: int = 0, verbose: int = 0, refine_steps: int = 0, timing: bool = False,
-                 floor_exp: int = 0, factor_path: int = 0, timing_classes=None, freeze_behind: int = 0):
+                 floor_exp: int = 0, factor_path: int = 0, timing_classes=None, freeze_behind: int = 0,
+                 attempt_graph: int = 0):
         """timing: time kernel classes with HIP events (stats()["kernel_ms"]);
         timing_classes: names from KERNEL_NAMES to time (None: all);
         freeze_behind: the behind-camera freeze (0 default: MIXED only, 1 also
-        FP64_FAST, -1 off; never FP64)."""
+        FP64_FAST, -1 off; never FP64);
+        attempt_graph: 0 replays each LM attempt as one captured HIP graph, -1
+        launches its kernels one by one."""
         self.lib = _lib.load()
         mask = 0
         if timing_classes:
@@ -49,7 +52,7 @@ class Graph:
                 mask |= 1 << _lib.KERNEL_NAMES.index(name)
         opts = _lib.Options(device=device, verbose=verbose, refine_steps=refine_steps, timing=int(bool(timing)),
                             floor_exp=floor_exp, factor_path=factor_path, timing_classes=mask,
-                            freeze_behind=freeze_behind)
+                            freeze_behind=freeze_behind, attempt_graph=attempt_graph)
         self.h = self.lib.zsba_create(C.byref(opts))
         if not self.h:
             raise MemoryError("zsba_create failed")
