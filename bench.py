@@ -257,7 +257,7 @@ def seed_compare(seed, config, iters):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_mixed_hbm_traffic.md")
 GROUP_KERNELS = {
     "cholesky": ["k_chol_begin", "k_chol_fused<true>"],
-    "linearisation": ["k_lin_points_f", "k_reduce_pose_part_f", "k_reduce_pose_f"],
+    "linearisation": ["k_lin_blocks_f", "k_lin_finish_f"],
     "schur": ["k_damp_invert_f", "k_form_Y_f", "k_fill_zero_d", "k_schur_f<2>"],
     "cost": ["k_cost"],
     "backsub": ["k_backsub_update<float>"],
@@ -392,10 +392,10 @@ def bench_batch(args) -> int:
                    "phase_ms_per_window": {k: round(v, 4) for k, v in info["phase_ms"].items()},
                    "attempts_per_window": attempts / (args.steps * n_windows),
                    "accepted_per_window": accepted / (args.steps * n_windows)},
-        # the grid-wide linearisation (k_batch_lin_points / _chunks / _entries per round)
+        # the grid-wide linearisation (k_batch_lin_blocks, then k_batch_lin_finish, per round)
         "roofline": {"bound": "hbm", "achieved": lin_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": lin_achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "linearisation (k_batch_lin_points, k_batch_lin_chunks, k_batch_lin_entries)",
+                     "kernel": "linearisation (k_batch_lin_blocks, k_batch_lin_finish)",
                      "algorithmic_bytes": lin_bytes, "avg_ms": lin_ms},
         "roofline_whole_solve": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": per_step_bytes,
