@@ -246,25 +246,25 @@ int32_t zsba_restore(zsba_graph* graph);
  * graphs must all use one device and stay alive for the life of the batch. */
 typedef struct zsba_batch zsba_batch;
 typedef struct zsba_batch_info {
-    int32_t windows;          /* graphs solved by the batched launch                 */
+    int32_t windows;          /* graphs solved by the batched launches               */
     int32_t individual;       /* graphs solved one by one                            */
-    int32_t threads;          /* workgroup size per window                           */
+    int32_t threads;          /* workgroup size of the factorisation and solves      */
     int32_t graph_launches;   /* hipGraph launches by this solve                     */
-    float tune_ms[3];         /* autotuner trial times for 256, 512, 1024 threads    */
+    float tune_ms[3];         /* autotuner: fastest trial with 256, 512, 1024 threads
+                                 (0 = not tried)                                     */
     float pad;
     double autotune_wall_ms;  /* time spent autotuning in this solve (0 when cached) */
-    double kernel_wall_ms;    /* batched launch, host-measured, synchronised         */
+    double kernel_wall_ms;    /* batched launches, host-measured, synchronised       */
     double solve_ms;          /* whole zsba_batch_solve call                         */
-    double phase_ms[8];       /* mean per-window device time by phase: cost,
-                                 linearisation, damping + inverses + Y, Schur,
-                                 factorisation, triangular solve, refinement,
-                                 update + controller; linearisation is the
-                                 grid-wide phase that every window shares, so its
-                                 entry is lin_ms                                     */
-    double lin_ms;            /* grid-wide linearisation launches of the solve,
-                                 device clock (first workgroup of a round's
-                                 linearisation to first workgroup of its attempts)  */
-    double reserved[3];
+    double phase_ms[8];       /* device time of the batched launches by phase (all
+                                 windows together): cost, linearisation, damping +
+                                 inverses + Y, Schur, factorisation, refinement
+                                 solves, refinement, update + controller             */
+    double lin_ms;            /* phase_ms[1]: the linearisation launches             */
+    int32_t steps;            /* lockstep steps launched (one attempt per unfinished
+                                 window each)                                        */
+    int32_t schur_inflight;   /* Schur contributions in flight per lane (autotuned)  */
+    double reserved[2];
 } zsba_batch_info;
 zsba_batch* zsba_batch_create(zsba_graph* const* graphs, int32_t count);
 void        zsba_batch_destroy(zsba_batch* batch);

@@ -71,7 +71,8 @@ class BatchInfo(C.Structure):
     _fields_ = [("windows", C.c_int32), ("individual", C.c_int32), ("threads", C.c_int32),
                 ("graph_launches", C.c_int32), ("tune_ms", C.c_float * 3), ("pad", C.c_float),
                 ("autotune_wall_ms", C.c_double), ("kernel_wall_ms", C.c_double), ("solve_ms", C.c_double),
-                ("phase_ms", C.c_double * 8), ("lin_ms", C.c_double), ("reserved", C.c_double * 3)]
+                ("phase_ms", C.c_double * 8), ("lin_ms", C.c_double), ("steps", C.c_int32),
+                ("schur_inflight", C.c_int32), ("reserved", C.c_double * 2)]
 
     PHASES = ("cost", "linearise", "invert", "schur", "factor", "solve", "refine", "update")
 

@@ -318,9 +318,11 @@ void launch_axpby(double a, const double* x, double b, double* y, int64_t n, hip
 void launch_obs_grad(const dev_problem& p, const dev_work64& w, const double* lam, double* dobs_by_e, hipStream_t s);
 
 // ---- batched local BA (kernels_batch.hip) --------------------------------------
-// One workgroup per window runs the window's LM attempts; the linearisations of
-// every window run as grid-wide launches between them.  Windows are limited to
-// reduced systems of kBatchMaxN rows (42 poses).
+// Windows solve in lockstep: each step every unfinished window linearises (when
+// its last attempt was accepted) and makes one LM attempt, each phase one
+// grid-wide launch over every window's items; the factorisation and the
+// substitutions run one workgroup per window.  Windows are limited to reduced
+// systems of kBatchMaxN rows (42 poses).
 constexpr int kBatchMaxN = 256;
 struct batch_window {
     dev_problem p;
@@ -328,32 +330,40 @@ struct batch_window {
     int32_t iterations = 0, use_relative = 0, refine_steps = 0, pad = 0;
     double floor_rel = 0.0;
 };
-// Phases timed inside the batched solver.
+// Phases of the batched solver's device clock.
 enum batch_phase : int {
     PH_COST = 0, PH_LINEARISE = 1, PH_INVERT = 2, PH_SCHUR = 3, PH_FACTOR = 4, PH_SOLVE = 5, PH_REFINE = 6, PH_UPDATE = 7
 };
 constexpr int kBatchPhases = 8;
-// Sizes of the grid-wide linearisation launches: prefix sums (count + 1 entries
-// each, on the device) of the windows' points, pose chunks (np * kPoseChunks),
-// pose-block entries (np * 27) and linearisation blocks, with their totals.
-struct batch_grid {
-    const int32_t* points = nullptr;
-    const int32_t* chunks = nullptr;
-    const int32_t* entries = nullptr;
-    const int32_t* lblocks = nullptr;  // linearisation blocks (windows with p.n_lblk > 0)
-    int32_t n_points = 0, n_chunks = 0, n_entries = 0, n_lblocks = 0;
-    int32_t n_per_point = 0;  // windows linearised per point (p.n_lblk == 0)
-    int32_t pad = 0;
+// Workgroup tables of the batched launches.  Table k holds, per workgroup of its
+// launch, the window (-1: none) and the workgroup's index inside the window's
+// share: BM_WIN one per window; BM_LBLK one per linearisation block and BM_LPOSE
+// one per pose (windows linearised by blocks); BM_LPTS 256 points, BM_LCHUNK 4
+// pose chunks, BM_LENTRY 256 pose-block entries (windows linearised per point);
+// BM_PTS 256 points; BM_CHUNK 4 pose chunks; BM_ROWS 256 pose rows; BM_INVZ 256
+// of the nl + n^2 items of the damping and the cleared reduced system; BM_PAIR
+// 256 (pose, point) pairs; BM_SBLK 4 reduced-system blocks; BM_PTSN 256 of nl + n;
+// BM_APPLY and BM_COST one per block of the per-graph grids of the update (np + nl
+// items) and of the cost (no items); BM_SLOT 256 of np + nl.
+enum batch_map : int {
+    BM_WIN = 0, BM_LBLK, BM_LPOSE, BM_LPTS, BM_LCHUNK, BM_LENTRY, BM_PTS, BM_CHUNK, BM_ROWS, BM_INVZ, BM_PAIR, BM_SBLK,
+    BM_PTSN, BM_APPLY, BM_COST, BM_SLOT, kBatchMaps
 };
-// Launches of one batched solve: `rounds` rounds of (linearisation of the
-// windows that need one, then every unfinished window's attempts up to its next
-// accepted step).  threads per window: 256, 512 or 1024; out[i] receives window
-// i's controller, phase_ticks[i * kBatchPhases + k] its time in phase k (100 MHz
-// ticks), lin_clock[2 r], lin_clock[2 r + 1] the device clock at the start of
-// round r's linearisation and of its attempts (earliest workgroup).
-void launch_batch_solve(const batch_window* wins, int count, int threads, bool mixed_path, lm_state* out,
-                        int64_t* phase_ticks, const batch_grid& grid, uint64_t* lin_clock, int rounds,
-                        hipStream_t s);
+struct batch_grid {
+    const int2* map[kBatchMaps] = {};
+    int32_t n_wg[kBatchMaps] = {};
+};
+// k_batch_begin: every window's controller reset, initial cost, first
+// linearisation requested; the phase clock pc (2 + kBatchPhases values) cleared.
+void launch_batch_begin(const batch_window* wins, int count, uint64_t* pc, hipStream_t s);
+// `steps` lockstep steps.  threads: the factorisation and solve workgroup size
+// (256, 512 or 1024); inflight: Schur contributions in flight per lane (1 or 2);
+// refine_rounds: the largest refine_steps of the windows (mixed).
+void launch_batch_steps(const batch_window* wins, const batch_grid& g, int threads, int inflight, bool mixed_path,
+                        int refine_rounds, int steps, uint64_t* pc, hipStream_t s);
+// Every controller to out[i]; *all_done = 1 when every window has finished.
+void launch_batch_finish(const batch_window* wins, int count, lm_state* out, int32_t* all_done, uint64_t* pc,
+                         hipStream_t s);
 
 // ---- dense Cholesky of the reduced system (kernels_chol.hip) ------------------
 // Factorises w.S (n x n, lower triangle) in place and solves for w.dxp from

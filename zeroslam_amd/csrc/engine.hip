@@ -1256,14 +1256,17 @@ int32_t zsba_graph::adjoint(const double* dL_dpoints, const double* dL_dposes, d
 }
 
 // ---- batched local BA -----------------------------------------------------------
-// One sequence of launches solves every window whose reduced system fits one
-// workgroup (zb::kBatchMaxN rows); larger windows and the reference-order FP64
-// precision go through the per-graph solve.  The sequence (iterations + 1 rounds
-// of grid-wide linearisation and per-window attempts) is captured once into a
-// hipGraph and replayed by later solves of the same batch; the window table and
-// the linearisation grid it reads are refreshed before each replay.  The
-// workgroup size is chosen per batch shape by timing the candidates on the batch
-// itself (states restored after each trial) and remembered for the process.
+// Every window whose reduced system fits one workgroup (zb::kBatchMaxN rows) is
+// solved in lockstep by one sequence of launches; larger windows and the
+// reference-order FP64 precision go through the per-graph solve.  A solve is one
+// hipGraph launch of the start and the first `iterations` steps (an accepted
+// attempt per iteration), then launches of kMoreSteps steps until every window
+// has finished, each checked one launch behind through a flag in pinned memory.
+// Both graphs are captured once per batch shape and replayed; the window table
+// and the workgroup tables are refreshed before each solve.  The workgroup size
+// of the factorisation and the Schur contributions in flight are chosen per
+// batch shape by timing the candidates on the batch itself (states restored
+// after each trial) and remembered for the process.
 namespace {
 struct tune_key {
     int32_t count, max_n, obs_bucket, mixed;
@@ -1272,7 +1275,9 @@ struct tune_key {
     }
 };
 std::mutex tune_mutex;
-std::map<tune_key, int32_t> tune_cache;
+std::map<tune_key, std::pair<int32_t, int32_t>> tune_cache;  // threads, Schur contributions in flight
+constexpr int32_t kMoreSteps = 2;
+constexpr int kXcds = 8;
 }  // namespace
 
 struct zsba_batch {
@@ -1284,50 +1289,52 @@ struct zsba_batch {
     zb::batch_window* d_wins = nullptr;
     lm_state* d_out = nullptr;
     lm_state* h_out = nullptr;
-    int64_t* d_ticks = nullptr;
-    int64_t* h_ticks = nullptr;
-    int32_t* d_prefix = nullptr;     // 4 (capacity + 1): points, pose chunks, pose-block entries, blocks
-    uint64_t* d_linclk = nullptr;    // 2 rounds
-    uint64_t* h_linclk = nullptr;
-    int32_t capacity = 0, clock_capacity = 0;
+    uint64_t* d_pc = nullptr;  // phase clock: last stamp, its phase, ticks per phase
+    uint64_t* h_pc = nullptr;
+    int32_t* d_flag = nullptr;
+    int32_t* h_flag = nullptr;  // 2 slots
+    int2* d_maps = nullptr;
+    size_t maps_capacity = 0;
+    int32_t capacity = 0;
     std::vector<zb::batch_window> h_wins;
-    int32_t min_threads = 256;       // one factorisation row thread per row below a diagonal tile
-    std::vector<int32_t> h_prefix;
+    std::vector<int2> h_maps;
+    int32_t min_threads = 256;  // one factorisation row thread per row below a diagonal tile
     zb::batch_grid grid{};
-    int32_t rounds = 0;
+    int32_t first_steps = 0, refine_rounds = 0, max_steps = 0;
     std::vector<int32_t> win_graph;  // window -> graph index
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int32_t graph_threads = 0, graph_mixed = -1, graph_count = -1, graph_rounds = -1;
-    int32_t graph_points = -1, graph_chunks = -1, graph_entries = -1, graph_lblocks = -1, graph_per_point = -1;
+    hipGraphExec_t exec_first = nullptr, exec_more = nullptr;
+    std::vector<int32_t> graph_key;
 
     ~zsba_batch() { release(); }
+    void drop_graphs() {
+        if (exec_first) (void)hipGraphExecDestroy(exec_first);
+        if (exec_more) (void)hipGraphExecDestroy(exec_more);
+        exec_first = exec_more = nullptr;
+        graph_key.clear();
+    }
     void release() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
+        drop_graphs();
         if (stream) (void)hipStreamSynchronize(stream);
         if (d_wins) (void)hipFree(d_wins);
         if (d_out) (void)hipFree(d_out);
         if (h_out) (void)hipHostFree(h_out);
-        if (d_ticks) (void)hipFree(d_ticks);
-        if (h_ticks) (void)hipHostFree(h_ticks);
-        if (d_prefix) (void)hipFree(d_prefix);
-        if (d_linclk) (void)hipFree(d_linclk);
-        if (h_linclk) (void)hipHostFree(h_linclk);
+        if (d_pc) (void)hipFree(d_pc);
+        if (h_pc) (void)hipHostFree(h_pc);
+        if (d_flag) (void)hipFree(d_flag);
+        if (h_flag) (void)hipHostFree(h_flag);
+        if (d_maps) (void)hipFree(d_maps);
         if (stream) (void)hipStreamDestroy(stream);
         d_wins = nullptr;
         d_out = nullptr;
         h_out = nullptr;
-        d_ticks = nullptr;
-        h_ticks = nullptr;
-        d_prefix = nullptr;
-        d_linclk = nullptr;
-        h_linclk = nullptr;
+        d_pc = nullptr;
+        h_pc = nullptr;
+        d_flag = nullptr;
+        h_flag = nullptr;
+        d_maps = nullptr;
+        maps_capacity = 0;
         stream = nullptr;
         capacity = 0;
-        clock_capacity = 0;
     }
     int32_t fail(int32_t code, const std::string& m) {
         err = m;
@@ -1337,7 +1344,7 @@ struct zsba_batch {
         return fail(ZSBA_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
     }
 
-    int32_t ensure(int32_t count, int dev, int32_t n_rounds) {
+    int32_t ensure(int32_t count, int dev) {
         hipError_t e;
         if (device != dev) {
             release();
@@ -1346,45 +1353,24 @@ struct zsba_batch {
         if ((e = hipSetDevice(dev)) != hipSuccess) return hip_fail(e, "hipSetDevice");
         if (!stream && (e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)) != hipSuccess)
             return hip_fail(e, "hipStreamCreate");
-        if (n_rounds > clock_capacity) {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-            exec = nullptr;
-            graph = nullptr;
-            if (d_linclk) (void)hipFree(d_linclk);
-            if (h_linclk) (void)hipHostFree(h_linclk);
-            d_linclk = nullptr;
-            h_linclk = nullptr;
-            clock_capacity = 0;
-            const size_t bytes = sizeof(uint64_t) * 2 * n_rounds;
-            if ((e = hipMalloc(&d_linclk, bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
-            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_linclk), bytes, hipHostMallocDefault)) != hipSuccess)
+        if (!d_pc) {
+            const size_t bytes = sizeof(uint64_t) * (2 + zb::kBatchPhases);
+            if ((e = hipMalloc(&d_pc, bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_pc), bytes, hipHostMallocDefault)) != hipSuccess)
                 return hip_fail(e, "hipHostMalloc");
-            clock_capacity = n_rounds;
+            if ((e = hipMalloc(&d_flag, sizeof(int32_t))) != hipSuccess) return hip_fail(e, "hipMalloc");
+            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_flag), 2 * sizeof(int32_t), hipHostMallocDefault)) !=
+                hipSuccess)
+                return hip_fail(e, "hipHostMalloc");
         }
         if (count > capacity) {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-            exec = nullptr;
-            graph = nullptr;
+            drop_graphs();
             if (d_wins) (void)hipFree(d_wins);
             if (d_out) (void)hipFree(d_out);
             if (h_out) (void)hipHostFree(h_out);
-            if (d_ticks) (void)hipFree(d_ticks);
-            if (h_ticks) (void)hipHostFree(h_ticks);
-            if (d_prefix) (void)hipFree(d_prefix);
             d_wins = nullptr;
             d_out = nullptr;
             h_out = nullptr;
-            d_ticks = nullptr;
-            h_ticks = nullptr;
-            d_prefix = nullptr;
-            if ((e = hipMalloc(&d_prefix, sizeof(int32_t) * 4 * (count + 1))) != hipSuccess)
-                return hip_fail(e, "hipMalloc");
-            const size_t tick_bytes = sizeof(int64_t) * zb::kBatchPhases * count;
-            if ((e = hipMalloc(&d_ticks, tick_bytes)) != hipSuccess) return hip_fail(e, "hipMalloc");
-            if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_ticks), tick_bytes, hipHostMallocDefault)) != hipSuccess)
-                return hip_fail(e, "hipHostMalloc");
             if ((e = hipMalloc(&d_wins, sizeof(zb::batch_window) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipMalloc(&d_out, sizeof(lm_state) * count)) != hipSuccess) return hip_fail(e, "hipMalloc");
             if ((e = hipHostMalloc(reinterpret_cast<void**>(&h_out), sizeof(lm_state) * count, hipHostMallocDefault)) !=
@@ -1395,41 +1381,180 @@ struct zsba_batch {
         return 0;
     }
 
-    // One launch of the batch kernel for the current window table.
-    int32_t launch(int32_t threads, bool mixed) {
-        hipError_t e;
-        const int32_t count = static_cast<int32_t>(h_wins.size());
-        if (!exec || graph_threads != threads || graph_mixed != (mixed ? 1 : 0) || graph_count != count ||
-            graph_rounds != rounds || graph_points != grid.n_points || graph_chunks != grid.n_chunks ||
-            graph_entries != grid.n_entries || graph_lblocks != grid.n_lblocks || graph_per_point != grid.n_per_point) {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
-            exec = nullptr;
-            graph = nullptr;
-            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
-                return hip_fail(e, "hipStreamBeginCapture");
-            zb::launch_batch_solve(d_wins, count, threads, mixed, d_out, d_ticks, grid, d_linclk, rounds, stream);
-            if ((e = hipStreamEndCapture(stream, &graph)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
-            if ((e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0)) != hipSuccess)
-                return hip_fail(e, "hipGraphInstantiate");
-            graph_threads = threads;
-            graph_mixed = mixed ? 1 : 0;
-            graph_count = count;
-            graph_rounds = rounds;
-            graph_points = grid.n_points;
-            graph_chunks = grid.n_chunks;
-            graph_entries = grid.n_entries;
-            graph_lblocks = grid.n_lblocks;
-            graph_per_point = grid.n_per_point;
+    // Workgroups of window w in table k (see zb::batch_map).
+    static int64_t share(int k, const zb::batch_window& bw, bool mixed) {
+        const dev_problem& p = bw.p;
+        const bool by_blocks = mixed && p.n_lblk > 0;
+        auto cdiv = [](int64_t a, int64_t b) { return (a + b - 1) / b; };
+        auto sum_grid = [&](int64_t n) {
+            const int64_t G = cdiv(n, 256);
+            return G < 1 ? int64_t(1) : (G > zb::kPartials ? int64_t(zb::kPartials) : G);
+        };
+        switch (k) {
+            case zb::BM_WIN: return 1;
+            case zb::BM_LBLK: return by_blocks ? p.n_lblk : 0;
+            case zb::BM_LPOSE: return by_blocks ? p.np : 0;
+            case zb::BM_LPTS: return by_blocks ? 0 : cdiv(p.nl, 256);
+            case zb::BM_LCHUNK: return by_blocks ? 0 : cdiv(static_cast<int64_t>(p.np) * zb::kPoseChunks, 4);
+            case zb::BM_LENTRY: return by_blocks ? 0 : cdiv(27L * p.np, 256);
+            case zb::BM_PTS: return cdiv(p.nl, 256);
+            case zb::BM_CHUNK: return mixed ? cdiv(static_cast<int64_t>(p.np) * zb::kPoseChunks, 4) : 0;
+            case zb::BM_ROWS: return mixed ? cdiv(6L * p.np, 256) : 0;
+            case zb::BM_INVZ: return cdiv(p.nl + static_cast<int64_t>(p.n) * p.n, 256);
+            case zb::BM_PAIR: return cdiv(p.npair, 256);
+            case zb::BM_SBLK: return cdiv(p.nblk, 4);
+            case zb::BM_PTSN: return mixed ? cdiv(static_cast<int64_t>(p.nl) + p.n, 256) : 0;
+            case zb::BM_APPLY: return sum_grid(static_cast<int64_t>(p.np) + p.nl);
+            case zb::BM_COST: return sum_grid(p.no);
+            case zb::BM_SLOT: return cdiv(static_cast<int64_t>(p.np) + p.nl, 256);
+            default: return 0;
         }
-        if ((e = hipGraphLaunch(exec, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-        info.graph_launches += 1;
+    }
+
+    // The workgroup tables.  Windows go to XCDs balanced by observation count
+    // (largest first, each to the least-loaded XCD); in each table, XCD x's
+    // workgroups are x, x + 8, x + 16, ... (the dispatcher's round robin), holding
+    // its windows' shares in window order.  BM_WIN keeps window i at workgroup i.
+    int32_t build_maps(bool mixed) {
+        const int32_t nw = static_cast<int32_t>(h_wins.size());
+        std::vector<int32_t> order(nw), xcd(nw, 0);
+        for (int32_t i = 0; i < nw; ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return h_wins[a].p.no > h_wins[b].p.no; });
+        int64_t load[kXcds] = {};
+        for (int32_t w : order) {
+            int x = 0;
+            for (int y = 1; y < kXcds; ++y)
+                if (load[y] < load[x]) x = y;
+            xcd[w] = x;
+            load[x] += h_wins[w].p.no + 1;
+        }
+        h_maps.clear();
+        size_t offset[zb::kBatchMaps];
+        for (int k = 0; k < zb::kBatchMaps; ++k) {
+            offset[k] = h_maps.size();
+            if (k == zb::BM_WIN) {
+                for (int32_t i = 0; i < nw; ++i) h_maps.push_back(make_int2(i, 0));
+                grid.n_wg[k] = nw;
+                continue;
+            }
+            std::vector<int2> list[kXcds];
+            for (int32_t w = 0; w < nw; ++w) {
+                const int64_t c = share(k, h_wins[w], mixed);
+                for (int64_t j = 0; j < c; ++j) list[xcd[w]].push_back(make_int2(w, static_cast<int>(j)));
+            }
+            size_t J = 0;
+            for (int x = 0; x < kXcds; ++x) J = std::max(J, list[x].size());
+            if (J * kXcds > static_cast<size_t>(INT32_MAX)) return fail(ZSBA_EINVAL, "batch too large");
+            for (size_t j = 0; j < J; ++j)
+                for (int x = 0; x < kXcds; ++x) h_maps.push_back(j < list[x].size() ? list[x][j] : make_int2(-1, 0));
+            grid.n_wg[k] = static_cast<int32_t>(J * kXcds);
+        }
+        hipError_t e;
+        if (h_maps.size() > maps_capacity) {
+            if (d_maps) (void)hipFree(d_maps);
+            d_maps = nullptr;
+            maps_capacity = 0;
+            if ((e = hipMalloc(&d_maps, sizeof(int2) * h_maps.size())) != hipSuccess) return hip_fail(e, "hipMalloc");
+            maps_capacity = h_maps.size();
+        }
+        if (!h_maps.empty() &&
+            (e = hipMemcpyAsync(d_maps, h_maps.data(), sizeof(int2) * h_maps.size(), hipMemcpyHostToDevice, stream)) !=
+                hipSuccess)
+            return hip_fail(e, "upload workgroup tables");
+        for (int k = 0; k < zb::kBatchMaps; ++k) grid.map[k] = d_maps + offset[k];
         return 0;
     }
 
-    // Times every candidate workgroup size once on this batch, restoring the window
-    // states after each trial, and returns the fastest.
-    int32_t autotune(bool mixed, int32_t& best) {
+    void enqueue_first(int32_t threads, int32_t inflight, bool mixed) {
+        const int32_t nw = static_cast<int32_t>(h_wins.size());
+        zb::launch_batch_begin(d_wins, nw, d_pc, stream);
+        zb::launch_batch_steps(d_wins, grid, threads, inflight, mixed, refine_rounds, first_steps, d_pc, stream);
+        zb::launch_batch_finish(d_wins, nw, d_out, d_flag, d_pc, stream);
+    }
+    void enqueue_more(int32_t threads, int32_t inflight, bool mixed) {
+        const int32_t nw = static_cast<int32_t>(h_wins.size());
+        zb::launch_batch_steps(d_wins, grid, threads, inflight, mixed, refine_rounds, kMoreSteps, d_pc, stream);
+        zb::launch_batch_finish(d_wins, nw, d_out, d_flag, d_pc, stream);
+    }
+
+    int32_t capture(int32_t threads, int32_t inflight, bool mixed) {
+        std::vector<int32_t> key = {threads, inflight, mixed ? 1 : 0, static_cast<int32_t>(h_wins.size()), first_steps,
+                                    refine_rounds};
+        for (int k = 0; k < zb::kBatchMaps; ++k) key.push_back(grid.n_wg[k]);
+        if (exec_first && exec_more && key == graph_key) return 0;
+        drop_graphs();
+        hipError_t e;
+        for (int which = 0; which < 2; ++which) {
+            hipGraph_t gr = nullptr;
+            if ((e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return hip_fail(e, "hipStreamBeginCapture");
+            if (which == 0) enqueue_first(threads, inflight, mixed);
+            else enqueue_more(threads, inflight, mixed);
+            if ((e = hipStreamEndCapture(stream, &gr)) != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+            hipGraphExec_t ex = nullptr;
+            e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+            (which == 0 ? exec_first : exec_more) = ex;
+        }
+        graph_key = key;
+        return 0;
+    }
+
+    // One solve of the window table: the first launch, then more until every
+    // window has finished (or the attempt limit of the per-graph solve,
+    // 10 x iterations, is reached).
+    int32_t run(int32_t threads, int32_t inflight, bool mixed, bool use_graphs) {
+        hipError_t e;
+        if (use_graphs) {
+            int32_t rc = capture(threads, inflight, mixed);
+            if (rc) return rc;
+            if ((e = hipGraphLaunch(exec_first, stream)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+            info.graph_launches += 1;
+        } else {
+            enqueue_first(threads, inflight, mixed);
+        }
+        hipEvent_t ev[2];
+        (void)hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+        (void)hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+        (void)hipMemcpyAsync(&h_flag[0], d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+        (void)hipEventRecord(ev[0], stream);
+        int32_t steps = first_steps;
+        for (int32_t i = 0;; ++i) {
+            const bool more = steps < max_steps;
+            if (more) {
+                if (use_graphs) {
+                    if ((e = hipGraphLaunch(exec_more, stream)) != hipSuccess) {
+                        (void)hipStreamSynchronize(stream);
+                        (void)hipEventDestroy(ev[0]);
+                        (void)hipEventDestroy(ev[1]);
+                        return hip_fail(e, "hipGraphLaunch");
+                    }
+                    info.graph_launches += 1;
+                } else {
+                    enqueue_more(threads, inflight, mixed);
+                }
+                (void)hipMemcpyAsync(&h_flag[(i + 1) & 1], d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+                (void)hipEventRecord(ev[(i + 1) & 1], stream);
+                steps += kMoreSteps;
+            }
+            (void)hipEventSynchronize(ev[i & 1]);
+            if (h_flag[i & 1] || !more) break;
+        }
+        e = hipStreamSynchronize(stream);
+        (void)hipEventDestroy(ev[0]);
+        (void)hipEventDestroy(ev[1]);
+        if (e != hipSuccess) return hip_fail(e, "batch solve");
+        if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "kernel launch");
+        info.steps = steps;
+        return 0;
+    }
+
+    // Times every candidate (factorisation workgroup size x Schur contributions in
+    // flight) once on this batch, restoring the window states after each trial,
+    // and returns the fastest.
+    int32_t autotune(bool mixed, int32_t& best_threads, int32_t& best_inflight) {
         hipError_t e;
         std::vector<double*> saved;
         for (int32_t k : win_graph) {
@@ -1445,40 +1570,37 @@ struct zsba_batch {
                                  stream);
             saved.push_back(buf);
         }
-        hipEvent_t t0, t1;
-        (void)hipEventCreate(&t0);
-        (void)hipEventCreate(&t1);
-        float best_ms = 0.f;
-        best = 0;
+        (void)hipStreamSynchronize(stream);
+        double best_ms = 0.0;
+        best_threads = 0;
+        best_inflight = 1;
         int32_t rc = 0;
         for (int32_t threads : {256, 512, 1024}) {
             if (threads < min_threads) continue;
-            (void)hipEventRecord(t0, stream);
-            zb::launch_batch_solve(d_wins, static_cast<int32_t>(h_wins.size()), threads, mixed, d_out, d_ticks, grid,
-                                   d_linclk, rounds, stream);
-            (void)hipEventRecord(t1, stream);
-            if ((e = hipEventSynchronize(t1)) != hipSuccess) {
-                rc = hip_fail(e, "autotune");
-                break;
+            for (int32_t inflight : {1, 2}) {
+                const double t0 = now_ms();
+                if ((rc = run(threads, inflight, mixed, false))) break;
+                const double ms = now_ms() - t0;
+                float& slot = info.tune_ms[threads == 256 ? 0 : threads == 512 ? 1 : 2];
+                if (slot == 0.f || ms < slot) slot = static_cast<float>(ms);
+                if (best_threads == 0 || ms < best_ms) {
+                    best_threads = threads;
+                    best_inflight = inflight;
+                    best_ms = ms;
+                }
+                for (size_t i = 0; i < win_graph.size(); ++i) {
+                    zsba_graph* g = graphs[win_graph[i]];
+                    (void)hipMemcpyAsync(g->w.pose, saved[i], sizeof(double) * 7L * g->dp.np, hipMemcpyDeviceToDevice,
+                                         stream);
+                    (void)hipMemcpyAsync(g->w.point, saved[i] + 7L * g->dp.np, sizeof(double) * 3L * g->dp.nl,
+                                         hipMemcpyDeviceToDevice, stream);
+                }
+                (void)hipStreamSynchronize(stream);
             }
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, t0, t1);
-            info.tune_ms[threads == 256 ? 0 : threads == 512 ? 1 : 2] = ms;
-            if (best == 0 || ms < best_ms) {
-                best = threads;
-                best_ms = ms;
-            }
-            for (size_t i = 0; i < win_graph.size(); ++i) {
-                zsba_graph* g = graphs[win_graph[i]];
-                (void)hipMemcpyAsync(g->w.pose, saved[i], sizeof(double) * 7L * g->dp.np, hipMemcpyDeviceToDevice, stream);
-                (void)hipMemcpyAsync(g->w.point, saved[i] + 7L * g->dp.np, sizeof(double) * 3L * g->dp.nl,
-                                     hipMemcpyDeviceToDevice, stream);
-            }
+            if (rc) break;
         }
-        (void)hipStreamSynchronize(stream);
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
         for (double* buf : saved) (void)hipFree(buf);
+        info.steps = 0;
         return rc;
     }
 
@@ -1492,6 +1614,8 @@ struct zsba_batch {
         h_wins.clear();
         win_graph.clear();
         int dev = -1;
+        const bool mixed = precision == ZSBA_PRECISION_MIXED;
+        int32_t max_refine = 0;
         for (int32_t k = 0; k < count; ++k) {
             zsba_graph* g = graphs[k];
             accepted[k] = 0;
@@ -1521,7 +1645,8 @@ struct zsba_batch {
             bw.iterations = iterations;
             bw.use_relative = use_rel ? 1 : 0;
             bw.refine_steps = g->opt.refine_steps;
-            bw.floor_rel = precision == ZSBA_PRECISION_MIXED ? g->mixed_floor() : 0.0;
+            bw.floor_rel = mixed ? g->mixed_floor() : 0.0;
+            max_refine = std::max(max_refine, bw.refine_steps);
             h_wins.push_back(bw);
             win_graph.push_back(k);
         }
@@ -1531,49 +1656,19 @@ struct zsba_batch {
             info.solve_ms = now_ms() - t_start;
             return 0;
         }
-        // iterations + 1 rounds: a window linearises after each accepted step that
-        // does not end its solve, so it pauses at most iterations - 1 times
-        rounds = std::max(iterations, 0) + 1;
-        int32_t rc = ensure(nw, dev, rounds);
+        int32_t rc = ensure(nw, dev);
         if (rc) return rc;
         hipError_t e;
         if ((e = hipMemcpyAsync(d_wins, h_wins.data(), sizeof(zb::batch_window) * nw, hipMemcpyHostToDevice, stream)) !=
             hipSuccess)
             return hip_fail(e, "upload windows");
-        {
-            h_prefix.assign(4 * (nw + 1), 0);
-            int64_t pts = 0, chunks = 0, entries = 0, lblocks = 0;
-            int32_t per_point = 0;
-            for (int32_t i = 0; i < nw; ++i) {
-                h_prefix[i] = static_cast<int32_t>(pts);
-                h_prefix[(nw + 1) + i] = static_cast<int32_t>(chunks);
-                h_prefix[2 * (nw + 1) + i] = static_cast<int32_t>(entries);
-                h_prefix[3 * (nw + 1) + i] = static_cast<int32_t>(lblocks);
-                pts += h_wins[i].p.nl;
-                chunks += static_cast<int64_t>(h_wins[i].p.np) * zb::kPoseChunks;
-                entries += static_cast<int64_t>(h_wins[i].p.np) * 27;
-                lblocks += h_wins[i].p.n_lblk;
-                if (h_wins[i].p.n_lblk == 0) ++per_point;
-            }
-            if (entries > INT32_MAX || 64 * chunks > INT32_MAX) return fail(ZSBA_EINVAL, "batch too large");
-            h_prefix[nw] = static_cast<int32_t>(pts);
-            h_prefix[(nw + 1) + nw] = static_cast<int32_t>(chunks);
-            h_prefix[2 * (nw + 1) + nw] = static_cast<int32_t>(entries);
-            h_prefix[3 * (nw + 1) + nw] = static_cast<int32_t>(lblocks);
-            grid.points = d_prefix;
-            grid.chunks = d_prefix + (nw + 1);
-            grid.entries = d_prefix + 2 * (nw + 1);
-            grid.lblocks = d_prefix + 3 * (nw + 1);
-            grid.n_points = static_cast<int32_t>(pts);
-            grid.n_chunks = static_cast<int32_t>(chunks);
-            grid.n_entries = static_cast<int32_t>(entries);
-            grid.n_lblocks = static_cast<int32_t>(lblocks);
-            grid.n_per_point = per_point;
-            if ((e = hipMemcpyAsync(d_prefix, h_prefix.data(), sizeof(int32_t) * h_prefix.size(), hipMemcpyHostToDevice,
-                                    stream)) != hipSuccess)
-                return hip_fail(e, "upload linearisation grid");
-        }
-        const bool mixed = precision == ZSBA_PRECISION_MIXED;
+        if ((rc = build_maps(mixed))) return rc;
+        // an accepted attempt per iteration in the first launch; at most 10
+        // attempts per iteration (the controller ends an iteration after 10
+        // failures), the per-graph solve's limit
+        first_steps = std::max(iterations, 1);
+        max_steps = 10 * std::max(iterations, 1);
+        refine_rounds = mixed ? max_refine : 0;
         int32_t max_n = 0, max_panel = 1;
         int64_t obs = 0;
         for (int32_t i = 0; i < nw; ++i) {
@@ -1586,30 +1681,31 @@ struct zsba_batch {
         min_threads = (max_panel - 1) * 32 + 64 <= 256 ? 256 : 512;
         const tune_key key{nw, max_n * 64 + max_panel, static_cast<int32_t>(std::log2(static_cast<double>(obs) + 1.0)),
                            mixed ? 1 : 0};
-        int32_t threads = 0;
+        int32_t threads = 0, inflight = 1;
         {
             std::lock_guard<std::mutex> lock(tune_mutex);
             auto it = tune_cache.find(key);
-            if (it != tune_cache.end()) threads = it->second;
+            if (it != tune_cache.end()) {
+                threads = it->second.first;
+                inflight = it->second.second;
+            }
         }
         if (threads == 0) {
             const double t = now_ms();
-            if ((rc = autotune(mixed, threads))) return rc;
+            if ((rc = autotune(mixed, threads, inflight))) return rc;
             info.autotune_wall_ms = now_ms() - t;
             std::lock_guard<std::mutex> lock(tune_mutex);
-            tune_cache[key] = threads;
+            tune_cache[key] = {threads, inflight};
         }
         info.threads = threads;
+        info.schur_inflight = inflight;
         const double t0 = now_ms();
-        if ((rc = launch(threads, mixed))) return rc;
+        if ((rc = run(threads, inflight, mixed, true))) return rc;
         if ((e = hipMemcpyAsync(h_out, d_out, sizeof(lm_state) * nw, hipMemcpyDeviceToHost, stream)) != hipSuccess)
             return hip_fail(e, "download controllers");
-        if ((e = hipMemcpyAsync(h_ticks, d_ticks, sizeof(int64_t) * zb::kBatchPhases * nw, hipMemcpyDeviceToHost,
-                                stream)) != hipSuccess)
-            return hip_fail(e, "download phase clocks");
-        if ((e = hipMemcpyAsync(h_linclk, d_linclk, sizeof(uint64_t) * 2 * rounds, hipMemcpyDeviceToHost, stream)) !=
+        if ((e = hipMemcpyAsync(h_pc, d_pc, sizeof(uint64_t) * (2 + zb::kBatchPhases), hipMemcpyDeviceToHost, stream)) !=
             hipSuccess)
-            return hip_fail(e, "download linearisation clocks");
+            return hip_fail(e, "download phase clock");
         if ((e = hipStreamSynchronize(stream)) != hipSuccess) return hip_fail(e, "batch solve");
         info.kernel_wall_ms = now_ms() - t0;
         for (int32_t i = 0; i < nw; ++i) {
@@ -1629,16 +1725,9 @@ struct zsba_batch {
             g->device_newer = true;
             accepted[win_graph[i]] = s.success;
         }
-        // mean per-window phase times (the device clock runs at 100 MHz)
-        for (int k = 0; k < zb::kBatchPhases; ++k) {
-            double sum = 0.0;
-            for (int32_t i = 0; i < nw; ++i) sum += static_cast<double>(h_ticks[i * zb::kBatchPhases + k]);
-            info.phase_ms[k] = sum / nw * 1e-5;
-        }
-        info.lin_ms = 0.0;
-        for (int32_t r = 0; r < rounds; ++r)
-            if (h_linclk[2 * r + 1] >= h_linclk[2 * r]) info.lin_ms += static_cast<double>(h_linclk[2 * r + 1] - h_linclk[2 * r]) * 1e-5;
-        info.phase_ms[zb::PH_LINEARISE] = info.lin_ms;
+        // device time by phase over the whole batch (the clock runs at 100 MHz)
+        for (int k = 0; k < zb::kBatchPhases; ++k) info.phase_ms[k] = static_cast<double>(h_pc[2 + k]) * 1e-5;
+        info.lin_ms = info.phase_ms[zb::PH_LINEARISE];
         info.solve_ms = now_ms() - t_start;
         return 0;
     }
