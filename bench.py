@@ -184,7 +184,7 @@ def equal_cost(g, gfz, problem, iters, steps, warmup, chi64, ms64, P64, X64):
         print(f"bench: mixed settles at chi2 {cmix!r} (FP64 {chi64!r}); comparing at that level",
               file=sys.stderr, flush=True)
     kfast = iterations_to_reach(gfz, FP64_FAST, level, 1, 4 * iters)
-    msf_level = time_solves(gfz, FP64_FAST, kfast, steps, warmup)[1] if kfast else None
+    msf_level, st_fast = time_solves(gfz, FP64_FAST, kfast, steps, warmup)[1:3] if kfast else (None, None)
     vraw, ms, stats, kms, kl = time_solves(g, MIXED, kmix, steps, warmup)
     chi_mix = stats["final_chi2"]
     state_err = state_error(g.poses(), g.points(), P64, X64)
@@ -203,6 +203,10 @@ def equal_cost(g, gfz, problem, iters, steps, warmup, chi64, ms64, P64, X64):
            "frozen_points": stats["frozen_points"],
            "speedup_vs_fp64": ms64_level / ms,
            "speedup_vs_fp64_fast": (msf_level / ms) if msf_level else None,
+           # per LM attempt (independent of how many iterations each run draws)
+           "attempts_mixed": stats["attempts"], "attempts_fp64_fast": st_fast["attempts"] if st_fast else None,
+           "ms_per_attempt_mixed": ms / max(stats["attempts"], 1),
+           "ms_per_attempt_fp64_fast": (msf_level / max(st_fast["attempts"], 1)) if st_fast else None,
            "solver_rule": "behind-camera freeze (mixed and FP64-fast); FP64 is the reference's LM unchanged"}
     gnf = getattr(g, "_no_freeze", None)
     if gnf is not None:
@@ -248,7 +252,8 @@ def seed_compare(seed, config, iters):
     cmp, _, _ = equal_cost(g, gfz, pr, iters, 2, 1, st64["final_chi2"], ms64, g.poses(), g.points())
     keep = ("equal_cost_level", "iterations_mixed", "iterations_fp64", "iterations_fp64_fast", "ms_mixed", "ms_fp64",
             "ms_fp64_fast", "chi2_rel_vs_fp64", "chi2_rel_vs_fp64_at_equal_iterations", "frozen_points",
-            "speedup_vs_fp64", "speedup_vs_fp64_fast", "mixed_without_freeze")
+            "speedup_vs_fp64", "speedup_vs_fp64_fast", "ms_per_attempt_mixed", "ms_per_attempt_fp64_fast",
+            "mixed_without_freeze")
     out = {"seed": seed, "n_obs": pr.n_obs}
     out.update({k: cmp[k] for k in keep if k in cmp})
     return out

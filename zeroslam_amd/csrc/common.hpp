@@ -338,16 +338,16 @@ constexpr int kBatchPhases = 8;
 // Workgroup tables of the batched launches.  Table k holds, per workgroup of its
 // launch, the window (-1: none) and the workgroup's index inside the window's
 // share: BM_WIN one per window; BM_LBLK one per linearisation block and BM_LPOSE
-// one per pose (windows linearised by blocks); BM_LPTS 256 points, BM_LCHUNK 4
-// pose chunks, BM_LENTRY 256 pose-block entries (windows linearised per point);
-// BM_PTS 256 points; BM_CHUNK 4 pose chunks; BM_ROWS 256 pose rows; BM_INVZ 256
-// of the nl + n^2 items of the damping and the cleared reduced system; BM_PAIR
-// 256 (pose, point) pairs; BM_SBLK 4 reduced-system blocks; BM_PTSN 256 of nl + n;
+// one per pose (mixed windows with linearisation blocks: their linearisation and
+// refinement); BM_LPTS 256 points, BM_LCHUNK 4 pose chunks, BM_LENTRY 256
+// pose-block entries (the other windows); BM_PTS 256 points; BM_INVZ 256 of the
+// nl + n^2 items of the damping and the cleared reduced system; BM_PAIR 256
+// (pose, point) pairs; BM_SBLK 4 reduced-system blocks; BM_PTSN 256 of nl + n;
 // BM_APPLY and BM_COST one per block of the per-graph grids of the update (np + nl
 // items) and of the cost (no items); BM_SLOT 256 of np + nl.
 enum batch_map : int {
-    BM_WIN = 0, BM_LBLK, BM_LPOSE, BM_LPTS, BM_LCHUNK, BM_LENTRY, BM_PTS, BM_CHUNK, BM_ROWS, BM_INVZ, BM_PAIR, BM_SBLK,
-    BM_PTSN, BM_APPLY, BM_COST, BM_SLOT, kBatchMaps
+    BM_WIN = 0, BM_LBLK, BM_LPOSE, BM_LPTS, BM_LCHUNK, BM_LENTRY, BM_PTS, BM_INVZ, BM_PAIR, BM_SBLK, BM_PTSN, BM_APPLY,
+    BM_COST, BM_SLOT, kBatchMaps
 };
 struct batch_grid {
     const int2* map[kBatchMaps] = {};

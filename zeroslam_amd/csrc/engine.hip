@@ -1398,8 +1398,6 @@ struct zsba_batch {
             case zb::BM_LCHUNK: return by_blocks ? 0 : cdiv(static_cast<int64_t>(p.np) * zb::kPoseChunks, 4);
             case zb::BM_LENTRY: return by_blocks ? 0 : cdiv(27L * p.np, 256);
             case zb::BM_PTS: return cdiv(p.nl, 256);
-            case zb::BM_CHUNK: return mixed ? cdiv(static_cast<int64_t>(p.np) * zb::kPoseChunks, 4) : 0;
-            case zb::BM_ROWS: return mixed ? cdiv(6L * p.np, 256) : 0;
             case zb::BM_INVZ: return cdiv(p.nl + static_cast<int64_t>(p.n) * p.n, 256);
             case zb::BM_PAIR: return cdiv(p.npair, 256);
             case zb::BM_SBLK: return cdiv(p.nblk, 4);

@@ -366,7 +366,8 @@ __global__ void __launch_bounds__(kThreads) k_batch_factor(const batch_window* w
     }
 }
 
-// dx_l = H_ll^-1 (b_l - sum W^T dx_p), one thread per point.
+// dx_l = H_ll^-1 (b_l - sum W^T dx_p), one thread per point (mixed windows with
+// linearisation blocks form it in their first refinement round).
 template <class TS>
 __global__ void __launch_bounds__(256) k_batch_point_step(const batch_window* wins, const int2* map, uint64_t* pc) {
     stamp(pc, PH_UPDATE);
@@ -375,6 +376,7 @@ __global__ void __launch_bounds__(256) k_batch_point_step(const batch_window* wi
     const batch_window& bw = wins[m.x];
     const lm_state* lm = bw.w.lm;
     if (lm->done || !lm->solve_ok) return;
+    if (sizeof(TS) == 4 && bw.p.n_lblk > 0 && bw.refine_steps > 0) return;  // k_batch_refine_blocks forms them
     const int l = m.y * 256 + static_cast<int>(threadIdx.x);
     if (l >= bw.p.nl) return;
     const blocks<TS> B(bw.w);
@@ -387,6 +389,37 @@ __device__ __forceinline__ bool refining(const batch_window& bw, int k) {
     return !lm->done && lm->solve_ok && k < bw.refine_steps;
 }
 
+// Windows with linearisation blocks: one workgroup per block, then one wave per
+// pose (the per-graph k_refine_blocks, k_refine_finish).
+template <class TS>
+__global__ void __launch_bounds__(kLinBlock) k_batch_refine_blocks(const batch_window* wins, const int2* map,
+                                                                   uint64_t* pc, int k) {
+    stamp(pc, PH_REFINE);
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const batch_window& bw = wins[m.x];
+    if (!refining(bw, k)) return;
+    __shared__ mixed::refine_stage sh;
+    const dev_work64& w = bw.w;
+    const blocks<TS> B(w);
+    mixed::refine_block(bw.p, w.pose, w.point, B.W, B.Hinv, w.bl_m, w.dxp, w.dxl, w.zl, w.gl, w.lin_part_g,
+                        w.lm->lambda, k == 0, m.y, sh);
+}
+
+__global__ void __launch_bounds__(64) k_batch_refine_finish(const batch_window* wins, const int2* map, uint64_t* pc,
+                                                            int k) {
+    stamp(pc, PH_REFINE);
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;
+    const batch_window& bw = wins[m.x];
+    if (!refining(bw, k)) return;
+    const dev_work64& w = bw.w;
+    mixed::refine_pose_finish(bw.p, w.lin_part_g, w.bp_m, w.dxp, w.rvec, w.lm->lambda, m.y,
+                              static_cast<int>(threadIdx.x));
+}
+
+// The other windows: one thread per point, one wave per pose chunk, one thread
+// per pose row.
 template <class TS>
 __global__ void __launch_bounds__(256) k_batch_full_residual(const batch_window* wins, const int2* map, uint64_t* pc,
                                                              int k) {
@@ -614,9 +647,14 @@ void enqueue_step(const batch_window* wins, const batch_grid& g, int threads, in
     else launch_factor<1024>(wins, g, pc, mixed_path, s);
     if (wgs(g, BM_PTS)) k_batch_point_step<TS><<<wgs(g, BM_PTS), 256, 0, s>>>(wins, g.map[BM_PTS], pc);
     for (int k = 0; mixed_path && k < refine_rounds; ++k) {
-        if (wgs(g, BM_PTS)) k_batch_full_residual<TS><<<wgs(g, BM_PTS), 256, 0, s>>>(wins, g.map[BM_PTS], pc, k);
-        if (wgs(g, BM_CHUNK)) k_batch_pose_chunk<TS><<<wgs(g, BM_CHUNK), 256, 0, s>>>(wins, g.map[BM_CHUNK], pc, k);
-        if (wgs(g, BM_ROWS)) k_batch_pose_rows<<<wgs(g, BM_ROWS), 256, 0, s>>>(wins, g.map[BM_ROWS], pc, k);
+        if (wgs(g, BM_LBLK))
+            k_batch_refine_blocks<TS><<<wgs(g, BM_LBLK), kLinBlock, 0, s>>>(wins, g.map[BM_LBLK], pc, k);
+        if (wgs(g, BM_LPOSE)) k_batch_refine_finish<<<wgs(g, BM_LPOSE), 64, 0, s>>>(wins, g.map[BM_LPOSE], pc, k);
+        // windows linearised per point (the pose-row launch reuses the entry table:
+        // 27 np >= 6 np items)
+        if (wgs(g, BM_LPTS)) k_batch_full_residual<TS><<<wgs(g, BM_LPTS), 256, 0, s>>>(wins, g.map[BM_LPTS], pc, k);
+        if (wgs(g, BM_LCHUNK)) k_batch_pose_chunk<TS><<<wgs(g, BM_LCHUNK), 256, 0, s>>>(wins, g.map[BM_LCHUNK], pc, k);
+        if (wgs(g, BM_LENTRY)) k_batch_pose_rows<<<wgs(g, BM_LENTRY), 256, 0, s>>>(wins, g.map[BM_LENTRY], pc, k);
         if (threads == 256) launch_resolve<256>(wins, g, pc, k, s);
         else if (threads == 512) launch_resolve<512>(wins, g, pc, k, s);
         else launch_resolve<1024>(wins, g, pc, k, s);

@@ -734,6 +734,30 @@ __global__ void __launch_bounds__(kBlock) k_full_residual(dev_problem p, const d
                                   lambda_override ? *lambda_override : lm->lambda, l);
 }
 
+// The refinement residual by linearisation blocks (mixed::refine_block), then one
+// wave per pose (mixed::refine_pose_finish).
+template <class TS>
+__global__ void __launch_bounds__(kLinBlock) k_refine_blocks(dev_problem p, const double* pose, const double* point,
+                                                             const TS* Wbuf, const TS* Hinv, const double* bl,
+                                                             const double* dxp, double* dxl, double* rl, double* gl,
+                                                             double* part, const lm_state* lm,
+                                                             const double* lambda_override, const int32_t* status,
+                                                             int step_first) {
+    if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
+    __shared__ mixed::refine_stage sh;
+    mixed::refine_block(p, pose, point, Wbuf, Hinv, bl, dxp, dxl, rl, gl, part,
+                        lambda_override ? *lambda_override : lm->lambda, step_first != 0, static_cast<int>(blockIdx.x),
+                        sh);
+}
+
+__global__ void __launch_bounds__(64) k_refine_finish(dev_problem p, const double* part, const double* bp,
+                                                      const double* dxp, double* rs, const lm_state* lm,
+                                                      const double* lambda_override, const int32_t* status) {
+    if (refine_skip(lm, lambda_override ? 0 : 1, status)) return;
+    mixed::refine_pose_finish(p, part, bp, dxp, rs, lambda_override ? *lambda_override : lm->lambda,
+                              static_cast<int>(blockIdx.x), static_cast<int>(threadIdx.x));
+}
+
 // One wave per (pose, chunk), then one thread per (pose, row).
 template <class TS>
 __global__ void __launch_bounds__(256) k_pose_residual_part(dev_problem p, const double* pslots, const TS* Wbuf,
@@ -906,18 +930,28 @@ void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t
 void launch_refine_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, int steps, const double* lambda_override) {
     if (p.n <= 0 && p.nl <= 0) return;
     const int gated = lambda_override ? 0 : 1;
+    // graphs with linearisation blocks form the point steps and the residual by
+    // blocks of observations
+    const bool by_blocks = p.n_lblk > 0 && steps > 0;
     ZB_TIME_BEGIN(w, K_REFINE, s);
-    if (p.nl > 0)
+    if (p.nl > 0 && !by_blocks)
         k_point_step<float><<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.W_f, w.Hinv_f, w.bl_m, w.dxp, w.dxl, w.lm, gated,
                                                                       w.chol_status);
     ZB_TIME_END(w, s);
     for (int k = 0; k < steps; ++k) {
         ZB_TIME_BEGIN(w, K_REFINE, s);
-        if (p.nl > 0)
+        if (by_blocks) {
+            k_refine_blocks<float><<<p.n_lblk, kLinBlock, 0, s>>>(p, w.pose, w.point, w.W_f, w.Hinv_f, w.bl_m, w.dxp,
+                                                                  w.dxl, w.zl, w.gl, w.lin_part_g, w.lm, lambda_override,
+                                                                  w.chol_status, k == 0 ? 1 : 0);
+            if (p.np > 0)
+                k_refine_finish<<<p.np, 64, 0, s>>>(p, w.lin_part_g, w.bp_m, w.dxp, w.rvec, w.lm, lambda_override,
+                                                    w.chol_status);
+        } else if (p.nl > 0)
             k_full_residual<float><<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.dxp, w.dxl, w.bl_m,
                                                                              w.Hinv_f, w.pose_grad, w.zl, w.gl, w.lm,
                                                                              lambda_override, w.chol_status);
-        if (p.np > 0) {
+        if (p.np > 0 && !by_blocks) {
             k_pose_residual_part<float><<<grid_for(64L * p.np * kPoseChunks, 256), 256, 0, s>>>(
                 p, w.pose_grad, w.W_f, w.gl, w.pose_part, w.lm, lambda_override, w.chol_status);
             k_pose_residual<<<grid_for(6L * p.np, 256), 256, 0, s>>>(p, w.pose_part, w.bp_m, w.dxp, w.rvec, w.lm,

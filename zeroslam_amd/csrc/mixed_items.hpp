@@ -598,10 +598,10 @@ __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* 
 // for (r_p, r_l) by the same elimination:  rs = r_p - W H_ll^-1 r_l,
 // delta_p = S^-1 rs (the Cholesky factor of S), delta_l = H_ll^-1 (r_l - W^T delta_p).
 
-// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p(a)) for point l.
+// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p(a)) for point l, into o.
 template <class TS>
-__device__ inline void point_step_item(const dev_problem& p, const TS* Wbuf, const TS* Hinv, const double* bl,
-                                       const double* dxp, double* dxl, int l) {
+__device__ inline void point_step_value(const dev_problem& p, const TS* Wbuf, const TS* Hinv, const double* bl,
+                                        const double* dxp, int l, double (&o)[3]) {
     double t[3] = {0.0, 0.0, 0.0};
     for (int a = p.lm_pair_start[l]; a < p.lm_pair_start[l + 1]; ++a) {
         const int k = p.pair_pose[a];
@@ -615,11 +615,20 @@ __device__ inline void point_step_item(const dev_problem& p, const TS* Wbuf, con
     }
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-        double o = 0.0;
+        double v = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (bl[3L * l + c] - t[c]);
-        dxl[3L * l + r] = o;
+        for (int c = 0; c < 3; ++c) v += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (bl[3L * l + c] - t[c]);
+        o[r] = v;
     }
+}
+
+template <class TS>
+__device__ inline void point_step_item(const dev_problem& p, const TS* Wbuf, const TS* Hinv, const double* bl,
+                                       const double* dxp, double* dxl, int l) {
+    double o[3];
+    point_step_value(p, Wbuf, Hinv, bl, dxp, l, o);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dxl[3L * l + r] = o[r];
 }
 
 // Point l: r_l and g_l = H_ll^-1 r_l; the pose-side products J_p^T Omega' u of
@@ -760,6 +769,176 @@ __device__ inline void refine_point_item(const dev_problem& p, const TS* Wbuf, c
 #pragma unroll
         for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * (rl[3L * l + c] - tt[c]);
         dxl[3L * l + r] += o;
+    }
+}
+
+// ---- the refinement residual by blocks of observations -----------------------
+// Round k of the refinement on linearisation block b (whole points, one thread
+// per observation, kLinBlock threads).  Round 0 first forms each point's step
+// dx_l = H_ll^-1 (b_l - sum_a W_a^T dx_p) (point_step_value); later rounds read
+// dx_l.  Each observation's Jacobians, robust weight, u = J_p dx_p + J_l dx_l and
+// its pose and point products are formed as full_residual_item forms them; the
+// point sums (insertion order) give r_l and g_l = H_ll^-1 r_l exactly as there.
+// Then each free pose's share of  sum_obs J_p^T Omega' u + sum_a W_a g_l(a)  over
+// the block (the W term at its pair's first observation, the observations in
+// block order) goes to the entry's slot part[6 e_slot + r]; refine_pose_finish
+// adds a pose's slots.
+struct refine_stage {
+    double pv[kLinBlock][6];   // pose-side terms of each observation
+    double lv[kLinBlock][3];   // point-side terms
+    double dl[kLinBlock][3];   // dx_l of the block's points
+    double gl[kLinBlock][3];   // g_l of the block's points
+    int32_t pairf[kLinBlock];  // obs_record.pairf, or -2 for a fixed pose
+    int32_t pt[kLinBlock];     // the observation's point offset in the block
+    int32_t lo_s[kLinBlock + 1];
+    uint8_t lo_l[kLinBlock];
+};
+
+template <class TS>
+__device__ inline void refine_block(const dev_problem& p, const double* pose, const double* point, const TS* Wbuf,
+                                    const TS* Hinv, const double* bl, const double* dxp, double* dxl, double* rl,
+                                    double* gl, double* part, double lam, bool step_first, int b, refine_stage& sh) {
+    const int t = threadIdx.x;
+    const int s0 = p.lb_start[b], m = p.lb_start[b + 1] - s0;
+    const int e0 = p.lp_start[b], ne = p.lp_start[b + 1] - e0;
+    const int l0 = p.lo_start[e0];
+    const int pt0 = p.lb_point[b], npt = p.lb_point[b + 1] - pt0;
+    for (int i = t; i <= ne; i += blockDim.x) sh.lo_s[i] = p.lo_start[e0 + i] - l0;
+    for (int i = t; i < p.lo_start[e0 + ne] - l0; i += blockDim.x) sh.lo_l[i] = p.lo_list[l0 + i];
+    for (int q = t; q < npt; q += blockDim.x) {
+        const int l = pt0 + q;
+        double d[3];
+        if (step_first) {
+            point_step_value(p, Wbuf, Hinv, bl, dxp, l, d);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dxl[3L * l + c] = d[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) d[c] = dxl[3L * l + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sh.dl[q][c] = d[c];
+    }
+    __syncthreads();
+    if (t < m) {
+        const int s = s0 + t;
+        const obs_record rec = p.obs_rec[s];
+        const bool pfixed = p.pose_fixed[rec.pose] != 0, lfixed = p.point_fixed[rec.point] != 0;
+        const int q = rec.point - pt0;
+        double pv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, lv[3] = {0.0, 0.0, 0.0};
+        if (!(pfixed && lfixed)) {
+            const double* K = p.cams + 4 * (rec.param & 0xFFFF);
+            double P[7], Pc[3], qt[4], r[2], Jp[12], Jl[6], W[4], Wr[4], drho;
+#pragma unroll
+            for (int i = 0; i < 7; ++i) P[i] = pose[7L * rec.pose + i];
+            const double X[3] = {point[3L * rec.point], point[3L * rec.point + 1], point[3L * rec.point + 2]};
+            pose_apply(P, X, Pc);
+            const double uv[2] = {rec.u, rec.v};
+            residual(Pc, uv, K, r);
+            pose_quat(P, qt);
+            jacobians(Pc, qt, K, Jp, Jl);
+            obs_weight(p, s, W);
+            const int li = rec.param >> 16;
+            robust_info(p.loss_kind[li], p.loss_delta[li], W, r, drho, Wr);
+            double u[2] = {0.0, 0.0};
+            if (!pfixed) {
+#pragma unroll
+                for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) u[mm] += Jp[6 * mm + k] * dxp[6L * rec.pose + k];
+            }
+            if (!lfixed) {
+#pragma unroll
+                for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) u[mm] += Jl[3 * mm + k] * sh.dl[q][k];
+            }
+            const double wu[2] = {Wr[0] * u[0] + Wr[1] * u[1], Wr[2] * u[0] + Wr[3] * u[1]};
+            if (!pfixed) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) pv[k] = Jp[k] * wu[0] + Jp[6 + k] * wu[1];
+            }
+            if (!lfixed) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) lv[k] = Jl[k] * wu[0] + Jl[3 + k] * wu[1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sh.pv[t][k] = pv[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sh.lv[t][k] = lv[k];
+        sh.pairf[t] = pfixed ? -2 : rec.pairf;
+        sh.pt[t] = q;
+    }
+    __syncthreads();
+    // point sums in insertion order: r_l, g_l
+    for (int q = t; q < npt; q += blockDim.x) {
+        const int l = pt0 + q;
+        const bool lfixed = p.point_fixed[l] != 0;
+        const int k0 = p.lm_start[l] - s0, k1 = p.lm_start[l + 1] - s0;
+        double rr[3] = {0.0, 0.0, 0.0};
+        if (!lfixed) {
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int k = k0; k < k1; ++k) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] += sh.lv[k][c];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) rr[c] = bl[3L * l + c] - lam * sh.dl[q][c] - acc[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) rl[3L * l + c] = rr[c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            double o = 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) o += static_cast<double>(Hinv[9L * l + 3 * r + c]) * rr[c];
+            gl[3L * l + r] = o;
+            sh.gl[q][r] = o;
+        }
+    }
+    __syncthreads();
+    // W_a g_l at the first observation of each W block
+    if (t < m) {
+        const int pf = sh.pairf[t];
+        if (pf >= 0 && (pf & 1)) {
+            const TS* Wa = Wbuf + 18L * (pf >> 1);
+            const int q = sh.pt[t];
+            const double g0 = sh.gl[q][0], g1 = sh.gl[q][1], g2 = sh.gl[q][2];
+#pragma unroll
+            for (int r = 0; r < 6; ++r)
+                sh.pv[t][r] += static_cast<double>(Wa[3 * r]) * g0 + static_cast<double>(Wa[3 * r + 1]) * g1 +
+                               static_cast<double>(Wa[3 * r + 2]) * g2;
+        }
+    }
+    __syncthreads();
+    // pose partials, written at the entry's pose-major slot
+    for (int q = t; q < ne * 6; q += blockDim.x) {
+        const int ei = q / 6, r = q - ei * 6;
+        double acc = 0.0;
+        for (int o = sh.lo_s[ei]; o < sh.lo_s[ei + 1]; ++o) acc += sh.pv[sh.lo_l[o]][r];
+        part[6L * p.e_slot[e0 + ei] + r] = acc;
+    }
+}
+
+// Pose i by one wave: lane-strided sums of its slots, a fixed butterfly, then
+// rs = r_p - lambda dx_p - sum (zero for a fixed pose).
+__device__ inline void refine_pose_finish(const dev_problem& p, const double* part, const double* bp,
+                                          const double* dxp, double* rs, double lam, int i, int lane) {
+    const bool fixed = p.pose_fixed[i] != 0;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!fixed)
+        for (int k = p.pe_start[i] + lane; k < p.pe_start[i + 1]; k += 64) {
+#pragma unroll
+            for (int r = 0; r < 6; ++r) acc[r] += part[6L * k + r];
+        }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) acc[r] = wave_sum_d(acc[r]);
+    if (lane < 6) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) a = (q == lane) ? acc[q] : a;
+        rs[6L * i + lane] = fixed ? 0.0 : bp[6L * i + lane] - lam * dxp[6L * i + lane] - a;
     }
 }
 
