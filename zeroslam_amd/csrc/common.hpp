@@ -207,8 +207,8 @@ struct dev_work64 {
     double* partial = nullptr;      // reduction partials
     double* cbuf = nullptr;         // [no] per-observation robust chi2, insertion order (exact sums)
     double* sbuf = nullptr;         // [6np + 3nl] per-slot LM scale terms (exact sums)
-    double* Lt = nullptr;           // [n*n + n] transposed factor (exact backward substitution), or the
-                                    // banded factor (n rows of band entries and 1 / L_ii)
+    double* Lt = nullptr;           // [n*n + n] transposed factor (exact backward substitution)
+    double* Lband = nullptr;        // [band_buffer_doubles] band factor (kernels_band.hip), zeroed at allocation
     double* chol_scale = nullptr;   // [4] Cholesky epsilon scales
     int32_t* chol_status = nullptr; // [0] 0 ok, 1 decompose failed, 2 solve failed
     lm_state* lm = nullptr;
@@ -375,12 +375,19 @@ void launch_cholesky_solve64(int32_t n, const dev_work64& w, hipStream_t s, bool
 void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);
 
 // ---- banded factorisation and substitutions (kernels_band.hip) -----------------
-// For the reassociating precisions with factor path 3 when the reduced system's
-// band (w.band) is at most kBandMaxW columns: one workgroup factorises w.S into
-// the compact band w.Lt and solves for w.dxp from w.bs (same status and solve_ok
-// rules).
+// For the reassociating precisions when the reduced system's band (w.band) is at
+// most kBandMaxW columns: one workgroup factorises w.S into the band factor in
+// w.Lt (by columns, by rows, and the reciprocal pivots) and solves for w.dxp
+// from w.bs (same status and solve_ok rules).  Factor path 3 forces it; the
+// automatic choice takes it when the system has at least kBandMinRowsPerWindow
+// windows of rows (a narrow band, not a dense matrix).
 constexpr int kBandPanel = 8;
-constexpr int kBandMaxW = 120;
+constexpr int kBandMaxW = 121;
+constexpr int kBandMinRowsPerWindow = 3;
+// Window (a multiple of 16, 64 .. 128) for a band, 0 if the band is too wide.
+int band_window(int32_t band);
+// Doubles of w.Lt the band factor needs (0 if the band is too wide).
+size_t band_buffer_doubles(int32_t n, int32_t band);
 bool band_path(int32_t n, const dev_work64& w, bool exact);
 void launch_chol_band(int32_t n, const dev_work64& w, hipStream_t s, bool gated, double floor_rel);
 void launch_band_solve(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated);

@@ -693,6 +693,8 @@ int32_t zsba_graph::build_structure() {
     w.cbuf = mem.get<double>(no);
     w.sbuf = mem.get<double>(6L * np + 3L * nl);
     w.Lt = mem.get<double>(static_cast<size_t>(n * n + n));
+    const size_t band_doubles = zb::band_buffer_doubles(static_cast<int32_t>(n), band);
+    w.Lband = band_doubles ? mem.get<double>(band_doubles) : nullptr;
     w.pose_terms_f = mem.get<float>(21L * no); w.pose_grad = mem.get<double>(6L * no);
     w.Hpp_f = mem.get<float>(21L * np); w.bp_m = mem.get<double>(6L * np);
     w.Hll_f = mem.get<float>(9L * nl); w.bl_m = mem.get<double>(3L * nl);
@@ -738,6 +740,8 @@ int32_t zsba_graph::build_structure() {
         mem.release();
         return fail(ZSBA_ENOMEM, "device allocation of %.1f MB failed", mem.bytes / 1e6);
     }
+    // the band factor's entries outside the band are never written: zero once
+    if (w.Lband) (void)hipMemsetAsync(w.Lband, 0, band_doubles * sizeof(double), stream);
     std::vector<int32_t> orig(no);
     for (int32_t s = 0; s < no; ++s) orig[s] = sorted_e[s];
     p.obs_pose = up(d_obs_pose, obs_pose_s); p.obs_point = up(d_obs_point, obs_point_s);

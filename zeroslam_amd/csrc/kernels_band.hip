@@ -3,70 +3,93 @@
 // matrix::decompose_cholesky / matrix::solve_cholesky
 // (source/matrix_decomposition_cholesky.hpp:26-95, called at
 // factor_graph.hpp:782-793) when the reduced system is a narrow band: row r of S
-// is zero left of column first[r], and band = max_r (r - first[r] + 1) is at
-// most kBandMaxW.  Cholesky fill stays inside that row profile (L_rj = 0 for
-// j < first[r]), so the factor is a band of the same width.
+// is zero left of column r - band + 1 (sequential tracks couple each pose to a
+// few neighbours only).  Cholesky fill stays inside that profile, so the factor
+// is a band of the same width.
 //
-// One workgroup of four waves.  The band lives in LDS as a window of rows: row r
-// in slot r mod kRing holds its entries (r, c) for c = r - band + 1 .. r at
-// offsets 0 .. band - 1, its right-hand side at offset band, and a scratch entry
-// at band + 1 that masked lanes write instead of branching.  Per panel of
-// kBandPanel columns c0 ..:
-//   phase 1  wave 0 holds the panel's rows in registers (lane l: rows c0 + l and
-//            c0 + 64 + l) and factors it right-looking: pivot d, y = 1/sqrt(d)
-//            (the hardware estimate and two Newton steps), L_rj = a_rj y, the
-//            remaining panel columns updated with L_cj broadcast by readlane;
-//            the forward substitution runs along (y_j = b_j y, b_r -= L_rj y_j).
-//            A short last panel is padded with identity columns.  Meanwhile
-//            waves 1-3 store the previous panel's rows (the factor as a compact
-//            band Lb: row r's entries at Lb[r (band + 1) + 0 .. band - 1] and
-//            1 / L_rr at Lb[r (band + 1) + band]; the forward solution to x) and
-//            load the rows the next panel reaches, every load of a thread issued
-//            before its first store, so that memory latency hides behind the
-//            factor;
-//   phase 2  all waves update the trailing triangle of the window with the
-//            panel (4 x 4 blocks, operands from the transposed panel LpT).
-// Then the backward substitution L^T x = y runs over 64-row blocks from the
-// bottom, each block's rows of Lb staged in LDS one block ahead by waves 1-3:
-// wave 0 runs the block's recurrence, then all waves apply the block's solved
-// values to the rows above it.
+// Factorisation (k_band_factor, one workgroup of four waves).  The trailing
+// submatrix lives in a window of W rows (W >= band + 7, a multiple of 16): row r
+// sits at window position r mod W, so a row keeps its position from the panel
+// that brings it in until the panel that eliminates it.  The window's symmetric
+// W x W matrix is held in registers as the tiles (I, J), I >= J, of 16 x 16
+// positions, each owned by one of waves 1-3 (tile (I, J) by wave 1 + (I + J) mod
+// 3, which spreads every block row's tiles over the three).  Per panel of 8
+// columns c0 .. c0 + 7 (window positions pp0 .. pp0 + 7):
+//   phase A  wave 0 reads the panel's columns for every window row from LDS (PB:
+//            lane l holds rows c0 + l and c0 + 64 + l) and factors them
+//            right-looking: pivot d (readlane), y = 1/sqrt(d), L_rj = a_rj y,
+//            the remaining panel columns updated with L_cj broadcast by readlane;
+//            the forward substitution runs along on the right-hand side (RH,
+//            kept by position).  The panel's L goes to LP by position, with
+//            zeros at the panel's own positions (its diagonal block, with 1/L_jj
+//            on the diagonal, goes to DB).  Meanwhile waves 1-3 apply the
+//            previous panel's L to every tile not holding this panel's
+//            positions (two v_mfma_f64_16x16x4f64 per tile: acc -= L_I L_J^T)
+//            and copy the previous panel's L to the factor buffers.
+//   phase B  waves 1-3 load the rows entering the window into this panel's
+//            positions (staged in SR one panel ahead by wave 0; the zeros of LP
+//            there keep every later update of this panel off them), then apply
+//            this panel's L to the tiles holding the next panel's positions and
+//            publish those columns to PB (lookahead); wave 0 stages the rows
+//            entering with the next panel and loads those of the one after.
+// Every tile receives every panel's update exactly once, in panel order: in
+// phase B of panel k when it holds panel k + 1's positions, else in phase A of
+// panel k + 1.  The sums are reassociated (four products per matrix-core step),
+// so this path serves the reassociating precisions only.
+//
+// The factor is written twice, as rows of kRing entries: by columns
+// (Lc[c kRing + d] = L_{c+d, c}) for the forward substitution and by rows
+// (Lr[r kRing + d] = L_{r, r-d}) for the backward one, with 1 / L_ii at d = 0.
+// Entries outside the band are zero (the buffer is cleared when it is
+// allocated).
+//
+// Substitutions (band_sub): wave 0 runs a right-looking recurrence over a ring
+// of 128 unknowns (lane l holds ring slots l and l + 64).  Step j: the lane
+// holding unknown j multiplies it by 1 / L_jj (entry d = 0 of its factor row),
+// readlane broadcasts the result, every slot subtracts its factor entry times it,
+// and the freed slot takes the unknown 128 steps away.  The other waves stream
+// the factor rows into a double-buffered LDS ring, kB rows per block, with the
+// loads of up to kK blocks in flight in their registers, so wave 0 reads only
+// LDS.  One barrier per block.
 //
 // Failure rules are the reference's: the decomposition fails when a pivot is not
 // greater than 1e-14 max|S_ii| (NaN fails), the solve when some |L_ii| is not
 // greater than 1e-7 max|L_ii|.  With floor_rel > 0 every diagonal entry is first
 // raised by floor_rel max|S_ii| (in the window; S itself is only read).  Entries
 // of S's lower triangle outside the row profile must be zero (the reduced-system
-// kernels clear S before forming it).
-//
-// Selected by factor path 3 only: on the 100-keyframe workload (n = 600, band 72)
-// the single-wave recurrences cost 85-175 cycles per step, and the factor plus
-// both substitutions take 0.41 ms against 0.32 ms for the tiled one-workgroup
-// factorisation and its substitution.
+// kernels clear S before forming it).  Rows past n (up to a multiple of 8) are
+// identity rows.
 #include "common.hpp"
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
+#include <utility>
 
 namespace zb {
 namespace {
 
-constexpr int kP = kBandPanel;
-constexpr int kWmax = kBandMaxW;
-constexpr int kStMax = kWmax + 2;      // window slot stride: entries, right-hand side, scratch
-constexpr int kRing = 3 * kP + kWmax;  // rows resident: previous panel, this panel's reach, next panel's
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
-constexpr int kXRing = 256;            // solved / pending values of the substitutions (>= 64 + kWmax)
-constexpr int kLpN = 136;              // panel rows kept for the trailing update
-constexpr int kMaxT = (kWmax + 2) / 4; // 4-row blocks of the largest trailing triangle
-constexpr int kMaxTiles = kMaxT * (kMaxT + 1) / 2;
-constexpr int kBlk = 64;               // substitution block rows
-constexpr int kGroup = 8;              // recurrence steps whose entries load together
-constexpr int kBlkBuf = kBlk * (kWmax + 1) + 1;  // one substitution block of Lb rows
-static_assert(kP + kWmax - 1 <= 128, "the panel's rows fit two rows per lane of one wave");
-static_assert(kXRing >= kBlk + kWmax, "the substitution ring holds a block and the band above it");
-static_assert(kP + 4 * kMaxT <= kLpN, "trailing blocks read panel rows inside LpT");
-static_assert(2 * kBlkBuf <= kRing * kStMax, "two substitution blocks fit in the window's memory");
+typedef double band_d4 __attribute__((ext_vector_type(4)));
+
+constexpr int kP = kBandPanel;  // panel columns
+constexpr int kThreads = 256;   // wave 0: panels and recurrences; waves 1-3: tiles and loads
+constexpr int kTileLanes = kThreads - 64;
+constexpr int kLoaders = 3;     // loading waves of the substitutions
+constexpr int kRing = 128;      // unknowns held by wave 0; row stride of the factor buffers
+constexpr int kB = 16;          // substitution steps per block
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// f(std::integral_constant<int, i>) for i = 0 .. N - 1: every index a constant,
+// so arrays indexed by it stay in registers.
+template <class F, int... Is>
+__device__ __forceinline__ void static_for_seq(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 __device__ __forceinline__ double lane_value(double v, int j) {
     const long long bits = __double_as_longlong(v);
@@ -98,430 +121,645 @@ __device__ __forceinline__ int opaque_lane() {
     return lane;
 }
 
-// Copies cnt doubles (element q: ld(q), then stv(q, value)) with kBatch loads of
-// a thread issued before its first store; threads tid0 .. tid0 + nthr - 1 take
-// part.
-template <int kBatch, class Ld, class St>
-__device__ __forceinline__ void copy_batched(int cnt, int tid0, int nthr, Ld ld, St stv) {
-    for (int q0 = static_cast<int>(threadIdx.x) - tid0; q0 < cnt; q0 += kBatch * nthr) {
-        double v[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int q = q0 + u * nthr;
-            v[u] = (q < cnt) ? ld(q) : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int q = q0 + u * nthr;
-            if (q < cnt) stv(q, v[u]);
-        }
-    }
+// Window size for a band: the rows a panel of kP columns reaches, in blocks of 16.
+constexpr int window_for(int band) {
+    return band + kP - 1 <= 64 ? 64 : band + kP - 1 <= 80 ? 80 : band + kP - 1 <= 96 ? 96
+         : band + kP - 1 <= 112 ? 112 : band + kP - 1 <= 128 ? 128 : 0;
 }
 
-// Rows [lo, hi) of the compact band Lb into a substitution block buffer (same
-// layout, stride wb + 1) and their right-hand sides v[lo ..] into vb.
-__device__ __forceinline__ void load_block(double* buf, double* vb, int lo, int hi, int wb, const double* Lb,
-                                           const double* v, int tid0, int nthr) {
-    const double* src = Lb + static_cast<int64_t>(lo) * (wb + 1);
-    copy_batched<8>((hi - lo) * (wb + 1), tid0, nthr, [&](int q) { return src[q]; },
-                    [&](int q, double val) { buf[q] = val; });
-    for (int i = static_cast<int>(threadIdx.x) - tid0; i < hi - lo; i += nthr) vb[i] = v[lo + i];
+// Tiles (I, J), I >= J, of nb position blocks, dealt to the three tile waves by
+// (I + J) mod 3.
+constexpr int tiles_of_wave(int nb, int wv) {
+    int c = 0;
+    for (int a = 0; a < nb; ++a)
+        for (int b = 0; b <= a; ++b)
+            if ((a + b) % 3 == wv) ++c;
+    return c;
+}
+constexpr int max_tiles(int nb) {
+    const int c0 = tiles_of_wave(nb, 0), c1 = tiles_of_wave(nb, 1), c2 = tiles_of_wave(nb, 2);
+    return c0 > c1 ? (c0 > c2 ? c0 : c2) : (c1 > c2 ? c1 : c2);
 }
 
-// Backward substitution L^T x = y over blocks of kBlk rows from the bottom.  On
-// entry blk[(nb - 1) & 1] and vbk[(nb - 1) & 1] hold the last block (load_block
-// with y); the other blocks come from Lb and x (= y).  x receives the solution.
-// blk0, blk1: kBlkBuf each; xr, ya: rings of kXRing.
-__device__ __forceinline__ void band_backward(int n, int wb, const double* Lb, double* x, double* blk0, double* blk1,
-                                              double (*vbk)[kBlk], double* xr, double* ya, double (*part)[64]) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int st = wb + 1;
-    const int nb = (n + kBlk - 1) / kBlk;
-    double* flat = &part[0][0];
-    for (int b = nb - 1; b >= 0; --b) {
-        const int i0 = b * kBlk;
-        const int mb = (n - i0) < kBlk ? (n - i0) : kBlk;
-        double* cur = (b & 1) ? blk1 : blk0;
-        double* nxt = (b & 1) ? blk0 : blk1;
-        if (wave == 0) {
-            const int i = i0 + lane;
-            const bool real = lane < mb;
-            // pending updates from the blocks below reach rows i >= i0 + kBlk + 1 - wb
-            const bool touched = (b < nb - 1) && (i >= i0 + kBlk + 1 - wb);
-            double yv = real ? vbk[b & 1][lane] - (touched ? ya[i % kXRing] : 0.0) : 0.0;
-            const double dl = real ? cur[lane * st + wb] : 0.0;
-            // step k: x_k from lane k, then y_i -= L_{i0+k, i} x_k for i < k (row i0 + k
-            // of the block); the next kGroup steps' entries load during these
-            auto entry = [&](int k) {
-                const int ln = opaque_lane();
-                const int off = ln - k + wb - 1;
-                const bool in = k < mb && ln < k && off >= 0;
-                const double t = cur[k * st + (in ? off : 0)];
-                return in ? t : 0.0;
-            };
-            double tc[kGroup], tn[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) tc[u] = entry(kBlk - 1 - u);
-#pragma unroll
-            for (int g = 0; g < kBlk / kGroup; ++g) {
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u)
-                    tn[u] = (g + 1 < kBlk / kGroup) ? entry(kBlk - 1 - (g + 1) * kGroup - u) : 0.0;
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int k = kBlk - 1 - g * kGroup - u;
-                    const int ln = opaque_lane();
-                    const double xk = lane_value(yv * dl, k);
-                    yv = (ln == k) ? xk : fma(-tc[u], xk, yv);
-                }
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) tc[u] = tn[u];
-            }
-            if (real) {
-                x[i] = yv;
-                xr[i % kXRing] = yv;
-            }
-        } else if (b > 0) {
-            load_block(nxt, vbk[(b - 1) & 1], i0 - kBlk, i0, wb, Lb, x, 64, kThreads - 64);
-        }
-        __syncthreads();
-        if (b == 0) break;
-        // rows i in [max(0, i0 - wb + 1), i0): pending += sum over block rows r of L_ri x_r
-        {
-            const int t = tid & 127, h = tid >> 7;
-            const int i = i0 - 1 - t;
-            double acc = 0.0;
-            if (i >= 0 && i >= i0 - wb + 1) {
-                const int rlo = i0 + 32 * h, rhi = (i0 + 32 * h + 32) < i0 + mb ? (i0 + 32 * h + 32) : i0 + mb;
-                for (int r = rlo; r < rhi; ++r) {
-                    const int off = i - r + wb - 1;
-                    if (off >= 0) acc = fma(cur[(r - i0) * st + off], xr[r % kXRing], acc);
-                }
-            }
-            flat[tid] = acc;
-        }
-        __syncthreads();
-        if (tid < 128) {
-            const int i = i0 - 1 - tid;
-            if (i >= 0 && i >= i0 - wb + 1) {
-                const double s = flat[tid] + flat[128 + tid];
-                const bool fresh = (b == nb - 1) || (i < i0 + kBlk + 1 - wb);
-                ya[i % kXRing] = fresh ? s : ya[i % kXRing] + s;
-            }
-        }
-        __syncthreads();
-    }
+// Factor entries one loading lane holds per block, and blocks in flight (the
+// loads outstanding stay within the 63 the wave's counter tracks).
+template <int W>
+struct sub_shape {
+    static constexpr int kItems = kB * W;
+    static constexpr int kNI = (kItems + 64 * kLoaders - 1) / (64 * kLoaders);
+    static constexpr int kK = (63 / (kNI + 1) + 1) < 8 ? (63 / (kNI + 1) + 1) : 8;
+};
+
+// LDS of the factorisation and of the substitutions (they run one after the
+// other and share it), in doubles.
+template <int W>
+struct band_lds {
+    static constexpr int kPB = 0;                          // [W][kP + 1]
+    static constexpr int kLP = kPB + W * (kP + 1);         // [2][kP][W]
+    static constexpr int kDB = kLP + 2 * kP * W;           // [2][kP][kP]
+    static constexpr int kSR = kDB + 2 * kP * kP;          // [2][kP][W]
+    static constexpr int kSRb = kSR + 2 * kP * W;          // [2][kP]
+    static constexpr int kRH = kSRb + 2 * kP;              // [W]
+    static constexpr int kFactor = kRH + W;
+    static constexpr int kRingOff = 0;                     // [2][kB][kRing]
+    static constexpr int kSlot = 2 * kB * kRing;           // [2][kRing]
+    static constexpr int kSub = kSlot + 2 * kRing;
+    static constexpr int kTotal = kFactor > kSub ? kFactor : kSub;
+};
+
+// Entry (r, c), r >= c, of the system being factorised: S's lower triangle
+// inside the band, the floor on the diagonal, identity rows past n.  The load
+// address is always valid (element 0 when the entry is not in S).
+__device__ __forceinline__ int64_t entry_index(int n, int wb, int r, int c) {
+    return (r < n && r - c < wb) ? static_cast<int64_t>(r) * n + c : 0;
+}
+__device__ __forceinline__ double entry_finish(double raw, int n, int wb, double lift, int r, int c) {
+    if (r >= n) return r == c ? 1.0 : 0.0;
+    if (r - c >= wb) return 0.0;
+    return r == c ? raw + lift : raw;
 }
 
-__global__ void __launch_bounds__(kThreads) k_chol_band(int n, int wb, const double* A, const double* b, double* x,
-                                                        double* Lb, double floor_rel, double* eps_out,
-                                                        int32_t* status, lm_state* lm, int gated) {
-    if (gated && lm->done) return;
-    __shared__ double ring[kRing * kStMax];
-    __shared__ __attribute__((aligned(16))) double LpT[kP][kLpN];
-    __shared__ double xr[kXRing], ya[kXRing];
-    __shared__ double part[kWaves][64];
-    __shared__ double vbk[2][kBlk];
-    __shared__ int tiles[kMaxTiles];
-    __shared__ double dstage[2][kP];
-    __shared__ int flags[2];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int st = wb + 2, sb = wb + 1;
-    auto slot = [&](int r) -> double* { return ring + (r % kRing) * st; };
-    double* flat = &part[0][0];
+// Rows of the substitutions: n rounded up to whole blocks (factor rows past n
+// are zero, so their unknowns are zero and touch nothing).
+__device__ __forceinline__ int sub_rows(int n) { return (n + kB - 1) / kB * kB; }
 
-    // max |S_ii| (NaN never wins), the floor and the pivot threshold; the
-    // trailing blocks' (I, J) table; the panel copy cleared
-    double m = 0.0;
-    for (int i = tid; i < n; i += kThreads) m = dmax_nan(m, fabs(A[static_cast<int64_t>(i) * n + i]));
-    flat[tid] = m;
-    for (int q = tid; q < kMaxTiles; q += kThreads) {
-        int I = 0;
-        while ((I + 1) * (I + 2) / 2 <= q) ++I;
-        tiles[q] = (I << 8) | (q - I * (I + 1) / 2);
-    }
-    for (int q = tid; q < kP * kLpN; q += kThreads) (&LpT[0][0])[q] = 0.0;
-    if (tid == 0) flags[0] = 0;
+// Substitution over factor rows F (stride kRing; entry 0 is 1 / L_jj): forward
+// (kFwd: rows 0, 1, .. of the column-ordered factor) or backward (rows from the
+// last, of the row-ordered factor), in blocks of kB steps.  A block's unknowns
+// sit in one half of the ring (lane l: slot l in u0, slot l + 64 in u1), so each
+// step is one product, one broadcast and two fused multiply-adds; the freed slots
+// take their new unknowns at the end of the block.  rhs and out may be the same
+// array.  Every wave of the workgroup calls it; wave 0 computes, waves
+// 1 .. kLoaders load.
+template <int W, bool kFwd>
+__device__ __forceinline__ void band_sub(int n, const double* F, const double* rhs, double* out, double* lds,
+                                         int wave, int lane) {
+    using sh = sub_shape<W>;
+    constexpr int kNI = sh::kNI, kK = sh::kK;
+    double* ring = lds + band_lds<W>::kRingOff;
+    double* rslot = lds + band_lds<W>::kSlot;
+    const int np = sub_rows(n);
+    const int nblk = np / kB;
+    // row of step t of block m
+    auto row_of = [&](int m, int t) __attribute__((always_inline)) { return kFwd ? m * kB + t : np - 1 - m * kB - t; };
+    // entries past W stay zero
+    for (int i = static_cast<int>(threadIdx.x); i < 2 * kB * kRing; i += kThreads) ring[i] = 0.0;
     __syncthreads();
-    for (int w = kThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) flat[tid] = dmax_nan(flat[tid], flat[tid + w]);
-        __syncthreads();
+
+    const bool loader = wave >= 1 && wave <= kLoaders;
+    const int li = (wave - 1) * 64 + lane;
+    // loading lane's items: entry (t, d) of a block, idx = li + u 64 kLoaders
+    int it_t[kNI], it_d[kNI];
+    static_for<kNI>([&](auto U) __attribute__((always_inline)) {
+        constexpr int u = U;
+        const int idx = li + u * 64 * kLoaders;
+        it_t[u] = idx < sh::kItems ? idx / W : -1;
+        it_d[u] = idx < sh::kItems ? idx - (idx / W) * W : 0;
+    });
+    double ld[kK][kNI];
+    double lr[kK];
+    auto issue = [&](int m, double* dst, double& rdst) __attribute__((always_inline)) {
+        const int mm = m < nblk ? m : nblk - 1;
+        static_for<kNI>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = U;
+            const int j = row_of(mm, it_t[u] < 0 ? 0 : it_t[u]);
+            dst[u] = F[j * kRing + it_d[u]];
+        });
+        const int j = row_of(mm, lane & (kB - 1));
+        const int jr = kFwd ? j + kRing : j - kRing;
+        const bool ok = jr >= 0 && jr < n;
+        const double v = rhs[ok ? jr : 0];
+        rdst = ok ? v : 0.0;
+    };
+    auto store = [&](int m, const double* src, double rsrc) __attribute__((always_inline)) {
+        double* rg = ring + (m & 1) * kB * kRing;
+        static_for<kNI>([&](auto U) __attribute__((always_inline)) {
+            constexpr int u = U;
+            if (it_t[u] >= 0) rg[it_t[u] * kRing + it_d[u]] = src[u];
+        });
+        if (wave == 1 && lane < kB) rslot[(m & 1) * kRing + (row_of(m, lane) & (kRing - 1))] = rsrc;
+    };
+
+    // wave 0: the ring of unknowns; slot s holds unknown j + ((s - j) & 127)
+    // (forward) or j - ((j - s) & 127) (backward) at step j
+    double u0 = 0.0, u1 = 0.0;
+    if (wave == 0) {
+        if (kFwd) {
+            u0 = lane < n ? rhs[lane] : 0.0;
+            u1 = lane + 64 < n ? rhs[lane + 64] : 0.0;
+        } else {
+            const int j0 = np - 1;
+            const int c0 = j0 - ((j0 - lane) & (kRing - 1)), c1 = j0 - ((j0 - lane - 64) & (kRing - 1));
+            u0 = (c0 >= 0 && c0 < n) ? rhs[c0] : 0.0;
+            u1 = (c1 >= 0 && c1 < n) ? rhs[c1] : 0.0;
+        }
     }
-    const double mx = flat[0];
+    if (loader) {
+        static_for<kK>([&](auto S) __attribute__((always_inline)) { issue(S, ld[S], lr[S]); });
+        store(0, ld[0], lr[0]);
+    }
+    __syncthreads();
+    // one block: the kB steps on the unknowns in half kHi of the ring
+    auto block = [&](int m, const double* l0, const double* l1, auto HI) __attribute__((always_inline)) {
+        constexpr bool kHi = HI;
+        double xs = 0.0;
+        const int jb = row_of(m, 0);
+        static_for<kB>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = T;
+            const int sl = (kFwd ? jb + t : jb - t) & 63;
+            const double xj = lane_value(kHi ? u1 * l1[t] : u0 * l0[t], sl);
+            u0 = fma(-l0[t], xj, u0);
+            u1 = fma(-l1[t], xj, u1);
+            xs = (opaque_lane() == t) ? xj : xs;
+        });
+        // the block's slots (kB consecutive, from lo) take the unknowns 128 away
+        const int lo = (kFwd ? jb : jb - (kB - 1)) & 63;
+        const bool freed = lane >= lo && lane < lo + kB;
+        const double rs = rslot[(m & 1) * kRing + (kHi ? 64 : 0) + lane];
+        if (kHi) u1 = freed ? rs : u1;
+        else u0 = freed ? rs : u0;
+        if (lane < kB) {
+            const int j = kFwd ? jb + lane : jb - lane;
+            if (j < n) out[j] = xs;
+        }
+    };
+    for (int m0 = 0; m0 < nblk; m0 += kK) {
+        static_for<kK>([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = S;
+            const int m = m0 + s;
+            if (m >= nblk) return;
+            if (wave == 0) {
+                const double* rg = ring + (m & 1) * kB * kRing;
+                double l0[kB], l1[kB];
+                const int jb = row_of(m, 0);
+                static_for<kB>([&](auto T) __attribute__((always_inline)) {
+                    constexpr int t = T;
+                    const int j = kFwd ? jb + t : jb - t;
+                    const int d0 = kFwd ? ((lane - j) & (kRing - 1)) : ((j - lane) & (kRing - 1));
+                    l0[t] = rg[t * kRing + d0];
+                    l1[t] = rg[t * kRing + (d0 ^ 64)];
+                });
+                if ((jb & 64) != 0) block(m, l0, l1, std::integral_constant<bool, true>{});
+                else block(m, l0, l1, std::integral_constant<bool, false>{});
+            } else if (loader) {
+                constexpr int sn = (s + 1) % kK;
+                store(m + 1, ld[sn], lr[sn]);
+                issue(m + kK, ld[s], lr[s]);
+            }
+            __syncthreads();
+        });
+    }
+}
+
+template <int W>
+__global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const double* A, const double* b, double* x,
+                                                          double* Lband, double floor_rel, double* eps_out,
+                                                          int32_t* status, lm_state* lm, int gated) {
+    if (gated && lm->done) return;
+    constexpr int NB = W / 16;
+    constexpr int kMaxT = max_tiles(NB);
+    constexpr int kPre = kP * W / 64;                               // entering-row items per lane of wave 0
+    constexpr int kCP = (kP * W + kTileLanes - 1) / kTileLanes;     // factor-copy items per tile lane
+    using lay = band_lds<W>;
+    static_assert(W % 16 == 0 && W >= 64 && W <= kRing, "window of 64 .. 128 positions in blocks of 16");
+    static_assert(kP * W % 64 == 0, "entering rows split evenly over wave 0");
+    __shared__ double smem[lay::kTotal];
+    __shared__ double red[kThreads];
+    __shared__ int tab[3][kMaxT];
+    __shared__ int sflag;
+    double* PB = smem + lay::kPB;
+    double* LP = smem + lay::kLP;
+    double* DB = smem + lay::kDB;
+    double* SR = smem + lay::kSR;
+    double* SRb = smem + lay::kSRb;
+    double* RH = smem + lay::kRH;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = uniform(tid >> 6);
+    const int np8 = (n + kP - 1) / kP * kP, NP = np8 / kP;
+    double* Lc = Lband;
+    double* Lr = Lband + static_cast<int64_t>(sub_rows(n)) * kRing;
+
+    if (tid == 0) {
+        int c[3] = {0, 0, 0};
+        for (int wv = 0; wv < 3; ++wv)
+            for (int s = 0; s < kMaxT; ++s) tab[wv][s] = -1;
+        for (int a = 0; a < NB; ++a)
+            for (int bb = 0; bb <= a; ++bb) {
+                const int wv = (a + bb) % 3;
+                tab[wv][c[wv]++] = (a << 8) | bb;
+            }
+    }
+    // max |S_ii| (NaN never wins), the floor and the pivot threshold
+    {
+        double m = 0.0;
+        for (int i = tid; i < n; i += kThreads) m = dmax_nan(m, fabs(A[static_cast<int64_t>(i) * n + i]));
+        red[tid] = m;
+        __syncthreads();
+        for (int s = kThreads / 2; s > 0; s >>= 1) {
+            if (tid < s) red[tid] = dmax_nan(red[tid], red[tid + s]);
+            __syncthreads();
+        }
+    }
+    const double mx = red[0];
     const double lift = floor_rel > 0.0 ? floor_rel * mx : 0.0;
     const double eps = (mx + lift) * 1e-14;
     if (tid == 0) {
         eps_out[0] = eps;
         status[0] = 0;
     }
-    // rows [lo, hi) from S into the window (threads tid0 ..)
-    auto load_rows = [&](int lo, int hi, int tid0, int nthr) {
-        copy_batched<4>(
-            (hi - lo) * st, tid0, nthr,
-            [&](int q) {
-                const int i = q / st, o = q - i * st, r = lo + i;
-                const int c = r - wb + 1 + o;
-                double v = 0.0;
-                if (o < wb) {
-                    v = (c >= 0) ? A[static_cast<int64_t>(r) * n + c] : 0.0;
-                    if (c == r) v += lift;
-                } else if (o == wb) {
-                    v = b[r];
-                }
-                return v;
-            },
-            [&](int q, double v) {
-                const int i = q / st;
-                slot(lo + i)[q - i * st] = v;
-            });
+
+    // tile waves: slots, their tiles and operand offsets; the copy items
+    const int r16 = lane & 15, kq = lane >> 4;
+    double acc[kMaxT][4];  // tile slot s, element v (row l / 16 + 4 v, column l % 16)
+    int tI[kMaxT], tJ[kMaxT], oI[kMaxT], oJ[kMaxT];
+    int c_kk[kCP], c_q[kCP];
+    // wave 0: the entering-row items
+    int e_i[kPre], e_q[kPre];
+    double pre[kPre], pre_b = 0.0;
+    static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
+        constexpr int s = S;
+        const int t = wave > 0 ? uniform(tab[wave - 1][s]) : -1;
+        tI[s] = t < 0 ? -1 : (t >> 8);
+        tJ[s] = t < 0 ? -1 : (t & 255);
+        oI[s] = kq * W + 16 * (t < 0 ? 0 : tI[s]) + r16;
+        oJ[s] = kq * W + 16 * (t < 0 ? 0 : tJ[s]) + r16;
+        acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.0;
+    });
+    static_for<kCP>([&](auto T) __attribute__((always_inline)) {
+        constexpr int t = T;
+        const int idx = (tid - 64) + t * kTileLanes;
+        c_kk[t] = (idx >= 0 && idx < kP * W) ? idx / W : -1;
+        c_q[t] = (idx >= 0 && idx < kP * W) ? idx - (idx / W) * W : 0;
+    });
+    static_for<kPre>([&](auto T) __attribute__((always_inline)) {
+        constexpr int t = T;
+        const int idx = lane + 64 * t;
+        e_i[t] = idx / W;
+        e_q[t] = idx - (idx / W) * W;
+    });
+
+    // the rows entering the window with panel m: row kP m + W + i at position
+    // pp0(m) + i; item (i, q): its entry at the column position q holds in panel
+    // m + 1's window (npos: that panel's first position)
+    auto entering = [&](int m, int t, int npos, int& r, int& c) __attribute__((always_inline)) {
+        const int rn = kP * m + W + e_i[t];
+        int d = e_q[t] - npos;
+        d += d < 0 ? W : 0;
+        const int cq = kP * (m + 1) + d;
+        r = cq <= rn ? rn : cq;
+        c = cq <= rn ? cq : rn;
     };
-    // rows [lo, hi) of the factor to Lb with their 1 / L_ii (from dstage[s]), their
-    // forward solution to x
-    auto store_rows = [&](int lo, int hi, int s, int tid0, int nthr) {
-        double* dst = Lb + static_cast<int64_t>(lo) * sb;
-        const int cnt = (hi - lo) * sb;
-        for (int q = tid - tid0; q < cnt; q += nthr) {
-            const int i = q / sb, o = q - i * sb;
-            dst[q] = (o < wb) ? slot(lo + i)[o] : dstage[s][i];
+    auto prefetch = [&](int m, int npos) __attribute__((always_inline)) {
+        static_for<kPre>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = T;
+            int r, c;
+            entering(m, t, npos, r, c);
+            pre[t] = A[entry_index(n, wb, r, c)];
+        });
+        const int rn = kP * m + W + (lane & (kP - 1));
+        pre_b = b[rn < n ? rn : 0];
+        pre_b = rn < n ? pre_b : 0.0;
+    };
+    auto stage = [&](int m, int npos) __attribute__((always_inline)) {
+        double* S = SR + (m & 1) * kP * W;
+        static_for<kPre>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = T;
+            int r, c;
+            entering(m, t, npos, r, c);
+            S[e_i[t] * W + e_q[t]] = entry_finish(pre[t], n, wb, lift, r, c);
+        });
+        if (lane < kP) SRb[(m & 1) * kP + lane] = pre_b;
+    };
+    // tile update: acc -= L_I L_J^T over the panel's kP columns (lane l: operand
+    // row l % 16 at depth l / 16; result element v at row l / 16 + 4 v, column
+    // l % 16).  Operands of every slot load first, then the products run.
+    auto update = [&](const double* L, auto want) __attribute__((always_inline)) {
+        double a0[kMaxT], b0[kMaxT], a1[kMaxT], b1[kMaxT];
+        static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = S;
+            a0[s] = L[oI[s]];
+            b0[s] = L[oJ[s]];
+            a1[s] = L[oI[s] + 4 * W];
+            b1[s] = L[oJ[s] + 4 * W];
+        });
+        static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = S;
+            if (tI[s] >= 0 && want(tI[s], tJ[s])) {
+                band_d4 c = {acc[s][0], acc[s][1], acc[s][2], acc[s][3]};
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(-a0[s], b0[s], c, 0, 0, 0);
+                c = __builtin_amdgcn_mfma_f64_16x16x4f64(-a1[s], b1[s], c, 0, 0, 0);
+                acc[s][0] = c[0];
+                acc[s][1] = c[1];
+                acc[s][2] = c[2];
+                acc[s][3] = c[3];
+            }
+        });
+    };
+    // the columns of the panel at positions x0 .. x0 + 7 (block X, half h) to
+    // PB[q][kk] = entry (position q, position x0 + kk); a diagonal tile writes
+    // its rows only (its columns hold the same values)
+    auto publish = [&](int X, int h, int x0) __attribute__((always_inline)) {
+        static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = S;
+            const int I = tI[s], J = tJ[s];
+            if (I < 0) return;
+            const int q = 16 * J + r16;
+            if (I == X) {
+                static_for<4>([&](auto V) __attribute__((always_inline)) {
+                    constexpr int v = V;
+                    if ((v >> 1) == h) PB[q * (kP + 1) + 16 * I + kq + 4 * v - x0] = acc[s][v];
+                });
+            } else if (J == X && (r16 >> 3) == h) {
+                static_for<4>([&](auto V) __attribute__((always_inline)) {
+                    constexpr int v = V;
+                    PB[(16 * I + kq + 4 * v) * (kP + 1) + q - x0] = acc[s][v];
+                });
+            }
+        });
+    };
+    // the positions x0 .. x0 + 7 (block X, half h) take the entering rows:
+    // S[i][q] = entry (entering row i, position q)
+    auto reload = [&](const double* S, int X, int h, int x0) __attribute__((always_inline)) {
+        static_for<kMaxT>([&](auto SS) __attribute__((always_inline)) {
+            constexpr int s = SS;
+            const int I = tI[s], J = tJ[s];
+            if (I < 0) return;
+            const int q = 16 * J + r16;
+            // selects, not conditional stores, and addresses inside S (rows
+            // taken mod 8), so the slots stay in registers
+            if (I == X) {
+                static_for<4>([&](auto V) __attribute__((always_inline)) {
+                    constexpr int v = V;
+                    const double t = S[((kq + 4 * v - 8 * h) & (kP - 1)) * W + q];
+                    acc[s][v] = ((v >> 1) == h) ? t : acc[s][v];
+                });
+            }
+            if (J == X) {
+                const bool mine = (r16 >> 3) == h;
+                static_for<4>([&](auto V) __attribute__((always_inline)) {
+                    constexpr int v = V;
+                    const double t = S[((r16 - 8 * h) & (kP - 1)) * W + 16 * I + kq + 4 * v];
+                    acc[s][v] = (mine && (I != X || (v >> 1) != h)) ? t : acc[s][v];
+                });
+            }
+        });
+    };
+    // panel k's L to the factor buffers, by columns and by rows: the panel's own
+    // rows from its diagonal block DB (DB[kk][t] = L_{c0+t, c0+kk}, 1 / L_jj on
+    // the diagonal), the others from LP by position; rows past n get zeros
+    auto copy_panel = [&](int k, int pp) __attribute__((always_inline)) {
+        const double* L = LP + (k & 1) * kP * W;
+        const double* D = DB + (k & 1) * kP * kP;
+        const int c0 = kP * k;
+        static_for<kCP>([&](auto T) __attribute__((always_inline)) {
+            constexpr int t = T;
+            const int kk = c_kk[t], q = c_q[t];
+            int d = q - pp;
+            d += d < 0 ? W : 0;
+            if (kk >= 0 && d >= kk) {
+                const int r = c0 + d, c = c0 + kk;
+                double v = d < kP ? D[kk * kP + d] : L[kk * W + q];
+                v = r < n ? v : 0.0;
+                Lc[static_cast<int64_t>(c) * kRing + (d - kk)] = v;
+                Lr[static_cast<int64_t>(r) * kRing + (d - kk)] = v;
+            }
+        });
+    };
+
+    // the window of panel 0 (rows 0 .. W - 1 at positions 0 .. W - 1), its first
+    // panel's columns, the right-hand side; the rows entering with panel 0, and
+    // the loads of those entering with panel 1
+    if (wave == 0) {
+        for (int q = lane; q < W; q += 64) RH[q] = q < n ? b[q] : 0.0;
+        if (NP >= 2) {
+            prefetch(0, kP % W);
+            stage(0, kP % W);
         }
-        for (int i = tid - tid0; i < hi - lo; i += nthr) x[lo + i] = slot(lo + i)[wb];
-    };
-    int loaded = (kP + wb - 1) < n ? (kP + wb - 1) : n;
-    load_rows(0, loaded, 0, kThreads);
+        if (NP >= 3) prefetch(1, (2 * kP) % W);
+    } else {
+        static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
+            constexpr int s = S;
+            double raw[4];
+            int rr[4], cc[4];
+            static_for<4>([&](auto V) __attribute__((always_inline)) {
+                constexpr int v = V;
+                const int p = 16 * (tI[s] < 0 ? 0 : tI[s]) + kq + 4 * v, q = 16 * (tJ[s] < 0 ? 0 : tJ[s]) + r16;
+                rr[v] = p > q ? p : q;
+                cc[v] = p > q ? q : p;
+                raw[v] = A[entry_index(n, wb, rr[v], cc[v])];
+            });
+            static_for<4>([&](auto V) __attribute__((always_inline)) {
+                constexpr int v = V;
+                acc[s][v] = entry_finish(raw[v], n, wb, lift, rr[v], cc[v]);
+            });
+        });
+        publish(0, 0, 0);
+    }
     __syncthreads();
 
     bool ok = true;  // wave 0
     double lmax = 0.0, lmin = INFINITY;
-    int k = 0;
-    for (int c0 = 0; c0 < n; c0 += kP, ++k) {
-        const int nk = (n - c0) < kP ? (n - c0) : kP;
-        const int re = (c0 + nk + wb - 1) < n ? (c0 + nk + wb - 1) : n;  // rows reaching the panel
-        const int want = (c0 + 2 * kP + wb - 1) < n ? (c0 + 2 * kP + wb - 1) : n;
+    int pp0 = 0;     // window position of column c0
+    for (int k = 0; k < NP; ++k) {
+        const int c0 = kP * k;
+        const int kb = k & 1;
+        const int ppp = pp0 == 0 ? W - kP : pp0 - kP;  // the previous panel's positions
+        int ppn = pp0 + kP;                            // the next panel's
+        if (ppn == W) ppn = 0;
+        // ---- phase A
         if (wave == 0) {
-            const int r0 = c0 + lane, r1 = r0 + 64;
-            const bool v0 = r0 < re, v1 = r1 < re;
-            double* R0 = slot(v0 ? r0 : c0);
-            double* R1 = slot(v1 ? r1 : c0);
+            int q0 = pp0 + lane;
+            if (q0 >= W) q0 -= W;
+            const bool v1 = 64 + lane < W;
+            int q1 = pp0 + 64 + lane;
+            if (q1 >= W) q1 -= W;
+            if (!v1) q1 = q0;
             double a0[kP], a1[kP];
-            int o0[kP], o1[kP];
-#pragma unroll
-            for (int j = 0; j < kP; ++j) {
-                const int off0 = j - lane + wb - 1, off1 = off0 - 64;
-                const bool in0 = v0 && off0 >= 0 && j <= lane, in1 = v1 && off1 >= 0;
-                o0[j] = in0 ? off0 : wb + 1;
-                o1[j] = in1 ? off1 : wb + 1;
-                const double t0 = R0[o0[j]], t1 = R1[o1[j]];
-                // a short last panel: identity rows and columns past n
-                a0[j] = in0 ? t0 : ((r0 >= n && lane == j) ? 1.0 : 0.0);
-                a1[j] = in1 ? t1 : 0.0;
-            }
-            double b0 = v0 ? R0[wb] : 0.0, b1 = v1 ? R1[wb] : 0.0;
-            double yv = 0.0;
-#pragma unroll
-            for (int j = 0; j < kP; ++j) {
+            static_for<kP>([&](auto KK) __attribute__((always_inline)) {
+                constexpr int kk = KK;
+                a0[kk] = PB[q0 * (kP + 1) + kk];
+                const double t = PB[q1 * (kP + 1) + kk];
+                a1[kk] = v1 ? t : 0.0;
+            });
+            const double rb0 = RH[q0], rb1 = RH[q1];
+            double b0 = rb0, b1 = v1 ? rb1 : 0.0;
+            // lane j < kP ends with the panel's column j: 1 / sqrt(d_j), d_j and
+            // the forward value; entries above the diagonal (lanes < j of
+            // column j) are left unreduced: nothing reads them
+            double yv = 0.0, dv = 0.0, bv = 0.0;
+            static_for<kP>([&](auto JJ) __attribute__((always_inline)) {
+                constexpr int j = JJ;
                 const double d = lane_value(a0[j], j);
-                ok &= (j >= nk) | (d > eps);
                 const double y = rsqrt_nr(d);
-                const double l = d * y;
-                if (j < nk) {
-                    lmax = dmax_nan(lmax, fabs(l));
-                    lmin = fmin(lmin, fabs(l));
-                }
-                a0[j] = (lane == j) ? l : ((lane > j) ? a0[j] * y : 0.0);
+                a0[j] = a0[j] * y;  // lane j: L_jj = d y
                 a1[j] = a1[j] * y;
                 const double yj = lane_value(b0, j) * y;
-                b0 = (lane == j) ? yj : ((lane > j) ? fma(-a0[j], yj, b0) : b0);
+                b0 = fma(-a0[j], yj, b0);
                 b1 = fma(-a1[j], yj, b1);
-                yv = (lane == j) ? y : yv;
-#pragma unroll
-                for (int jj = j + 1; jj < kP; ++jj) {
-                    const double t = lane_value(a0[j], jj);
-                    a0[jj] = (lane >= jj) ? fma(-a0[j], t, a0[jj]) : a0[jj];
-                    a1[jj] = fma(-a1[j], t, a1[jj]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < kP; ++j) {
-                R0[o0[j]] = a0[j];
-                R1[o1[j]] = a1[j];
-                LpT[j][lane] = a0[j];
-                LpT[j][64 + lane] = a1[j];
-            }
-            R0[v0 ? wb : wb + 1] = b0;
-            R1[v1 ? wb : wb + 1] = b1;
-            if (lane < kP) dstage[k & 1][lane] = yv;
-            if (lane == 0 && !ok) flags[0] = 1;
-        } else {
-            // waves 1-3: the previous panel's rows out, the next panel's rows in
-            if (c0 > 0) store_rows(c0 - kP, c0, (k - 1) & 1, 64, kThreads - 64);
-            if (want > loaded) load_rows(loaded, want, 64, kThreads - 64);
-        }
-        loaded = want > loaded ? want : loaded;
-        __syncthreads();
-        if (flags[0]) break;
-        // trailing triangle: rows and columns t0 .. re - 1
-        const int t0 = c0 + nk, mt = re - t0;
-        if (mt > 0) {
-            const int T = (mt + 3) / 4, ntile = T * (T + 1) / 2;
-            for (int q = tid; q < ntile; q += kThreads) {
-                const int packed = tiles[q];
-                const int i0 = 4 * (packed >> 8), j0 = 4 * (packed & 255);
-                double* Rw[4];
-                int off[4][4];
-                double acc[4][4];
-#pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int i = i0 + a;
-                    Rw[a] = slot(t0 + (i < mt ? i : mt - 1));
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const int j = j0 + c;
-                        off[a][c] = (i < mt && j <= i) ? j - i + wb - 1 : wb + 1;
-                        acc[a][c] = Rw[a][off[a][c]];
+                const int ln = opaque_lane();
+                yv = (ln == j) ? y : yv;
+                dv = (ln == j) ? d : dv;
+                bv = (ln == j) ? yj : bv;
+                static_for<kP>([&](auto JN) __attribute__((always_inline)) {
+                    constexpr int jj = JN;
+                    if constexpr (jj > j) {
+                        const double t = lane_value(a0[j], jj);
+                        a0[jj] = fma(-a0[j], t, a0[jj]);
+                        a1[jj] = fma(-a1[j], t, a1[jj]);
                     }
-                }
-#pragma unroll
-                for (int kk = 0; kk < kP; ++kk) {
-                    const double2 li0 = *reinterpret_cast<const double2*>(&LpT[kk][nk + i0]);
-                    const double2 li1 = *reinterpret_cast<const double2*>(&LpT[kk][nk + i0 + 2]);
-                    const double2 lj0 = *reinterpret_cast<const double2*>(&LpT[kk][nk + j0]);
-                    const double2 lj1 = *reinterpret_cast<const double2*>(&LpT[kk][nk + j0 + 2]);
-                    const double li[4] = {li0.x, li0.y, li1.x, li1.y};
-                    const double lj[4] = {lj0.x, lj0.y, lj1.x, lj1.y};
-#pragma unroll
-                    for (int a = 0; a < 4; ++a)
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) acc[a][c] = fma(-li[a], lj[c], acc[a][c]);
-                }
-#pragma unroll
-                for (int a = 0; a < 4; ++a)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) Rw[a][off[a][c]] = acc[a][c];
+                });
+            });
+            // the pivot rules, one column per lane
+            if (lane < kP && c0 + lane < n) {
+                ok &= dv > eps;
+                const double l = dv * yv;
+                lmax = dmax_nan(lmax, fabs(l));
+                lmin = fmin(lmin, fabs(l));
+            }
+            double* L = LP + kb * kP * W;
+            double* D = DB + kb * kP * kP;
+            static_for<kP>([&](auto KK) __attribute__((always_inline)) {
+                constexpr int kk = KK;
+                L[kk * W + q0] = lane < kP ? 0.0 : a0[kk];
+                if (v1) L[kk * W + q1] = a1[kk];
+                if (lane < kP) D[kk * kP + lane] = (kk == lane) ? yv : a0[kk];
+            });
+            if (v1) RH[q1] = b1;
+            if (lane >= kP) {
+                RH[q0] = b0;
+            } else {
+                if (c0 + lane < n) x[c0 + lane] = bv;
+                RH[q0] = (k + 2 <= NP) ? SRb[kb * kP + lane] : 0.0;
+            }
+        } else if (k > 0) {
+            const int Xc = pp0 >> 4;
+            update(LP + (kb ^ 1) * kP * W, [&](int I, int J) __attribute__((always_inline)) { return I != Xc && J != Xc; });
+            copy_panel(k - 1, ppp);
+        }
+        __syncthreads();
+        // ---- phase B
+        if (k + 1 < NP) {
+            if (wave == 0) {
+                int np2 = ppn + kP;
+                np2 -= np2 >= W ? W : 0;
+                int np3 = np2 + kP;
+                np3 -= np3 >= W ? W : 0;
+                if (k + 1 <= NP - 2) stage(k + 1, np2);
+                if (k + 2 <= NP - 2) prefetch(k + 2, np3);
+            } else {
+                const int Xc = pp0 >> 4, hc = (pp0 >> 3) & 1;
+                const int Xn = ppn >> 4, hn = (ppn >> 3) & 1;
+                reload(SR + kb * kP * W, Xc, hc, pp0);
+                update(LP + kb * kP * W, [&](int I, int J) __attribute__((always_inline)) { return I == Xn || J == Xn; });
+                publish(Xn, hn, ppn);
             }
         }
         __syncthreads();
+        pp0 = ppn;
     }
-    if (wave == 0 && lane == 0) flags[1] = (ok && !(lmin > 1e-7 * lmax)) ? 1 : 0;
-    // the last panel's rows out
-    if (!flags[0] && k > 0) store_rows((k - 1) * kP, n, (k - 1) & 1, 0, kThreads);
+    // the last panel's L to the factor buffers; the status
+    {
+        int ppl = pp0 - kP;
+        if (ppl < 0) ppl += W;
+        if (wave > 0) copy_panel(NP - 1, ppl);
+    }
+    if (wave == 0) {
+        // lanes 0 .. kP - 1 hold the pivot rules of their columns
+        bool all_ok = true;
+        double mxl = 0.0, mnl = INFINITY;
+        static_for<kP>([&](auto J) __attribute__((always_inline)) {
+            constexpr int j = J;
+            all_ok &= __builtin_amdgcn_readlane(ok ? 1 : 0, j) != 0;
+            mxl = dmax_nan(mxl, lane_value(lmax, j));
+            mnl = fmin(mnl, lane_value(lmin, j));
+        });
+        if (lane == 0) sflag = !all_ok ? 1 : ((mnl > 1e-7 * mxl) ? 0 : 2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
-    if (flags[0] || flags[1]) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int st = sflag;
+    if (st != 0) {
         if (tid == 0) {
-            status[0] = flags[0] ? 1 : 2;
+            status[0] = st;
             lm->solve_ok = 0;
         }
         return;
     }
-    // backward substitution; the block buffers take the window's memory
-    double* blk0 = ring;
-    double* blk1 = ring + kBlkBuf;
-    const int nb = (n + kBlk - 1) / kBlk;
-    load_block(((nb - 1) & 1) ? blk1 : blk0, vbk[(nb - 1) & 1], (nb - 1) * kBlk, n, wb, Lb, x, 0, kThreads);
-    __syncthreads();
-    band_backward(n, wb, Lb, x, blk0, blk1, vbk, xr, ya, part);
+    band_sub<W, false>(n, Lr, x, x, smem, wave, lane);
     if (tid == 0) lm->solve_ok = 1;
 }
 
-// x = S^-1 b with the factor of k_chol_band in Lb: the forward substitution over
-// blocks of kBlk rows from the top (rows staged one block ahead), then
-// band_backward.
-__global__ void __launch_bounds__(kThreads) k_band_solve(int n, int wb, const double* Lb, const double* b, double* x,
+// x = S^-1 b with the factor of k_band_factor (forward, then backward).
+template <int W>
+__global__ void __launch_bounds__(kThreads) k_band_solve(int n, const double* Lband, const double* b, double* x,
                                                          const int32_t* status, lm_state* lm, int gated) {
     if (gated && lm->done) return;
     if (*status != 0) {
         if (threadIdx.x == 0) lm->solve_ok = 0;
         return;
     }
-    __shared__ double blkmem[2 * kBlkBuf];
-    __shared__ double vbk[2][kBlk];
-    __shared__ double xr[kXRing], ya[kXRing];
-    __shared__ double part[kWaves][64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int st = wb + 1;
-    double* blk0 = blkmem;
-    double* blk1 = blkmem + kBlkBuf;
-    const int nb = (n + kBlk - 1) / kBlk;
-    load_block(blk0, vbk[0], 0, (n < kBlk ? n : kBlk), wb, Lb, b, 0, kThreads);
+    __shared__ double smem[band_lds<W>::kSub];
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(static_cast<int>(threadIdx.x) >> 6);
+    const double* Lc = Lband;
+    const double* Lr = Lband + static_cast<int64_t>(sub_rows(n)) * kRing;
+    band_sub<W, true>(n, Lc, b, x, smem, wave, lane);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __syncthreads();
-    for (int bi = 0; bi < nb; ++bi) {
-        const int i0 = bi * kBlk;
-        const int mb = (n - i0) < kBlk ? (n - i0) : kBlk;
-        double* cur = (bi & 1) ? blk1 : blk0;
-        double* nxt = (bi & 1) ? blk0 : blk1;
-        // rows r of the block: partial = sum over j < i0 of L_rj y_j (a quarter of j per wave)
-        {
-            const int r = i0 + lane;
-            double acc = 0.0;
-            if (lane < mb && i0 > 0) {
-                const int lo = (r - wb + 1) > 0 ? (r - wb + 1) : 0;
-                for (int j = lo + wave; j < i0; j += kWaves)
-                    acc = fma(cur[lane * st + (j - r + wb - 1)], xr[j % kXRing], acc);
-            }
-            part[wave][lane] = acc;
-        }
-        __syncthreads();
-        if (wave == 0) {
-            const int r = i0 + lane;
-            const bool real = lane < mb;
-            const double* row = cur + (real ? lane : 0) * st;
-            double bv = real ? vbk[bi & 1][lane] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]))
-                             : 0.0;
-            const double dl = real ? row[wb] : 0.0;
-            // step k: y_k from lane k, then b_r -= L_{r, i0+k} y_k for r > k; the next
-            // kGroup steps' entries load during these
-            auto entry = [&](int k) {
-                const int ln = opaque_lane();
-                const int off = k - ln + wb - 1;
-                const bool in = real && k < ln && off >= 0;
-                const double t = row[in ? off : 0];
-                return in ? t : 0.0;
-            };
-            double tc[kGroup], tn[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) tc[u] = entry(u);
-#pragma unroll
-            for (int g = 0; g < kBlk / kGroup; ++g) {
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) tn[u] = (g + 1 < kBlk / kGroup) ? entry((g + 1) * kGroup + u) : 0.0;
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) {
-                    const int k = g * kGroup + u;
-                    const int ln = opaque_lane();
-                    const double yk = lane_value(bv * dl, k);
-                    bv = (ln == k) ? yk : fma(-tc[u], yk, bv);
-                }
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) tc[u] = tn[u];
-            }
-            if (real) {
-                x[r] = bv;
-                xr[r % kXRing] = bv;
-                vbk[bi & 1][lane] = bv;  // the backward pass starts from this block
-            }
-        } else if (bi + 1 < nb) {
-            const int hi = (i0 + 2 * kBlk) < n ? (i0 + 2 * kBlk) : n;
-            load_block(nxt, vbk[(bi + 1) & 1], i0 + kBlk, hi, wb, Lb, b, 64, kThreads - 64);
-        }
-        __syncthreads();
-    }
-    band_backward(n, wb, Lb, x, blk0, blk1, vbk, xr, ya, part);
-    if (tid == 0) lm->solve_ok = 1;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    band_sub<W, false>(n, Lr, x, x, smem, wave, lane);
+    if (threadIdx.x == 0) lm->solve_ok = 1;
 }
 
 }  // namespace
 
+int band_window(int32_t band) { return band > 0 ? window_for(band) : 0; }
+
+size_t band_buffer_doubles(int32_t n, int32_t band) {
+    const int W = band_window(band);
+    if (W == 0 || n <= 0) return 0;
+    // column-ordered rows [0, np), row-ordered rows [0, np + kRing)
+    const size_t np = (static_cast<size_t>(n) + kB - 1) / kB * kB;
+    return static_cast<size_t>(kRing) * (2 * np + kRing);
+}
+
 bool band_path(int32_t n, const dev_work64& w, bool exact) {
-    if (exact || n <= 0 || (n & 1) || w.band <= 0 || w.band > kBandMaxW || !w.Lt) return false;
-    return w.factor_path == 3;
+    if (exact || n <= 0 || w.band <= 0 || !w.Lband) return false;
+    const int W = band_window(w.band);
+    if (W == 0) return false;
+    if (w.factor_path == 3) return true;
+    return w.factor_path == 0 && n >= kBandMinRowsPerWindow * W;
 }
 
 void launch_chol_band(int32_t n, const dev_work64& w, hipStream_t s, bool gated, double floor_rel) {
-    k_chol_band<<<1, kThreads, 0, s>>>(n, w.band, w.S, w.bs, w.dxp, w.Lt, floor_rel, w.chol_scale, w.chol_status, w.lm,
-                                       gated ? 1 : 0);
+    const int g = gated ? 1 : 0;
+#define ZB_BAND_FACTOR(WW)                                                                                       \
+    k_band_factor<WW><<<1, kThreads, 0, s>>>(n, w.band, w.S, w.bs, w.dxp, w.Lband, floor_rel, w.chol_scale,  \
+                                             w.chol_status, w.lm, g)
+    switch (band_window(w.band)) {
+    case 64: ZB_BAND_FACTOR(64); break;
+    case 80: ZB_BAND_FACTOR(80); break;
+    case 96: ZB_BAND_FACTOR(96); break;
+    case 112: ZB_BAND_FACTOR(112); break;
+    case 128: ZB_BAND_FACTOR(128); break;
+    default: break;
+    }
+#undef ZB_BAND_FACTOR
 }
 
 void launch_band_solve(int32_t n, const dev_work64& w, const double* b, double* x, hipStream_t s, bool gated) {
-    k_band_solve<<<1, kThreads, 0, s>>>(n, w.band, w.Lt, b, x, w.chol_status, w.lm, gated ? 1 : 0);
+    const int g = gated ? 1 : 0;
+#define ZB_BAND_SOLVE(WW) k_band_solve<WW><<<1, kThreads, 0, s>>>(n, w.Lband, b, x, w.chol_status, w.lm, g)
+    switch (band_window(w.band)) {
+    case 64: ZB_BAND_SOLVE(64); break;
+    case 80: ZB_BAND_SOLVE(80); break;
+    case 96: ZB_BAND_SOLVE(96); break;
+    case 112: ZB_BAND_SOLVE(112); break;
+    case 128: ZB_BAND_SOLVE(128); break;
+    default: break;
+    }
+#undef ZB_BAND_SOLVE
 }
 
 }  // namespace zb
