@@ -225,11 +225,11 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
             const int j = row_of(mm, it_t[u] < 0 ? 0 : it_t[u]);
             dst[u] = F[j * kRing + it_d[u]];
         });
+        // the refill value's validity is applied when it is stored: a select
+        // here would wait for this load, and with it every load before it
         const int j = row_of(mm, lane & (kB - 1));
         const int jr = kFwd ? j + kRing : j - kRing;
-        const bool ok = jr >= 0 && jr < n;
-        const double v = rhs[ok ? jr : 0];
-        rdst = ok ? v : 0.0;
+        rdst = rhs[(jr >= 0 && jr < n) ? jr : 0];
     };
     auto store = [&](int m, const double* src, double rsrc) __attribute__((always_inline)) {
         double* rg = ring + (m & 1) * kB * kRing;
@@ -237,7 +237,12 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
             constexpr int u = U;
             if (it_t[u] >= 0) rg[it_t[u] * kRing + it_d[u]] = src[u];
         });
-        if (wave == 1 && lane < kB) rslot[(m & 1) * kRing + (row_of(m, lane) & (kRing - 1))] = rsrc;
+        if (wave == 1 && lane < kB) {
+            const int mm = m < nblk ? m : nblk - 1;
+            const int j = row_of(mm, lane);
+            const int jr = kFwd ? j + kRing : j - kRing;
+            rslot[(m & 1) * kRing + (row_of(m, lane) & (kRing - 1))] = (jr >= 0 && jr < n) ? rsrc : 0.0;
+        }
     };
 
     // wave 0: the ring of unknowns; slot s holds unknown j + ((s - j) & 127)
@@ -270,7 +275,7 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
             const double xj = lane_value(kHi ? u1 * l1[t] : u0 * l0[t], sl);
             u0 = fma(-l0[t], xj, u0);
             u1 = fma(-l1[t], xj, u1);
-            xs = (opaque_lane() == t) ? xj : xs;
+            xs = (lane == t) ? xj : xs;
         });
         // the block's slots (kB consecutive, from lo) take the unknowns 128 away
         const int lo = (kFwd ? jb : jb - (kB - 1)) & 63;
@@ -416,9 +421,9 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
             entering(m, t, npos, r, c);
             pre[t] = A[entry_index(n, wb, r, c)];
         });
+        // validity applied when staged (a select here would wait for the loads)
         const int rn = kP * m + W + (lane & (kP - 1));
         pre_b = b[rn < n ? rn : 0];
-        pre_b = rn < n ? pre_b : 0.0;
     };
     auto stage = [&](int m, int npos) __attribute__((always_inline)) {
         double* S = SR + (m & 1) * kP * W;
@@ -428,7 +433,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
             entering(m, t, npos, r, c);
             S[e_i[t] * W + e_q[t]] = entry_finish(pre[t], n, wb, lift, r, c);
         });
-        if (lane < kP) SRb[(m & 1) * kP + lane] = pre_b;
+        if (lane < kP) SRb[(m & 1) * kP + lane] = (kP * m + W + lane < n) ? pre_b : 0.0;
     };
     // tile update: acc -= L_I L_J^T over the panel's kP columns (lane l: operand
     // row l % 16 at depth l / 16; result element v at row l / 16 + 4 v, column
