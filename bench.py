@@ -261,7 +261,7 @@ def seed_compare(seed, config, iters):
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "c1_mixed_hbm_traffic.md")
 GROUP_KERNELS = {
-    "cholesky": ["k_chol_begin", "k_chol_fused<true>"],
+    "cholesky": ["k_band_factor<80>"],
     "linearisation": ["k_lin_blocks_f", "k_lin_finish_f"],
     "schur": ["k_damp_invert_f", "k_form_Y_f", "k_fill_zero_d", "k_schur_f<2>"],
     "cost": ["k_cost"],
