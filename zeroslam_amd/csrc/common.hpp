@@ -342,7 +342,7 @@ constexpr int kBatchPhases = 8;
 // refinement); BM_LPTS 256 points, BM_LCHUNK 4 pose chunks, BM_LENTRY 256
 // pose-block entries (the other windows); BM_PTS 256 points; BM_INVZ 256 of the
 // nl + n^2 items of the damping and the cleared reduced system; BM_PAIR 256
-// (pose, point) pairs; BM_SBLK 4 reduced-system blocks; BM_PTSN 256 of nl + n;
+// (pose, point) pairs; BM_SBLK one reduced-system block; BM_PTSN 256 of nl + n;
 // BM_APPLY and BM_COST one per block of the per-graph grids of the update (np + nl
 // items) and of the cost (no items); BM_SLOT 256 of np + nl.
 enum batch_map : int {

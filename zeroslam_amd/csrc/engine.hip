@@ -1404,7 +1404,7 @@ struct zsba_batch {
             case zb::BM_PTS: return cdiv(p.nl, 256);
             case zb::BM_INVZ: return cdiv(p.nl + static_cast<int64_t>(p.n) * p.n, 256);
             case zb::BM_PAIR: return cdiv(p.npair, 256);
-            case zb::BM_SBLK: return cdiv(p.nblk, 4);
+            case zb::BM_SBLK: return p.nblk;
             case zb::BM_PTSN: return mixed ? cdiv(static_cast<int64_t>(p.nl) + p.n, 256) : 0;
             case zb::BM_APPLY: return sum_grid(static_cast<int64_t>(p.np) + p.nl);
             case zb::BM_COST: return sum_grid(p.no);

@@ -306,21 +306,23 @@ __global__ void __launch_bounds__(256) k_batch_form_Y(const batch_window* wins, 
     mixed::form_Y_item(bw.p, B.W, B.Hinv, B.Y, a);
 }
 
-// One wave per reduced-system block (four per workgroup).
+// One workgroup per reduced-system block.
 template <class TS, int kInFlight>
-__global__ void __launch_bounds__(256) k_batch_schur(const batch_window* wins, const int2* map, uint64_t* pc) {
+__global__ void __launch_bounds__(mixed::kSchurThreads) k_batch_schur(const batch_window* wins, const int2* map,
+                                                                       uint64_t* pc) {
     stamp(pc, PH_SCHUR);
     const int2 m = map[blockIdx.x];
     if (m.x < 0) return;
     const batch_window& bw = wins[m.x];
     const lm_state* lm = bw.w.lm;
     if (lm->done) return;
-    const int blk = m.y * 4 + static_cast<int>(threadIdx.x >> 6);
+    const int blk = m.y;
     if (blk >= bw.p.nblk) return;
+    __shared__ mixed::schur_lds<TS> red;
     const blocks<TS> B(bw.w);
     const dev_work64& w = bw.w;
     mixed::schur_block<TS, kInFlight>(bw.p, B.Y, B.W, B.Hpp, w.bp_m, w.bl_m, w.S, w.bs, lm->lambda, blk,
-                                      threadIdx.x & 63);
+                                      static_cast<int>(threadIdx.x), red);
 }
 
 // One workgroup per window: the diagonal floor, the envelope factorisation and
@@ -639,8 +641,10 @@ void enqueue_step(const batch_window* wins, const batch_grid& g, int threads, in
     if (wgs(g, BM_INVZ)) k_batch_invert<TS><<<wgs(g, BM_INVZ), 256, 0, s>>>(wins, g.map[BM_INVZ], pc);
     if (wgs(g, BM_PAIR)) k_batch_form_Y<TS><<<wgs(g, BM_PAIR), 256, 0, s>>>(wins, g.map[BM_PAIR], pc);
     if (wgs(g, BM_SBLK)) {
-        if (inflight == 2) k_batch_schur<TS, 2><<<wgs(g, BM_SBLK), 256, 0, s>>>(wins, g.map[BM_SBLK], pc);
-        else k_batch_schur<TS, 1><<<wgs(g, BM_SBLK), 256, 0, s>>>(wins, g.map[BM_SBLK], pc);
+        if (inflight == 2)
+            k_batch_schur<TS, 2><<<wgs(g, BM_SBLK), mixed::kSchurThreads, 0, s>>>(wins, g.map[BM_SBLK], pc);
+        else
+            k_batch_schur<TS, 1><<<wgs(g, BM_SBLK), mixed::kSchurThreads, 0, s>>>(wins, g.map[BM_SBLK], pc);
     }
     if (threads == 256) launch_factor<256>(wins, g, pc, mixed_path, s);
     else if (threads == 512) launch_factor<512>(wins, g, pc, mixed_path, s);

@@ -94,17 +94,19 @@ __global__ void __launch_bounds__(kBlock) k_form_Y_f(dev_problem p, const float*
     if (a < p.npair) mixed::form_Y_item(p, Wbuf, Hinv, Y, a);
 }
 
-// One wave per reduced-system block; kInFlight as in mixed::schur_block (the
-// same result either way).
+// One workgroup per reduced-system block; kInFlight as in mixed::schur_block
+// (the same result either way).
 template <int kInFlight>
-__global__ void __launch_bounds__(256) k_schur_f(dev_problem p, const float* Y, const float* Wbuf, const float* Hpp,
-                                                 const double* bp, const double* bl, double* S, double* bs,
-                                                 const lm_state* lm, const double* lambda_override) {
+__global__ void __launch_bounds__(mixed::kSchurThreads) k_schur_f(dev_problem p, const float* Y, const float* Wbuf,
+                                                                   const float* Hpp, const double* bp, const double* bl,
+                                                                   double* S, double* bs, const lm_state* lm,
+                                                                   const double* lambda_override) {
     if (!lambda_override && lm->done) return;
-    const int blk = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
+    __shared__ mixed::schur_lds<float> red;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
     if (blk < p.nblk)
-        mixed::schur_block<float, kInFlight>(p, Y, Wbuf, Hpp, bp, bl, S, bs, lambda_override ? *lambda_override : lm->lambda, blk,
-                           threadIdx.x & 63);
+        mixed::schur_block<float, kInFlight>(p, Y, Wbuf, Hpp, bp, bl, S, bs,
+                                             lambda_override ? *lambda_override : lm->lambda, blk, threadIdx.x, red);
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_zero_d(double* a, int64_t n, const lm_state* lm, int gated) {
@@ -114,7 +116,8 @@ __global__ void __launch_bounds__(kBlock) k_fill_zero_d(double* a, int64_t n, co
 
 inline int grid_of(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
 
-// Below this many workgroups the Schur kernels keep two contributions in flight.
+// Below this many reduced-system blocks the Schur kernel keeps two contributions
+// in flight per lane.
 constexpr int kSchurSmallGrid = 2048;
 
 }  // namespace
@@ -174,12 +177,12 @@ void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipS
         // Two contributions in flight per lane hide more latency while the grid
         // is small; with more workgroups than the GPU holds at once, one (and
         // the higher occupancy it allows) is faster.
-        if (grid_of(64L * p.nblk, 256) < kSchurSmallGrid)
-            k_schur_f<2><<<grid_of(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y_f, w.W_f, w.Hpp_f, w.bp_m, w.bl_m, w.S, w.bs, w.lm,
-                                                              lambda_override);
+        if (p.nblk < kSchurSmallGrid)
+            k_schur_f<2><<<p.nblk, mixed::kSchurThreads, 0, s>>>(p, w.Y_f, w.W_f, w.Hpp_f, w.bp_m, w.bl_m, w.S, w.bs,
+                                                                 w.lm, lambda_override);
         else
-            k_schur_f<1><<<grid_of(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y_f, w.W_f, w.Hpp_f, w.bp_m, w.bl_m, w.S, w.bs, w.lm,
-                                                              lambda_override);
+            k_schur_f<1><<<p.nblk, mixed::kSchurThreads, 0, s>>>(p, w.Y_f, w.W_f, w.Hpp_f, w.bp_m, w.bl_m, w.S, w.bs,
+                                                                 w.lm, lambda_override);
         ZB_TIME_END(w, s);
     }
 }

@@ -32,6 +32,53 @@ template <class T> __device__ __forceinline__ T wave_sum(T v);
 template <> __device__ __forceinline__ float wave_sum<float>(float v) { return wave_sum_f(v); }
 template <> __device__ __forceinline__ double wave_sum<double>(double v) { return wave_sum_d(v); }
 
+// Reduce-scatter over the wave by recursive halving.  On entry slot k of every
+// lane holds a partial of entry k (M slots).  The step over lane bit b pairs each
+// lane with the lane across that bit: the lane with b = 0 keeps the lower half of
+// the slots, its partner the upper half, and each adds the partner's partials of
+// the half it keeps.  After the six steps the lane holds the wave total of entry
+// `off` when `m` is 1 (every entry ends in exactly one lane; m is 0 on the lanes
+// left without one).  Slots past a lane's own count only ever meet other such
+// slots, so they never reach a total.  M entries cost M/2 + M/4 + ... shuffles
+// (38 for M = 36) where one butterfly per entry costs 6 M.
+template <class T, int M, int kBit>
+struct reduce_scatter_step {
+    static __device__ __forceinline__ T run(const T (&v)[M], int lane, int& off, int& m) {
+        constexpr int H = (M + 1) / 2;
+        const bool hi = (lane >> kBit) & 1;
+        T u[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k) {
+            const T lo_v = v[k];
+            const T hi_v = (H + k < M) ? v[(H + k < M) ? H + k : 0] : T(0);
+            const T send = hi ? lo_v : hi_v;
+            const T keep = hi ? hi_v : lo_v;
+            u[k] = keep + __shfl_xor(send, 1 << kBit, 64);
+        }
+        if (hi) {
+            off += H;
+            m = m > H ? m - H : 0;
+        } else {
+            m = m < H ? m : H;
+        }
+        return reduce_scatter_step<T, H, kBit - 1>::run(u, lane, off, m);
+    }
+};
+template <class T, int M>
+struct reduce_scatter_step<T, M, -1> {
+    static __device__ __forceinline__ T run(const T (&v)[M], int, int&, int&) { return v[0]; }
+};
+
+// Wave total of entry `entry` (or -1: this lane holds none) from M per-lane slots.
+template <class T, int M>
+__device__ __forceinline__ T wave_reduce_scatter(const T (&v)[M], int lane, int& entry) {
+    static_assert(M >= 1 && M <= 64, "one entry per lane at most");
+    int off = 0, m = M;
+    const T total = reduce_scatter_step<T, M, 5>::run(v, lane, off, m);
+    entry = m == 1 ? off : -1;
+    return total;
+}
+
 __device__ __forceinline__ void obs_weight(const dev_problem& p, int s, double W[4]) {
     if (p.identity_info) {
         W[0] = 1.0; W[1] = 0.0; W[2] = 0.0; W[3] = 1.0;
@@ -497,17 +544,30 @@ __device__ inline void form_Y_item(const dev_problem& p, const TS* Wbuf, const T
     for (int i = 0; i < 18; ++i) Y[18L * a + i] = o[i];
 }
 
-// One wave per 6x6 lower-triangle block of the reduced camera system: the
-// products Y_a W_c^T accumulate in TS per lane and across the wave, Y b_l in
-// binary64; the block is assembled in binary64:
+// One workgroup of kSchurWaves waves per 6x6 lower-triangle block of the
+// reduced camera system: the products Y_a W_c^T accumulate in TS per lane, Y b_l
+// in binary64; the block is assembled in binary64:
 //   S_ik = [i == k] (H_pp,i + lambda I) - sum Y W^T,  b_s,i = b_p,i - sum Y b_l
 // (factor_graph.hpp:768-781, matrix_sparse_block.hpp:158-196).
+// Lane j of wave w adds contributions c0 + 64 w + j, then + 64 kSchurWaves, ...
+// in order; each wave reduce-scatters its 36 (and, on a diagonal block, 6)
+// sums, and the waves' totals are added in wave order in binary64.
 // kInFlight: contributions per lane loaded before they are added (2 hides the
-// re-reads' latency; 1 needs fewer registers).  Either way each lane adds its
-// contributions c, c + 64, ... in order, so the result is the same.
+// re-reads' latency; 1 needs fewer registers); the result is the same.
+constexpr int kSchurWaves = 4;
+constexpr int kSchurThreads = 64 * kSchurWaves;
+
+template <class TS>
+struct schur_lds {
+    TS s[kSchurWaves][36];
+    double b[kSchurWaves][6];
+};
+
 template <class TS, int kInFlight = 2>
 __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* Wbuf, const TS* Hpp, const double* bp,
-                                   const double* bl, double* S, double* bs, double lam, int blk, int lane) {
+                                   const double* bl, double* S, double* bs, double lam, int blk, int tid,
+                                   schur_lds<TS>& red) {
+    const int lane = tid & 63, wave = tid >> 6;
     const int row = p.blk_row[blk], col = p.blk_col[blk];
     const bool diag = row == col;
     TS acc[36];
@@ -515,7 +575,8 @@ __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* 
     for (int i = 0; i < 36; ++i) acc[i] = TS(0);
     double accb[6] = {0, 0, 0, 0, 0, 0};
     const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
-    // contributions c, c + 64 of this lane: two in flight (loads first), added in order
+    constexpr int kStride = kSchurThreads;
+    // contributions c, c + kStride of this lane: two in flight (loads first), added in order
     auto add = [&](const TS (&Ya)[18], const TS (&Wc)[18], const double (&bv)[3]) {
 #pragma unroll
         for (int r = 0; r < 6; ++r)
@@ -543,47 +604,56 @@ __device__ inline void schur_block(const dev_problem& p, const TS* Y, const TS* 
             bv[2] = bl[3L * l + 2];
         }
     };
-    int64_t c = c0 + lane;
+    int64_t c = c0 + 64 * wave + lane;
     if (kInFlight == 2) {
-        for (; c + 64 < c1; c += 128) {
+        for (; c + kStride < c1; c += 2 * kStride) {
             TS Ya0[18], Wc0[18], Ya1[18], Wc1[18];
             double b0[3], b1[3];
             fetch(c, Ya0, Wc0, b0);
-            fetch(c + 64, Ya1, Wc1, b1);
+            fetch(c + kStride, Ya1, Wc1, b1);
             add(Ya0, Wc0, b0);
             add(Ya1, Wc1, b1);
         }
     }
-    for (; c < c1; c += 64) {
+    for (; c < c1; c += kStride) {
         TS Ya0[18], Wc0[18];
         double b0[3];
         fetch(c, Ya0, Wc0, b0);
         add(Ya0, Wc0, b0);
     }
-    const int64_t n = p.n;
-    double* Sblk = S + (6L * row) * n + 6L * col;
-#pragma unroll
-    for (int i = 0; i < 36; ++i) {
-        const TS tot = wave_sum<TS>(acc[i]);
-        const int r = i / 6, c = i % 6;
-        if (lane == 0) {
-            if (diag) {
-                if (c <= r) {
-                    double base = static_cast<double>(Hpp[21L * row + r * (r + 1) / 2 + c]);
-                    if (r == c) base += lam;
-                    Sblk[r * n + c] = base - static_cast<double>(tot);
-                }
-            } else {
-                Sblk[r * n + c] = -static_cast<double>(tot);
-            }
-        }
-    }
+    // Waves with no contributions hold zeros and still take part: every wave's
+    // total enters the sum, so the result does not depend on the list length.
+    int e;
+    const TS tot = wave_reduce_scatter<TS, 36>(acc, lane, e);
+    if (e >= 0) red.s[wave][e] = tot;
     if (diag) {
+        int eb;
+        const double totb = wave_reduce_scatter<double, 6>(accb, lane, eb);
+        if (eb >= 0) red.b[wave][eb] = totb;
+    }
+    __syncthreads();
+    if (tid < 36) {
+        const int r = tid / 6, cc = tid % 6;
+        double sum = static_cast<double>(red.s[0][tid]);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const double tot = wave_sum_d(accb[i]);
-            if (lane == 0) bs[6L * row + i] = bp[6L * row + i] - tot;
+        for (int w = 1; w < kSchurWaves; ++w) sum += static_cast<double>(red.s[w][tid]);
+        const int64_t n = p.n;
+        double* Sblk = S + (6L * row) * n + 6L * col;
+        if (diag) {
+            if (cc <= r) {
+                double base = static_cast<double>(Hpp[21L * row + r * (r + 1) / 2 + cc]);
+                if (r == cc) base += lam;
+                Sblk[r * n + cc] = base - sum;
+            }
+        } else {
+            Sblk[r * n + cc] = -sum;
         }
+    } else if (diag && tid >= 64 && tid < 70) {
+        const int i = tid - 64;
+        double sum = red.b[0][i];
+#pragma unroll
+        for (int w = 1; w < kSchurWaves; ++w) sum += red.b[w][i];
+        bs[6L * row + i] = bp[6L * row + i] - sum;
     }
 }
 
