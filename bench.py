@@ -485,6 +485,81 @@ def bench_match(args) -> int:
     return 0
 
 
+def bench_track(args) -> int:
+    """The tracking step's BA, map::optimise(1, true, 50) (slam.hpp:316,387,469):
+    the last frame of the 100-keyframe workload free, every landmark it observes
+    fixed (zeroslam_amd.problem.motion_only), solve(50) with relative
+    convergence.  One step = one frame's BA as a tracker calls it: a fresh handle
+    is built, loaded and solved, and its state read back; the numeric solve alone
+    (restore + solve on a built handle) is reported beside it, for each
+    precision.  The CPU baseline runs the same graph through the reference's own
+    optimiser (oracle/_ref), one thread."""
+    from zeroslam_amd.engine import FP64, FP64_FAST, MIXED, Graph
+    from zeroslam_amd.problem import motion_only, synth
+
+    pr = motion_only(synth(1), 1)
+    iters = 50
+    print(f"bench: tracking BA, one frame, {pr.n_obs} observations of {pr.n_points} fixed landmarks",
+          file=sys.stderr, flush=True)
+
+    def frame(prec):
+        g = Graph.from_problem(pr)
+        g.solve(iters, True, prec)
+        pose = g.poses()
+        g.close()
+        return pose
+
+    def numeric(prec, steps):
+        g = Graph.from_problem(pr)
+        g.solve(0)
+        g.snapshot()
+        for _ in range(args.warmup):
+            g.restore()
+            g.solve(iters, True, prec)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.restore()
+            g.solve(iters, True, prec)
+        dt = (time.perf_counter() - t0) / steps
+        st = g.stats()
+        g.close()
+        return dt, st
+
+    for _ in range(args.warmup):
+        frame(FP64)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame(FP64)
+    per_frame = (time.perf_counter() - t0) / args.steps
+    paths = {}
+    for name, prec in (("fp64", FP64), ("fp64_fast", FP64_FAST), ("mixed", MIXED)):
+        dt, st = numeric(prec, max(args.steps, 20))
+        paths[name] = {"ms_per_solve": dt * 1e3, "iterations": st["iterations"], "attempts": st["attempts"],
+                       "final_chi2": st["final_chi2"]}
+    out = {"metric": "tracking-step BA frames per second (map::optimise(1, true, 50), FP64 reference order)",
+           "value": 1.0 / per_frame, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": per_frame * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"tracking BA: last frame of the 100-keyframe workload, {pr.n_obs} observations, "
+                                  f"{pr.n_points} fixed landmarks, solve({iters}) with relative convergence"},
+           "numeric_solve": paths}
+    if not args.no_cpu:
+        from oracle.oracle import CpuGraph, have_reference
+        kind = "reference" if have_reference() else "port"
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            o = CpuGraph(pr, kind if kind == "reference" else "oracle")
+            o.solve(iters, True)
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        out["cpu_baseline"] = {"value": 1.0 / dt, "unit": "frames/s", "cores": 1, "kind": kind,
+                               "sample": f"{reps} frames (graph build + solve({iters}, relative)) in "
+                                         f"{dt * reps:.1f} s, single-threaded on {cpu_model()}"}
+    print(json.dumps(out))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -499,6 +574,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--match", action="store_true",
                     help="bench the projection search (data association) instead of the solver")
+    ap.add_argument("--track", action="store_true",
+                    help="bench the tracking step's motion-only BA (one free frame) instead of the headline")
     ap.add_argument("--seeds", type=int, default=4,
                     help="repeat the equal-cost comparison on this many other instances of the workload")
     args = ap.parse_args()
@@ -511,6 +588,8 @@ def main() -> int:
         return bench_batch(args)
     if args.match:
         return bench_match(args)
+    if args.track:
+        return bench_track(args)
 
     from zeroslam_amd.engine import FP64, FP64_FAST, MIXED, Graph
     from zeroslam_amd.problem import ITERATIONS, synth

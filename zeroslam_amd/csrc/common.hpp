@@ -21,8 +21,9 @@ namespace zb {
 // keep the XCDs evenly loaded.  It is a bijection of [0, n) (the tail that
 // does not fill a round of chunks keeps its numbers): the same work is done,
 // by other workgroups.
+template <int kChunk = 16>
 __device__ __forceinline__ int xcd_block(int bid, int n) {
-    constexpr int kXcds = 8, kChunk = 16;
+    constexpr int kXcds = 8;
     const int full = n / (kXcds * kChunk) * (kXcds * kChunk);
     if (bid >= full) return bid;
     const int x = bid % kXcds, i = bid / kXcds;

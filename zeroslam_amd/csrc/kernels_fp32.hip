@@ -95,7 +95,9 @@ __global__ void __launch_bounds__(kBlock) k_form_Y_f(dev_problem p, const float*
 }
 
 // One workgroup per reduced-system block; kInFlight as in mixed::schur_block
-// (the same result either way).
+// (the same result either way).  Blocks are numbered row by row, so a chunk of
+// consecutive blocks shares its block rows' Y blocks: 64 per XCD chunk.
+constexpr int kSchurChunk = 64;
 template <int kInFlight>
 __global__ void __launch_bounds__(mixed::kSchurThreads) k_schur_f(dev_problem p, const float* Y, const float* Wbuf,
                                                                    const float* Hpp, const double* bp, const double* bl,
@@ -103,7 +105,7 @@ __global__ void __launch_bounds__(mixed::kSchurThreads) k_schur_f(dev_problem p,
                                                                    const double* lambda_override) {
     if (!lambda_override && lm->done) return;
     __shared__ mixed::schur_lds<float> red;
-    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int blk = xcd_block<kSchurChunk>(blockIdx.x, gridDim.x);
     if (blk < p.nblk)
         mixed::schur_block<float, kInFlight>(p, Y, Wbuf, Hpp, bp, bl, S, bs,
                                              lambda_override ? *lambda_override : lm->lambda, blk, threadIdx.x, red);
