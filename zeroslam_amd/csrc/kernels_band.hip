@@ -160,7 +160,8 @@ struct band_lds {
     static constexpr int kSR = kDB + 2 * kP * kP;          // [2][kP][W]
     static constexpr int kSRb = kSR + 2 * kP * W;          // [2][kP]
     static constexpr int kRH = kSRb + 2 * kP;              // [W]
-    static constexpr int kFactor = kRH + W;
+    static constexpr int kSink = kRH + W;                  // [64]: wave 0's lanes with nothing to store
+    static constexpr int kFactor = kSink + 64;
     static constexpr int kRingOff = 0;                     // [2][kB][kRing]
     static constexpr int kSlot = 2 * kB * kRing;           // [2][kRing]
     static constexpr int kSub = kSlot + 2 * kRing;
@@ -170,13 +171,21 @@ struct band_lds {
 // Entry (r, c), r >= c, of the system being factorised: S's lower triangle
 // inside the band, the floor on the diagonal, identity rows past n.  The load
 // address is always valid (element 0 when the entry is not in S).
+// Both are written as selects over values computed for every lane: wave 0 runs
+// them for ten items per lane per panel with no other wave on its SIMD, and
+// branches around the multiply or the early returns (exec-mask updates per item)
+// cost it several times the arithmetic.
 __device__ __forceinline__ int64_t entry_index(int n, int wb, int r, int c) {
-    return (r < n && r - c < wb) ? static_cast<int64_t>(r) * n + c : 0;
+    // r, c >= 0: one unsigned 32 x 32 -> 64 multiply-add
+    const uint64_t idx = static_cast<uint64_t>(static_cast<uint32_t>(r)) * static_cast<uint32_t>(n) +
+                         static_cast<uint32_t>(c);
+    const bool in = (r < n) & (r - c < wb);
+    return in ? static_cast<int64_t>(idx) : 0;
 }
 __device__ __forceinline__ double entry_finish(double raw, int n, int wb, double lift, int r, int c) {
-    if (r >= n) return r == c ? 1.0 : 0.0;
-    if (r - c >= wb) return 0.0;
-    return r == c ? raw + lift : raw;
+    const bool diag = r == c;
+    const double in_band = r - c < wb ? (diag ? raw + lift : raw) : 0.0;
+    return r < n ? in_band : (diag ? 1.0 : 0.0);
 }
 
 // Rows of the substitutions: n rounded up to whole blocks (factor rows past n
@@ -339,6 +348,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     double* SRb = smem + lay::kSRb;
     double* RH = smem + lay::kRH;
     const int tid = threadIdx.x, lane = tid & 63;
+    double* sink = smem + lay::kSink + lane;
     const int wave = uniform(tid >> 6);
     const int np8 = (n + kP - 1) / kP * kP, NP = np8 / kP;
     double* Lc = Lband;
@@ -587,6 +597,8 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
                 a1[kk] = v1 ? t : 0.0;
             });
             const double rb0 = RH[q0], rb1 = RH[q1];
+            // the value refilling lane j < kP's position (the row entering there)
+            const double refill = SRb[kb * kP + (lane & (kP - 1))];
             double b0 = rb0, b1 = v1 ? rb1 : 0.0;
             // lane j < kP ends with the panel's column j: 1 / sqrt(d_j), d_j and
             // the forward value; entries above the diagonal (lanes < j of
@@ -621,21 +633,19 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
                 lmax = dmax_nan(lmax, fabs(l));
                 lmin = fmin(lmin, fabs(l));
             }
+            // Every lane stores every value: those with nothing to store write
+            // their slot of the sink, so no store waits on an exec-mask update.
             double* L = LP + kb * kP * W;
             double* D = DB + kb * kP * kP;
             static_for<kP>([&](auto KK) __attribute__((always_inline)) {
                 constexpr int kk = KK;
                 L[kk * W + q0] = lane < kP ? 0.0 : a0[kk];
-                if (v1) L[kk * W + q1] = a1[kk];
-                if (lane < kP) D[kk * kP + lane] = (kk == lane) ? yv : a0[kk];
+                *(v1 ? L + kk * W + q1 : sink) = a1[kk];
+                *(lane < kP ? D + kk * kP + lane : sink) = (kk == lane) ? yv : a0[kk];
             });
-            if (v1) RH[q1] = b1;
-            if (lane >= kP) {
-                RH[q0] = b0;
-            } else {
-                if (c0 + lane < n) x[c0 + lane] = bv;
-                RH[q0] = (k + 2 <= NP) ? SRb[kb * kP + lane] : 0.0;
-            }
+            *(v1 ? RH + q1 : sink) = b1;
+            RH[q0] = lane >= kP ? b0 : ((k + 2 <= NP) ? refill : 0.0);
+            if (lane < kP && c0 + lane < n) x[c0 + lane] = bv;
         } else if (k > 0) {
             const int Xc = pp0 >> 4;
             update(LP + (kb ^ 1) * kP * W, [&](int I, int J) __attribute__((always_inline)) { return I != Xc && J != Xc; });
