@@ -19,6 +19,12 @@ CASES = [
     (1, dict(n_frames=21, n_points=1500)),               # n = 126, not a multiple of the panel width
     (1, dict(n_frames=100)),                             # the headline workload: band 72, n = 600
     (3, dict(n_frames=50, n_points=3000)),
+    # longer tracks widen the band: windows of 96, 112 and 128 positions (the
+    # four-wave factorisation; the cases above take windows of 64 and 80)
+    (1, dict(n_frames=60, track_max=13)),                # band 78
+    (1, dict(n_frames=60, track_max=16)),                # band 96
+    (1, dict(n_frames=60, track_max=18)),                # band 108 (window 128)
+    (1, dict(n_frames=60, track_max=20)),                # band 120: past the limit, the tiled path
 ]
 
 

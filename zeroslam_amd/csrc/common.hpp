@@ -383,7 +383,9 @@ void launch_cholesky_resolve64(int32_t n, const dev_work64& w, const double* b, 
 // automatic choice takes it when the system has at least kBandMinRowsPerWindow
 // windows of rows (a narrow band, not a dense matrix).
 constexpr int kBandPanel = 8;
-constexpr int kBandMaxW = 121;
+// The substitutions hold 128 unknowns and refill the 16 slots a block of steps
+// frees at the block's end, so a factor row may reach at most 128 - 15 columns.
+constexpr int kBandMaxW = 113;
 constexpr int kBandMinRowsPerWindow = 3;
 // Window (a multiple of 16, 64 .. 128) for a band, 0 if the band is too wide.
 int band_window(int32_t band);

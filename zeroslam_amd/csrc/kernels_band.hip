@@ -72,8 +72,26 @@ namespace {
 typedef double band_d4 __attribute__((ext_vector_type(4)));
 
 constexpr int kP = kBandPanel;  // panel columns
-constexpr int kThreads = 256;   // wave 0: panels and recurrences; waves 1-3: tiles and loads
-constexpr int kTileLanes = kThreads - 64;
+constexpr int kThreads = 256;   // the substitutions: wave 0 the recurrence, waves 1-3 the loads
+
+// Wave roles of the factorisation for a window of W positions.  Up to W = 80,
+// eight waves: wave 0 the panels, wave 4 (on wave 0's SIMD) half of the rows
+// entering the window, waves 1-3 and 5-7 the tiles.  Wider windows keep four
+// waves (wave 0 the panels and all entering rows, waves 1-3 the tiles): at two
+// waves per SIMD their registers would spill.
+template <int W>
+struct band_roles {
+    static constexpr bool kWide = W > 80;
+    static constexpr int kThreads = kWide ? 256 : 512;
+    static constexpr int kTileWaves = kWide ? 3 : 6;
+    static constexpr int kTileLanes = 64 * kTileWaves;
+    static constexpr int kStagers = kWide ? 1 : 2;
+    static constexpr int kStageWave = kWide ? -1 : 4;  // the second staging wave
+    // tile wave index of a wave (-1: none)
+    static __device__ __forceinline__ int tile_wave(int wave) {
+        return (wave == 0 || wave == kStageWave) ? -1 : ((kStageWave < 0 || wave < kStageWave) ? wave - 1 : wave - 2);
+    }
+};
 constexpr int kLoaders = 3;     // loading waves of the substitutions
 constexpr int kRing = 128;      // unknowns held by wave 0; row stride of the factor buffers
 constexpr int kB = 16;          // substitution steps per block
@@ -127,18 +145,19 @@ constexpr int window_for(int band) {
          : band + kP - 1 <= 112 ? 112 : band + kP - 1 <= 128 ? 128 : 0;
 }
 
-// Tiles (I, J), I >= J, of nb position blocks, dealt to the three tile waves by
-// (I + J) mod 3.
-constexpr int tiles_of_wave(int nb, int wv) {
+// Tiles (I, J), I >= J, of nb position blocks, dealt to the nt tile waves by
+// (I + J) mod nt.
+constexpr int tiles_of_wave(int nb, int nt, int wv) {
     int c = 0;
     for (int a = 0; a < nb; ++a)
         for (int b = 0; b <= a; ++b)
-            if ((a + b) % 3 == wv) ++c;
+            if ((a + b) % nt == wv) ++c;
     return c;
 }
-constexpr int max_tiles(int nb) {
-    const int c0 = tiles_of_wave(nb, 0), c1 = tiles_of_wave(nb, 1), c2 = tiles_of_wave(nb, 2);
-    return c0 > c1 ? (c0 > c2 ? c0 : c2) : (c1 > c2 ? c1 : c2);
+constexpr int max_tiles(int nb, int nt) {
+    int m = 0;
+    for (int wv = 0; wv < nt; ++wv) m = tiles_of_wave(nb, nt, wv) > m ? tiles_of_wave(nb, nt, wv) : m;
+    return m;
 }
 
 // Factor entries one loading lane holds per block, and blocks in flight (the
@@ -212,7 +231,7 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
     // row of step t of block m
     auto row_of = [&](int m, int t) __attribute__((always_inline)) { return kFwd ? m * kB + t : np - 1 - m * kB - t; };
     // entries past W stay zero
-    for (int i = static_cast<int>(threadIdx.x); i < 2 * kB * kRing; i += kThreads) ring[i] = 0.0;
+    for (int i = static_cast<int>(threadIdx.x); i < 2 * kB * kRing; i += static_cast<int>(blockDim.x)) ring[i] = 0.0;
     __syncthreads();
 
     const bool loader = wave >= 1 && wave <= kLoaders;
@@ -326,20 +345,23 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
 }
 
 template <int W>
-__global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const double* A, const double* b, double* x,
+__global__ void __launch_bounds__(band_roles<W>::kThreads) k_band_factor(int n, int wb, const double* A, const double* b, double* x,
                                                           double* Lband, double floor_rel, double* eps_out,
                                                           int32_t* status, lm_state* lm, int gated) {
     if (gated && lm->done) return;
+    using roles = band_roles<W>;
+    constexpr int kFactorThreads = roles::kThreads, kTileWaves = roles::kTileWaves, kTileLanes = roles::kTileLanes;
+    constexpr int kStageWave = roles::kStageWave;
     constexpr int NB = W / 16;
-    constexpr int kMaxT = max_tiles(NB);
-    constexpr int kPre = kP * W / 64;                               // entering-row items per lane of wave 0
+    constexpr int kMaxT = max_tiles(NB, kTileWaves);
+    constexpr int kPre = kP * W / (64 * roles::kStagers);           // entering-row items per lane of a staging wave
     constexpr int kCP = (kP * W + kTileLanes - 1) / kTileLanes;     // factor-copy items per tile lane
     using lay = band_lds<W>;
     static_assert(W % 16 == 0 && W >= 64 && W <= kRing, "window of 64 .. 128 positions in blocks of 16");
-    static_assert(kP * W % 64 == 0, "entering rows split evenly over wave 0");
+    static_assert(kP * W % (64 * roles::kStagers) == 0, "entering rows split evenly over the staging waves");
     __shared__ double smem[lay::kTotal];
-    __shared__ double red[kThreads];
-    __shared__ int tab[3][kMaxT];
+    __shared__ double red[kFactorThreads];
+    __shared__ int tab[kTileWaves][kMaxT];
     __shared__ int sflag;
     double* PB = smem + lay::kPB;
     double* LP = smem + lay::kLP;
@@ -350,27 +372,29 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     const int tid = threadIdx.x, lane = tid & 63;
     double* sink = smem + lay::kSink + lane;
     const int wave = uniform(tid >> 6);
+    const int tw = roles::tile_wave(wave);
+    const bool stager = wave == 0 || wave == kStageWave;
     const int np8 = (n + kP - 1) / kP * kP, NP = np8 / kP;
     double* Lc = Lband;
     double* Lr = Lband + static_cast<int64_t>(sub_rows(n)) * kRing;
 
     if (tid == 0) {
-        int c[3] = {0, 0, 0};
-        for (int wv = 0; wv < 3; ++wv)
+        int c[kTileWaves] = {};
+        for (int wv = 0; wv < kTileWaves; ++wv)
             for (int s = 0; s < kMaxT; ++s) tab[wv][s] = -1;
         for (int a = 0; a < NB; ++a)
             for (int bb = 0; bb <= a; ++bb) {
-                const int wv = (a + bb) % 3;
+                const int wv = (a + bb) % kTileWaves;
                 tab[wv][c[wv]++] = (a << 8) | bb;
             }
     }
     // max |S_ii| (NaN never wins), the floor and the pivot threshold
     {
         double m = 0.0;
-        for (int i = tid; i < n; i += kThreads) m = dmax_nan(m, fabs(A[static_cast<int64_t>(i) * n + i]));
+        for (int i = tid; i < n; i += kFactorThreads) m = dmax_nan(m, fabs(A[static_cast<int64_t>(i) * n + i]));
         red[tid] = m;
         __syncthreads();
-        for (int s = kThreads / 2; s > 0; s >>= 1) {
+        for (int s = kFactorThreads / 2; s > 0; s >>= 1) {
             if (tid < s) red[tid] = dmax_nan(red[tid], red[tid + s]);
             __syncthreads();
         }
@@ -393,7 +417,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     double pre[kPre], pre_b = 0.0;
     static_for<kMaxT>([&](auto S) __attribute__((always_inline)) {
         constexpr int s = S;
-        const int t = wave > 0 ? uniform(tab[wave - 1][s]) : -1;
+        const int t = tw >= 0 ? uniform(tab[tw][s]) : -1;
         tI[s] = t < 0 ? -1 : (t >> 8);
         tJ[s] = t < 0 ? -1 : (t & 255);
         oI[s] = kq * W + 16 * (t < 0 ? 0 : tI[s]) + r16;
@@ -402,13 +426,14 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     });
     static_for<kCP>([&](auto T) __attribute__((always_inline)) {
         constexpr int t = T;
-        const int idx = (tid - 64) + t * kTileLanes;
+        const int idx = tw >= 0 ? tw * 64 + lane + t * kTileLanes : -1;
         c_kk[t] = (idx >= 0 && idx < kP * W) ? idx / W : -1;
         c_q[t] = (idx >= 0 && idx < kP * W) ? idx - (idx / W) * W : 0;
     });
     static_for<kPre>([&](auto T) __attribute__((always_inline)) {
         constexpr int t = T;
-        const int idx = lane + 64 * t;
+        // the staging waves take alternate groups of 64 items
+        const int idx = lane + 64 * (roles::kStagers * t + (wave == kStageWave ? 1 : 0));
         e_i[t] = idx / W;
         e_q[t] = idx - (idx / W) * W;
     });
@@ -443,7 +468,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
             entering(m, t, npos, r, c);
             S[e_i[t] * W + e_q[t]] = entry_finish(pre[t], n, wb, lift, r, c);
         });
-        if (lane < kP) SRb[(m & 1) * kP + lane] = (kP * m + W + lane < n) ? pre_b : 0.0;
+        if (wave == 0 && lane < kP) SRb[(m & 1) * kP + lane] = (kP * m + W + lane < n) ? pre_b : 0.0;
     };
     // tile update: acc -= L_I L_J^T over the panel's kP columns (lane l: operand
     // row l % 16 at depth l / 16; result element v at row l / 16 + 4 v, column
@@ -544,8 +569,9 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     // the window of panel 0 (rows 0 .. W - 1 at positions 0 .. W - 1), its first
     // panel's columns, the right-hand side; the rows entering with panel 0, and
     // the loads of those entering with panel 1
-    if (wave == 0) {
-        for (int q = lane; q < W; q += 64) RH[q] = q < n ? b[q] : 0.0;
+    if (stager) {
+        if (wave == 0)
+            for (int q = lane; q < W; q += 64) RH[q] = q < n ? b[q] : 0.0;
         if (NP >= 2) {
             prefetch(0, kP % W);
             stage(0, kP % W);
@@ -646,7 +672,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
             *(v1 ? RH + q1 : sink) = b1;
             RH[q0] = lane >= kP ? b0 : ((k + 2 <= NP) ? refill : 0.0);
             if (lane < kP && c0 + lane < n) x[c0 + lane] = bv;
-        } else if (k > 0) {
+        } else if (k > 0 && tw >= 0) {
             const int Xc = pp0 >> 4;
             update(LP + (kb ^ 1) * kP * W, [&](int I, int J) __attribute__((always_inline)) { return I != Xc && J != Xc; });
             copy_panel(k - 1, ppp);
@@ -654,14 +680,14 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
         __syncthreads();
         // ---- phase B
         if (k + 1 < NP) {
-            if (wave == 0) {
+            if (stager) {
                 int np2 = ppn + kP;
                 np2 -= np2 >= W ? W : 0;
                 int np3 = np2 + kP;
                 np3 -= np3 >= W ? W : 0;
                 if (k + 1 <= NP - 2) stage(k + 1, np2);
                 if (k + 2 <= NP - 2) prefetch(k + 2, np3);
-            } else {
+            } else if (tw >= 0) {
                 const int Xc = pp0 >> 4, hc = (pp0 >> 3) & 1;
                 const int Xn = ppn >> 4, hn = (ppn >> 3) & 1;
                 reload(SR + kb * kP * W, Xc, hc, pp0);
@@ -676,7 +702,7 @@ __global__ void __launch_bounds__(kThreads) k_band_factor(int n, int wb, const d
     {
         int ppl = pp0 - kP;
         if (ppl < 0) ppl += W;
-        if (wave > 0) copy_panel(NP - 1, ppl);
+        if (tw >= 0) copy_panel(NP - 1, ppl);
     }
     if (wave == 0) {
         // lanes 0 .. kP - 1 hold the pivot rules of their columns
@@ -729,7 +755,10 @@ __global__ void __launch_bounds__(kThreads) k_band_solve(int n, const double* Lb
 
 }  // namespace
 
-int band_window(int32_t band) { return band > 0 ? window_for(band) : 0; }
+static_assert(kBandMaxW + kB - 1 <= kRing, "a block's steps reach only unknowns held in the ring");
+static_assert(window_for(kBandMaxW) != 0, "the widest band has a window");
+
+int band_window(int32_t band) { return (band > 0 && band <= kBandMaxW) ? window_for(band) : 0; }
 
 size_t band_buffer_doubles(int32_t n, int32_t band) {
     const int W = band_window(band);
@@ -750,7 +779,7 @@ bool band_path(int32_t n, const dev_work64& w, bool exact) {
 void launch_chol_band(int32_t n, const dev_work64& w, hipStream_t s, bool gated, double floor_rel) {
     const int g = gated ? 1 : 0;
 #define ZB_BAND_FACTOR(WW)                                                                                       \
-    k_band_factor<WW><<<1, kThreads, 0, s>>>(n, w.band, w.S, w.bs, w.dxp, w.Lband, floor_rel, w.chol_scale,  \
+    k_band_factor<WW><<<1, band_roles<WW>::kThreads, 0, s>>>(n, w.band, w.S, w.bs, w.dxp, w.Lband, floor_rel, w.chol_scale,  \
                                              w.chol_status, w.lm, g)
     switch (band_window(w.band)) {
     case 64: ZB_BAND_FACTOR(64); break;
