@@ -7,14 +7,16 @@
 // few neighbours only).  Cholesky fill stays inside that profile, so the factor
 // is a band of the same width.
 //
-// Factorisation (k_band_factor, one workgroup of four waves).  The trailing
-// submatrix lives in a window of W rows (W >= band + 7, a multiple of 16): row r
-// sits at window position r mod W, so a row keeps its position from the panel
-// that brings it in until the panel that eliminates it.  The window's symmetric
-// W x W matrix is held in registers as the tiles (I, J), I >= J, of 16 x 16
-// positions, each owned by one of waves 1-3 (tile (I, J) by wave 1 + (I + J) mod
-// 3, which spreads every block row's tiles over the three).  Per panel of 8
-// columns c0 .. c0 + 7 (window positions pp0 .. pp0 + 7):
+// Factorisation (k_band_factor, one workgroup: band_roles gives its waves).  The
+// trailing submatrix lives in a window of W rows (W >= band + 7, a multiple of
+// 16): row r sits at window position r mod W, so a row keeps its position from
+// the panel that brings it in until the panel that eliminates it.  The window's
+// symmetric W x W matrix is held in registers as the tiles (I, J), I >= J, of
+// 16 x 16 positions, each owned by one tile wave (tile (I, J) by tile wave
+// (I + J) mod the tile-wave count, which spreads every block row's tiles over
+// them).  Below, "waves 1-3" stands for the tile waves and "wave 0" for the
+// staging waves where entering rows are staged.  Per panel of 8 columns
+// c0 .. c0 + 7 (window positions pp0 .. pp0 + 7):
 //   phase A  wave 0 reads the panel's columns for every window row from LDS (PB:
 //            lane l holds rows c0 + l and c0 + 64 + l) and factors them
 //            right-looking: pivot d (readlane), y = 1/sqrt(d), L_rj = a_rj y,
