@@ -183,7 +183,11 @@ struct lm_state {
     int32_t done, need_linearize, need_rollback, solve_ok, step_applied, executed_iterations;
     int32_t lin_count;       // linearisation sequences ended (the per-graph path ends one per attempt)
     int32_t linearisations;  // linearisations performed in this solve (make_hessian calls)
-    int32_t pad[2];
+    // 1 on the reassociating paths: a step is accepted only if it lowers the
+    // cost, besides rho > 0 (the two agree whenever the step solves the damped
+    // system well enough that the predicted decrease is positive)
+    int32_t require_descent;
+    int32_t pad;
 };
 
 // Working buffers of the binary64 path.
@@ -328,7 +332,7 @@ constexpr int kBatchMaxN = 256;
 struct batch_window {
     dev_problem p;
     dev_work64 w;
-    int32_t iterations = 0, use_relative = 0, refine_steps = 0, pad = 0;
+    int32_t iterations = 0, use_relative = 0, refine_steps = 0, require_descent = 1;
     double floor_rel = 0.0;
 };
 // Phases of the batched solver's device clock.

@@ -942,6 +942,7 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
     std::memset(lm_pinned, 0, sizeof(lm_state));
     lm_pinned->iterations = iterations;
     lm_pinned->use_relative = use_rel ? 1 : 0;
+    lm_pinned->require_descent = exact ? 0 : 1;
     if ((e = hipMemcpyAsync(w.lm, lm_pinned, sizeof(lm_state), hipMemcpyHostToDevice, stream)) != hipSuccess)
         return hip_fail(e, "upload controller");
     // every point fixed (map::optimise(n, true, ...)): the reference-order solve

@@ -148,6 +148,7 @@ __global__ void __launch_bounds__(256) k_batch_begin(const batch_window* wins, u
         *lm = lm_state{};
         lm->iterations = bw.iterations;
         lm->use_relative = bw.use_relative;
+        lm->require_descent = bw.require_descent;
         lm->need_linearize = 1;
     }
     const double c = grid_sum(

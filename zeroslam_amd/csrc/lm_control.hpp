@@ -107,13 +107,19 @@ __device__ inline void lm_after_solve(lm_state* lm) {
 }
 
 // ---- accept / reject, factor_graph.hpp:566-605 ---------------------------------
+// The reference accepts a step when rho = (chi2 - chi2_new) / scale > 0.  With an
+// exact solve of the damped system the predicted decrease (scale) is positive, so
+// that means the cost fell.  The reassociating paths (require_descent) also ask
+// for chi2_new < chi2: with binary32 blocks and almost no damping their step can
+// miss the system by enough to make scale negative, and the reference's test
+// alone would then accept a step that raises the cost.
 __device__ inline void lm_decide(lm_state* lm) {
     if (lm->done || !lm->step_applied) return;
     const double temp_chi = lm->chi2_new;
     const double rho = (lm->chi2 - temp_chi) / lm->scale;
     lm->rho = rho;
     bool good = false;
-    if ((rho > 0) && isfinite(temp_chi)) {
+    if ((rho > 0) && isfinite(temp_chi) && (!lm->require_descent || temp_chi < lm->chi2)) {
         double alpha = 1.0 - cr_cube(2.0 * rho - 1.0);  // math::pow(x, 3.0), correctly rounded
         alpha = (2.0 / 3.0 < alpha) ? 2.0 / 3.0 : alpha;
         const double scale_factor = (1.0 / 3.0 < alpha) ? alpha : 1.0 / 3.0;
