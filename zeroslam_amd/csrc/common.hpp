@@ -274,11 +274,15 @@ constexpr int kLinBlock = 128;
 // `exact`: reproduce the reference's summation orders (sequential sums over all
 // observations / slots, hash-map orders in the Schur products); otherwise use
 // fixed-shape parallel reductions.
+// decide: the accept / reject decision follows in the same launch as the sum
+// (the LM attempt's sequence; a launch of its own on the exact path).
 void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target /*0 chi2, 1 chi2_new*/,
-                   bool only_if_step, bool exact = false);
+                   bool only_if_step, bool exact = false, bool decide = false);
 // exact: per-pose sums in insertion order (the reference's); otherwise in the
-// fixed chunks of the mixed path.
-void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact = true);
+// fixed chunks of the mixed path.  end_attempt (gated runs): the attempt's
+// controller reset follows the linearisation's bookkeeping in one launch.
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact = true,
+                        bool end_attempt = false);
 void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override,
                              bool exact = false);
@@ -296,7 +300,8 @@ void launch_backsub_only64(const dev_problem& p, const dev_work64& w, hipStream_
 void launch_gn_apply64(const dev_problem& p, const dev_work64& w, hipStream_t s);
 
 // ---- mixed path: binary32 linearisation and Schur (kernels_fp32.hip) -----------
-void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated);
+void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated,
+                            bool end_attempt = false);
 void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override);
 // ---- mixed path: binary64 arithmetic over the binary32 blocks (kernels_fp64.hip)
 void launch_init_lm_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s);

@@ -976,12 +976,12 @@ int32_t zsba_graph::run(int32_t iterations, int32_t use_rel, int32_t precision) 
             zb::launch_after_solve64(w, stream);
             zb::launch_backsub_update64(dp, w, stream, exact);
         }
-        zb::launch_cost64(dp, w, stream, 1, true, exact);
-        zb::launch_decide64(w, stream);
+        // the decision rides on the cost's final sum, the controller reset on the
+        // linearisation's bookkeeping (the same state updates, in the same order)
+        zb::launch_cost64(dp, w, stream, 1, true, exact, true);
         zb::launch_rollback64(dp, w, stream);
-        if (mixed) zb::launch_linearise_mixed(dp, w, stream, true);
-        else zb::launch_linearise64(dp, w, stream, true, exact);
-        zb::launch_end_attempt64(w, stream);
+        if (mixed) zb::launch_linearise_mixed(dp, w, stream, true, true);
+        else zb::launch_linearise64(dp, w, stream, true, exact, true);
     };
     hipGraphExec_t att = nullptr;
     if (max_attempts > 0 && !w.timer && opt.attempt_graph >= 0 && !att_capture_failed) {

@@ -79,6 +79,10 @@ __global__ void __launch_bounds__(256) k_reduce_pose_f(dev_problem p, const doub
 }
 
 __global__ void k_linearise_done_f(lm_state* lm) { lm_linearise_done(lm); }
+__global__ void k_linearise_done_end_f(lm_state* lm) {
+    lm_linearise_done(lm);
+    lm_end_attempt(lm);
+}
 
 __global__ void __launch_bounds__(kBlock) k_damp_invert_f(dev_problem p, const float* Hll, float* Hinv, const lm_state* lm,
                                                           const double* lambda_override, const uint8_t* frozen) {
@@ -124,7 +128,7 @@ constexpr int kSchurSmallGrid = 2048;
 
 }  // namespace
 
-void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated) {
+void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool end_attempt) {
     if (p.n_lblk > 0) {
         float* part_h = static_cast<float*>(w.lin_part_h);
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
@@ -137,7 +141,8 @@ void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream
                                                                  gated ? 1 : 0);
             ZB_TIME_END(w, s);
         }
-        if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);
+        if (gated && end_attempt) k_linearise_done_end_f<<<1, 1, 0, s>>>(w.lm);
+        else if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);
         return;
     }
     if (p.nl > 0) {
@@ -153,7 +158,8 @@ void launch_linearise_mixed(const dev_problem& p, const dev_work64& w, hipStream
         k_reduce_pose_f<<<grid_of(27L * p.np, 256), 256, 0, s>>>(p, w.pose_part, w.Hpp_f, w.bp_m, w.lm, gated ? 1 : 0);
         ZB_TIME_END(w, s);
     }
-    if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);
+    if (gated && end_attempt) k_linearise_done_end_f<<<1, 1, 0, s>>>(w.lm);
+    else if (gated) k_linearise_done_f<<<1, 1, 0, s>>>(w.lm);
 }
 
 void launch_reduced_system_mixed(const dev_problem& p, const dev_work64& w, hipStream_t s, const double* lambda_override) {

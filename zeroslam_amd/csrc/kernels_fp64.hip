@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(64) k_seqsum(const double* a, int64_t n, lm_st
 }
 
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int n, double* out, lm_state* lm,
-                                                       int target, int gate) {
+                                                       int target, int gate, int decide = 0) {
     if (gate == 1 && lm->done) return;
     if (gate == 2 && (lm->done || !lm->step_applied)) return;
     __shared__ double sh[1024];
@@ -168,6 +168,7 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, in
         else if (target == 1) lm->chi2_new = total;
         else if (target == 2) lm->scale = total + 1e-3;  // factor_graph.hpp:556-560
         else if (out) *out = total;
+        if (decide) lm_decide(lm);  // the attempt's accept / reject on the new cost
     }
 }
 
@@ -325,6 +326,10 @@ __global__ void __launch_bounds__(256) k_reduce_pose_chunks(dev_problem p, const
 }
 
 __global__ void k_linearise_done(lm_state* lm) { lm_linearise_done(lm); }
+__global__ void k_linearise_done_end(lm_state* lm) {
+    lm_linearise_done(lm);
+    lm_end_attempt(lm);
+}
 
 // ---- LM initialisation, factor_graph.hpp:511-522 -----------------------------
 template <class TS>
@@ -873,7 +878,8 @@ inline int cost_grid(const dev_problem& p) {
 }  // namespace
 
 // target: 0 -> lm->chi2, 1 -> lm->chi2_new (gated on an applied step), 3 -> partial[kPartials]
-void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step, bool exact) {
+void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int target, bool only_if_step, bool exact,
+                   bool decide) {
     const int g = cost_grid(p);
     const int gate = only_if_step ? 2 : (target == 3 ? 0 : 1);
     ZB_TIME_BEGIN(w, K_COST, s);
@@ -882,9 +888,10 @@ void launch_cost64(const dev_problem& p, const dev_work64& w, hipStream_t s, int
         k_seqsum<<<1, 64, 0, s>>>(w.cbuf, p.no, w.lm, target, gate, w.partial + kPartials);
     } else {
         k_cost<<<g, kBlock, 0, s>>>(p, w.pose, w.point, w.residual, w.partial, w.lm, gate, 1, nullptr);
-        k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate);
+        k_sum_partials<<<1, 1024, 0, s>>>(w.partial, g, w.partial + kPartials, w.lm, target, gate, decide ? 1 : 0);
     }
     ZB_TIME_END(w, s);
+    if (exact && decide) launch_decide64(w, s);
 }
 
 // chi2 of the stored residuals (get_current_chi(false)), summed in insertion order.
@@ -897,7 +904,8 @@ void launch_math_probe(int fn, int n, const double* in, double* out, hipStream_t
     if (n > 0) k_math_probe<<<grid_for(n, kBlock), kBlock, 0, s>>>(fn, n, in, out);
 }
 
-void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact) {
+void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s, bool gated, bool exact,
+                        bool end_attempt) {
     if (p.nl > 0) {
         ZB_TIME_BEGIN(w, K_LINEARISE, s);
         k_linearise_points<<<grid_for(p.nl, kBlock), kBlock, 0, s>>>(p, w.pose, w.point, w.pose_terms, w.Hll, w.bl, w.W,
@@ -916,7 +924,8 @@ void launch_linearise64(const dev_problem& p, const dev_work64& w, hipStream_t s
         }
         ZB_TIME_END(w, s);
     }
-    if (gated) k_linearise_done<<<1, 1, 0, s>>>(w.lm);
+    if (gated && end_attempt) k_linearise_done_end<<<1, 1, 0, s>>>(w.lm);
+    else if (gated) k_linearise_done<<<1, 1, 0, s>>>(w.lm);
 }
 
 void launch_init_lm64(const dev_problem& p, const dev_work64& w, hipStream_t s) {
