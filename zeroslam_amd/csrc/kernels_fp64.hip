@@ -408,106 +408,27 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Reduced camera system, one wave per 6x6 block (row >= col):
+// Reduced camera system, one workgroup per 6x6 block (row >= col):
 //   S_ik = [i == k] (H_pp,i + lambda I) - sum_l Y_il W_kl^T
 //   b_s,i = b_p,i - sum_l Y_il b_l
-// (factor_graph.hpp:768-781 with matrix_sparse_block.hpp:158-196).
-// Each lane adds its contributions c, c + 64, ... in order, kInFlight of them
-// loaded before they are added; each product entry is formed from zero in
-// ascending k and added to the lane's sum, so kInFlight changes the registers
-// in use, not the result.
+// (factor_graph.hpp:768-781 with matrix_sparse_block.hpp:158-196), by the
+// binary64 instance of the mixed path's block code (mixed::schur_block: four
+// waves over the block's contributions, a reduce-scatter per wave, the waves'
+// totals added in order), which the batched binary64 windows use too.
+// kInFlight changes the registers in use, not the result.  Blocks are numbered
+// row by row, so the XCD chunks of 64 consecutive blocks share their rows' Y.
 template <int kInFlight>
-__global__ void __launch_bounds__(256) k_schur(dev_problem p, const double* Y, const double* Wbuf, const double* Hpp,
-                                               const double* bp, const double* bl, double* S, double* bs,
-                                               const lm_state* lm, const double* lambda_override) {
+__global__ void __launch_bounds__(mixed::kSchurThreads) k_schur(dev_problem p, const double* Y, const double* Wbuf,
+                                                                 const double* Hpp, const double* bp, const double* bl,
+                                                                 double* S, double* bs, const lm_state* lm,
+                                                                 const double* lambda_override) {
     if (!lambda_override && lm->done) return;
-    const int blk = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (blk >= p.nblk) return;
-    const int row = p.blk_row[blk], col = p.blk_col[blk];
-    const bool diag = row == col;
-    double acc[36];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) acc[i] = 0.0;
-    double accb[6] = {0, 0, 0, 0, 0, 0};
-    const int64_t c0 = p.blk_start[blk], c1 = p.blk_start[blk + 1];
-    auto fetch = [&](int64_t c, double (&Ya)[18], double (&Wc)[18], double (&b3)[3]) {
-        const int2 ab = p.contrib[c];
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Ya[i] = Y[18L * ab.x + i];
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Wc[i] = Wbuf[18L * ab.y + i];
-        b3[0] = b3[1] = b3[2] = 0.0;
-        if (diag) {
-            const int l = p.pair_point[ab.x];
-            b3[0] = bl[3L * l];
-            b3[1] = bl[3L * l + 1];
-            b3[2] = bl[3L * l + 2];
-        }
-    };
-    auto add = [&](const double (&Ya)[18], const double (&Wc)[18], const double (&b3)[3]) {
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-#pragma unroll
-            for (int cc = 0; cc < 6; ++cc) {
-                double t = 0.0;
-                t = t + Ya[r * 3] * Wc[cc * 3];
-                t = t + Ya[r * 3 + 1] * Wc[cc * 3 + 1];
-                t = t + Ya[r * 3 + 2] * Wc[cc * 3 + 2];
-                acc[r * 6 + cc] = acc[r * 6 + cc] + t;
-            }
-        if (diag) {
-#pragma unroll
-            for (int r = 0; r < 6; ++r) {
-                double t = 0.0;
-                t = t + Ya[r * 3] * b3[0];
-                t = t + Ya[r * 3 + 1] * b3[1];
-                t = t + Ya[r * 3 + 2] * b3[2];
-                accb[r] = accb[r] + t;
-            }
-        }
-    };
-    int64_t c = c0 + lane;
-    if (kInFlight == 2) {
-        for (; c + 64 < c1; c += 128) {
-            double Ya0[18], Wc0[18], Ya1[18], Wc1[18], b0[3], b1[3];
-            fetch(c, Ya0, Wc0, b0);
-            fetch(c + 64, Ya1, Wc1, b1);
-            add(Ya0, Wc0, b0);
-            add(Ya1, Wc1, b1);
-        }
-    }
-    for (; c < c1; c += 64) {
-        double Ya0[18], Wc0[18], b0[3];
-        fetch(c, Ya0, Wc0, b0);
-        add(Ya0, Wc0, b0);
-    }
-    const double lam = lambda_override ? *lambda_override : lm->lambda;
-    const int64_t n = p.n;
-    double* Sblk = S + (6L * row) * n + 6L * col;
-#pragma unroll
-    for (int i = 0; i < 36; ++i) {
-        const double tot = wave_sum(acc[i]);
-        const int r = i / 6, c = i % 6;
-        if (lane == 0) {
-            if (diag) {
-                if (c <= r) {
-                    double base = Hpp[21L * row + r * (r + 1) / 2 + c];
-                    if (r == c) base += lam;
-                    Sblk[r * n + c] = base - tot;
-                }
-            } else {
-                Sblk[r * n + c] = 0.0 - tot;
-            }
-        }
-    }
-    if (diag) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            const double tot = wave_sum(accb[i]);
-            if (lane == 0) bs[6L * row + i] = bp[6L * row + i] - tot;
-        }
-    }
+    __shared__ mixed::schur_lds<double> red;
+    const int blk = xcd_block<64>(blockIdx.x, gridDim.x);
+    if (blk < p.nblk)
+        mixed::schur_block<double, kInFlight>(p, Y, Wbuf, Hpp, bp, bl, S, bs,
+                                              lambda_override ? *lambda_override : lm->lambda, blk,
+                                              static_cast<int>(threadIdx.x), red);
 }
 
 __device__ __forceinline__ int order_set(const lm_state* lm) { return lm->lin_count <= 1 ? 0 : 1; }
@@ -867,8 +788,8 @@ __global__ void k_math_probe(int fn, int n, const double* in, double* out) {
 
 inline int grid_for(int64_t n, int block) { return static_cast<int>((n + block - 1) / block); }
 
-// Below this many workgroups k_schur keeps two contributions in flight per
-// lane (more latency hidden); above it, one (higher occupancy).
+// Below this many reduced-system blocks k_schur keeps two contributions in
+// flight per lane (more latency hidden); above it, one (higher occupancy).
 constexpr int kSchurSmallGrid = 2048;
 inline int cost_grid(const dev_problem& p) {
     const int g = grid_for(p.no, kBlock);
@@ -1001,12 +922,12 @@ void launch_reduced_system64(const dev_problem& p, const dev_work64& w, hipStrea
             k_schur_exact<<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.S, w.lm);
             k_bs_exact<<<grid_for(64L * p.np, 256), 256, 0, s>>>(p, w.Y, w.bp, w.bl, w.bs, w.lm);
         } else {
-            if (grid_for(64L * p.nblk, 256) < kSchurSmallGrid)
-                k_schur<2><<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
-                                                                lambda_override);
+            if (p.nblk < kSchurSmallGrid)
+                k_schur<2><<<p.nblk, mixed::kSchurThreads, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+                                                                   lambda_override);
             else
-                k_schur<1><<<grid_for(64L * p.nblk, 256), 256, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
-                                                                lambda_override);
+                k_schur<1><<<p.nblk, mixed::kSchurThreads, 0, s>>>(p, w.Y, w.W, w.Hpp, w.bp, w.bl, w.S, w.bs, w.lm,
+                                                                   lambda_override);
         }
         ZB_TIME_END(w, s);
     }
