@@ -185,7 +185,8 @@ struct band_lds {
     static constexpr int kFactor = kSink + 64;
     static constexpr int kRingOff = 0;                     // [2][kB][kRing]
     static constexpr int kSlot = 2 * kB * kRing;           // [2][kRing]
-    static constexpr int kSub = kSlot + 2 * kRing;
+    static constexpr int kOut = kSlot + 2 * kRing;         // [2][kB]: a block's unknowns, stored a block later
+    static constexpr int kSub = kOut + 2 * kB;
     static constexpr int kTotal = kFactor > kSub ? kFactor : kSub;
 };
 
@@ -228,6 +229,7 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
     constexpr int kNI = sh::kNI, kK = sh::kK;
     double* ring = lds + band_lds<W>::kRingOff;
     double* rslot = lds + band_lds<W>::kSlot;
+    double* xo = lds + band_lds<W>::kOut;
     const int np = sub_rows(n);
     const int nblk = np / kB;
     // row of step t of block m
@@ -313,9 +315,14 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
         const double rs = rslot[(m & 1) * kRing + (kHi ? 64 : 0) + lane];
         if (kHi) u1 = freed ? rs : u1;
         else u0 = freed ? rs : u0;
+        // the block's unknowns go out through LDS: loader wave 1 stores them
+        // during the next block, so wave 0 has no global store to wait on
+        if (lane < kB) xo[(m & 1) * kB + lane] = xs;
+    };
+    auto store_out = [&](int m) __attribute__((always_inline)) {
         if (lane < kB) {
-            const int j = kFwd ? jb + lane : jb - lane;
-            if (j < n) out[j] = xs;
+            const int j = row_of(m, lane);
+            if (j < n) out[j] = xo[(m & 1) * kB + lane];
         }
     };
     for (int m0 = 0; m0 < nblk; m0 += kK) {
@@ -338,12 +345,14 @@ __device__ __forceinline__ void band_sub(int n, const double* F, const double* r
                 else block(m, l0, l1, std::integral_constant<bool, false>{});
             } else if (loader) {
                 constexpr int sn = (s + 1) % kK;
+                if (wave == 1 && m > 0) store_out(m - 1);
                 store(m + 1, ld[sn], lr[sn]);
                 issue(m + kK, ld[s], lr[s]);
             }
             __syncthreads();
         });
     }
+    if (wave == 1 && nblk > 0) store_out(nblk - 1);
 }
 
 template <int W>
