@@ -163,12 +163,14 @@ constexpr int max_tiles(int nb, int nt) {
 }
 
 // Factor entries one loading lane holds per block, and blocks in flight (the
-// loads outstanding stay within the 63 the wave's counter tracks).
+// loads outstanding stay within the 63 the wave's counter tracks).  Four blocks
+// keep three blocks' loads ahead, more than an HBM round trip at the recurrence's
+// pace, with half the registers and unrolled code of eight.
 template <int W>
 struct sub_shape {
     static constexpr int kItems = kB * W;
     static constexpr int kNI = (kItems + 64 * kLoaders - 1) / (64 * kLoaders);
-    static constexpr int kK = (63 / (kNI + 1) + 1) < 8 ? (63 / (kNI + 1) + 1) : 8;
+    static constexpr int kK = (63 / (kNI + 1) + 1) < 4 ? (63 / (kNI + 1) + 1) : 4;
 };
 
 // LDS of the factorisation and of the substitutions (they run one after the
